@@ -1,0 +1,190 @@
+/*
+ * tci.h — C ABI of libtci_hip.so, the MI355X (gfx950) TCI2 sweep.
+ *
+ * Everything is extern "C", returns a tci_status, throws nothing, and takes
+ * plain pointers and sizes.  A Julia `ccall` shim (or ctypes, or C++) binds it;
+ * each entry point names the reference interface it replaces
+ * (TensorCrossInterpolation.jl v0.9.19, paths relative to its repository).
+ *
+ * Threading: calls on distinct contexts are independent; calls on one
+ * context are serialized by the caller.  Every context owns one HIP stream.
+ * There is no CPU fallback: without a usable gfx950 device every compute
+ * entry point returns TCI_ERR_DEVICE.
+ */
+#ifndef TCI_H
+#define TCI_H
+
+#include <stdint.h>
+#include "tci_functors.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes; 1..9 map one-to-one onto the reference's exceptions. */
+typedef enum tci_status {
+    TCI_OK = 0,
+    TCI_ERR_NAN_L = 1,          /* ErrorException("lu.L contains NaNs")    matrixlu.jl:164-166 */
+    TCI_ERR_NAN_U = 2,          /* ErrorException("lu.U contains NaNs")    matrixlu.jl:167-169 */
+    TCI_ERR_ARG = 3,            /* ArgumentError                           tensorci2.jl:736-754 */
+    TCI_ERR_ZERO_MAXSAMPLE = 4, /* ErrorException("maxsamplevalue is zero!") tensorci2.jl:50  */
+    TCI_ERR_NAN_TENSOR = 5,     /* ErrorException("Error: NaN in tensor T[b]") tensorci2.jl:439 */
+    TCI_ERR_NONSQUARE = 6,      /* "Pivot matrix at bond b is not square!" tensorci2.jl:388    */
+    TCI_ERR_DIM = 7,            /* DimensionMismatch                       tensorci2.jl:197-199 */
+    TCI_ERR_GENERIC = 8,        /* other ErrorException                    tensorci2.jl:730-732 */
+    TCI_ERR_SINGULAR = 9,       /* SingularException from P \ Pi1          tensorci2.jl:391    */
+    TCI_ERR_CAPACITY = 10,      /* a set or Pi matrix outgrew the capacities chosen at submit */
+    TCI_ERR_OOM = 11,           /* device memory / HBM budget exhausted                       */
+    TCI_ERR_DEVICE = 12,        /* no gfx950 device, or a HIP runtime error                   */
+    TCI_ERR_PENDING = 13        /* result not ready yet (non-blocking queries)                */
+} tci_status;
+
+#define TCI_UNLIMITED INT64_MAX
+
+typedef struct tci_ctx tci_ctx;
+
+/* Library / context ---------------------------------------------------- */
+const char *tci_last_error(void);             /* thread-local message of the last failure */
+const char *tci_version(void);
+/* One context = one device + one HIP stream + an HBM budget for solve state. */
+tci_status tci_ctx_create(tci_ctx **out, int32_t device, uint64_t hbm_budget_bytes);
+tci_status tci_ctx_destroy(tci_ctx *ctx);
+tci_status tci_ctx_synchronize(tci_ctx *ctx);
+
+/* rrLU ----------------------------------------------------------------- */
+/* Replaces rrlu(A; maxrank, reltol, abstol, leftorthogonal)::rrLU{Float64}
+ * (matrixlu.jl:217-225, i.e. rrlu! 194-204 and _optimizerrlu! 141-181).
+ * Synchronous; every pointer is host memory owned by the caller.
+ * A: m×n column-major with leading dimension lda (not modified).
+ * Out: rowperm[m], colperm[n] (1-based, rrLU.rowpermutation/colpermutation),
+ *      L (m×p, leading dim ldl, cap columns), U (p×n, leading dim ldu >= cap),
+ *      *npivot = p <= cap = min(m, n, maxrank), *error = rrLU.error.
+ * Pivot order, L, U and error are bit-identical to the reference algorithm. */
+tci_status tci_rrlu_f64(tci_ctx *ctx, const double *A, int64_t lda, int64_t m, int64_t n,
+                        int64_t maxrank, double reltol, double abstol, int32_t leftorth,
+                        int64_t *rowperm, int64_t *colperm, double *L, int64_t ldl,
+                        double *U, int64_t ldu, int64_t *npivot, double *error);
+
+/* One factorization job of a batch (device memory).  A is overwritten with
+ * the packed factors exactly as rrlu!(A) leaves it (matrixlu.jl:194-204):
+ * tril(A[:,1:p]) holds L (before its unit diagonal is imposed when
+ * leftorth), triu(A[1:p,:]) holds U. */
+typedef struct tci_lu_job {
+    double *A;             /* device, m×n column-major, lda = m, in/out       */
+    int32_t *rowperm;      /* device, m entries, 1-based on return             */
+    int32_t *colperm;      /* device, n entries, 1-based on return             */
+    int64_t m, n;
+    int64_t maxrank;
+    double reltol, abstol;
+    int32_t leftorth;
+    int32_t status;        /* out: tci_status of this job                      */
+    int64_t npivot;        /* out                                              */
+    double error;          /* out                                              */
+} tci_lu_job;
+
+/* Batched rrLU of `count` independent matrices of uneven sizes in ONE kernel
+ * launch (no counterpart in the reference, which factors one matrix per
+ * call; each job's result equals rrlu! on that matrix).  `jobs` is a device
+ * array; the call is asynchronous on the context stream. */
+tci_status tci_rrlu_batched_f64(tci_ctx *ctx, tci_lu_job *jobs_device, int64_t count);
+
+/* Pi-tensor evaluation ------------------------------------------------- */
+/* Replaces _batchevaluate_dispatch(Float64, f, localdims, Iset, Jset, Val(M))
+ * (batcheval.jl:32-61) for a device functor f (include/tci_functors.h).
+ * left: nI multi-indices of length nl (nI×nl bytes, row-major, 1-based);
+ * right: nJ multi-indices of length nr; M = L - nl - nr central indices.
+ * out: (nI, d_{nl+1}, …, d_{nl+M}, nJ) column-major, host memory.
+ * *maxabs_inout = max(|*maxabs_inout|, max|out|) with NaN propagation
+ * (updatemaxsample!, tensorci2.jl:397-399).  Synchronous. */
+typedef struct tci_functor {
+    int32_t kind;          /* tci_functor_kind                                 */
+    int32_t nparams;
+    const double *params;  /* host                                             */
+    int32_t niparams;
+    const int32_t *iparams;/* host                                             */
+} tci_functor;
+
+tci_status tci_pi_eval_f64(tci_ctx *ctx, const tci_functor *f, int32_t L,
+                           const int32_t *localdims, const uint8_t *left, int64_t nI,
+                           int32_t nl, const uint8_t *right, int64_t nJ, int32_t nr,
+                           double *out, double *maxabs_inout);
+
+/* Whole TCI2 solves ----------------------------------------------------- */
+/* optimize! keyword arguments (tensorci2.jl:700-720) plus two device
+ * capacities.  Defaults: tci_options_default(). */
+typedef struct tci_options {
+    double tolerance;              /* 1e-8                                    */
+    int64_t maxbonddim;            /* TCI_UNLIMITED                           */
+    int32_t maxiter;               /* 20                                      */
+    int32_t sweepstrategy;         /* 0 :backandforth, 1 :forward, 2 :backward */
+    int32_t normalizeerror;        /* 1                                       */
+    int32_t ncheckhistory;         /* 3                                       */
+    int32_t maxnglobalpivot;       /* 5                                       */
+    int32_t nsearchglobalpivot;    /* 5                                       */
+    double tolmarginglobalsearch;  /* 10.0                                    */
+    int32_t strictlynested;        /* 0                                       */
+    int32_t checkconvglobalpivot;  /* 1                                       */
+    uint64_t seed;                 /* global pivot search PRNG seed            */
+    int32_t chi_cap;               /* device capacity for any |Iset|/|Jset|;
+                                      0 = min(maxbonddim, 64)                  */
+    int32_t reserved;
+} tci_options;
+
+void tci_options_default(tci_options *o);
+
+/* One TCI problem: crossinterpolate2(Float64, f, localdims, initialpivots;
+ * opts...) (tensorci2.jl:943-953).  Pointers are host memory, copied at
+ * submit time. */
+typedef struct tci_problem {
+    int32_t L;
+    const int32_t *localdims;
+    tci_functor f;
+    int32_t npivots;
+    const int32_t *initialpivots;  /* npivots × L, 1-based; NULL = [ones(L)]   */
+    tci_options opts;
+} tci_problem;
+
+typedef struct tci_batch tci_batch;
+
+/* Submit `count` independent solves as one launch: one workgroup per solve,
+ * every solve's pivot sets, Pi buffers and site tensors resident in HBM for
+ * the whole solve.  Asynchronous on the context stream. */
+tci_status tci_solve_submit(tci_ctx *ctx, int32_t count, const tci_problem *problems,
+                            tci_batch **out);
+/* TCI_OK when finished, TCI_ERR_PENDING while running. */
+tci_status tci_batch_query(tci_batch *b);
+tci_status tci_batch_wait(tci_batch *b);
+/* Fetch results of solve i after the batch finished (host copies). */
+typedef struct tci_result {
+    int32_t status;                /* tci_status of this solve                */
+    int32_t niter;                 /* length of ranks/errors                  */
+    int32_t L;
+    int32_t npivoterrors;
+    double maxsamplevalue;
+    double workstats[6];           /* Pi elements, argmax bytes, fused rrLU bytes,
+                                      Schur flops, #rrLU, #evaluations        */
+} tci_result;
+tci_status tci_batch_result(tci_batch *b, int32_t i, tci_result *out);
+/* ranks[niter], errors[niter] (normalized as optimize! returns them),
+ * nglobalpivots[niter]; any pointer may be NULL. */
+tci_status tci_batch_history(tci_batch *b, int32_t i, int64_t *ranks, double *errors,
+                             int64_t *nglobalpivots);
+/* linkdims[L-1] = |Iset[b+1]| (tensorci2.jl:83-85) */
+tci_status tci_batch_linkdims(tci_batch *b, int32_t i, int64_t *linkdims);
+/* which = 0: Iset[site] (entries of length site-1), 1: Jset[site] (length
+ * L-site); site is 1-based.  out receives count×len bytes, 1-based values. */
+tci_status tci_batch_indexset(tci_batch *b, int32_t i, int32_t which, int32_t site,
+                              int32_t *count, uint8_t *out, int64_t cap_bytes);
+/* site tensor T[site] (χ_{site-1} × d × χ_site column-major). */
+tci_status tci_batch_sitetensor(tci_batch *b, int32_t i, int32_t site, int64_t *dims3,
+                                double *out, int64_t cap);
+tci_status tci_batch_pivoterrors(tci_batch *b, int32_t i, double *out, int64_t cap);
+tci_status tci_batch_bonderrors(tci_batch *b, int32_t i, double *out);
+tci_status tci_batch_free(tci_batch *b);
+/* Device time of the batch's solve kernel (HIP events), milliseconds. */
+tci_status tci_batch_elapsed_ms(tci_batch *b, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TCI_H */

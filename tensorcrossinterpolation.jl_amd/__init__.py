@@ -1,0 +1,471 @@
+"""tci_amd — MI355X (gfx950) TCI2 sweep behind the reference's API.
+
+Python mirror of TensorCrossInterpolation.jl's hot-path interface
+(`crossinterpolate2`, `TensorCI2`, `optimize!` keywords, `rrlu`, `rrLU`
+accessors, batch evaluation), calling libtci_hip.so through its C ABI
+(include/tci.h).  All arithmetic runs in the gfx950 kernels; this module
+only marshals arguments and results.  There is no CPU fallback: without the
+built library or a gfx950 device every call raises.
+
+Load it with importlib (the directory name is not a Python identifier), as
+tests/conftest.py, bench.py and __graft_entry__.py do, under the name
+`tci_amd`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import warnings
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+from ._abi import (F_GAUSS_SUM, F_LORENTZ_INT, F_LORENTZ_WEIGHTED, F_QUANTICS_EXP,
+                   F_QUANTICS_GAUSS_COS, F_SUM, F_TABLE, F_TT, TCI_UNLIMITED, ProblemHolder)
+
+__all__ = [
+    "TCIError", "ArgumentError", "Context", "default_context", "rrLU", "rrlu", "rrlu_batched",
+    "left", "right", "rowindices", "colindices", "npivots", "pivoterrors", "lastpivoterror",
+    "DeviceFunctor", "Lorentz", "WeightedLorentz", "QuanticsGaussCos", "GaussSum", "SumFunctor",
+    "TableFunctor", "TTFunctor", "QuanticsExp", "batchevaluate", "TensorCI2", "crossinterpolate2",
+    "crossinterpolate2_batch", "TCI_UNLIMITED",
+]
+
+
+# --------------------------------------------------------------------------
+# errors (tci_status → the reference's exception)
+# --------------------------------------------------------------------------
+class TCIError(RuntimeError):
+    """ErrorException of the reference, or a device error; .status = tci_status."""
+
+    def __init__(self, status, msg):
+        super().__init__(f"{_abi.STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class ArgumentError(TCIError, ValueError):
+    """ArgumentError of the reference (tensorci2.jl:736-754)."""
+
+
+_MESSAGES = {
+    1: "lu.L contains NaNs", 2: "lu.U contains NaNs", 4: "maxsamplevalue is zero!",
+    5: "NaN in tensor", 6: "Pivot matrix is not square!", 9: "singular pivot matrix",
+    10: "capacity exceeded (raise chi_cap)",
+}
+
+
+def _raise(status, msg=None):
+    if msg is None:
+        msg = _abi.lib().tci_last_error().decode() or _MESSAGES.get(status, "")
+    if status == 3:
+        raise ArgumentError(status, msg)
+    raise TCIError(status, msg)
+
+
+def _check(status):
+    if status != 0:
+        _raise(status)
+
+
+# --------------------------------------------------------------------------
+# context
+# --------------------------------------------------------------------------
+class Context:
+    """One device + one HIP stream + an HBM budget (tci_ctx)."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0):
+        self._h = C.c_void_p()
+        _check(_abi.lib().tci_ctx_create(C.byref(self._h), int(device), int(hbm_budget_bytes)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self):
+        _check(_abi.lib().tci_ctx_synchronize(self._h))
+
+    def close(self):
+        if self._h:
+            _abi.lib().tci_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_DEFAULT = None
+
+
+def default_context():
+    global _DEFAULT
+    if _DEFAULT is None:
+        _DEFAULT = Context(0)
+    return _DEFAULT
+
+
+# --------------------------------------------------------------------------
+# rrLU (matrixlu.jl)
+# --------------------------------------------------------------------------
+@dataclass
+class rrLU:
+    """Fields of rrLU{Float64} (matrixlu.jl:71-91)."""
+    rowpermutation: np.ndarray
+    colpermutation: np.ndarray
+    L: np.ndarray
+    U: np.ndarray
+    leftorthogonal: bool
+    npivot: int
+    error: float
+
+    @property
+    def shape(self):
+        return (self.L.shape[0], self.U.shape[1])
+
+
+def rrlu(A, maxrank=TCI_UNLIMITED, reltol=1e-14, abstol=0.0, leftorthogonal=True, ctx=None) -> rrLU:
+    """rrlu(A; maxrank, reltol, abstol, leftorthogonal) (matrixlu.jl:217-225) on the GPU."""
+    ctx = ctx or default_context()
+    A = np.asfortranarray(A, dtype=np.float64)
+    m, n = A.shape
+    cap = max(min(m, n, int(maxrank)), 0)
+    rp = np.zeros(max(m, 1), np.int64)
+    cp = np.zeros(max(n, 1), np.int64)
+    Lb = np.zeros((max(m, 1), max(cap, 1)), order="F")
+    Ub = np.zeros((max(cap, 1), max(n, 1)), order="F")
+    p = C.c_int64()
+    err = C.c_double()
+    st = _abi.lib().tci_rrlu_f64(ctx.handle, _abi.dptr(A) if A.size else None, max(m, 1), m, n,
+                                 int(maxrank), float(reltol), float(abstol), int(bool(leftorthogonal)),
+                                 _abi.i64ptr(rp), _abi.i64ptr(cp), _abi.dptr(Lb), max(m, 1),
+                                 _abi.dptr(Ub), max(cap, 1), C.byref(p), C.byref(err))
+    _check(st)
+    pv = p.value
+    return rrLU(rp[:m].copy(), cp[:n].copy(), np.array(Lb[:m, :pv]), np.array(Ub[:pv, :n]),
+                bool(leftorthogonal), pv, err.value)
+
+
+def left(lu: rrLU, permute=True):
+    """left(lu; permute) (matrixlu.jl:374-382)."""
+    if not permute:
+        return lu.L
+    out = np.empty_like(lu.L)
+    out[lu.rowpermutation - 1, :] = lu.L
+    return out
+
+
+def right(lu: rrLU, permute=True):
+    """right(lu; permute) (matrixlu.jl:384-392)."""
+    if not permute:
+        return lu.U
+    out = np.empty_like(lu.U)
+    out[:, lu.colpermutation - 1] = lu.U
+    return out
+
+
+def rowindices(lu: rrLU):
+    return lu.rowpermutation[:lu.npivot]
+
+
+def colindices(lu: rrLU):
+    return lu.colpermutation[:lu.npivot]
+
+
+def npivots(lu: rrLU):
+    return lu.npivot
+
+
+def pivoterrors(lu: rrLU):
+    """[|diag|..., error] (matrixlu.jl:394-416)."""
+    p = lu.npivot
+    d = np.diag(lu.U[:p, :p]) if lu.leftorthogonal else np.diag(lu.L[:p, :p])
+    return np.concatenate([np.abs(d), [lu.error]])
+
+
+def lastpivoterror(lu: rrLU):
+    return lu.error
+
+
+def rrlu_batched(mats, maxrank=TCI_UNLIMITED, reltol=1e-14, abstol=0.0, leftorthogonal=True,
+                 ctx=None, device_buffers=None):
+    """Factor a list of matrices in one launch (tci_rrlu_batched_f64).
+
+    Returns (rowperms, colperms, npivots, errors, statuses) per matrix and
+    leaves the packed factors on the device.  `device_buffers` may pass an
+    already-resident torch tensor holding the concatenated column-major
+    matrices (as returned by `upload_matrices`) to skip the upload.
+    """
+    import torch
+    ctx = ctx or default_context()
+    if device_buffers is None:
+        device_buffers = upload_matrices(mats)
+    A, offs, shapes = device_buffers
+    cnt = len(shapes)
+    tot_perm = sum(m + n for m, n in shapes)
+    perm = torch.zeros(max(tot_perm, 1), dtype=torch.int32, device=A.device)
+    jobs = (_abi.tci_lu_job * cnt)()
+    po = 0
+    for i, (m, n) in enumerate(shapes):
+        j = jobs[i]
+        j.A = A.data_ptr() + 8 * offs[i]
+        j.rowperm = perm.data_ptr() + 4 * po
+        j.colperm = perm.data_ptr() + 4 * (po + m)
+        j.m, j.n, j.maxrank = m, n, int(maxrank)
+        j.reltol, j.abstol, j.leftorth = float(reltol), float(abstol), int(bool(leftorthogonal))
+        po += m + n
+    jb = torch.frombuffer(bytearray(jobs), dtype=torch.uint8).to(A.device)
+    torch.cuda.synchronize(A.device)
+    _check(_abi.lib().tci_rrlu_batched_f64(ctx.handle, C.c_void_p(jb.data_ptr()), cnt))
+    ctx.synchronize()
+    C.memmove(jobs, bytes(jb.cpu().numpy().tobytes()), C.sizeof(jobs))
+    permh = perm.cpu().numpy()
+    out = []
+    po = 0
+    for i, (m, n) in enumerate(shapes):
+        j = jobs[i]
+        out.append(dict(rowpermutation=permh[po:po + m].astype(np.int64),
+                        colpermutation=permh[po + m:po + m + n].astype(np.int64),
+                        npivot=int(j.npivot), error=float(j.error), status=int(j.status)))
+        po += m + n
+    return out
+
+
+def upload_matrices(mats):
+    """Concatenate column-major FP64 matrices into one device tensor."""
+    import torch
+    shapes = [tuple(int(x) for x in np.shape(a)) for a in mats]
+    offs, o = [], 0
+    for m, n in shapes:
+        offs.append(o)
+        o += m * n
+    host = np.empty(max(o, 1))
+    for a, off, (m, n) in zip(mats, offs, shapes):
+        host[off:off + m * n] = np.asarray(a, np.float64).ravel(order="F")
+    return torch.from_numpy(host).to("cuda"), offs, shapes
+
+
+# --------------------------------------------------------------------------
+# device functors (include/tci_functors.h)
+# --------------------------------------------------------------------------
+@dataclass
+class DeviceFunctor:
+    """A function f(σ) evaluated on the device: kind tag + parameter block."""
+    kind: int
+    params: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    iparams: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+
+    def struct(self):
+        self._p = np.ascontiguousarray(self.params, np.float64)
+        self._ip = np.ascontiguousarray(self.iparams, np.int32)
+        return _abi.tci_functor(self.kind, len(self._p), _abi.dptr(self._p) if len(self._p) else None,
+                                len(self._ip), _abi.i32ptr(self._ip) if len(self._ip) else None)
+
+
+def Lorentz(a=1.0):
+    """f(v) = a / (1 + Σ v_i²) (README.md:24-28)."""
+    return DeviceFunctor(F_LORENTZ_INT, np.array([a], np.float64))
+
+
+def WeightedLorentz(c, D=16.0):
+    """f(v) = 1 / (1 + Σ c_i (v_i/D)²)."""
+    return DeviceFunctor(F_LORENTZ_WEIGHTED, np.concatenate([[float(D)], np.asarray(c, np.float64)]))
+
+
+def QuanticsGaussCos(a, c, w, A=0.1, F=2.0 ** 20):
+    """x = Σ(σ_i−1)2^{−i}; f = Σ a_k exp(−(x−c_k)²/(2w_k²)) + A cos(F x)."""
+    K = len(a)
+    trip = np.stack([np.asarray(a, float), np.asarray(c, float), np.asarray(w, float)], 1).ravel()
+    return DeviceFunctor(F_QUANTICS_GAUSS_COS, np.concatenate([[K], trip, [A, F]]))
+
+
+def GaussSum(a, w, c, d):
+    """x_i = (σ_i−1)/(d−1); f = Σ_k a_k exp(−Σ_i (x_i−c_{k,i})²/(2w_k²)); c is K×L."""
+    a = np.asarray(a, float)
+    K = len(a)
+    return DeviceFunctor(F_GAUSS_SUM, np.concatenate([[K, float(d)], a, np.asarray(w, float),
+                                                       np.asarray(c, float).ravel()]))
+
+
+def SumFunctor(s=1.0):
+    """f(v) = s·Σ v_i (test_batcheval.jl:13-35)."""
+    return DeviceFunctor(F_SUM, np.array([s], np.float64))
+
+
+def TableFunctor(table):
+    """f(v) = table[v_1, …, v_L] for a small dense table (column-major)."""
+    t = np.asarray(table, np.float64)
+    return DeviceFunctor(F_TABLE, t.ravel(order="F"), np.array(t.shape, np.int32))
+
+
+def TTFunctor(cores):
+    """f(v) = T_1[1,v_1,:]·…·T_L[:,v_L,1] for given cores (χ_{i-1}, d_i, χ_i)."""
+    L = len(cores)
+    chis = [cores[0].shape[0]] + [c.shape[2] for c in cores]
+    ds = [c.shape[1] for c in cores]
+    params = np.concatenate([np.asarray(c, np.float64).ravel(order="F") for c in cores])
+    return DeviceFunctor(F_TT, params, np.array(chis + ds, np.int32))
+
+
+def QuanticsExp(alpha1, beta1, alpha2=0.0, beta2=0.0):
+    """x = Σ(σ_i−1)2^{−i}; f = α₁ exp(β₁x) + α₂ exp(β₂x)."""
+    return DeviceFunctor(F_QUANTICS_EXP, np.array([alpha1, beta1, alpha2, beta2], np.float64))
+
+
+def batchevaluate(f: DeviceFunctor, localdims, leftindexset, rightindexset, M, ctx=None):
+    """_batchevaluate_dispatch(Float64, f, localdims, Iset, Jset, Val(M)) (batcheval.jl:32-61).
+
+    Returns (array of shape (|Iset|, d…, |Jset|), max|array|)."""
+    ctx = ctx or default_context()
+    localdims = np.ascontiguousarray(localdims, np.int32)
+    L = len(localdims)
+    nI, nJ = len(leftindexset), len(rightindexset)
+    nl = len(leftindexset[0]) if nI else 0
+    nr = len(rightindexset[0]) if nJ else 0
+    if nI and nJ and nl + M + nr != L:
+        raise ArgumentError(3, "Invalid number of central indices")
+    shape = (nI,) + tuple(int(d) for d in localdims[nl:nl + M]) + (nJ,)
+    if nI * nJ == 0:
+        return np.zeros(tuple(0 for _ in shape)), 0.0
+    left = np.ascontiguousarray(np.asarray(leftindexset, np.uint8).reshape(nI, nl))
+    right = np.ascontiguousarray(np.asarray(rightindexset, np.uint8).reshape(nJ, nr))
+    out = np.zeros(int(np.prod(shape)))
+    mx = C.c_double(0.0)
+    fs = f.struct()
+    _check(_abi.lib().tci_pi_eval_f64(ctx.handle, C.byref(fs), L, _abi.i32ptr(localdims),
+                                      _abi.u8ptr(left), nI, nl, _abi.u8ptr(right), nJ, nr,
+                                      _abi.dptr(out), C.byref(mx)))
+    return out.reshape(shape, order="F"), mx.value
+
+
+# --------------------------------------------------------------------------
+# TensorCI2 / crossinterpolate2 (tensorci2.jl)
+# --------------------------------------------------------------------------
+class TensorCI2:
+    """Result of a device solve with the fields of TensorCI2{Float64} (tensorci2.jl:6-40).
+
+    Iset[b]/Jset[b] are lists of 1-based multi-indices (b = 0..L-1 for sites
+    1..L); sitetensors[b] has shape (χ_{b}, d_b, χ_{b+1})."""
+
+    def __init__(self, batch, i, localdims):
+        lib = _abi.lib()
+        self.localdims = [int(d) for d in localdims]
+        L = len(self.localdims)
+        r = _abi.tci_result()
+        _check(lib.tci_batch_result(batch, i, C.byref(r)))
+        self.status = r.status
+        self.niter = r.niter
+        self.maxsamplevalue = r.maxsamplevalue
+        self.workstats = dict(zip(["pi_elements", "rrlu_argmax_bytes", "rrlu_fused_bytes",
+                                   "schur_flops", "n_rrlu", "n_eval"], list(r.workstats)))
+        n = max(r.niter, 1)
+        ranks = np.zeros(n, np.int64)
+        errs = np.zeros(n)
+        ng = np.zeros(n, np.int64)
+        lib.tci_batch_history(batch, i, _abi.i64ptr(ranks), _abi.dptr(errs), _abi.i64ptr(ng))
+        self.ranks, self.errors, self.nglobalpivots = ranks[:r.niter], errs[:r.niter], ng[:r.niter]
+        pe = np.zeros(max(r.npivoterrors, 1))
+        lib.tci_batch_pivoterrors(batch, i, _abi.dptr(pe), len(pe))
+        self.pivoterrors = pe[:r.npivoterrors]
+        be = np.zeros(max(L - 1, 1))
+        lib.tci_batch_bonderrors(batch, i, _abi.dptr(be))
+        self.bonderrors = be[:L - 1]
+        self.Iset, self.Jset = [], []
+        for which, dst in ((0, self.Iset), (1, self.Jset)):
+            for site in range(1, L + 1):
+                ln = site - 1 if which == 0 else L - site
+                cnt = C.c_int32()
+                buf = np.zeros(1 << 16, np.uint8)
+                lib.tci_batch_indexset(batch, i, which, site, C.byref(cnt), _abi.u8ptr(buf), len(buf))
+                if cnt.value * ln > len(buf):
+                    buf = np.zeros(cnt.value * ln, np.uint8)
+                    lib.tci_batch_indexset(batch, i, which, site, C.byref(cnt), _abi.u8ptr(buf), len(buf))
+                dst.append([list(map(int, buf[e * ln:(e + 1) * ln])) for e in range(cnt.value)])
+        self.sitetensors = []
+        if self.status == 0:
+            for site in range(1, L + 1):
+                d3 = np.zeros(3, np.int64)
+                lib.tci_batch_sitetensor(batch, i, site, _abi.i64ptr(d3), None, 0)
+                out = np.zeros(max(int(np.prod(d3)), 1))
+                _check(lib.tci_batch_sitetensor(batch, i, site, _abi.i64ptr(d3), _abi.dptr(out), len(out)))
+                self.sitetensors.append(out[:int(np.prod(d3))].reshape(tuple(d3), order="F"))
+
+    def __len__(self):
+        return len(self.localdims)
+
+    def linkdims(self):
+        """[|Iset[b+1]|] (tensorci2.jl:83-85)."""
+        return [len(self.Iset[b]) for b in range(1, len(self.localdims))]
+
+    def rank(self):
+        return max(self.linkdims())
+
+    def pivoterror(self):
+        """max(bonderrors) (tensorci2.jl:157-159)."""
+        return float(np.max(self.bonderrors))
+
+
+def _resolve_tolerance(tolerance, pivottolerance):
+    """optimize! tolerance/pivottolerance resolution (tensorci2.jl:734-746)."""
+    if pivottolerance is not None:
+        if tolerance is not None and tolerance != pivottolerance:
+            raise ArgumentError(3, "Got different values for pivottolerance and tolerance in "
+                                   "optimize!(TCI2). For TCI2, both of these options have the same "
+                                   "meaning. Please assign only `tolerance`.")
+        warnings.warn("The option `pivottolerance` of `optimize!(tci::TensorCI2, f)` is deprecated. "
+                      "Please update your code to use `tolerance`, as `pivottolerance` will be "
+                      "removed in the future.", DeprecationWarning, stacklevel=3)
+        return pivottolerance
+    return 1e-8 if tolerance is None else tolerance
+
+
+def _holder(f, localdims, initialpivots, kw):
+    kw = dict(kw)
+    tol = _resolve_tolerance(kw.pop("tolerance", None), kw.pop("pivottolerance", None))
+    if kw.pop("pivotsearch", "full") != "full":
+        raise ArgumentError(3, "only pivotsearch=:full runs on the device")
+    kw.pop("checkbatchevaluatable", None)   # device functors are batch evaluators
+    kw.pop("verbosity", None)
+    kw.pop("loginterval", None)
+    if not isinstance(f, DeviceFunctor):
+        raise TypeError("f must be a DeviceFunctor (a Julia/Python closure cannot run on the GPU)")
+    return ProblemHolder(localdims, f.kind, f.params, f.iparams, initialpivots, tolerance=tol, **kw)
+
+
+def crossinterpolate2_batch(problems, ctx=None, return_batch=False):
+    """Submit many crossinterpolate2 problems as ONE launch.
+
+    problems: list of (f, localdims, initialpivots_or_None, kwargs).
+    Returns a list of (TensorCI2, ranks, errors) in order."""
+    ctx = ctx or default_context()
+    holders = [_holder(f, ld, piv, kw) for f, ld, piv, kw in problems]
+    arr = (_abi.tci_problem * len(holders))(*[h.st for h in holders])
+    b = C.c_void_p()
+    _check(_abi.lib().tci_solve_submit(ctx.handle, len(holders), arr, C.byref(b)))
+    try:
+        _check(_abi.lib().tci_batch_wait(b))
+        out = []
+        for i, (f, ld, piv, kw) in enumerate(problems):
+            t = TensorCI2(b, i, ld)
+            out.append(t)
+        if return_batch:
+            ms = C.c_float()
+            _abi.lib().tci_batch_elapsed_ms(b, C.byref(ms))
+            return out, ms.value
+        return out
+    finally:
+        _abi.lib().tci_batch_free(b)
+
+
+def crossinterpolate2(ValueType, f, localdims, initialpivots=None, ctx=None, **kwargs):
+    """crossinterpolate2(Float64, f, localdims, initialpivots; kwargs...) (tensorci2.jl:943-953).
+
+    Returns (tci, ranks, errors) like the reference; raises the reference's
+    errors (ArgumentError, ErrorException) as TCIError subclasses."""
+    if ValueType not in (float, np.float64, "Float64"):
+        raise TypeError("only Float64 solves run on the device")
+    (t,) = crossinterpolate2_batch([(f, localdims, initialpivots, kwargs)], ctx=ctx)
+    if t.status != 0:
+        _raise(t.status, _MESSAGES.get(t.status, ""))
+    return t, t.ranks, t.errors

@@ -1,0 +1,173 @@
+"""ctypes mirror of include/tci.h and the loader of libtci_hip.so.
+
+The library is built in-tree (`make -C tensorcrossinterpolation.jl_amd/csrc`,
+or `__graft_entry__.build()`); importing this module never builds it.  There
+is no CPU fallback: if the library or a gfx950 device is missing, the entry
+points raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libtci_hip.so")
+
+TCI_UNLIMITED = (1 << 63) - 1
+
+# tci_status
+STATUS = {
+    0: "TCI_OK", 1: "TCI_ERR_NAN_L", 2: "TCI_ERR_NAN_U", 3: "TCI_ERR_ARG",
+    4: "TCI_ERR_ZERO_MAXSAMPLE", 5: "TCI_ERR_NAN_TENSOR", 6: "TCI_ERR_NONSQUARE",
+    7: "TCI_ERR_DIM", 8: "TCI_ERR_GENERIC", 9: "TCI_ERR_SINGULAR", 10: "TCI_ERR_CAPACITY",
+    11: "TCI_ERR_OOM", 12: "TCI_ERR_DEVICE", 13: "TCI_ERR_PENDING",
+}
+TCI_OK, TCI_ERR_PENDING = 0, 13
+
+# tci_functor_kind
+F_LORENTZ_INT = 1
+F_LORENTZ_WEIGHTED = 2
+F_QUANTICS_GAUSS_COS = 3
+F_GAUSS_SUM = 4
+F_SUM = 5
+F_TABLE = 6
+F_TT = 7
+F_QUANTICS_EXP = 8
+
+
+class tci_functor(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("nparams", C.c_int32), ("params", C.POINTER(C.c_double)),
+                ("niparams", C.c_int32), ("iparams", C.POINTER(C.c_int32))]
+
+
+class tci_options(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("maxbonddim", C.c_int64), ("maxiter", C.c_int32),
+                ("sweepstrategy", C.c_int32), ("normalizeerror", C.c_int32),
+                ("ncheckhistory", C.c_int32), ("maxnglobalpivot", C.c_int32),
+                ("nsearchglobalpivot", C.c_int32), ("tolmarginglobalsearch", C.c_double),
+                ("strictlynested", C.c_int32), ("checkconvglobalpivot", C.c_int32),
+                ("seed", C.c_uint64), ("chi_cap", C.c_int32), ("reserved", C.c_int32)]
+
+
+class tci_problem(C.Structure):
+    _fields_ = [("L", C.c_int32), ("localdims", C.POINTER(C.c_int32)), ("f", tci_functor),
+                ("npivots", C.c_int32), ("initialpivots", C.POINTER(C.c_int32)),
+                ("opts", tci_options)]
+
+
+class tci_result(C.Structure):
+    _fields_ = [("status", C.c_int32), ("niter", C.c_int32), ("L", C.c_int32),
+                ("npivoterrors", C.c_int32), ("maxsamplevalue", C.c_double),
+                ("workstats", C.c_double * 6)]
+
+
+class tci_lu_job(C.Structure):
+    _fields_ = [("A", C.c_void_p), ("rowperm", C.c_void_p), ("colperm", C.c_void_p),
+                ("m", C.c_int64), ("n", C.c_int64), ("maxrank", C.c_int64),
+                ("reltol", C.c_double), ("abstol", C.c_double), ("leftorth", C.c_int32),
+                ("status", C.c_int32), ("npivot", C.c_int64), ("error", C.c_double)]
+
+
+def dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def i64ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def i32ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def u8ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+SWEEP = {"backandforth": 0, "forward": 1, "backward": 2}
+
+
+def make_options(tolerance=1e-8, maxbonddim=TCI_UNLIMITED, maxiter=20, sweepstrategy="backandforth",
+                 normalizeerror=True, ncheckhistory=3, maxnglobalpivot=5, nsearchglobalpivot=5,
+                 tolmarginglobalsearch=10.0, strictlynested=False, checkconvglobalpivot=True,
+                 seed=0, chi_cap=0):
+    if isinstance(sweepstrategy, str):
+        sweepstrategy = SWEEP[sweepstrategy]
+    return tci_options(float(tolerance), int(maxbonddim), int(maxiter), int(sweepstrategy),
+                       int(bool(normalizeerror)), int(ncheckhistory), int(maxnglobalpivot),
+                       int(nsearchglobalpivot), float(tolmarginglobalsearch), int(bool(strictlynested)),
+                       int(bool(checkconvglobalpivot)), int(seed) & ((1 << 64) - 1), int(chi_cap), 0)
+
+
+class ProblemHolder:
+    """Owns the numpy buffers a tci_problem points into."""
+
+    def __init__(self, localdims, kind, params=(), iparams=(), initialpivots=None, **opts):
+        self.localdims = np.ascontiguousarray(localdims, np.int32)
+        self.L = len(self.localdims)
+        self.params = np.ascontiguousarray(np.asarray(params, np.float64).ravel())
+        self.iparams = np.ascontiguousarray(np.asarray(iparams, np.int32).ravel())
+        if initialpivots is None:
+            initialpivots = [[1] * self.L]
+        self.pivots = np.ascontiguousarray(np.asarray(initialpivots, np.int32).reshape(-1, self.L))
+        f = tci_functor(int(kind), len(self.params), dptr(self.params) if len(self.params) else None,
+                        len(self.iparams), i32ptr(self.iparams) if len(self.iparams) else None)
+        self.st = tci_problem(self.L, i32ptr(self.localdims), f, self.pivots.shape[0],
+                              i32ptr(self.pivots), make_options(**opts))
+
+
+_LIB = None
+
+
+def lib():
+    """Load libtci_hip.so (built in-tree).  Raises if it is missing."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise OSError(f"libtci_hip.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    vp = C.c_void_p
+    L.tci_last_error.restype = C.c_char_p
+    L.tci_version.restype = C.c_char_p
+    L.tci_ctx_create.argtypes = [P(vp), C.c_int32, C.c_uint64]
+    L.tci_ctx_destroy.argtypes = [vp]
+    L.tci_ctx_synchronize.argtypes = [vp]
+    L.tci_rrlu_f64.argtypes = [vp, P(C.c_double), C.c_int64, C.c_int64, C.c_int64, C.c_int64,
+                               C.c_double, C.c_double, C.c_int32, P(C.c_int64), P(C.c_int64),
+                               P(C.c_double), C.c_int64, P(C.c_double), C.c_int64, P(C.c_int64),
+                               P(C.c_double)]
+    L.tci_rrlu_batched_f64.argtypes = [vp, vp, C.c_int64]
+    L.tci_pi_eval_f64.argtypes = [vp, P(tci_functor), C.c_int32, P(C.c_int32), P(C.c_uint8),
+                                  C.c_int64, C.c_int32, P(C.c_uint8), C.c_int64, C.c_int32,
+                                  P(C.c_double), P(C.c_double)]
+    L.tci_options_default.argtypes = [P(tci_options)]
+    L.tci_options_default.restype = None
+    L.tci_solve_submit.argtypes = [vp, C.c_int32, P(tci_problem), P(vp)]
+    L.tci_batch_query.argtypes = [vp]
+    L.tci_batch_wait.argtypes = [vp]
+    L.tci_batch_result.argtypes = [vp, C.c_int32, P(tci_result)]
+    L.tci_batch_history.argtypes = [vp, C.c_int32, P(C.c_int64), P(C.c_double), P(C.c_int64)]
+    L.tci_batch_linkdims.argtypes = [vp, C.c_int32, P(C.c_int64)]
+    L.tci_batch_indexset.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, P(C.c_int32),
+                                     P(C.c_uint8), C.c_int64]
+    L.tci_batch_sitetensor.argtypes = [vp, C.c_int32, C.c_int32, P(C.c_int64), P(C.c_double),
+                                       C.c_int64]
+    L.tci_batch_pivoterrors.argtypes = [vp, C.c_int32, P(C.c_double), C.c_int64]
+    L.tci_batch_bonderrors.argtypes = [vp, C.c_int32, P(C.c_double)]
+    L.tci_batch_free.argtypes = [vp]
+    L.tci_batch_elapsed_ms.argtypes = [vp, P(C.c_float)]
+    _LIB = L
+    return L
+
+
+EXPORTED_SYMBOLS = [
+    "tci_last_error", "tci_version", "tci_ctx_create", "tci_ctx_destroy", "tci_ctx_synchronize",
+    "tci_rrlu_f64", "tci_rrlu_batched_f64", "tci_pi_eval_f64", "tci_options_default",
+    "tci_solve_submit", "tci_batch_query", "tci_batch_wait", "tci_batch_result",
+    "tci_batch_history", "tci_batch_linkdims", "tci_batch_indexset", "tci_batch_sitetensor",
+    "tci_batch_pivoterrors", "tci_batch_bonderrors", "tci_batch_free", "tci_batch_elapsed_ms",
+]

@@ -1,0 +1,546 @@
+// tci_api.hip — the C ABI of libtci_hip.so (declared in include/tci.h).
+//
+// Host-side plumbing only: argument checks, device memory, copies and
+// launches on the context's HIP stream.  All arithmetic of the TCI path runs
+// in the gfx950 kernels of tci_kernels.hip; there is no CPU fallback.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/tci.h"
+#include "tci_kernels.h"
+#include "tci_layout.h"
+#include "tci_problem.h"
+
+using namespace tci;
+
+static thread_local std::string g_err;
+
+// Base.max on Float64 (NaN-propagating)
+static double nan_max(double a, double b)
+{
+    if (a != a || b != b) return a + b;
+    return a > b ? a : b;
+}
+
+static tci_status set_err(tci_status code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return set_err(TCI_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+struct tci_ctx {
+    int device;
+    hipStream_t stream;
+    uint64_t budget;
+    uint64_t used;
+};
+
+struct tci_batch {
+    tci_ctx *ctx;
+    int count;
+    std::vector<SolveDesc> desc;      // host copy (device pointers)
+    std::vector<size_t> rep_off;
+    uint8_t *d_in = nullptr, *d_state = nullptr, *d_rep = nullptr;
+    SolveDesc *d_desc = nullptr;
+    size_t bytes = 0, rep_bytes = 0;
+    std::vector<uint8_t> h_in;        // upload image, alive until the copy ran
+    std::vector<uint8_t> h_rep;
+    bool fetched = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+extern "C" {
+
+const char *tci_last_error(void) { return g_err.c_str(); }
+const char *tci_version(void) { return "tci-mi355x 0.1 (gfx950, FP64)"; }
+
+tci_status tci_ctx_create(tci_ctx **out, int32_t device, uint64_t hbm_budget_bytes)
+{
+    if (!out) return set_err(TCI_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        return set_err(TCI_ERR_DEVICE, "no HIP device available (this library has no CPU fallback)");
+    if (device < 0 || device >= n) return set_err(TCI_ERR_DEVICE, "device %d out of range", device);
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return set_err(TCI_ERR_DEVICE, "device %d is %s; this library is built for gfx950 only",
+                       device, prop.gcnArchName);
+    HIPCHK(hipSetDevice(device));
+    tci_ctx *c = new tci_ctx();
+    c->device = device;
+    c->used = 0;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return set_err(TCI_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    if (hbm_budget_bytes == 0) {
+        size_t fr = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fr, &tot));
+        hbm_budget_bytes = (uint64_t)(0.8 * (double)fr);
+    }
+    c->budget = hbm_budget_bytes;
+    *out = c;
+    return TCI_OK;
+}
+
+tci_status tci_ctx_destroy(tci_ctx *ctx)
+{
+    if (!ctx) return TCI_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return TCI_OK;
+}
+
+tci_status tci_ctx_synchronize(tci_ctx *ctx)
+{
+    if (!ctx) return set_err(TCI_ERR_ARG, "ctx is NULL");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return TCI_OK;
+}
+
+// ------------------------------------------------------------------- rrLU
+tci_status tci_rrlu_batched_f64(tci_ctx *ctx, tci_lu_job *jobs, int64_t count)
+{
+    if (!ctx || (!jobs && count > 0) || count < 0) return set_err(TCI_ERR_ARG, "bad arguments");
+    if (count == 0) return TCI_OK;
+    if (count > INT32_MAX) return set_err(TCI_ERR_ARG, "too many jobs");
+    HIPCHK(hipSetDevice(ctx->device));
+    // job shapes decide the staging placement; read them once
+    std::vector<tci_lu_job> h((size_t)count);
+    HIPCHK(hipMemcpyAsync(h.data(), jobs, sizeof(tci_lu_job) * (size_t)count, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    int64_t mx = 0;
+    for (const tci_lu_job &j : h) {
+        if (j.m < 0 || j.n < 0 || j.m > INT32_MAX / 2 || j.n > INT32_MAX / 2 || j.m * j.n > ((int64_t)1 << 31))
+            return set_err(TCI_ERR_ARG, "job shape out of range");
+        if ((j.m > 0 && (!j.A || !j.rowperm)) || (j.n > 0 && (!j.A || !j.colperm)))
+            return set_err(TCI_ERR_ARG, "job has NULL buffers");
+        if (j.maxrank < 0) return set_err(TCI_ERR_ARG, "maxrank must be >= 0");
+        int64_t s = j.m > j.n ? j.m : j.n;
+        mx = s > mx ? s : mx;
+    }
+    double *gstage = nullptr;
+    const int64_t lds_cap = 48 * 1024 / 16;
+    if (mx > lds_cap) HIPCHK(hipMallocAsync((void **)&gstage, sizeof(double) * 2 * mx * count, ctx->stream));
+    hipError_t e = launch_rrlu(jobs, count, mx, gstage, 2 * mx, ctx->stream);
+    if (gstage) hipFreeAsync(gstage, ctx->stream);
+    if (e != hipSuccess) return set_err(TCI_ERR_DEVICE, "rrlu launch: %s", hipGetErrorString(e));
+    return TCI_OK;
+}
+
+tci_status tci_rrlu_f64(tci_ctx *ctx, const double *A, int64_t lda, int64_t m, int64_t n,
+                        int64_t maxrank, double reltol, double abstol, int32_t leftorth,
+                        int64_t *rowperm, int64_t *colperm, double *Lh, int64_t ldl, double *Uh,
+                        int64_t ldu, int64_t *npivot, double *error)
+{
+    if (!ctx || m < 0 || n < 0 || lda < (m > 0 ? m : 1) || maxrank < 0 || !npivot || !error)
+        return set_err(TCI_ERR_ARG, "bad arguments");
+    if ((m > 0 && n > 0 && !A) || (m > 0 && !rowperm) || (n > 0 && !colperm))
+        return set_err(TCI_ERR_ARG, "NULL buffer");
+    int64_t cap = m < n ? m : n;
+    if (maxrank < cap) cap = maxrank;
+    if ((cap > 0 && m > 0 && (!Lh || ldl < m)) || (cap > 0 && n > 0 && (!Uh || ldu < cap)))
+        return set_err(TCI_ERR_ARG, "L/U buffers too small");
+    if (m * n > ((int64_t)1 << 31)) return set_err(TCI_ERR_ARG, "matrix too large");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const size_t mn = (size_t)(m * n);
+    double *dA = nullptr, *dL = nullptr, *dU = nullptr;
+    int32_t *dperm = nullptr;
+    tci_lu_job *dj = nullptr;
+    HIPCHK(hipMallocAsync((void **)&dA, sizeof(double) * (mn ? mn : 1), st));
+    HIPCHK(hipMallocAsync((void **)&dperm, sizeof(int32_t) * (size_t)(m + n + 1), st));
+    HIPCHK(hipMallocAsync((void **)&dj, sizeof(tci_lu_job), st));
+    if (mn) HIPCHK(hipMemcpy2DAsync(dA, sizeof(double) * m, A, sizeof(double) * lda, sizeof(double) * m,
+                                    n, hipMemcpyHostToDevice, st));
+    tci_lu_job j;
+    memset(&j, 0, sizeof j);
+    j.A = dA;
+    j.rowperm = dperm;
+    j.colperm = dperm + m;
+    j.m = m;
+    j.n = n;
+    j.maxrank = maxrank;
+    j.reltol = reltol;
+    j.abstol = abstol;
+    j.leftorth = leftorth ? 1 : 0;
+    HIPCHK(hipMemcpyAsync(dj, &j, sizeof j, hipMemcpyHostToDevice, st));
+    tci_status s = tci_rrlu_batched_f64(ctx, dj, 1);
+    if (s != TCI_OK) return s;
+    HIPCHK(hipMemcpyAsync(&j, dj, sizeof j, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const int64_t p = j.npivot;
+    if (p > 0) {
+        HIPCHK(hipMallocAsync((void **)&dL, sizeof(double) * (size_t)(m * p + 1), st));
+        HIPCHK(hipMallocAsync((void **)&dU, sizeof(double) * (size_t)(p * n + 1), st));
+        HIPCHK(launch_lu_unpack(dA, m, n, p, j.leftorth, dL, m, dU, p, st));
+        HIPCHK(hipMemcpy2DAsync(Lh, sizeof(double) * ldl, dL, sizeof(double) * m, sizeof(double) * m, p,
+                                hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpy2DAsync(Uh, sizeof(double) * ldu, dU, sizeof(double) * p, sizeof(double) * p, n,
+                                hipMemcpyDeviceToHost, st));
+    }
+    std::vector<int32_t> perm((size_t)(m + n + 1));
+    HIPCHK(hipMemcpyAsync(perm.data(), dperm, sizeof(int32_t) * (size_t)(m + n), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int64_t i = 0; i < m; ++i) rowperm[i] = perm[(size_t)i];
+    for (int64_t q = 0; q < n; ++q) colperm[q] = perm[(size_t)(m + q)];
+    hipFreeAsync(dA, st);
+    hipFreeAsync(dperm, st);
+    hipFreeAsync(dj, st);
+    if (dL) hipFreeAsync(dL, st);
+    if (dU) hipFreeAsync(dU, st);
+    *npivot = p;
+    *error = j.error;
+    if (j.status == TCI_ERR_NAN_L) return set_err(TCI_ERR_NAN_L, "lu.L contains NaNs");
+    if (j.status == TCI_ERR_NAN_U) return set_err(TCI_ERR_NAN_U, "lu.U contains NaNs");
+    return TCI_OK;
+}
+
+// ------------------------------------------------------------ Pi tensors
+tci_status tci_pi_eval_f64(tci_ctx *ctx, const tci_functor *f, int32_t L, const int32_t *localdims,
+                           const uint8_t *left, int64_t nI, int32_t nl, const uint8_t *right,
+                           int64_t nJ, int32_t nr, double *out, double *maxabs_inout)
+{
+    if (!ctx || !f || !localdims || L < 1 || nl < 0 || nr < 0 || nl + nr > L || nI < 0 || nJ < 0)
+        return set_err(TCI_ERR_ARG, "bad arguments");
+    const int M = L - nl - nr;
+    int64_t nc = 1;
+    for (int t = 0; t < M; ++t) nc *= localdims[nl + t];
+    const int64_t nrow = nI * nc, tot = nrow * nJ;
+    if (tot == 0) return TCI_OK;   // empty Pi (batcheval.jl:40-42)
+    if (!out || (nI > 0 && nl > 0 && !left) || (nJ > 0 && nr > 0 && !right))
+        return set_err(TCI_ERR_ARG, "NULL buffer");
+    // validate the functor against a one-point problem description
+    tci_problem pr;
+    memset(&pr, 0, sizeof pr);
+    pr.L = L;
+    pr.localdims = localdims;
+    pr.f = *f;
+    tci_options_default(&pr.opts);
+    SolveCaps c;
+    std::string msg;
+    if (L >= 2) {
+        int s = check_problem(pr, c, &msg);
+        if (s != TCI_OK) return set_err((tci_status)s, "%s", msg.c_str());
+    }
+    for (int64_t i = 0; i < nI * nl; ++i)
+        if (left[i] < 1 || left[i] > localdims[i % nl]) return set_err(TCI_ERR_ARG, "left index out of range");
+    for (int64_t j = 0; j < nJ * nr; ++j)
+        if (right[j] < 1 || right[j] > localdims[L - nr + j % nr]) return set_err(TCI_ERR_ARG, "right index out of range");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int blocks = eval_blocks(tot);
+    size_t need = 0;
+    auto a = [&](size_t b) { size_t at = need; need = lay_align(need + (b ? b : 1)); return at; };
+    size_t o_out = a(sizeof(double) * (size_t)tot);
+    size_t o_bm = a(sizeof(double) * (size_t)(blocks + 1));
+    size_t o_left = a((size_t)(nI * nl));
+    size_t o_right = a((size_t)(nJ * nr));
+    size_t o_rows = a((size_t)(nrow * (nl + M)));
+    size_t o_cd = a(sizeof(int32_t) * (size_t)(M + 1));
+    size_t o_par = a(sizeof(double) * (size_t)f->nparams);
+    size_t o_ipar = a(sizeof(int32_t) * (size_t)f->niparams);
+    uint8_t *d = nullptr;
+    HIPCHK(hipMallocAsync((void **)&d, need, st));
+    if (nI * nl) HIPCHK(hipMemcpyAsync(d + o_left, left, (size_t)(nI * nl), hipMemcpyHostToDevice, st));
+    if (nJ * nr) HIPCHK(hipMemcpyAsync(d + o_right, right, (size_t)(nJ * nr), hipMemcpyHostToDevice, st));
+    if (M) HIPCHK(hipMemcpyAsync(d + o_cd, localdims + nl, sizeof(int32_t) * M, hipMemcpyHostToDevice, st));
+    if (f->nparams) HIPCHK(hipMemcpyAsync(d + o_par, f->params, sizeof(double) * f->nparams, hipMemcpyHostToDevice, st));
+    if (f->niparams) HIPCHK(hipMemcpyAsync(d + o_ipar, f->iparams, sizeof(int32_t) * f->niparams, hipMemcpyHostToDevice, st));
+    const uint8_t *rows = d + o_left;
+    int rstride = nl;
+    if (M > 0) {
+        HIPCHK(launch_kron_center(d + o_left, nI, nl, (const int32_t *)(d + o_cd), M, nc, d + o_rows,
+                                  nl + M, st));
+        rows = d + o_rows;
+        rstride = nl + M;
+    }
+    HIPCHK(launch_eval(f->kind, (const double *)(d + o_par), (const int32_t *)(d + o_ipar), L, rows,
+                       nrow, rstride, nl + M, d + o_right, nJ, nr, (double *)(d + o_out),
+                       (double *)(d + o_bm), st));
+    HIPCHK(hipMemcpyAsync(out, d + o_out, sizeof(double) * (size_t)tot, hipMemcpyDeviceToHost, st));
+    std::vector<double> bm((size_t)blocks);
+    HIPCHK(hipMemcpyAsync(bm.data(), d + o_bm, sizeof(double) * blocks, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    hipFreeAsync(d, st);
+    if (maxabs_inout) {
+        double mx = fabs(*maxabs_inout);
+        for (double v : bm) mx = nan_max(mx, v);
+        *maxabs_inout = mx;
+    }
+    return TCI_OK;
+}
+
+// ------------------------------------------------------------- solves
+void tci_options_default(tci_options *o)
+{
+    if (!o) return;
+    memset(o, 0, sizeof *o);
+    o->tolerance = 1e-8;
+    o->maxbonddim = TCI_UNLIMITED;
+    o->maxiter = 20;
+    o->sweepstrategy = 0;
+    o->normalizeerror = 1;
+    o->ncheckhistory = 3;
+    o->maxnglobalpivot = 5;
+    o->nsearchglobalpivot = 5;
+    o->tolmarginglobalsearch = 10.0;
+    o->strictlynested = 0;
+    o->checkconvglobalpivot = 1;
+    o->seed = 0;
+    o->chi_cap = 0;
+}
+
+static void batch_release(tci_batch *b)
+{
+    if (!b) return;
+    if (b->d_desc) hipFree(b->d_desc);
+    if (b->d_in) hipFree(b->d_in);
+    if (b->d_state) hipFree(b->d_state);
+    if (b->d_rep) hipFree(b->d_rep);
+    if (b->ev0) hipEventDestroy(b->ev0);
+    if (b->ev1) hipEventDestroy(b->ev1);
+    if (b->ctx) b->ctx->used -= b->bytes;
+    delete b;
+}
+
+tci_status tci_solve_submit(tci_ctx *ctx, int32_t count, const tci_problem *problems, tci_batch **out)
+{
+    if (!ctx || !out || count < 1 || !problems) return set_err(TCI_ERR_ARG, "bad arguments");
+    *out = nullptr;
+    std::vector<SolveCaps> caps((size_t)count);
+    std::vector<size_t> ib((size_t)count), sb((size_t)count), rb((size_t)count);
+    size_t tin = 0, tstate = 0, trep = 0;
+    int max_mr = 0;
+    for (int i = 0; i < count; ++i) {
+        std::string msg;
+        int s = check_problem(problems[i], caps[(size_t)i], &msg);
+        if (s != TCI_OK) return set_err((tci_status)s, "problem %d: %s", i, msg.c_str());
+        layout_solve(caps[(size_t)i], nullptr, nullptr, nullptr, nullptr, &ib[(size_t)i], &sb[(size_t)i], &rb[(size_t)i]);
+        tin += ib[(size_t)i];
+        tstate += sb[(size_t)i];
+        trep += rb[(size_t)i];
+        if (caps[(size_t)i].mrcap > max_mr) max_mr = caps[(size_t)i].mrcap;
+    }
+    const size_t total = tin + tstate + trep + sizeof(SolveDesc) * (size_t)count;
+    if (ctx->used + total > ctx->budget)
+        return set_err(TCI_ERR_OOM, "batch needs %zu bytes, HBM budget has %llu left", total,
+                       (unsigned long long)(ctx->budget - ctx->used));
+    HIPCHK(hipSetDevice(ctx->device));
+    tci_batch *b = new tci_batch();
+    b->ctx = nullptr;
+    b->count = count;
+    auto fail = [&](hipError_t e, const char *what) {
+        batch_release(b);
+        return set_err(e == hipErrorOutOfMemory ? TCI_ERR_OOM : TCI_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+    };
+    hipError_t e;
+    if ((e = hipMalloc((void **)&b->d_in, tin)) != hipSuccess) return fail(e, "hipMalloc(inputs)");
+    if ((e = hipMalloc((void **)&b->d_state, tstate)) != hipSuccess) return fail(e, "hipMalloc(state)");
+    if ((e = hipMalloc((void **)&b->d_rep, trep)) != hipSuccess) return fail(e, "hipMalloc(report)");
+    if ((e = hipMalloc((void **)&b->d_desc, sizeof(SolveDesc) * (size_t)count)) != hipSuccess) return fail(e, "hipMalloc(desc)");
+    b->ctx = ctx;
+    b->bytes = total;
+    ctx->used += total;
+    b->rep_bytes = trep;
+    b->desc.resize((size_t)count);
+    b->rep_off.resize((size_t)count);
+    b->h_in.assign(tin, 0);
+    uint8_t *himg = b->h_in.data();
+    size_t oin = 0, ost = 0, orp = 0;
+    for (int i = 0; i < count; ++i) {
+        SolveDesc &D = b->desc[(size_t)i];
+        memset(&D, 0, sizeof D);
+        size_t x, y, z;
+        layout_solve(caps[(size_t)i], &D, b->d_in + oin, b->d_state + ost, b->d_rep + orp, &x, &y, &z);
+        fill_desc_scalars(problems[i], D);
+        pack_inputs(problems[i], caps[(size_t)i], D, himg + oin, b->d_in + oin);
+        b->rep_off[(size_t)i] = orp;
+        oin += ib[(size_t)i];
+        ost += sb[(size_t)i];
+        orp += rb[(size_t)i];
+    }
+    hipStream_t st = ctx->stream;
+    if ((e = hipMemcpyAsync(b->d_in, himg, tin, hipMemcpyHostToDevice, st)) != hipSuccess) return fail(e, "upload inputs");
+    if ((e = hipMemcpyAsync(b->d_desc, b->desc.data(), sizeof(SolveDesc) * (size_t)count, hipMemcpyHostToDevice, st)) != hipSuccess)
+        return fail(e, "upload descriptors");
+    if ((e = hipEventCreate(&b->ev0)) != hipSuccess) return fail(e, "hipEventCreate");
+    if ((e = hipEventCreate(&b->ev1)) != hipSuccess) return fail(e, "hipEventCreate");
+    if ((e = hipEventRecord(b->ev0, st)) != hipSuccess) return fail(e, "hipEventRecord");
+    if ((e = launch_solve(b->d_desc, count, max_mr, st)) != hipSuccess) return fail(e, "solve launch");
+    if ((e = hipEventRecord(b->ev1, st)) != hipSuccess) return fail(e, "hipEventRecord");
+    *out = b;
+    return TCI_OK;
+}
+
+static tci_status fetch_reports(tci_batch *b)
+{
+    if (b->fetched) return TCI_OK;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    HIPCHK(hipEventSynchronize(b->ev1));
+    b->h_rep.resize(b->rep_bytes);
+    HIPCHK(hipMemcpy(b->h_rep.data(), b->d_rep, b->rep_bytes, hipMemcpyDeviceToHost));
+    b->fetched = true;
+    return TCI_OK;
+}
+
+tci_status tci_batch_query(tci_batch *b)
+{
+    if (!b) return set_err(TCI_ERR_ARG, "batch is NULL");
+    hipError_t e = hipEventQuery(b->ev1);
+    if (e == hipErrorNotReady) return TCI_ERR_PENDING;
+    if (e != hipSuccess) return set_err(TCI_ERR_DEVICE, "hipEventQuery: %s", hipGetErrorString(e));
+    return TCI_OK;
+}
+
+tci_status tci_batch_wait(tci_batch *b)
+{
+    if (!b) return set_err(TCI_ERR_ARG, "batch is NULL");
+    return fetch_reports(b);
+}
+
+// host view of solve i's report block
+static bool report_view(tci_batch *b, int32_t i, SolveDesc &R)
+{
+    if (!b || i < 0 || i >= b->count) return false;
+    if (fetch_reports(b) != TCI_OK) return false;
+    R = b->desc[(size_t)i];
+    const uint8_t *dev = b->d_rep + b->rep_off[(size_t)i];
+    uint8_t *host = b->h_rep.data() + b->rep_off[(size_t)i];
+    auto h = [&](const void *p) { return host + ((const uint8_t *)p - dev); };
+    R.ints = (int32_t *)h(R.ints);
+    R.scal = (double *)h(R.scal);
+    R.ranks = (int64_t *)h(R.ranks);
+    R.nglob = (int64_t *)h(R.nglob);
+    R.errors = (double *)h(R.errors);
+    R.pivoterrors = (double *)h(R.pivoterrors);
+    R.bonderrors = (double *)h(R.bonderrors);
+    R.setn = (int32_t *)h(R.setn);
+    return true;
+}
+
+tci_status tci_batch_result(tci_batch *b, int32_t i, tci_result *out)
+{
+    SolveDesc R;
+    if (!out || !report_view(b, i, R)) return set_err(TCI_ERR_ARG, "bad batch/index");
+    out->status = R.ints[INT_DONE] ? R.ints[INT_STATUS] : TCI_ERR_DEVICE;
+    out->niter = R.ints[INT_NITER] < R.maxiter_cap ? R.ints[INT_NITER] : R.maxiter_cap;
+    out->L = R.L;
+    out->npivoterrors = R.ints[INT_NPE];
+    out->maxsamplevalue = R.scal[SCAL_MAXSAMPLE];
+    for (int k = 0; k < 6; ++k) out->workstats[k] = R.scal[SCAL_WS_PI + k];
+    return TCI_OK;
+}
+
+tci_status tci_batch_history(tci_batch *b, int32_t i, int64_t *ranks, double *errors, int64_t *ng)
+{
+    SolveDesc R;
+    if (!report_view(b, i, R)) return set_err(TCI_ERR_ARG, "bad batch/index");
+    int n = R.ints[INT_NITER] < R.maxiter_cap ? R.ints[INT_NITER] : R.maxiter_cap;
+    for (int k = 0; k < n; ++k) {
+        if (ranks) ranks[k] = R.ranks[k];
+        if (errors) errors[k] = R.errors[k];
+        if (ng) ng[k] = R.nglob[k];
+    }
+    return TCI_OK;
+}
+
+tci_status tci_batch_linkdims(tci_batch *b, int32_t i, int64_t *linkdims)
+{
+    SolveDesc R;
+    if (!linkdims || !report_view(b, i, R)) return set_err(TCI_ERR_ARG, "bad batch/index");
+    for (int s = 1; s < R.L; ++s) linkdims[s - 1] = R.setn[SET_I * R.L + s];
+    return TCI_OK;
+}
+
+tci_status tci_batch_indexset(tci_batch *b, int32_t i, int32_t which, int32_t site, int32_t *count,
+                              uint8_t *out, int64_t cap_bytes)
+{
+    SolveDesc R;
+    if (!count || !report_view(b, i, R) || site < 1 || site > R.L || which < 0 || which > 1)
+        return set_err(TCI_ERR_ARG, "bad arguments");
+    const int L = R.L, b0 = site - 1, kind = which ? SET_J : SET_I;
+    const int len = which ? L - 1 - b0 : b0;
+    const int n = R.setn[kind * L + b0];
+    *count = n;
+    if ((int64_t)n * len > cap_bytes || (!out && n * len > 0)) return set_err(TCI_ERR_ARG, "output buffer too small");
+    if (n == 0 || len == 0) return TCI_OK;
+    std::vector<uint8_t> raw((size_t)n * L);
+    const uint8_t *src = R.sets + ((int64_t)(kind * L + b0) * R.setcap) * L;
+    HIPCHK(hipMemcpy(raw.data(), src, raw.size(), hipMemcpyDeviceToHost));
+    for (int e = 0; e < n; ++e) memcpy(out + (size_t)e * len, raw.data() + (size_t)e * L, (size_t)len);
+    return TCI_OK;
+}
+
+tci_status tci_batch_sitetensor(tci_batch *b, int32_t i, int32_t site, int64_t *dims3, double *out, int64_t cap)
+{
+    SolveDesc R;
+    if (!dims3 || !report_view(b, i, R) || site < 1 || site > R.L) return set_err(TCI_ERR_ARG, "bad arguments");
+    HIPCHK(hipMemcpy(dims3, R.Tdims + 3 * (site - 1), sizeof(int64_t) * 3, hipMemcpyDeviceToHost));
+    const int64_t n = dims3[0] * dims3[1] * dims3[2];
+    if (n > cap || (!out && n > 0)) return set_err(TCI_ERR_ARG, "output buffer too small");
+    if (n > 0) HIPCHK(hipMemcpy(out, R.T + R.tstride * (site - 1), sizeof(double) * n, hipMemcpyDeviceToHost));
+    return TCI_OK;
+}
+
+tci_status tci_batch_pivoterrors(tci_batch *b, int32_t i, double *out, int64_t cap)
+{
+    SolveDesc R;
+    if (!report_view(b, i, R)) return set_err(TCI_ERR_ARG, "bad batch/index");
+    const int n = R.ints[INT_NPE];
+    if (n > cap || (!out && n > 0)) return set_err(TCI_ERR_ARG, "output buffer too small");
+    for (int k = 0; k < n; ++k) out[k] = R.pivoterrors[k];
+    return TCI_OK;
+}
+
+tci_status tci_batch_bonderrors(tci_batch *b, int32_t i, double *out)
+{
+    SolveDesc R;
+    if (!out || !report_view(b, i, R)) return set_err(TCI_ERR_ARG, "bad batch/index");
+    for (int k = 0; k < R.L - 1; ++k) out[k] = R.bonderrors[k];
+    return TCI_OK;
+}
+
+tci_status tci_batch_elapsed_ms(tci_batch *b, float *ms)
+{
+    if (!b || !ms) return set_err(TCI_ERR_ARG, "bad arguments");
+    HIPCHK(hipEventSynchronize(b->ev1));
+    HIPCHK(hipEventElapsedTime(ms, b->ev0, b->ev1));
+    return TCI_OK;
+}
+
+tci_status tci_batch_free(tci_batch *b)
+{
+    if (!b) return TCI_OK;
+    if (b->ctx) {
+        hipSetDevice(b->ctx->device);
+        if (b->ev1) hipEventSynchronize(b->ev1);
+    }
+    batch_release(b);
+    return TCI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,900 @@
+// tci_engine.h — the TCI2 solve engine: one team (a gfx950 workgroup) runs
+// one whole crossinterpolate2 solve with every piece of state in device
+// memory: pivot sets, Pi matrices, rrLU, site tensors, global pivot search.
+//
+// The engine is written once against a "Team" interface:
+//   tm.tid, tm.nt            thread id / team size
+//   tm.sync()                team barrier
+//   tm.argmax(ArgBest)       team-wide argmax, result in every thread
+//   tm.max_nan(double)       team-wide NaN-propagating max
+//   tm.any(int)              team-wide logical or
+//   tm.xs, tm.ys             staging vectors (mrcap doubles each)
+// The device implementation lives in tci_kernels.hip.  Every parallel loop
+// writes disjoint elements and every reduction is over a total order, so the
+// result does not depend on the team size: a one-thread team on the CPU
+// (used only by the tests to check the control logic) produces the same bits.
+//
+// Arithmetic: compiled with -ffp-contract=off; x*y and a-b round separately
+// as in the reference (matrixlu.jl:132); divisions are IEEE divisions.
+#pragma once
+
+#include <math.h>
+#include <stdint.h>
+
+#include "../../include/tci_functors.h"
+#include "tci_desc.h"
+
+#if defined(__HIPCC__)
+#define ENG __device__
+#else
+#define ENG
+#endif
+
+namespace tci {
+
+enum {
+    ST_OK = 0, ST_NAN_L = 1, ST_NAN_U = 2, ST_ARG = 3, ST_ZERO_MAXSAMPLE = 4,
+    ST_NAN_TENSOR = 5, ST_NONSQUARE = 6, ST_DIM = 7, ST_GENERIC = 8, ST_SINGULAR = 9,
+    ST_CAPACITY = 10
+};
+
+struct ArgBest {
+    double v;      // abs2 value, -1 = no candidate (NaN entries never count)
+    int64_t lin;   // column-major linear position
+    double a;      // the entry itself
+};
+
+ENG inline bool arg_better(const ArgBest &a, const ArgBest &b)
+{
+    return a.v > b.v || (a.v == b.v && a.lin < b.lin);
+}
+
+// Base.max on Float64: NaN-propagating
+ENG inline double jl_max(double a, double b)
+{
+    if (a != a || b != b) return a + b;
+    return a > b ? a : b;
+}
+
+// Scalars shared by the team (written by thread 0 between barriers).
+struct EngShared {
+    double maxsample;
+    double abstol;
+    double ws[6];
+    uint64_t rng[4];
+    int32_t status;
+    int32_t cnt;       // scratch count (union compaction)
+    int32_t nf;        // global pivots found
+    int32_t pad;
+};
+
+// ---- project PRNG: SplitMix64-seeded xoshiro256** --------------------------
+ENG inline uint64_t eng_splitmix(uint64_t *x)
+{
+    uint64_t z = (*x += 0x9e3779b97f4a7c15ULL);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+ENG inline void eng_rng_seed(uint64_t *s, uint64_t seed)
+{
+    uint64_t x = seed;
+    for (int i = 0; i < 4; ++i) s[i] = eng_splitmix(&x);
+}
+ENG inline uint64_t eng_rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+ENG inline uint64_t eng_rng_next(uint64_t *s)
+{
+    uint64_t r = eng_rotl(s[1] * 5, 7) * 9;
+    uint64_t t = s[1] << 17;
+    s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = eng_rotl(s[3], 45);
+    return r;
+}
+
+// rrlu!(A) with full pivoting (matrixlu.jl:114-181) on A (m×n, lda = m),
+// run by the whole team.  Output: rowperm/colperm (0-based original index at
+// each position), pivval[k] = k-th pivot (may be null), *perr = rrLU.error,
+// stats[0] = Σ_{k<s}(m-k)(n-k) (argmax passes), stats[1] = Σ_{k=1..p}(m-k)(n-k)
+// (Schur updates).  Returns p, or -status for the NaN errors.
+template <class Team>
+ENG int rrlu_team(Team &tm, double *A, int m, int n, int64_t maxrank, double reltol,
+              double abstol, bool leftorth, int32_t *rowperm, int32_t *colperm,
+              double *pivval, double *perr, double *stats)
+{
+    const int64_t lda = m;
+    for (int i = tm.tid; i < m; i += tm.nt) rowperm[i] = i;
+    for (int j = tm.tid; j < n; j += tm.nt) colperm[j] = j;
+    // first argmax over the whole matrix (submatrixargmax, matrixlu.jl:1-32)
+    ArgBest best;
+    best.v = -1.0;
+    best.lin = INT64_MAX;
+    best.a = 0.0;
+    const int64_t cnt = (int64_t)m * n;
+    for (int64_t q = tm.tid; q < cnt; q += tm.nt) {
+        double a = A[q];
+        double v = a * a;
+        if (v == v) {
+            ArgBest c;
+            c.v = v;
+            c.lin = q;
+            c.a = a;
+            if (arg_better(c, best)) best = c;
+        }
+    }
+    best = tm.argmax(best);
+    int64_t mr = maxrank;
+    if (m < mr) mr = m;
+    if (n < mr) mr = n;
+    double maxerr = 0.0, err = 0.0 / 0.0;
+    double sarg = 0.0, sschur = 0.0;
+    int p = 0;
+    while (p < mr) {
+        const int k = p;
+        int r, c;
+        if (best.v < 0.0) { r = k; c = k; }
+        else { c = (int)(best.lin / lda); r = (int)(best.lin - (int64_t)c * lda); }
+        sarg += (double)(m - k) * (double)(n - k);
+        // the candidate carries its value, so no thread re-reads A here
+        const double e = best.v < 0.0 ? fabs(A[k + lda * k]) : fabs(best.a);
+        err = e;
+        if ((e < reltol * maxerr || e < abstol) && p > 0) break;
+        maxerr = jl_max(maxerr, e);
+        // swaprow!(k, r) then swapcol!(k, c) (matrixlu.jl:98-117), one pass:
+        // A'[i,j] = A[σ(i), τ(j)] with σ = (k r), τ = (k c)
+        if (r != k)
+            for (int j = tm.tid; j < n; j += tm.nt) {
+                if (j == k || j == c) continue;
+                double t = A[k + lda * j];
+                A[k + lda * j] = A[r + lda * j];
+                A[r + lda * j] = t;
+            }
+        if (c != k)
+            for (int i = tm.tid; i < m; i += tm.nt) {
+                if (i == k || i == r) continue;
+                double t = A[i + lda * k];
+                A[i + lda * k] = A[i + lda * c];
+                A[i + lda * c] = t;
+            }
+        if (tm.tid == 0) {
+            double akk = A[k + lda * k], akc = A[k + lda * c];
+            double ark = A[r + lda * k], arc = A[r + lda * c];
+            A[k + lda * k] = arc;
+            A[k + lda * c] = ark;
+            A[r + lda * k] = akc;
+            A[r + lda * c] = akk;
+            int32_t t = rowperm[k]; rowperm[k] = rowperm[r]; rowperm[r] = t;
+            t = colperm[k]; colperm[k] = colperm[c]; colperm[c] = t;
+        }
+        tm.sync();
+        const double piv = A[k + lda * k];
+        if (tm.tid == 0 && pivval) pivval[k] = piv;
+        // scale the pivot column (leftorth) or row by true division, stage x, y
+        if (leftorth) {
+            for (int i = k + 1 + tm.tid; i < m; i += tm.nt) {
+                double x = A[i + lda * k] / piv;
+                A[i + lda * k] = x;
+                tm.xs[i] = x;
+            }
+            for (int j = k + 1 + tm.tid; j < n; j += tm.nt) tm.ys[j] = A[k + lda * j];
+        } else {
+            for (int j = k + 1 + tm.tid; j < n; j += tm.nt) {
+                double y = A[k + lda * j] / piv;
+                A[k + lda * j] = y;
+                tm.ys[j] = y;
+            }
+            for (int i = k + 1 + tm.tid; i < m; i += tm.nt) tm.xs[i] = A[i + lda * k];
+        }
+        tm.sync();
+        // A[k+1:,k+1:] -= x·yᵀ, unfused, fused with the next argmax
+        best.v = -1.0;
+        best.lin = INT64_MAX;
+        best.a = 0.0;
+        const int mm = m - k - 1, nn = n - k - 1;
+        const int64_t tot = (int64_t)mm * nn;
+        for (int64_t q = tm.tid; q < tot; q += tm.nt) {
+            int64_t jj = q / mm, ii = q - jj * mm;
+            int64_t i = k + 1 + ii, j = k + 1 + jj;
+            double t = tm.xs[i] * tm.ys[j];
+            double a = A[i + lda * j] - t;
+            A[i + lda * j] = a;
+            double v = a * a;
+            if (v == v) {
+                ArgBest cnd;
+                cnd.v = v;
+                cnd.lin = i + lda * j;
+                cnd.a = a;
+                if (arg_better(cnd, best)) best = cnd;
+            }
+        }
+        best = tm.argmax(best);
+        p = k + 1;
+        sschur += (double)(m - p) * (double)(n - p);
+    }
+    // L = tril(A[:,1:p]), U = triu(A[1:p,:]); NaN checks (matrixlu.jl:162-169)
+    int nanL = 0, nanU = 0;
+    for (int64_t q = tm.tid; q < (int64_t)m * p; q += tm.nt) {
+        int64_t j = q / m, i = q - j * m;
+        if (i >= j) { double v = A[i + lda * j]; if (v != v) nanL = 1; }
+    }
+    for (int64_t q = tm.tid; q < (int64_t)p * n; q += tm.nt) {
+        int64_t j = q / p, i = q - j * p;
+        if (j >= i) { double v = A[i + lda * j]; if (v != v) nanU = 1; }
+    }
+    nanL = tm.any(nanL);
+    nanU = tm.any(nanU);
+    stats[0] = sarg;
+    stats[1] = sschur;
+    if (p >= (m < n ? m : n)) err = 0.0;   // matrixlu.jl:176-178
+    *perr = err;
+    tm.sync();
+    if (nanL) return -ST_NAN_L;
+    if (nanU) return -ST_NAN_U;
+    return p;
+}
+
+
+template <class Team>
+struct Engine {
+    Team &tm;
+    const SolveDesc &D;
+    EngShared &S;
+    int L;
+
+    ENG Engine(Team &t, const SolveDesc &d, EngShared &s) : tm(t), D(d), S(s), L(d.L) {}
+
+    // ---------------------------------------------------------------- sets
+    ENG uint8_t *setp(int kind, int b) const
+    {
+        return D.sets + ((int64_t)(kind * L + b) * D.setcap) * L;
+    }
+    ENG int32_t &setn(int kind, int b) const { return D.setn[kind * L + b]; }
+    ENG static int ilen(int b) { return b; }                 // |Iset[b]| entries (0-based site b)
+    ENG int jlen(int b) const { return L - 1 - b; }          // |Jset[b]| entries
+
+    ENG static bool eq(const uint8_t *a, const uint8_t *b, int len)
+    {
+        for (int t = 0; t < len; ++t)
+            if (a[t] != b[t]) return false;
+        return true;
+    }
+
+    // pushunique! (util.jl:16-20), called by thread 0 only
+    ENG int pushunique(int kind, int b, const uint8_t *x, int len)
+    {
+        uint8_t *base = setp(kind, b);
+        int32_t &n = setn(kind, b);
+        for (int i = 0; i < n; ++i)
+            if (eq(base + (int64_t)i * L, x, len)) return ST_OK;
+        if (n >= D.setcap) return ST_CAPACITY;
+        for (int t = 0; t < len; ++t) base[(int64_t)n * L + t] = x[t];
+        n += 1;
+        return ST_OK;
+    }
+
+    // addglobalpivots! (tensorci2.jl:193-213); thread 0, then barrier
+    ENG int addglobalpivots(const uint8_t *piv, int npiv)
+    {
+        if (tm.tid == 0) {
+            int st = ST_OK;
+            for (int q = 0; q < npiv && st == ST_OK; ++q) {
+                const uint8_t *p = piv + (int64_t)q * L;
+                for (int b = 0; b < L && st == ST_OK; ++b) {
+                    st = pushunique(SET_I, b, p, ilen(b));
+                    if (st == ST_OK) st = pushunique(SET_J, b, p + b + 1, jlen(b));
+                }
+            }
+            if (st != ST_OK) S.status = st;
+        }
+        tm.sync();
+        return S.status;
+    }
+
+    // ------------------------------------------------------ index tables
+    // rows = kronecker(Iset[b], d_b) (tensorci2.jl:315-320): entry q = i + nI·s
+    ENG int kron_rows(int b)
+    {
+        const int nI = setn(SET_I, b), d = D.localdims[b];
+        const uint8_t *I = setp(SET_I, b);
+        const int m = nI * d;
+        if (m > D.mrcap) return m;   // caller reports ST_CAPACITY
+        for (int q = tm.tid; q < m; q += tm.nt) {
+            int s = q / nI, i = q - s * nI;
+            uint8_t *r = D.rows + (int64_t)q * L;
+            for (int t = 0; t < b; ++t) r[t] = I[(int64_t)i * L + t];
+            r[b] = (uint8_t)(s + 1);
+        }
+        return m;
+    }
+
+    // cols = kronecker(d_{b+1}, Jset[b+1]) (tensorci2.jl:322-327): q = s + d·j
+    ENG int kron_cols(int b)
+    {
+        const int nJ = setn(SET_J, b + 1), d = D.localdims[b + 1];
+        const uint8_t *J = setp(SET_J, b + 1);
+        const int len = jlen(b + 1);
+        const int n = nJ * d;
+        if (n > D.mrcap) return n;   // caller reports ST_CAPACITY
+        for (int q = tm.tid; q < n; q += tm.nt) {
+            int j = q / d, s = q - j * d;
+            uint8_t *c = D.cols + (int64_t)q * L;
+            c[0] = (uint8_t)(s + 1);
+            for (int t = 0; t < len; ++t) c[1 + t] = J[(int64_t)j * L + t];
+        }
+        return n;
+    }
+
+    // union(kron, extras) keeping first-occurrence order.  An extra prefix e
+    // (length b+1) lies in kronecker(Iset[b], d_b) iff e[1:b] ∈ Iset[b]; an
+    // extra suffix lies in kronecker(d, Jset[b+1]) iff e[2:end] ∈ Jset[b+1].
+    // Extras come from an earlier Iset/Jset and are unique already.
+    ENG int union_extras(bool rowside, int b, int m0, int ekind)
+    {
+        const int ne = setn(ekind, rowside ? b + 1 : b);
+        if (ne == 0) return m0;
+        const uint8_t *E = setp(ekind, rowside ? b + 1 : b);
+        const int refkind = rowside ? SET_I : SET_J;
+        const int refb = rowside ? b : b + 1;
+        const int nref = setn(refkind, refb);
+        const uint8_t *R = setp(refkind, refb);
+        const int cmplen = rowside ? b : jlen(b + 1);
+        const int off = rowside ? 0 : 1;
+        const int elen = rowside ? b + 1 : jlen(b);
+        for (int e = tm.tid; e < ne; e += tm.nt) {
+            const uint8_t *x = E + (int64_t)e * L + off;
+            bool found = false;
+            for (int i = 0; i < nref && !found; ++i) found = eq(R + (int64_t)i * L, x, cmplen);
+            D.flags[e] = found ? 0 : 1;
+        }
+        tm.sync();
+        if (tm.tid == 0) {
+            int m = m0;
+            uint8_t *dst = rowside ? D.rows : D.cols;
+            for (int e = 0; e < ne; ++e) {
+                if (!D.flags[e]) continue;
+                if (m >= D.mrcap) { S.status = ST_CAPACITY; break; }
+                for (int t = 0; t < elen; ++t) dst[(int64_t)m * L + t] = E[(int64_t)e * L + t];
+                ++m;
+            }
+            S.cnt = m;
+        }
+        tm.sync();
+        return S.cnt;
+    }
+
+    // copy a set into the rows or cols table
+    ENG void load_set(uint8_t *dst, int kind, int b, int len)
+    {
+        const int n = setn(kind, b);
+        const uint8_t *src = setp(kind, b);
+        for (int64_t q = tm.tid; q < (int64_t)n * len; q += tm.nt) {
+            int64_t e = q / len, t = q - e * len;
+            dst[e * L + t] = src[e * L + t];
+        }
+    }
+
+    // set[kind][b] = table[perm[0..p-1]] (pivot order kept, tensorci2.jl:599-600)
+    ENG void select_into(int kind, int b, const uint8_t *table, const int32_t *perm, int p, int len)
+    {
+        uint8_t *dst = setp(kind, b);
+        for (int64_t q = tm.tid; q < (int64_t)p * len; q += tm.nt) {
+            int64_t k = q / len, t = q - k * len;
+            dst[k * L + t] = table[(int64_t)perm[k] * L + t];
+        }
+        if (tm.tid == 0) setn(kind, b) = p;
+    }
+
+    // ---------------------------------------------------------- evaluation
+    // out[i + ld·j] = f(rows[i] ⊕ cols[j]) (batcheval.jl:50-57); returns the
+    // NaN-propagating max of |out| folded into S.maxsample when fold != 0.
+    // transpose != 0 writes out[j + ld·i].
+    ENG void eval(const uint8_t *rows, int nl, int m, const uint8_t *cols, int n, double *out,
+                  int64_t ld, bool transpose, bool fold)
+    {
+        double lmax = 0.0;
+        const int64_t cnt = (int64_t)m * n;
+        for (int64_t q = tm.tid; q < cnt; q += tm.nt) {
+            int64_t j = q / m, i = q - j * m;
+            tci_point x;
+            x.a = rows + i * L;
+            x.b = cols + j * L;
+            x.na = nl;
+            x.L = L;
+            double v = tci_functor_eval(D.fkind, D.params, D.iparams, &x);
+            if (out) out[transpose ? (j + ld * i) : (i + ld * j)] = v;
+            lmax = jl_max(lmax, fabs(v));
+        }
+        if (fold) {
+            double g = tm.max_nan(lmax);
+            if (tm.tid == 0) {
+                S.maxsample = jl_max(fabs(S.maxsample), g);
+                S.ws[0] += (double)cnt;
+                S.ws[5] += 1.0;
+            }
+        }
+        tm.sync();
+    }
+
+    // --------------------------------------------------------------- rrLU
+    ENG int rrlu(int m, int n, int64_t maxrank, double reltol, double abstol, bool leftorth,
+                 double *perr)
+    {
+        double st[4];
+        int p = rrlu_team(tm, D.pi, m, n, maxrank, reltol, abstol, leftorth, D.rowperm,
+                          D.colperm, D.pivval, perr, st);
+        if (tm.tid == 0) {
+            S.ws[1] += 8.0 * st[0];
+            S.ws[2] += 8.0 * (double)m * (double)n + 16.0 * st[1] +
+                       8.0 * (double)(p > 0 ? p : 0) * (double)(m + n);
+            S.ws[3] += 2.0 * st[1];
+            S.ws[4] += 1.0;
+        }
+        return p;
+    }
+
+    // updateerrors! (tensorci2.jl:161-169); thread 0
+    ENG void updateerrors(int b, int p, double err)
+    {
+        if (tm.tid == 0) {
+            D.bonderrors[b] = err;
+            int npe = D.ints[INT_NPE];
+            int n = npe > p + 1 ? npe : p + 1;
+            for (int i = 0; i < n; ++i) {
+                double a = i < npe ? D.pivoterrors[i] : 0.0;
+                double e = i < p ? fabs(D.pivval[i]) : (i == p ? err : 0.0);
+                D.pivoterrors[i] = jl_max(a, e);
+            }
+            D.ints[INT_NPE] = n;
+        }
+    }
+
+    ENG int fail(int st)
+    {
+        if (tm.tid == 0 && S.status == ST_OK) S.status = st;
+        tm.sync();
+        return S.status;
+    }
+
+    // ------------------------------------------------------ updatepivots!
+    // tensorci2.jl:510-551, 599-607 (pivotsearch = :full).  ekind < 0: no extras.
+    ENG int updatepivots(int b, bool leftorth, double abstol, int ekind_i, int ekind_j)
+    {
+        int m = kron_rows(b);
+        int n = kron_cols(b);
+        if (m > D.mrcap || n > D.mrcap) return fail(ST_CAPACITY);
+        tm.sync();
+        if (ekind_i >= 0) {
+            m = union_extras(true, b, m, ekind_i);
+            if (S.status) return S.status;
+            n = union_extras(false, b, n, ekind_j);
+            if (S.status) return S.status;
+        }
+        eval(D.rows, b + 1, m, D.cols, n, D.pi, m, false, true);
+        double err;
+        int p = rrlu(m, n, D.maxbonddim, 1e-14, abstol, leftorth, &err);
+        if (p < 0) return fail(-p);
+        if (p > D.setcap) return fail(ST_CAPACITY);
+        select_into(SET_I, b + 1, D.rows, D.rowperm, p, b + 1);
+        select_into(SET_J, b, D.cols, D.colperm, p, jlen(b));
+        updateerrors(b, p, err);
+        tm.sync();
+        return ST_OK;
+    }
+
+    // ------------------------------------------------- Pi1·P⁻¹ (site tensor)
+    // T_b = transpose(transpose(P) \ transpose(Pi1)) (tensorci2.jl:391):
+    // LU with partial pivoting of Pᵀ, then substitution per row of Pi1.
+    ENG int solve_right(const double *Pi1, int r, double *A, int c, double *X)
+    {
+        int singular = 0;
+        for (int k = 0; k < c; ++k) {
+            if (tm.tid == 0) {
+                int pr = k;
+                double bestv = fabs(A[k + (int64_t)c * k]);
+                for (int i = k + 1; i < c; ++i) {
+                    double v = fabs(A[i + (int64_t)c * k]);
+                    if (v > bestv) { bestv = v; pr = i; }
+                }
+                D.ipiv[k] = pr;
+            }
+            tm.sync();
+            const int pr = D.ipiv[k];
+            if (pr != k)
+                for (int j = tm.tid; j < c; j += tm.nt) {
+                    double x = A[k + (int64_t)c * j];
+                    A[k + (int64_t)c * j] = A[pr + (int64_t)c * j];
+                    A[pr + (int64_t)c * j] = x;
+                }
+            tm.sync();
+            const double d = A[k + (int64_t)c * k];
+            if (d == 0.0) singular = 1;
+            for (int i = k + 1 + tm.tid; i < c; i += tm.nt) A[i + (int64_t)c * k] = A[i + (int64_t)c * k] / d;
+            tm.sync();
+            const int w = c - k - 1;
+            for (int64_t q = tm.tid; q < (int64_t)w * w; q += tm.nt) {
+                int64_t jj = q / w, ii = q - jj * w;
+                int64_t i = k + 1 + ii, j = k + 1 + jj;
+                A[i + c * j] = A[i + c * j] - A[i + c * k] * A[k + c * j];
+            }
+            tm.sync();
+        }
+        if (singular) return ST_SINGULAR;
+        for (int q = tm.tid; q < r; q += tm.nt) {
+            for (int i = 0; i < c; ++i) X[q + (int64_t)r * i] = Pi1[q + (int64_t)r * i];
+            for (int k = 0; k < c; ++k) {
+                int pk = D.ipiv[k];
+                if (pk != k) {
+                    double x = X[q + (int64_t)r * k];
+                    X[q + (int64_t)r * k] = X[q + (int64_t)r * pk];
+                    X[q + (int64_t)r * pk] = x;
+                }
+            }
+            for (int i = 0; i < c; ++i) {
+                double s = X[q + (int64_t)r * i];
+                for (int t = 0; t < i; ++t) s = s - A[i + (int64_t)c * t] * X[q + (int64_t)r * t];
+                X[q + (int64_t)r * i] = s;
+            }
+            for (int i = c - 1; i >= 0; --i) {
+                double s = X[q + (int64_t)r * i];
+                for (int t = i + 1; t < c; ++t) s = s - A[i + (int64_t)c * t] * X[q + (int64_t)r * t];
+                X[q + (int64_t)r * i] = s / A[i + (int64_t)c * i];
+            }
+        }
+        tm.sync();
+        return ST_OK;
+    }
+
+    ENG double *Tslot(int b) const { return D.T + D.tstride * b; }
+    ENG void setTdims(int b, int64_t a, int64_t d, int64_t c)
+    {
+        if (tm.tid == 0) { D.Tdims[3 * b] = a; D.Tdims[3 * b + 1] = d; D.Tdims[3 * b + 2] = c; }
+    }
+
+    // fillsitetensors! (globalsearch.jl:97-104) → setsitetensor!(tci, f, b)
+    // (tensorci2.jl:367-394).  When no global search follows (nsearch == 0)
+    // the tensors are never read before sweep1site! rebuilds all of them, so
+    // only max|Pi1| (which feeds maxsamplevalue) is computed.
+    ENG int fillsitetensors(bool need_tensors)
+    {
+        for (int b = 0; b < L; ++b) {
+            const int nI = setn(SET_I, b), d = D.localdims[b], nJ = setn(SET_J, b);
+            const int m = kron_rows(b);
+            if (m > D.mrcap) return fail(ST_CAPACITY);
+            load_set(D.cols, SET_J, b, jlen(b));
+            tm.sync();
+            if (!need_tensors) {
+                eval(D.rows, b + 1, m, D.cols, nJ, nullptr, 0, false, true);
+                continue;
+            }
+            if ((int64_t)m * nJ > D.tstride) return fail(ST_CAPACITY);
+            if (b == L - 1) {
+                eval(D.rows, b + 1, m, D.cols, nJ, Tslot(b), m, false, true);
+                setTdims(b, nI, d, nJ);
+                continue;
+            }
+            eval(D.rows, b + 1, m, D.cols, nJ, D.pi, m, false, true);
+            const int np = setn(SET_I, b + 1);
+            if (np != nJ) {
+                if (tm.tid == 0) D.ints[INT_ERRSITE] = b + 1;
+                return fail(ST_NONSQUARE);
+            }
+            // Pᵀ[jj, ii] = f(Iset[b+1][ii] ⊕ Jset[b][jj])
+            eval(setp(SET_I, b + 1), b + 1, np, D.cols, nJ, D.pmat, np, true, false);
+            int st = solve_right(D.pi, m, D.pmat, np, Tslot(b));
+            if (st != ST_OK) {
+                if (tm.tid == 0) D.ints[INT_ERRSITE] = b + 1;
+                return fail(st);
+            }
+            setTdims(b, nI, d, np);
+            tm.sync();
+        }
+        return ST_OK;
+    }
+
+    // ---------------------------------------------------------- sweep2site!
+    // tensorci2.jl:855-916 with niter = 2
+    ENG int sweep2site(double abstol)
+    {
+        for (int iter = 1; iter <= 2; ++iter) {
+            const int hist = D.ints[INT_HIST];
+            const bool extras = !D.strictlynested && hist != 0;
+            const int ek_i = hist == 1 ? SET_HA_I : SET_HB_I;
+            const int ek_j = hist == 1 ? SET_HA_J : SET_HB_J;
+            const int nh_i = hist == 1 ? SET_HB_I : SET_HA_I;   // new history slot
+            const int nh_j = hist == 1 ? SET_HB_J : SET_HA_J;
+            // push!(Iset_history, deepcopy(Iset)) — only history[end] is ever read
+            for (int b = 0; b < L; ++b) {
+                const int nI = setn(SET_I, b), nJ = setn(SET_J, b);
+                const uint8_t *si = setp(SET_I, b), *sj = setp(SET_J, b);
+                uint8_t *di = setp(nh_i, b), *dj = setp(nh_j, b);
+                for (int64_t q = tm.tid; q < (int64_t)nI * L; q += tm.nt) di[q] = si[q];
+                for (int64_t q = tm.tid; q < (int64_t)nJ * L; q += tm.nt) dj[q] = sj[q];
+            }
+            tm.sync();
+            if (tm.tid == 0) {
+                for (int b = 0; b < L; ++b) {
+                    setn(nh_i, b) = setn(SET_I, b);
+                    setn(nh_j, b) = setn(SET_J, b);
+                }
+                D.ints[INT_HIST] = hist == 1 ? 2 : 1;
+                D.ints[INT_NPE] = 0;   // flushpivoterror!
+            }
+            tm.sync();
+            const bool fwd = D.sweepstrategy == 1 || (D.sweepstrategy == 0 && (iter & 1));
+            for (int s = 0; s < L - 1; ++s) {
+                const int b = fwd ? s : L - 2 - s;
+                int st = updatepivots(b, fwd, abstol, extras ? ek_i : -1, extras ? ek_j : -1);
+                if (st != ST_OK) {
+                    if (tm.tid == 0) D.ints[INT_ERRSITE] = b + 1;
+                    return st;
+                }
+            }
+        }
+        return fillsitetensors(D.nsearchglobalpivot > 0);
+    }
+
+    // ------------------------------------------------------ global search
+    // DefaultGlobalPivotFinder (globalpivotfinder.jl:143-195), project PRNG.
+    ENG double tt_eval(const uint8_t *x, double *v, double *w) const
+    {
+        int64_t chl = D.Tdims[0];
+        for (int64_t i = 0; i < chl; ++i) v[i] = 1.0;
+        for (int t = 0; t < L; ++t) {
+            const int64_t d0 = D.Tdims[3 * t], d = D.Tdims[3 * t + 1], d2 = D.Tdims[3 * t + 2];
+            const int64_t s = x[t] - 1;
+            const double *T = D.T + D.tstride * t;
+            for (int64_t j = 0; j < d2; ++j) {
+                double acc = 0.0;
+                for (int64_t i = 0; i < d0; ++i) acc = acc + v[i] * T[i + d0 * (s + d * j)];
+                w[j] = acc;
+            }
+            double *tmp = v; v = w; w = tmp;
+            chl = d2;
+        }
+        return chl > 0 ? v[0] : 0.0;
+    }
+
+    ENG int globalsearch(double abstol)
+    {
+        const int ns = D.nsearchglobalpivot;
+        uint8_t *pts = D.gpts;                          // ns × L
+        uint8_t *found = D.gpts + (int64_t)ns * L;      // ns × L
+        uint8_t *cand = found + (int64_t)ns * L;        // ns·Σd × L
+        const int sumd = D.sumd;
+        const int64_t ncand = (int64_t)ns * sumd;
+        double *errs = D.work;                          // ncand
+        double *vec = D.work + ncand;                   // ncand × 2·setcap
+        if (tm.tid == 0) {
+            for (int q = 0; q < ns; ++q)
+                for (int p = 0; p < L; ++p) {
+                    uint64_t r = eng_rng_next(S.rng) >> 32;
+                    pts[(int64_t)q * L + p] = (uint8_t)(1 + (int)(r % (uint64_t)D.localdims[p]));
+                }
+        }
+        tm.sync();
+        for (int64_t c = tm.tid; c < ncand; c += tm.nt) {
+            const int64_t q = c / sumd;
+            int r = (int)(c - q * sumd), p = 0;
+            while (r >= D.localdims[p]) { r -= D.localdims[p]; ++p; }
+            uint8_t *x = cand + c * L;
+            for (int t = 0; t < L; ++t) x[t] = pts[q * L + t];
+            x[p] = (uint8_t)(r + 1);
+            tci_point pt;
+            pt.a = x; pt.b = x; pt.na = L; pt.L = L;
+            const double fv = tci_functor_eval(D.fkind, D.params, D.iparams, &pt);
+            double *v = vec + c * 2 * D.setcap;
+            const double tv = tt_eval(x, v, v + D.setcap);
+            errs[c] = fabs(fv - tv);
+        }
+        tm.sync();
+        if (tm.tid == 0) {
+            const double thr = abstol * D.tolmargin;
+            int nf = 0;
+            for (int q = 0; q < ns; ++q) {
+                double best_err = 0.0;
+                int64_t bi = -1;
+                for (int r = 0; r < sumd; ++r) {
+                    double e = errs[(int64_t)q * sumd + r];
+                    if (e > best_err) { best_err = e; bi = (int64_t)q * sumd + r; }
+                }
+                if (best_err > thr) {
+                    const uint8_t *src = bi >= 0 ? cand + bi * L : pts + (int64_t)q * L;
+                    for (int t = 0; t < L; ++t) found[(int64_t)nf * L + t] = src[t];
+                    ++nf;
+                }
+            }
+            if (nf > D.maxnglobalpivot) nf = D.maxnglobalpivot;
+            S.nf = nf;
+        }
+        tm.sync();
+        const int nf = S.nf;
+        return addglobalpivots(found, nf) == ST_OK ? nf : -1;
+    }
+
+    // convergencecriterion (tensorci2.jl:609-628); thread 0
+    ENG bool converged(int n, double tol) const
+    {
+        const int nch = D.ncheckhistory;
+        if (n < nch) return false;
+        bool all_err = true, all_ng = true, all_max = true;
+        int64_t minrank = D.ranks[n - nch];
+        for (int i = n - nch; i < n; ++i) {
+            if (!(D.errors[i] < tol)) all_err = false;
+            if (D.nglob[i] != 0) all_ng = false;
+            if (D.ranks[i] < minrank) minrank = D.ranks[i];
+            if (!(D.ranks[i] >= D.maxbonddim)) all_max = false;
+        }
+        if (!D.checkconvglobalpivot) all_ng = true;
+        return (all_err && all_ng && minrank == D.ranks[n - 1]) || all_max;
+    }
+
+    // ------------------------------------------------ final sweep1site!
+    // tensorci2.jl:402-461, forward, updatetensors = true
+    ENG int sweep1site_forward(double abstol)
+    {
+        if (tm.tid == 0) D.ints[INT_NPE] = 0;
+        for (int b = 0; b < L - 1; ++b) {
+            const int nI = setn(SET_I, b), d = D.localdims[b], nJ = setn(SET_J, b);
+            const int m = kron_rows(b);
+            if (m > D.mrcap || nJ > D.mrcap) return fail(ST_CAPACITY);
+            load_set(D.cols, SET_J, b, jlen(b));
+            tm.sync();
+            eval(D.rows, b + 1, m, D.cols, nJ, D.pi, m, false, true);
+            double err;
+            int p = rrlu(m, nJ, D.maxbonddim, 1e-14, abstol, true, &err);
+            if (p < 0) {
+                if (tm.tid == 0) D.ints[INT_ERRSITE] = b + 1;
+                return fail(-p);
+            }
+            if (p > D.setcap || (int64_t)m * p > D.tstride) return fail(ST_CAPACITY);
+            select_into(SET_I, b + 1, D.rows, D.rowperm, p, b + 1);
+            select_into(SET_J, b, D.cols, D.colperm, p, jlen(b));
+            // left(luci) = colstimespivotinv (matrixluci.jl:48-57): rows at
+            // positions < p are unit vectors, the rest solve x·L11 = L[i,1:p]
+            double *T = Tslot(b);
+            const double *A = D.pi;
+            for (int64_t q = tm.tid; q < (int64_t)p * p; q += tm.nt) {
+                int64_t j = q / p, i = q - j * p;
+                T[D.rowperm[i] + (int64_t)m * j] = (i == j) ? 1.0 : 0.0;
+            }
+            for (int i = p + tm.tid; i < m; i += tm.nt) {
+                double *row = T + D.rowperm[i];
+                for (int j = p - 1; j >= 0; --j) {
+                    double s = A[i + (int64_t)m * j];
+                    for (int t = j + 1; t < p; ++t) s = s - row[(int64_t)m * t] * A[t + (int64_t)m * j];
+                    row[(int64_t)m * j] = s;    // L11 has a unit diagonal: s / 1.0 == s
+                }
+            }
+            setTdims(b, nI, d, p);
+            tm.sync();
+            int nan = 0;
+            for (int64_t q = tm.tid; q < (int64_t)m * p; q += tm.nt) {
+                double v = T[q];
+                if (v != v) nan = 1;
+            }
+            if (tm.any(nan)) {
+                if (tm.tid == 0) D.ints[INT_ERRSITE] = b + 1;
+                return fail(ST_NAN_TENSOR);
+            }
+            updateerrors(b, p, err);
+            tm.sync();
+        }
+        // last site: T_L = filltensor(Iset[L], Jset[L], Val(1)) (tensorci2.jl:449-459)
+        const int b = L - 1;
+        const int nI = setn(SET_I, b), d = D.localdims[b], nJ = setn(SET_J, b);
+        const int m = kron_rows(b);
+        if (m > D.mrcap) return fail(ST_CAPACITY);
+        load_set(D.cols, SET_J, b, jlen(b));
+        tm.sync();
+        if ((int64_t)m * nJ > D.tstride) return fail(ST_CAPACITY);
+        eval(D.rows, b + 1, m, D.cols, nJ, Tslot(b), m, false, false);
+        setTdims(b, nI, d, nJ);
+        tm.sync();
+        return ST_OK;
+    }
+
+    ENG int64_t rank() const
+    {
+        int64_t r = 0;
+        for (int b = 1; b < L; ++b) if (setn(SET_I, b) > r) r = setn(SET_I, b);
+        return r;
+    }
+
+    // ------------------------------------------------ crossinterpolate2
+    // tensorci2.jl:943-953 → TensorCI2{T}(f, localdims, initialpivots)
+    // (42-53) → optimize! (700-850)
+    ENG int run()
+    {
+        if (tm.tid == 0) {
+            S.status = ST_OK;
+            for (int i = 0; i < 6; ++i) S.ws[i] = 0.0;
+            for (int q = 0; q < NSETKIND * L; ++q) D.setn[q] = 0;
+            D.ints[INT_NITER] = 0;
+            D.ints[INT_NPE] = 0;
+            D.ints[INT_HIST] = 0;
+            D.ints[INT_ERRSITE] = 0;
+            for (int b = 0; b < L - 1; ++b) D.bonderrors[b] = 0.0;
+            for (int b = 0; b < L; ++b) { D.Tdims[3 * b] = 0; D.Tdims[3 * b + 1] = 0; D.Tdims[3 * b + 2] = 0; }
+        }
+        tm.sync();
+        int st = addglobalpivots(D.initpivots, D.npivinit);
+        if (st != ST_OK) return finish(st);
+        // maxsamplevalue = maximum(abs, f(x) for x in initialpivots) (49-50)
+        if (tm.tid == 0) {
+            double mx = 0.0;
+            for (int q = 0; q < D.npivinit; ++q) {
+                tci_point pt;
+                pt.a = D.initpivots + (int64_t)q * L; pt.b = pt.a; pt.na = L; pt.L = L;
+                double v = fabs(tci_functor_eval(D.fkind, D.params, D.iparams, &pt));
+                mx = q == 0 ? v : jl_max(mx, v);
+            }
+            S.maxsample = mx;
+            if (!(fabs(mx) > 0.0)) S.status = ST_ZERO_MAXSAMPLE;
+        }
+        tm.sync();
+        if (S.status) return finish(S.status);
+        // optimize! argument checks (730-754)
+        if (D.nsearchglobalpivot > 0 && D.nsearchglobalpivot < D.maxnglobalpivot) return finish(ST_GENERIC);
+        const double tol = D.tolerance;
+        if (D.maxbonddim == INT64_MAX && tol <= 0) return finish(ST_ARG);
+        if (tm.tid == 0) eng_rng_seed(S.rng, D.seed);
+        tm.sync();
+        int niter = 0;
+        for (int iter = 1; iter <= D.maxiter; ++iter) {
+            const double norm = D.normalizeerror ? S.maxsample : 1.0;
+            const double abstol = tol * norm;
+            st = sweep2site(abstol);
+            if (st != ST_OK) return finish(st);
+            double perr = D.bonderrors[0];
+            for (int b = 1; b < L - 1; ++b) perr = jl_max(perr, D.bonderrors[b]);
+            int ng = 0;
+            if (D.nsearchglobalpivot > 0) {
+                ng = globalsearch(abstol);
+                if (ng < 0) return finish(S.status);
+            }
+            if (tm.tid == 0) {
+                if (niter < D.maxiter_cap) {
+                    D.ranks[niter] = rank();
+                    D.errors[niter] = perr;
+                    D.nglob[niter] = ng;
+                }
+                D.ints[INT_NITER] = niter + 1;
+            }
+            ++niter;
+            tm.sync();
+            bool conv = false;
+            if (tm.tid == 0) S.cnt = converged(niter, abstol) ? 1 : 0;
+            tm.sync();
+            conv = S.cnt != 0;
+            tm.sync();
+            if (conv) break;
+        }
+        const double norm = D.normalizeerror ? S.maxsample : 1.0;
+        st = sweep1site_forward(tol * norm);
+        if (st != ST_OK) return finish(st);
+        // _sanitycheck (globalsearch.jl:106-111)
+        for (int b = 0; b < L - 1; ++b)
+            if (setn(SET_I, b + 1) != setn(SET_J, b)) {
+                if (tm.tid == 0) D.ints[INT_ERRSITE] = b + 1;
+                return finish(ST_NONSQUARE);
+            }
+        if (tm.tid == 0)
+            for (int i = 0; i < niter; ++i) D.errors[i] = D.errors[i] / norm;
+        return finish(ST_OK);
+    }
+
+    ENG int finish(int st)
+    {
+        tm.sync();
+        if (tm.tid == 0) {
+            D.ints[INT_STATUS] = st;
+            D.scal[SCAL_MAXSAMPLE] = S.maxsample;
+            for (int i = 0; i < 6; ++i) D.scal[SCAL_WS_PI + i] = S.ws[i];
+            D.ints[INT_DONE] = 1;
+        }
+        tm.sync();
+        return st;
+    }
+};
+
+}  // namespace tci

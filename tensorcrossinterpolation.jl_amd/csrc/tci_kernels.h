@@ -1,0 +1,23 @@
+// tci_kernels.h — launchers of the gfx950 kernels (defined in tci_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/tci.h"
+#include "tci_desc.h"
+
+namespace tci {
+
+hipError_t launch_solve(const SolveDesc *descs_device, int count, int max_mrcap, hipStream_t st);
+hipError_t launch_rrlu(tci_lu_job *jobs_device, int64_t count, int64_t max_side, double *gstage,
+                       int64_t gstage_stride, hipStream_t st);
+hipError_t launch_lu_unpack(const double *A, int64_t m, int64_t n, int64_t p, int leftorth,
+                            double *L, int64_t ldl, double *U, int64_t ldu, hipStream_t st);
+hipError_t launch_kron_center(const uint8_t *left, int64_t nI, int nl, const int32_t *cdims, int M,
+                              int64_t nc, uint8_t *rowsx, int stride, hipStream_t st);
+int eval_blocks(int64_t tot);
+hipError_t launch_eval(int kind, const double *params, const int32_t *iparams, int L,
+                       const uint8_t *rows, int64_t nrow, int rstride, int nl, const uint8_t *cols,
+                       int64_t ncol, int cstride, double *out, double *blockmax, hipStream_t st);
+
+}  // namespace tci

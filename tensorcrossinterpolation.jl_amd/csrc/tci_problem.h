@@ -1,0 +1,156 @@
+// tci_problem.h — validation of a tci_problem and its device capacities.
+// Host-side plumbing shared by the library's submit path (and the tests'
+// CPU build of the engine): checks every argument that would otherwise make
+// a device read go out of bounds, then fixes the capacities of the solve.
+#pragma once
+#include <stdint.h>
+#include <string.h>
+
+#include <string>
+
+#include "../../include/tci.h"
+#include "tci_layout.h"
+
+namespace tci {
+
+// Returns TCI_OK or an error with *msg set.
+inline int check_problem(const tci_problem &p, SolveCaps &c, std::string *msg)
+{
+    auto bad = [&](int code, const char *m) { if (msg) *msg = m; return code; };
+    if (p.L < 2) return bad(TCI_ERR_GENERIC, "localdims should have at least 2 elements!");
+    if (p.L > 250) return bad(TCI_ERR_ARG, "at most 250 sites are supported");
+    if (!p.localdims) return bad(TCI_ERR_ARG, "localdims is NULL");
+    const int L = p.L;
+    int dmax = 0, sumd = 0;
+    for (int b = 0; b < L; ++b) {
+        int d = p.localdims[b];
+        if (d < 1 || d > 255) return bad(TCI_ERR_ARG, "local dimensions must be in 1..255");
+        dmax = d > dmax ? d : dmax;
+        sumd += d;
+    }
+    const int npiv = p.initialpivots ? p.npivots : 1;
+    if (npiv < 1) return bad(TCI_ERR_ARG, "at least one initial pivot is required");
+    if (p.initialpivots)
+        for (int q = 0; q < npiv; ++q)
+            for (int b = 0; b < L; ++b) {
+                int v = p.initialpivots[(int64_t)q * L + b];
+                if (v < 1 || v > p.localdims[b]) return bad(TCI_ERR_ARG, "initial pivot out of range");
+            }
+    // functor parameter blocks must cover every read the functor makes
+    const tci_functor &f = p.f;
+    const int np = f.nparams, nip = f.niparams;
+    if (np > 0 && !f.params) return bad(TCI_ERR_ARG, "functor params is NULL");
+    if (nip > 0 && !f.iparams) return bad(TCI_ERR_ARG, "functor iparams is NULL");
+    switch (f.kind) {
+    case TCI_F_LORENTZ_INT:
+    case TCI_F_SUM:
+        if (np < 1) return bad(TCI_ERR_ARG, "functor needs 1 parameter");
+        break;
+    case TCI_F_LORENTZ_WEIGHTED:
+        if (np < 1 + L) return bad(TCI_ERR_ARG, "weighted Lorentzian needs 1+L parameters");
+        break;
+    case TCI_F_QUANTICS_GAUSS_COS: {
+        if (L > 52) return bad(TCI_ERR_ARG, "quantics functors support at most 52 sites");
+        if (np < 1) return bad(TCI_ERR_ARG, "missing K");
+        int K = (int)f.params[0];
+        if (K < 0 || (double)K != f.params[0] || np < 1 + 3 * K + 2)
+            return bad(TCI_ERR_ARG, "quantics Gaussian+cos parameter block too short");
+        break;
+    }
+    case TCI_F_GAUSS_SUM: {
+        if (np < 2) return bad(TCI_ERR_ARG, "missing K, d");
+        int K = (int)f.params[0];
+        if (K < 0 || (double)K != f.params[0] || (int64_t)np < 2 + 2 * (int64_t)K + (int64_t)K * L)
+            return bad(TCI_ERR_ARG, "Gaussian-sum parameter block too short");
+        break;
+    }
+    case TCI_F_TABLE: {
+        if (nip < L) return bad(TCI_ERR_ARG, "table functor needs L dims");
+        int64_t tot = 1;
+        for (int b = 0; b < L; ++b) {
+            if (f.iparams[b] != p.localdims[b]) return bad(TCI_ERR_ARG, "table dims differ from localdims");
+            tot *= f.iparams[b];
+            if (tot > ((int64_t)1 << 31)) return bad(TCI_ERR_ARG, "table too large");
+        }
+        if (np < tot) return bad(TCI_ERR_ARG, "table parameter block too short");
+        break;
+    }
+    case TCI_F_TT: {
+        if (nip < 2 * L + 1) return bad(TCI_ERR_ARG, "TT functor needs 2L+1 iparams");
+        if (f.iparams[0] != 1 || f.iparams[L] != 1) return bad(TCI_ERR_ARG, "TT end bonds must be 1");
+        int64_t tot = 0;
+        for (int b = 0; b < L; ++b) {
+            int cl = f.iparams[b], cr = f.iparams[b + 1], d = f.iparams[L + 1 + b];
+            if (cl < 1 || cl > 64 || cr < 1 || cr > 64) return bad(TCI_ERR_ARG, "TT bond dims must be in 1..64");
+            if (d != p.localdims[b]) return bad(TCI_ERR_ARG, "TT local dims differ from localdims");
+            tot += (int64_t)cl * d * cr;
+        }
+        if (np < tot) return bad(TCI_ERR_ARG, "TT parameter block too short");
+        break;
+    }
+    case TCI_F_QUANTICS_EXP:
+        if (L > 52) return bad(TCI_ERR_ARG, "quantics functors support at most 52 sites");
+        if (np < 4) return bad(TCI_ERR_ARG, "quantics exp needs 4 parameters");
+        break;
+    default:
+        return bad(TCI_ERR_ARG, "unknown functor kind");
+    }
+    const tci_options &o = p.opts;
+    if (o.maxiter < 0 || o.maxiter > 100000) return bad(TCI_ERR_ARG, "maxiter out of range");
+    if (o.ncheckhistory < 1) return bad(TCI_ERR_ARG, "ncheckhistory must be >= 1");
+    if (o.maxnglobalpivot < 0 || o.nsearchglobalpivot < 0 || o.nsearchglobalpivot > 4096)
+        return bad(TCI_ERR_ARG, "global pivot search sizes out of range");
+    if (o.sweepstrategy < 0 || o.sweepstrategy > 2) return bad(TCI_ERR_ARG, "unknown sweep strategy");
+    if (o.maxbonddim < 0) return bad(TCI_ERR_ARG, "maxbonddim must be >= 0");
+    int64_t chi = o.chi_cap > 0 ? o.chi_cap : (o.maxbonddim < 64 ? o.maxbonddim : 64);
+    if (chi < 1) chi = 1;
+    if (chi > 8192) return bad(TCI_ERR_ARG, "chi_cap too large");
+    int extra = o.maxnglobalpivot > npiv ? o.maxnglobalpivot : npiv;
+    c.L = L;
+    c.dmax = dmax;
+    c.sumd = sumd;
+    c.setcap = (int32_t)chi + extra;
+    c.mrcap = dmax * c.setcap + c.setcap;
+    c.npivinit = npiv;
+    c.nparams = np;
+    c.niparams = nip;
+    c.nsearch = o.nsearchglobalpivot;
+    c.maxiter = o.maxiter > 0 ? o.maxiter : 1;
+    return TCI_OK;
+}
+
+// Copy the problem-owned fields into a descriptor (pointers excluded).
+inline void fill_desc_scalars(const tci_problem &p, SolveDesc &D)
+{
+    const tci_options &o = p.opts;
+    D.fkind = p.f.kind;
+    D.maxiter = o.maxiter;
+    D.sweepstrategy = o.sweepstrategy;
+    D.normalizeerror = o.normalizeerror;
+    D.ncheckhistory = o.ncheckhistory;
+    D.maxnglobalpivot = o.maxnglobalpivot;
+    D.nsearchglobalpivot = o.nsearchglobalpivot;
+    D.strictlynested = o.strictlynested;
+    D.checkconvglobalpivot = o.checkconvglobalpivot;
+    D.tolerance = o.tolerance;
+    D.tolmargin = o.tolmarginglobalsearch;
+    D.maxbonddim = o.maxbonddim;
+    D.seed = o.seed;
+}
+
+// Pack the inputs (localdims, params, iparams, initial pivots) into a host
+// image of the solve's input region: `dst_base` is the image of `dev_base`.
+inline void pack_inputs(const tci_problem &p, const SolveCaps &c, const SolveDesc &Ddev,
+                        uint8_t *dst_base, const uint8_t *dev_base)
+{
+    auto at = [&](const void *devp) { return dst_base + ((const uint8_t *)devp - dev_base); };
+    memcpy(at(Ddev.localdims), p.localdims, sizeof(int32_t) * c.L);
+    if (c.nparams) memcpy(at(Ddev.params), p.f.params, sizeof(double) * c.nparams);
+    if (c.niparams) memcpy(at(Ddev.iparams), p.f.iparams, sizeof(int32_t) * c.niparams);
+    uint8_t *piv = at(Ddev.initpivots);
+    for (int q = 0; q < c.npivinit; ++q)
+        for (int b = 0; b < c.L; ++b)
+            piv[(int64_t)q * c.L + b] = p.initialpivots ? (uint8_t)p.initialpivots[(int64_t)q * c.L + b] : 1;
+}
+
+}  // namespace tci

@@ -1,0 +1,87 @@
+"""ctypes wrapper of the TEST-ONLY CPU build of the device solve engine."""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_L = None
+
+
+def lib():
+    global _L
+    if _L is None:
+        L = C.CDLL(os.path.join(_HERE, "libengine_host.so"))
+        vp = C.c_void_p
+        L.eh_solve.restype = vp
+        L.eh_solve.argtypes = [vp, C.POINTER(C.c_int)]
+        L.eh_last_error.restype = C.c_char_p
+        for nm in ("eh_free",):
+            getattr(L, nm).argtypes = [vp]
+        L.eh_niter.argtypes = [vp]
+        L.eh_errsite.argtypes = [vp]
+        L.eh_history.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.eh_setn.argtypes = [vp, C.c_int, C.c_int]
+        L.eh_set.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int32)]
+        L.eh_Tdims.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
+        L.eh_T.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
+        L.eh_maxsample.restype = C.c_double
+        L.eh_maxsample.argtypes = [vp]
+        L.eh_npivoterrors.argtypes = [vp]
+        L.eh_pivoterrors.argtypes = [vp, C.POINTER(C.c_double)]
+        L.eh_bonderrors.argtypes = [vp, C.POINTER(C.c_double)]
+        L.eh_workstats.argtypes = [vp, C.POINTER(C.c_double)]
+        _L = L
+    return _L
+
+
+class EngineSolve:
+    def __init__(self, holder):
+        st = C.c_int()
+        self.L = holder.L
+        self.h = lib().eh_solve(C.byref(holder.st), C.byref(st))
+        self.status = st.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().eh_free(self.h)
+
+    @property
+    def niter(self):
+        return lib().eh_niter(self.h)
+
+    def history(self):
+        n = self.niter
+        r = np.zeros(max(n, 1), np.int64); e = np.zeros(max(n, 1)); g = np.zeros(max(n, 1), np.int64)
+        lib().eh_history(self.h, r.ctypes.data_as(C.POINTER(C.c_int64)), e.ctypes.data_as(C.POINTER(C.c_double)),
+                         g.ctypes.data_as(C.POINTER(C.c_int64)))
+        return r[:n], e[:n], g[:n]
+
+    def set(self, which, b):
+        n = lib().eh_setn(self.h, which, b)
+        ln = b if which == 0 else self.L - 1 - b
+        out = np.zeros(max(n * ln, 1), np.int32)
+        lib().eh_set(self.h, which, b, out.ctypes.data_as(C.POINTER(C.c_int32)))
+        return [list(map(int, out[i * ln:(i + 1) * ln])) for i in range(n)]
+
+    def sitetensor(self, b):
+        d = np.zeros(3, np.int64)
+        lib().eh_Tdims(self.h, b, d.ctypes.data_as(C.POINTER(C.c_int64)))
+        out = np.zeros(max(int(np.prod(d)), 1))
+        lib().eh_T(self.h, b, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out[:int(np.prod(d))].reshape(tuple(d), order="F")
+
+    @property
+    def maxsample(self):
+        return lib().eh_maxsample(self.h)
+
+    def pivoterrors(self):
+        n = lib().eh_npivoterrors(self.h)
+        out = np.zeros(max(n, 1))
+        lib().eh_pivoterrors(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out[:n]
+
+    def bonderrors(self):
+        out = np.zeros(self.L - 1)
+        lib().eh_bonderrors(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out

@@ -1,0 +1,118 @@
+// engine_host.cpp — TEST-ONLY CPU build of the device solve engine.
+//
+// Compiles tci_engine.h (the code the gfx950 kernel runs) with a one-thread
+// "team" so the tests can check the engine's control logic and arithmetic
+// against the oracle on the CPU.  The product library never contains or
+// calls this build: it exists only under tests/.
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "tci_engine.h"
+#include "tci_problem.h"
+
+using namespace tci;
+
+struct HostTeam {
+    int tid = 0, nt = 1;
+    double *xs = nullptr, *ys = nullptr;
+    void sync() {}
+    ArgBest argmax(ArgBest b) { return b; }
+    double max_nan(double v) { return v; }
+    int any(int v) { return v; }
+};
+
+struct EhSolve {
+    std::vector<uint8_t> inputs, arena, report;
+    SolveDesc D;
+    int status;
+};
+
+static std::string g_msg;
+
+extern "C" const char *eh_last_error(void) { return g_msg.c_str(); }
+
+extern "C" void *eh_solve(const tci_problem *p, int *status)
+{
+    SolveCaps c;
+    int st = check_problem(*p, c, &g_msg);
+    if (st != TCI_OK) { *status = st; return nullptr; }
+    EhSolve *h = new EhSolve();
+    size_t ib, sb, rb;
+    layout_solve(c, nullptr, nullptr, nullptr, nullptr, &ib, &sb, &rb);
+    h->inputs.assign(ib, 0);
+    h->arena.assign(sb, 0);
+    h->report.assign(rb, 0);
+    memset(&h->D, 0, sizeof(SolveDesc));
+    layout_solve(c, &h->D, h->inputs.data(), h->arena.data(), h->report.data(), &ib, &sb, &rb);
+    fill_desc_scalars(*p, h->D);
+    pack_inputs(*p, c, h->D, h->inputs.data(), h->inputs.data());
+    HostTeam tm;
+    tm.xs = h->D.stage;
+    tm.ys = h->D.stage + c.mrcap;
+    EngShared S;
+    memset(&S, 0, sizeof S);
+    Engine<HostTeam> e(tm, h->D, S);
+    h->status = e.run();
+    *status = h->status;
+    return h;
+}
+
+extern "C" void eh_free(void *hp) { delete (EhSolve *)hp; }
+extern "C" int eh_niter(void *hp) { return ((EhSolve *)hp)->D.ints[INT_NITER]; }
+extern "C" int eh_errsite(void *hp) { return ((EhSolve *)hp)->D.ints[INT_ERRSITE]; }
+extern "C" void eh_history(void *hp, int64_t *ranks, double *errors, int64_t *nglob)
+{
+    const SolveDesc &D = ((EhSolve *)hp)->D;
+    int n = D.ints[INT_NITER];
+    if (n > D.maxiter_cap) n = D.maxiter_cap;
+    for (int i = 0; i < n; ++i) {
+        ranks[i] = D.ranks[i];
+        errors[i] = D.errors[i];
+        nglob[i] = D.nglob[i];
+    }
+}
+extern "C" int eh_setn(void *hp, int which, int b)
+{
+    const SolveDesc &D = ((EhSolve *)hp)->D;
+    return D.setn[(which ? SET_J : SET_I) * D.L + b];
+}
+extern "C" void eh_set(void *hp, int which, int b, int32_t *out)
+{
+    const SolveDesc &D = ((EhSolve *)hp)->D;
+    const int L = D.L, kind = which ? SET_J : SET_I, len = which ? L - 1 - b : b;
+    const int n = D.setn[kind * L + b];
+    const uint8_t *s = D.sets + ((int64_t)(kind * L + b) * D.setcap) * L;
+    for (int i = 0; i < n; ++i)
+        for (int t = 0; t < len; ++t) out[i * len + t] = s[(int64_t)i * L + t];
+}
+extern "C" void eh_Tdims(void *hp, int b, int64_t *d3)
+{
+    const SolveDesc &D = ((EhSolve *)hp)->D;
+    for (int i = 0; i < 3; ++i) d3[i] = D.Tdims[3 * b + i];
+}
+extern "C" void eh_T(void *hp, int b, double *out)
+{
+    const SolveDesc &D = ((EhSolve *)hp)->D;
+    int64_t n = D.Tdims[3 * b] * D.Tdims[3 * b + 1] * D.Tdims[3 * b + 2];
+    memcpy(out, D.T + D.tstride * b, sizeof(double) * n);
+}
+extern "C" double eh_maxsample(void *hp) { return ((EhSolve *)hp)->D.scal[SCAL_MAXSAMPLE]; }
+extern "C" int eh_npivoterrors(void *hp) { return ((EhSolve *)hp)->D.ints[INT_NPE]; }
+extern "C" void eh_pivoterrors(void *hp, double *out)
+{
+    const SolveDesc &D = ((EhSolve *)hp)->D;
+    memcpy(out, D.pivoterrors, sizeof(double) * D.ints[INT_NPE]);
+}
+extern "C" void eh_bonderrors(void *hp, double *out)
+{
+    const SolveDesc &D = ((EhSolve *)hp)->D;
+    memcpy(out, D.bonderrors, sizeof(double) * (D.L - 1));
+}
+extern "C" void eh_workstats(void *hp, double *out6)
+{
+    const SolveDesc &D = ((EhSolve *)hp)->D;
+    for (int i = 0; i < 6; ++i) out6[i] = D.scal[SCAL_WS_PI + i];
+}
