@@ -183,6 +183,12 @@ tci_status tci_batch_bonderrors(tci_batch *b, int32_t i, double *out);
 tci_status tci_batch_free(tci_batch *b);
 /* Device time of the batch's solve kernel (HIP events), milliseconds. */
 tci_status tci_batch_elapsed_ms(tci_batch *b, float *ms);
+/* Diagnostic (no counterpart in the reference, whose only timing split is the
+ * verbosity>2 print at tensorci2.jl:547-550): microseconds solve i spent in
+ * each phase — [0] Pi evaluation, [1] rrLU, [2] pivot-set updates (unused),
+ * [3] fillsitetensors!, [4] global pivot search, [5] final sweep1site!,
+ * [6] whole solve, [7] unused. */
+tci_status tci_batch_profile(tci_batch *b, int32_t i, double *us8);
 
 #ifdef __cplusplus
 }

@@ -15,7 +15,7 @@
 #include <stdlib.h>
 #include <string.h>
 
-static char g_err[256];
+static _Thread_local char g_err[256];
 static int fail(int code, const char *fmt, ...)
 {
     va_list ap;
@@ -98,7 +98,7 @@ static void addpivot(double *A, int64_t m, int64_t n, int64_t *rowperm, int64_t 
 }
 
 /* work accounting for the roofline: Σ_{k<s}(m-k)(n-k) and Σ_{k=1..p}(m-k)(n-k) */
-static double g_stat_arg, g_stat_schur;
+static _Thread_local double g_stat_arg, g_stat_schur;
 
 /* rrlu (matrixlu.jl:217-225) → rrlu! (194-204) → _optimizerrlu! (141-181). */
 int orc_rrlu(const double *A0, int64_t m, int64_t n, int64_t maxrank, double reltol,

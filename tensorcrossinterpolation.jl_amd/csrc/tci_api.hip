@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <string>
@@ -45,12 +46,71 @@ static tci_status set_err(tci_status code, const char *fmt, ...)
             return set_err(TCI_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
 
+// Device buffers of finished batches are kept for reuse, so a serving loop
+// does not pay hipMalloc/hipFree per launch.
+struct PoolBuf {
+    void *ptr;
+    size_t bytes;
+};
+
 struct tci_ctx {
     int device;
     hipStream_t stream;
     uint64_t budget;
     uint64_t used;
+    std::vector<PoolBuf> pool;
+    int lds_max;        // dynamic LDS bytes a workgroup may use
+    int team;           // threads per solve workgroup
+    int pi_lds_kb;      // LDS budget for Pi matrices (KiB)
 };
+
+// Launch shape of the solve kernel: a team of `team` threads per solve; LDS
+// holds the rrLU staging vectors, the index tables and as much of a Pi
+// matrix as the budget allows (one resident workgroup per CU when large).
+static LdsPlan make_plan(const tci_ctx *c, int max_mrcap, int max_tab)
+{
+    LdsPlan p;
+    memset(&p, 0, sizeof p);
+    int64_t left = c->lds_max;
+    if ((int64_t)2 * max_mrcap * 8 <= left) {
+        p.stage_cap = max_mrcap;
+        left -= (int64_t)2 * max_mrcap * 8;
+    }
+    int64_t tab = ((int64_t)max_tab + 15) & ~(int64_t)15;
+    if (2 * tab <= left) {
+        p.tab_cap = (int32_t)tab;
+        left -= 2 * tab;
+    }
+    int64_t pib = (int64_t)c->pi_lds_kb * 1024;
+    if (pib > left) pib = left;
+    p.pi_cap = pib > 0 ? pib / 8 : 0;
+    return p;
+}
+
+static hipError_t pool_get(tci_ctx *c, size_t bytes, void **out)
+{
+    size_t best = (size_t)-1;
+    for (size_t i = 0; i < c->pool.size(); ++i)
+        if (c->pool[i].bytes >= bytes && c->pool[i].bytes <= 2 * bytes + (1 << 20) &&
+            (best == (size_t)-1 || c->pool[i].bytes < c->pool[best].bytes))
+            best = i;
+    if (best != (size_t)-1) {
+        *out = c->pool[best].ptr;
+        c->pool.erase(c->pool.begin() + (long)best);
+        return hipSuccess;
+    }
+    return hipMalloc(out, bytes);
+}
+
+static void pool_put(tci_ctx *c, void *p, size_t bytes)
+{
+    if (!p) return;
+    c->pool.push_back({p, bytes});
+    while (c->pool.size() > 16) {
+        (void)hipFree(c->pool.front().ptr);
+        c->pool.erase(c->pool.begin());
+    }
+}
 
 struct tci_batch {
     tci_ctx *ctx;
@@ -59,7 +119,7 @@ struct tci_batch {
     std::vector<size_t> rep_off;
     uint8_t *d_in = nullptr, *d_state = nullptr, *d_rep = nullptr;
     SolveDesc *d_desc = nullptr;
-    size_t bytes = 0, rep_bytes = 0;
+    size_t bytes = 0, rep_bytes = 0, in_bytes = 0, state_bytes = 0, desc_bytes = 0;
     std::vector<uint8_t> h_in;        // upload image, alive until the copy ran
     std::vector<uint8_t> h_rep;
     bool fetched = false;
@@ -99,6 +159,14 @@ tci_status tci_ctx_create(tci_ctx **out, int32_t device, uint64_t hbm_budget_byt
         hbm_budget_bytes = (uint64_t)(0.8 * (double)fr);
     }
     c->budget = hbm_budget_bytes;
+    int lds = 0;
+    HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    c->lds_max = lds - 4096;   // static LDS of the solve kernel stays below 4 KiB
+    const char *ev = getenv("TCI_TEAM");
+    c->team = ev ? atoi(ev) : 512;
+    if (c->team != 256 && c->team != 512 && c->team != 1024) c->team = 512;
+    ev = getenv("TCI_PI_LDS_KB");
+    c->pi_lds_kb = ev ? atoi(ev) : 1 << 20;
     *out = c;
     return TCI_OK;
 }
@@ -106,9 +174,10 @@ tci_status tci_ctx_create(tci_ctx **out, int32_t device, uint64_t hbm_budget_byt
 tci_status tci_ctx_destroy(tci_ctx *ctx)
 {
     if (!ctx) return TCI_OK;
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
-    hipStreamDestroy(ctx->stream);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (PoolBuf &b : ctx->pool) (void)hipFree(b.ptr);
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return TCI_OK;
 }
@@ -146,7 +215,7 @@ tci_status tci_rrlu_batched_f64(tci_ctx *ctx, tci_lu_job *jobs, int64_t count)
     const int64_t lds_cap = 48 * 1024 / 16;
     if (mx > lds_cap) HIPCHK(hipMallocAsync((void **)&gstage, sizeof(double) * 2 * mx * count, ctx->stream));
     hipError_t e = launch_rrlu(jobs, count, mx, gstage, 2 * mx, ctx->stream);
-    if (gstage) hipFreeAsync(gstage, ctx->stream);
+    if (gstage) (void)hipFreeAsync(gstage, ctx->stream);
     if (e != hipSuccess) return set_err(TCI_ERR_DEVICE, "rrlu launch: %s", hipGetErrorString(e));
     return TCI_OK;
 }
@@ -207,11 +276,11 @@ tci_status tci_rrlu_f64(tci_ctx *ctx, const double *A, int64_t lda, int64_t m, i
     HIPCHK(hipStreamSynchronize(st));
     for (int64_t i = 0; i < m; ++i) rowperm[i] = perm[(size_t)i];
     for (int64_t q = 0; q < n; ++q) colperm[q] = perm[(size_t)(m + q)];
-    hipFreeAsync(dA, st);
-    hipFreeAsync(dperm, st);
-    hipFreeAsync(dj, st);
-    if (dL) hipFreeAsync(dL, st);
-    if (dU) hipFreeAsync(dU, st);
+    (void)hipFreeAsync(dA, st);
+    (void)hipFreeAsync(dperm, st);
+    (void)hipFreeAsync(dj, st);
+    if (dL) (void)hipFreeAsync(dL, st);
+    if (dU) (void)hipFreeAsync(dU, st);
     *npivot = p;
     *error = j.error;
     if (j.status == TCI_ERR_NAN_L) return set_err(TCI_ERR_NAN_L, "lu.L contains NaNs");
@@ -285,7 +354,7 @@ tci_status tci_pi_eval_f64(tci_ctx *ctx, const tci_functor *f, int32_t L, const 
     std::vector<double> bm((size_t)blocks);
     HIPCHK(hipMemcpyAsync(bm.data(), d + o_bm, sizeof(double) * blocks, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    hipFreeAsync(d, st);
+    (void)hipFreeAsync(d, st);
     if (maxabs_inout) {
         double mx = fabs(*maxabs_inout);
         for (double v : bm) mx = nan_max(mx, v);
@@ -317,13 +386,20 @@ void tci_options_default(tci_options *o)
 static void batch_release(tci_batch *b)
 {
     if (!b) return;
-    if (b->d_desc) hipFree(b->d_desc);
-    if (b->d_in) hipFree(b->d_in);
-    if (b->d_state) hipFree(b->d_state);
-    if (b->d_rep) hipFree(b->d_rep);
-    if (b->ev0) hipEventDestroy(b->ev0);
-    if (b->ev1) hipEventDestroy(b->ev1);
-    if (b->ctx) b->ctx->used -= b->bytes;
+    if (b->ctx) {
+        pool_put(b->ctx, b->d_desc, b->desc_bytes);
+        pool_put(b->ctx, b->d_in, b->in_bytes);
+        pool_put(b->ctx, b->d_state, b->state_bytes);
+        pool_put(b->ctx, b->d_rep, b->rep_bytes);
+        b->ctx->used -= b->bytes;
+    } else {
+        if (b->d_desc) (void)hipFree(b->d_desc);
+        if (b->d_in) (void)hipFree(b->d_in);
+        if (b->d_state) (void)hipFree(b->d_state);
+        if (b->d_rep) (void)hipFree(b->d_rep);
+    }
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
     delete b;
 }
 
@@ -334,8 +410,10 @@ tci_status tci_solve_submit(tci_ctx *ctx, int32_t count, const tci_problem *prob
     std::vector<SolveCaps> caps((size_t)count);
     std::vector<size_t> ib((size_t)count), sb((size_t)count), rb((size_t)count);
     size_t tin = 0, tstate = 0, trep = 0;
-    int max_mr = 0;
+    int max_mr = 0, max_tab = 0;
+    int fkind = problems[0].f.kind;
     for (int i = 0; i < count; ++i) {
+        if (problems[i].f.kind != fkind) fkind = 0;
         std::string msg;
         int s = check_problem(problems[i], caps[(size_t)i], &msg);
         if (s != TCI_OK) return set_err((tci_status)s, "problem %d: %s", i, msg.c_str());
@@ -344,6 +422,8 @@ tci_status tci_solve_submit(tci_ctx *ctx, int32_t count, const tci_problem *prob
         tstate += sb[(size_t)i];
         trep += rb[(size_t)i];
         if (caps[(size_t)i].mrcap > max_mr) max_mr = caps[(size_t)i].mrcap;
+        const int tab = caps[(size_t)i].mrcap * caps[(size_t)i].L;
+        if (tab > max_tab) max_tab = tab;
     }
     const size_t total = tin + tstate + trep + sizeof(SolveDesc) * (size_t)count;
     if (ctx->used + total > ctx->budget)
@@ -358,13 +438,16 @@ tci_status tci_solve_submit(tci_ctx *ctx, int32_t count, const tci_problem *prob
         return set_err(e == hipErrorOutOfMemory ? TCI_ERR_OOM : TCI_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
     };
     hipError_t e;
-    if ((e = hipMalloc((void **)&b->d_in, tin)) != hipSuccess) return fail(e, "hipMalloc(inputs)");
-    if ((e = hipMalloc((void **)&b->d_state, tstate)) != hipSuccess) return fail(e, "hipMalloc(state)");
-    if ((e = hipMalloc((void **)&b->d_rep, trep)) != hipSuccess) return fail(e, "hipMalloc(report)");
-    if ((e = hipMalloc((void **)&b->d_desc, sizeof(SolveDesc) * (size_t)count)) != hipSuccess) return fail(e, "hipMalloc(desc)");
     b->ctx = ctx;
     b->bytes = total;
     ctx->used += total;
+    b->in_bytes = tin;
+    b->state_bytes = tstate;
+    b->desc_bytes = sizeof(SolveDesc) * (size_t)count;
+    if ((e = pool_get(ctx, tin, (void **)&b->d_in)) != hipSuccess) return fail(e, "hipMalloc(inputs)");
+    if ((e = pool_get(ctx, tstate, (void **)&b->d_state)) != hipSuccess) return fail(e, "hipMalloc(state)");
+    if ((e = pool_get(ctx, trep, (void **)&b->d_rep)) != hipSuccess) return fail(e, "hipMalloc(report)");
+    if ((e = pool_get(ctx, b->desc_bytes, (void **)&b->d_desc)) != hipSuccess) return fail(e, "hipMalloc(desc)");
     b->rep_bytes = trep;
     b->desc.resize((size_t)count);
     b->rep_off.resize((size_t)count);
@@ -390,7 +473,8 @@ tci_status tci_solve_submit(tci_ctx *ctx, int32_t count, const tci_problem *prob
     if ((e = hipEventCreate(&b->ev0)) != hipSuccess) return fail(e, "hipEventCreate");
     if ((e = hipEventCreate(&b->ev1)) != hipSuccess) return fail(e, "hipEventCreate");
     if ((e = hipEventRecord(b->ev0, st)) != hipSuccess) return fail(e, "hipEventRecord");
-    if ((e = launch_solve(b->d_desc, count, max_mr, st)) != hipSuccess) return fail(e, "solve launch");
+    const LdsPlan plan = make_plan(ctx, max_mr, max_tab);
+    if ((e = launch_solve(b->d_desc, count, ctx->team, fkind, plan, st)) != hipSuccess) return fail(e, "solve launch");
     if ((e = hipEventRecord(b->ev1, st)) != hipSuccess) return fail(e, "hipEventRecord");
     *out = b;
     return TCI_OK;
@@ -532,12 +616,23 @@ tci_status tci_batch_elapsed_ms(tci_batch *b, float *ms)
     return TCI_OK;
 }
 
+tci_status tci_batch_profile(tci_batch *b, int32_t i, double *us8)
+{
+    SolveDesc R;
+    if (!us8 || !report_view(b, i, R)) return set_err(TCI_ERR_ARG, "bad batch/index");
+    int khz = 0;
+    HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->ctx->device));
+    const double us_per_tick = khz > 0 ? 1e3 / (double)khz : 0.0;
+    for (int k = 0; k < 8; ++k) us8[k] = R.scal[SCAL_TPH + k] * us_per_tick;
+    return TCI_OK;
+}
+
 tci_status tci_batch_free(tci_batch *b)
 {
     if (!b) return TCI_OK;
     if (b->ctx) {
-        hipSetDevice(b->ctx->device);
-        if (b->ev1) hipEventSynchronize(b->ev1);
+        (void)hipSetDevice(b->ctx->device);
+        if (b->ev1) (void)hipEventSynchronize(b->ev1);
     }
     batch_release(b);
     return TCI_OK;

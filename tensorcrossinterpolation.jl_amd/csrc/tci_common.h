@@ -1,0 +1,85 @@
+// tci_common.h — definitions shared by the solve engine and the register-tile
+// rrLU: status codes, the argmax key, Julia's NaN-propagating max, the
+// team-shared scalars and the project PRNG.
+#pragma once
+
+#include <math.h>
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define ENG __device__
+#define ENG_INLINE __device__ __forceinline__
+#define ENG_NOINLINE __device__ __attribute__((noinline))
+#else
+#define ENG
+#define ENG_INLINE inline
+#define ENG_NOINLINE __attribute__((noinline))
+#endif
+
+namespace tci {
+
+enum {
+    ST_OK = 0, ST_NAN_L = 1, ST_NAN_U = 2, ST_ARG = 3, ST_ZERO_MAXSAMPLE = 4,
+    ST_NAN_TENSOR = 5, ST_NONSQUARE = 6, ST_DIM = 7, ST_GENERIC = 8, ST_SINGULAR = 9,
+    ST_CAPACITY = 10
+};
+
+struct ArgBest {
+    double v;      // abs2 value, -1 = no candidate (NaN entries never count)
+    int64_t lin;   // column-major linear position
+    double a;      // the entry itself
+    int32_t orig;  // original row + 65536·original column (register-tile rrLU)
+    int32_t pad;
+};
+
+ENG inline bool arg_better(const ArgBest &a, const ArgBest &b)
+{
+    return a.v > b.v || (a.v == b.v && a.lin < b.lin);
+}
+
+// Base.max on Float64: NaN-propagating
+ENG inline double jl_max(double a, double b)
+{
+    if (a != a || b != b) return a + b;
+    return a > b ? a : b;
+}
+
+// Scalars shared by the team (written by thread 0 between barriers).
+struct EngShared {
+    double maxsample;
+    double abstol;
+    double ws[6];
+    uint64_t rng[4];
+    uint64_t tph[8];   // clock ticks per phase (thread 0): see PH_*
+    int32_t status;
+    int32_t cnt;       // scratch count (union compaction)
+    int32_t nf;        // global pivots found
+    int32_t pad;
+};
+
+// ---- project PRNG: SplitMix64-seeded xoshiro256** --------------------------
+ENG inline uint64_t eng_splitmix(uint64_t *x)
+{
+    uint64_t z = (*x += 0x9e3779b97f4a7c15ULL);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+ENG inline void eng_rng_seed(uint64_t *s, uint64_t seed)
+{
+    uint64_t x = seed;
+    for (int i = 0; i < 4; ++i) s[i] = eng_splitmix(&x);
+}
+ENG inline uint64_t eng_rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+ENG inline uint64_t eng_rng_next(uint64_t *s)
+{
+    uint64_t r = eng_rotl(s[1] * 5, 7) * 9;
+    uint64_t t = s[1] << 17;
+    s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = eng_rotl(s[3], 45);
+    return r;
+}
+
+
+}  // namespace tci

@@ -33,7 +33,9 @@ enum {
     SCAL_WS_SCHUR = 4,    // Schur flops        2·Σ_{k=1..p}(m-k)(n-k)
     SCAL_WS_NRRLU = 5,
     SCAL_WS_NEVAL = 6,
-    NSCAL = 8
+    SCAL_TPH = 8,         // 8 phase timers (device clock ticks): eval, rrlu, sets,
+                          // fill, global search, sweep1site, total
+    NSCAL = 16
 };
 
 struct SolveDesc {

@@ -4,11 +4,18 @@
 //
 // The engine is written once against a "Team" interface:
 //   tm.tid, tm.nt            thread id / team size
+//   tm.lane, tm.wsize        lane within a wave / wave width (64 on gfx950)
+//   tm.wave, tm.nwaves       wave id / waves per team
 //   tm.sync()                team barrier
 //   tm.argmax(ArgBest)       team-wide argmax, result in every thread
 //   tm.max_nan(double)       team-wide NaN-propagating max
 //   tm.any(int)              team-wide logical or
 //   tm.xs, tm.ys             staging vectors (mrcap doubles each)
+//   tm.lpi, tm.lpi_cap       on-chip (LDS) buffer for Pi matrices, in doubles
+//   tm.lrows, tm.lcols       on-chip index tables (null: use the HBM ones)
+//   tm.lperm                 LDS ints for the register-tile rrLU permutations
+//   tm.uniform(x)            x as a wave-uniform value (all lanes hold the same x)
+//   Team::TR, RA, CB         register-tile shape (tci_regtile.h); RA = 0: none
 // The device implementation lives in tci_kernels.hip.  Every parallel loop
 // writes disjoint elements and every reduction is over a total order, so the
 // result does not depend on the team size: a one-thread team on the CPU
@@ -24,73 +31,10 @@
 #include "../../include/tci_functors.h"
 #include "tci_desc.h"
 
-#if defined(__HIPCC__)
-#define ENG __device__
-#else
-#define ENG
-#endif
+#include "tci_common.h"
+#include "tci_regtile.h"
 
 namespace tci {
-
-enum {
-    ST_OK = 0, ST_NAN_L = 1, ST_NAN_U = 2, ST_ARG = 3, ST_ZERO_MAXSAMPLE = 4,
-    ST_NAN_TENSOR = 5, ST_NONSQUARE = 6, ST_DIM = 7, ST_GENERIC = 8, ST_SINGULAR = 9,
-    ST_CAPACITY = 10
-};
-
-struct ArgBest {
-    double v;      // abs2 value, -1 = no candidate (NaN entries never count)
-    int64_t lin;   // column-major linear position
-    double a;      // the entry itself
-};
-
-ENG inline bool arg_better(const ArgBest &a, const ArgBest &b)
-{
-    return a.v > b.v || (a.v == b.v && a.lin < b.lin);
-}
-
-// Base.max on Float64: NaN-propagating
-ENG inline double jl_max(double a, double b)
-{
-    if (a != a || b != b) return a + b;
-    return a > b ? a : b;
-}
-
-// Scalars shared by the team (written by thread 0 between barriers).
-struct EngShared {
-    double maxsample;
-    double abstol;
-    double ws[6];
-    uint64_t rng[4];
-    int32_t status;
-    int32_t cnt;       // scratch count (union compaction)
-    int32_t nf;        // global pivots found
-    int32_t pad;
-};
-
-// ---- project PRNG: SplitMix64-seeded xoshiro256** --------------------------
-ENG inline uint64_t eng_splitmix(uint64_t *x)
-{
-    uint64_t z = (*x += 0x9e3779b97f4a7c15ULL);
-    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-    return z ^ (z >> 31);
-}
-ENG inline void eng_rng_seed(uint64_t *s, uint64_t seed)
-{
-    uint64_t x = seed;
-    for (int i = 0; i < 4; ++i) s[i] = eng_splitmix(&x);
-}
-ENG inline uint64_t eng_rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
-ENG inline uint64_t eng_rng_next(uint64_t *s)
-{
-    uint64_t r = eng_rotl(s[1] * 5, 7) * 9;
-    uint64_t t = s[1] << 17;
-    s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3];
-    s[2] ^= t;
-    s[3] = eng_rotl(s[3], 45);
-    return r;
-}
 
 // rrlu!(A) with full pivoting (matrixlu.jl:114-181) on A (m×n, lda = m),
 // run by the whole team.  Output: rowperm/colperm (0-based original index at
@@ -110,16 +54,18 @@ ENG int rrlu_team(Team &tm, double *A, int m, int n, int64_t maxrank, double rel
     best.v = -1.0;
     best.lin = INT64_MAX;
     best.a = 0.0;
-    const int64_t cnt = (int64_t)m * n;
-    for (int64_t q = tm.tid; q < cnt; q += tm.nt) {
-        double a = A[q];
-        double v = a * a;
-        if (v == v) {
-            ArgBest c;
-            c.v = v;
-            c.lin = q;
-            c.a = a;
-            if (arg_better(c, best)) best = c;
+    for (int j = tm.wave; j < n; j += tm.nwaves) {
+        const double *col = A + lda * j;
+        for (int i = tm.lane; i < m; i += tm.wsize) {
+            double a = col[i];
+            double v = a * a;
+            if (v == v) {
+                ArgBest c;
+                c.v = v;
+                c.lin = i + lda * j;
+                c.a = a;
+                if (arg_better(c, best)) best = c;
+            }
         }
     }
     best = tm.argmax(best);
@@ -190,21 +136,22 @@ ENG int rrlu_team(Team &tm, double *A, int m, int n, int64_t maxrank, double rel
         best.v = -1.0;
         best.lin = INT64_MAX;
         best.a = 0.0;
-        const int mm = m - k - 1, nn = n - k - 1;
-        const int64_t tot = (int64_t)mm * nn;
-        for (int64_t q = tm.tid; q < tot; q += tm.nt) {
-            int64_t jj = q / mm, ii = q - jj * mm;
-            int64_t i = k + 1 + ii, j = k + 1 + jj;
-            double t = tm.xs[i] * tm.ys[j];
-            double a = A[i + lda * j] - t;
-            A[i + lda * j] = a;
-            double v = a * a;
-            if (v == v) {
-                ArgBest cnd;
-                cnd.v = v;
-                cnd.lin = i + lda * j;
-                cnd.a = a;
-                if (arg_better(cnd, best)) best = cnd;
+        // columns over waves, rows over lanes: coalesced, no index division
+        for (int j = k + 1 + tm.wave; j < n; j += tm.nwaves) {
+            double *col = A + lda * j;
+            const double yj = tm.ys[j];
+            for (int i = k + 1 + tm.lane; i < m; i += tm.wsize) {
+                double t = tm.xs[i] * yj;
+                double a = col[i] - t;
+                col[i] = a;
+                double v = a * a;
+                if (v == v) {
+                    ArgBest cnd;
+                    cnd.v = v;
+                    cnd.lin = i + lda * j;
+                    cnd.a = a;
+                    if (arg_better(cnd, best)) best = cnd;
+                }
             }
         }
         best = tm.argmax(best);
@@ -213,13 +160,17 @@ ENG int rrlu_team(Team &tm, double *A, int m, int n, int64_t maxrank, double rel
     }
     // L = tril(A[:,1:p]), U = triu(A[1:p,:]); NaN checks (matrixlu.jl:162-169)
     int nanL = 0, nanU = 0;
-    for (int64_t q = tm.tid; q < (int64_t)m * p; q += tm.nt) {
-        int64_t j = q / m, i = q - j * m;
-        if (i >= j) { double v = A[i + lda * j]; if (v != v) nanL = 1; }
-    }
-    for (int64_t q = tm.tid; q < (int64_t)p * n; q += tm.nt) {
-        int64_t j = q / p, i = q - j * p;
-        if (j >= i) { double v = A[i + lda * j]; if (v != v) nanU = 1; }
+    for (int j = tm.wave; j < p; j += tm.nwaves)
+        for (int i = j + tm.lane; i < m; i += tm.wsize) {
+            double v = A[i + lda * j];
+            if (v != v) nanL = 1;
+        }
+    for (int j = tm.wave; j < n; j += tm.nwaves) {
+        const int imax = j < p - 1 ? j : p - 1;
+        for (int i = tm.lane; i <= imax; i += tm.wsize) {
+            double v = A[i + lda * j];
+            if (v != v) nanU = 1;
+        }
     }
     nanL = tm.any(nanL);
     nanU = tm.any(nanU);
@@ -234,14 +185,28 @@ ENG int rrlu_team(Team &tm, double *A, int m, int n, int64_t maxrank, double rel
 }
 
 
-template <class Team>
+// phases timed by the engine (ticks of tm.clock(), accumulated by thread 0)
+enum { PH_EVAL = 0, PH_RRLU = 1, PH_SETS = 2, PH_FILL = 3, PH_GSEARCH = 4, PH_SWEEP1 = 5, PH_TOTAL = 6 };
+
+// FK > 0 compiles the engine for one functor kind (the switch folds away);
+// FK = 0 dispatches on the descriptor's kind at run time.
+template <class Team, int FK = 0>
 struct Engine {
     Team &tm;
     const SolveDesc &D;
     EngShared &S;
     int L;
 
-    ENG Engine(Team &t, const SolveDesc &d, EngShared &s) : tm(t), D(d), S(s), L(d.L) {}
+    uint8_t *rows, *cols;   // combined index tables (LDS when the team has them)
+
+    ENG Engine(Team &t, const SolveDesc &d, EngShared &s)
+        : tm(t), D(d), S(s), L(d.L), rows(t.lrows ? t.lrows : d.rows), cols(t.lcols ? t.lcols : d.cols) {}
+
+    // a Pi matrix of m×n doubles lives in LDS when it fits, else in HBM
+    ENG double *pibuf(int m, int n) const
+    {
+        return (int64_t)m * n <= tm.lpi_cap ? tm.lpi : D.pi;
+    }
 
     // ---------------------------------------------------------------- sets
     ENG uint8_t *setp(int kind, int b) const
@@ -300,7 +265,7 @@ struct Engine {
         if (m > D.mrcap) return m;   // caller reports ST_CAPACITY
         for (int q = tm.tid; q < m; q += tm.nt) {
             int s = q / nI, i = q - s * nI;
-            uint8_t *r = D.rows + (int64_t)q * L;
+            uint8_t *r = rows + (int64_t)q * L;
             for (int t = 0; t < b; ++t) r[t] = I[(int64_t)i * L + t];
             r[b] = (uint8_t)(s + 1);
         }
@@ -317,7 +282,7 @@ struct Engine {
         if (n > D.mrcap) return n;   // caller reports ST_CAPACITY
         for (int q = tm.tid; q < n; q += tm.nt) {
             int j = q / d, s = q - j * d;
-            uint8_t *c = D.cols + (int64_t)q * L;
+            uint8_t *c = cols + (int64_t)q * L;
             c[0] = (uint8_t)(s + 1);
             for (int t = 0; t < len; ++t) c[1 + t] = J[(int64_t)j * L + t];
         }
@@ -349,7 +314,7 @@ struct Engine {
         tm.sync();
         if (tm.tid == 0) {
             int m = m0;
-            uint8_t *dst = rowside ? D.rows : D.cols;
+            uint8_t *dst = rowside ? rows : cols;
             for (int e = 0; e < ne; ++e) {
                 if (!D.flags[e]) continue;
                 if (m >= D.mrcap) { S.status = ST_CAPACITY; break; }
@@ -391,19 +356,20 @@ struct Engine {
     ENG void eval(const uint8_t *rows, int nl, int m, const uint8_t *cols, int n, double *out,
                   int64_t ld, bool transpose, bool fold)
     {
+        const uint64_t t0 = tm.clock();
         double lmax = 0.0;
         const int64_t cnt = (int64_t)m * n;
-        for (int64_t q = tm.tid; q < cnt; q += tm.nt) {
-            int64_t j = q / m, i = q - j * m;
-            tci_point x;
-            x.a = rows + i * L;
-            x.b = cols + j * L;
-            x.na = nl;
-            x.L = L;
-            double v = tci_functor_eval(D.fkind, D.params, D.iparams, &x);
-            if (out) out[transpose ? (j + ld * i) : (i + ld * j)] = v;
-            lmax = jl_max(lmax, fabs(v));
-        }
+        for (int j = tm.wave; j < n; j += tm.nwaves)
+            for (int i = tm.lane; i < m; i += tm.wsize) {
+                tci_point x;
+                x.a = rows + (int64_t)i * L;
+                x.b = cols + (int64_t)j * L;
+                x.na = nl;
+                x.L = L;
+                double v = tci_functor_eval(FK ? FK : D.fkind, D.params, D.iparams, &x);
+                if (out) out[transpose ? (j + ld * i) : (i + ld * j)] = v;
+                lmax = jl_max(lmax, fabs(v));
+            }
         if (fold) {
             double g = tm.max_nan(lmax);
             if (tm.tid == 0) {
@@ -413,16 +379,45 @@ struct Engine {
             }
         }
         tm.sync();
+        if (tm.tid == 0) S.tph[PH_EVAL] += tm.clock() - t0;
     }
 
     // --------------------------------------------------------------- rrLU
-    ENG int rrlu(int m, int n, int64_t maxrank, double reltol, double abstol, bool leftorth,
-                 double *perr)
+    // rrlu!(A) on the m×n matrix A (lda = m).  With need_left, A's first p
+    // columns hold the factor L in position order afterwards (used by
+    // colstimespivotinv); otherwise only the permutations, pivots and error
+    // are produced.  Small matrices are factored in registers.
+    // (not inlined: one copy of the register tile code per kernel)
+    ENG_NOINLINE int rrlu(double *A, int m, int n, int64_t maxrank, double reltol, double abstol,
+                          bool leftorth, double *perr, bool need_left = false,
+                          double *lscratch = nullptr)
     {
         double st[4];
-        int p = rrlu_team(tm, D.pi, m, n, maxrank, reltol, abstol, leftorth, D.rowperm,
-                          D.colperm, D.pivval, perr, st);
+        const uint64_t t0 = tm.clock();
+        int p;
+        if (Team::RA > 0 && RegLU<Team, Team::TR, Team::RA, Team::CB>::fits(tm, m, n)) {
+            RegLU<Team, Team::TR, Team::RA, Team::CB> R;
+            R.setup(tm, m, n);
+            R.load(A, m);
+            // L recorded by original row in lscratch, then permuted into A
+            double *Lorig = need_left ? lscratch : nullptr;
+            p = R.factor_lean(tm, maxrank, reltol, abstol, leftorth, tm.lperm, tm.lperm + m, tm.xs,
+                              tm.ys, D.pivval, perr, st, Lorig);
+            for (int i = tm.tid; i < m; i += tm.nt) D.rowperm[i] = tm.lperm[i];
+            for (int j = tm.tid; j < n; j += tm.nt) D.colperm[j] = tm.lperm[m + j];
+            if (need_left && p > 0) {
+                tm.sync();
+                for (int t = tm.wave; t < p; t += tm.nwaves)
+                    for (int pos = t + 1 + tm.lane; pos < m; pos += tm.wsize)
+                        A[pos + (int64_t)m * t] = Lorig[tm.lperm[pos] + (int64_t)m * t];
+            }
+            tm.sync();
+        } else {
+            p = rrlu_team(tm, A, m, n, maxrank, reltol, abstol, leftorth, D.rowperm, D.colperm,
+                          D.pivval, perr, st);
+        }
         if (tm.tid == 0) {
+            S.tph[PH_RRLU] += tm.clock() - t0;
             S.ws[1] += 8.0 * st[0];
             S.ws[2] += 8.0 * (double)m * (double)n + 16.0 * st[1] +
                        8.0 * (double)(p > 0 ? p : 0) * (double)(m + n);
@@ -455,6 +450,61 @@ struct Engine {
         return S.status;
     }
 
+    // Pi evaluation fused into the register-tile rrLU for the Lorentzian
+    // functor: the matrix is built in registers from integer row and column
+    // sums of v², never stored.  Returns p or -status like rrlu().
+    ENG_INLINE bool lorentz_tile(int m, int n) const
+    {
+        return (FK == TCI_F_LORENTZ_INT || (FK == 0 && D.fkind == TCI_F_LORENTZ_INT)) &&
+               Team::RA > 0 && RegLU<Team, Team::TR, Team::RA, Team::CB>::fits(tm, m, n) &&
+               m <= D.mrcap && n <= D.mrcap;
+    }
+
+    ENG_NOINLINE int eval_rrlu_lorentz(int m, int nl, int n, int64_t maxrank, double reltol,
+                                       double abstol, bool leftorth, double *perr)
+    {
+        const uint64_t t0 = tm.clock();
+        // integer sums of squares of the row part and the column part (exact
+        // in double: at most 250·255² < 2^53)
+        for (int q = tm.tid; q < m; q += tm.nt) {
+            int64_t acc = 0;
+            for (int t = 0; t < nl; ++t) { int64_t v = rows[(int64_t)q * L + t]; acc += v * v; }
+            tm.xs[q] = (double)acc;
+        }
+        for (int q = tm.tid; q < n; q += tm.nt) {
+            int64_t acc = 0;
+            for (int t = 0; t < L - nl; ++t) { int64_t v = cols[(int64_t)q * L + t]; acc += v * v; }
+            tm.ys[q] = (double)acc;
+        }
+        tm.sync();
+        RegLU<Team, Team::TR, Team::RA, Team::CB> R;
+        R.setup(tm, m, n);
+        const double lmax = R.fill_lorentz(D.params[0], tm.xs, tm.ys);
+        const double g = tm.max_nan(lmax);
+        if (tm.tid == 0) {
+            S.maxsample = jl_max(fabs(S.maxsample), g);
+            S.ws[0] += (double)m * (double)n;
+            S.ws[5] += 1.0;
+            S.tph[PH_EVAL] += tm.clock() - t0;
+        }
+        const uint64_t t1 = tm.clock();
+        double st[4];
+        int p = R.factor_lean(tm, maxrank, reltol, abstol, leftorth, tm.lperm, tm.lperm + m,
+                              tm.xs, tm.ys, D.pivval, perr, st, nullptr);
+        for (int i = tm.tid; i < m; i += tm.nt) D.rowperm[i] = tm.lperm[i];
+        for (int j = tm.tid; j < n; j += tm.nt) D.colperm[j] = tm.lperm[m + j];
+        tm.sync();
+        if (tm.tid == 0) {
+            S.tph[PH_RRLU] += tm.clock() - t1;
+            S.ws[1] += 8.0 * st[0];
+            S.ws[2] += 8.0 * (double)m * (double)n + 16.0 * st[1] +
+                       8.0 * (double)(p > 0 ? p : 0) * (double)(m + n);
+            S.ws[3] += 2.0 * st[1];
+            S.ws[4] += 1.0;
+        }
+        return p;
+    }
+
     // ------------------------------------------------------ updatepivots!
     // tensorci2.jl:510-551, 599-607 (pivotsearch = :full).  ekind < 0: no extras.
     ENG int updatepivots(int b, bool leftorth, double abstol, int ekind_i, int ekind_j)
@@ -469,13 +519,19 @@ struct Engine {
             n = union_extras(false, b, n, ekind_j);
             if (S.status) return S.status;
         }
-        eval(D.rows, b + 1, m, D.cols, n, D.pi, m, false, true);
         double err;
-        int p = rrlu(m, n, D.maxbonddim, 1e-14, abstol, leftorth, &err);
+        int p;
+        if (lorentz_tile(m, n)) {
+            p = eval_rrlu_lorentz(m, b + 1, n, D.maxbonddim, 1e-14, abstol, leftorth, &err);
+        } else {
+            double *A = pibuf(m, n);
+            eval(rows, b + 1, m, cols, n, A, m, false, true);
+            p = rrlu(A, m, n, D.maxbonddim, 1e-14, abstol, leftorth, &err);
+        }
         if (p < 0) return fail(-p);
         if (p > D.setcap) return fail(ST_CAPACITY);
-        select_into(SET_I, b + 1, D.rows, D.rowperm, p, b + 1);
-        select_into(SET_J, b, D.cols, D.colperm, p, jlen(b));
+        select_into(SET_I, b + 1, rows, D.rowperm, p, b + 1);
+        select_into(SET_J, b, cols, D.colperm, p, jlen(b));
         updateerrors(b, p, err);
         tm.sync();
         return ST_OK;
@@ -483,12 +539,15 @@ struct Engine {
 
     // ------------------------------------------------- Pi1·P⁻¹ (site tensor)
     // T_b = transpose(transpose(P) \ transpose(Pi1)) (tensorci2.jl:391):
-    // LU with partial pivoting of Pᵀ, then substitution per row of Pi1.
+    // LU with partial pivoting of A = Pᵀ (c×c, first maximum |.| per column),
+    // done by one thread (c is a bond dimension, small), then forward and
+    // back substitution for each of the r rows of Pi1 in parallel.  A, Pi1
+    // and X may live in LDS; X (r×c) is the result.
     ENG int solve_right(const double *Pi1, int r, double *A, int c, double *X)
     {
-        int singular = 0;
-        for (int k = 0; k < c; ++k) {
-            if (tm.tid == 0) {
+        if (tm.tid == 0) {
+            int singular = 0;
+            for (int k = 0; k < c; ++k) {
                 int pr = k;
                 double bestv = fabs(A[k + (int64_t)c * k]);
                 for (int i = k + 1; i < c; ++i) {
@@ -496,29 +555,24 @@ struct Engine {
                     if (v > bestv) { bestv = v; pr = i; }
                 }
                 D.ipiv[k] = pr;
-            }
-            tm.sync();
-            const int pr = D.ipiv[k];
-            if (pr != k)
-                for (int j = tm.tid; j < c; j += tm.nt) {
-                    double x = A[k + (int64_t)c * j];
-                    A[k + (int64_t)c * j] = A[pr + (int64_t)c * j];
-                    A[pr + (int64_t)c * j] = x;
+                if (pr != k)
+                    for (int j = 0; j < c; ++j) {
+                        double x = A[k + (int64_t)c * j];
+                        A[k + (int64_t)c * j] = A[pr + (int64_t)c * j];
+                        A[pr + (int64_t)c * j] = x;
+                    }
+                const double d = A[k + (int64_t)c * k];
+                if (d == 0.0) singular = 1;
+                for (int i = k + 1; i < c; ++i) A[i + (int64_t)c * k] = A[i + (int64_t)c * k] / d;
+                for (int j = k + 1; j < c; ++j) {
+                    const double u = A[k + (int64_t)c * j];
+                    for (int i = k + 1; i < c; ++i) A[i + (int64_t)c * j] = A[i + (int64_t)c * j] - A[i + (int64_t)c * k] * u;
                 }
-            tm.sync();
-            const double d = A[k + (int64_t)c * k];
-            if (d == 0.0) singular = 1;
-            for (int i = k + 1 + tm.tid; i < c; i += tm.nt) A[i + (int64_t)c * k] = A[i + (int64_t)c * k] / d;
-            tm.sync();
-            const int w = c - k - 1;
-            for (int64_t q = tm.tid; q < (int64_t)w * w; q += tm.nt) {
-                int64_t jj = q / w, ii = q - jj * w;
-                int64_t i = k + 1 + ii, j = k + 1 + jj;
-                A[i + c * j] = A[i + c * j] - A[i + c * k] * A[k + c * j];
             }
-            tm.sync();
+            S.cnt = singular;
         }
-        if (singular) return ST_SINGULAR;
+        tm.sync();
+        if (S.cnt) return ST_SINGULAR;
         for (int q = tm.tid; q < r; q += tm.nt) {
             for (int i = 0; i < c; ++i) X[q + (int64_t)r * i] = Pi1[q + (int64_t)r * i];
             for (int k = 0; k < c; ++k) {
@@ -544,6 +598,15 @@ struct Engine {
         return ST_OK;
     }
 
+    // LDS scratch carved from the team's Pi area; null when it does not fit
+    ENG double *lds_take(int64_t n, int64_t &off) const
+    {
+        if (!tm.lpi || off + n > tm.lpi_cap) return nullptr;
+        double *p = tm.lpi + off;
+        off += (n + 1) & ~(int64_t)1;
+        return p;
+    }
+
     ENG double *Tslot(int b) const { return D.T + D.tstride * b; }
     ENG void setTdims(int b, int64_t a, int64_t d, int64_t c)
     {
@@ -560,31 +623,39 @@ struct Engine {
             const int nI = setn(SET_I, b), d = D.localdims[b], nJ = setn(SET_J, b);
             const int m = kron_rows(b);
             if (m > D.mrcap) return fail(ST_CAPACITY);
-            load_set(D.cols, SET_J, b, jlen(b));
+            load_set(cols, SET_J, b, jlen(b));
             tm.sync();
             if (!need_tensors) {
-                eval(D.rows, b + 1, m, D.cols, nJ, nullptr, 0, false, true);
+                eval(rows, b + 1, m, cols, nJ, nullptr, 0, false, true);
                 continue;
             }
             if ((int64_t)m * nJ > D.tstride) return fail(ST_CAPACITY);
             if (b == L - 1) {
-                eval(D.rows, b + 1, m, D.cols, nJ, Tslot(b), m, false, true);
+                eval(rows, b + 1, m, cols, nJ, Tslot(b), m, false, true);
                 setTdims(b, nI, d, nJ);
                 continue;
             }
-            eval(D.rows, b + 1, m, D.cols, nJ, D.pi, m, false, true);
             const int np = setn(SET_I, b + 1);
+            int64_t off = 0;
+            double *Pi1 = lds_take((int64_t)m * nJ, off);
+            double *At = lds_take((int64_t)np * np, off);
+            double *X = lds_take((int64_t)m * np, off);
+            if (!Pi1 || !At || !X) { Pi1 = D.pi; At = D.pmat; X = Tslot(b); }
+            eval(rows, b + 1, m, cols, nJ, Pi1, m, false, true);
             if (np != nJ) {
                 if (tm.tid == 0) D.ints[INT_ERRSITE] = b + 1;
                 return fail(ST_NONSQUARE);
             }
             // Pᵀ[jj, ii] = f(Iset[b+1][ii] ⊕ Jset[b][jj])
-            eval(setp(SET_I, b + 1), b + 1, np, D.cols, nJ, D.pmat, np, true, false);
-            int st = solve_right(D.pi, m, D.pmat, np, Tslot(b));
+            eval(setp(SET_I, b + 1), b + 1, np, cols, nJ, At, np, true, false);
+            int st = solve_right(Pi1, m, At, np, X);
             if (st != ST_OK) {
                 if (tm.tid == 0) D.ints[INT_ERRSITE] = b + 1;
                 return fail(st);
             }
+            double *T = Tslot(b);
+            if (X != T)
+                for (int64_t q = tm.tid; q < (int64_t)m * np; q += tm.nt) T[q] = X[q];
             setTdims(b, nI, d, np);
             tm.sync();
         }
@@ -630,30 +701,14 @@ struct Engine {
                 }
             }
         }
-        return fillsitetensors(D.nsearchglobalpivot > 0);
+        const uint64_t t0 = tm.clock();
+        int st = fillsitetensors(D.nsearchglobalpivot > 0);
+        if (tm.tid == 0) S.tph[PH_FILL] += tm.clock() - t0;
+        return st;
     }
 
     // ------------------------------------------------------ global search
     // DefaultGlobalPivotFinder (globalpivotfinder.jl:143-195), project PRNG.
-    ENG double tt_eval(const uint8_t *x, double *v, double *w) const
-    {
-        int64_t chl = D.Tdims[0];
-        for (int64_t i = 0; i < chl; ++i) v[i] = 1.0;
-        for (int t = 0; t < L; ++t) {
-            const int64_t d0 = D.Tdims[3 * t], d = D.Tdims[3 * t + 1], d2 = D.Tdims[3 * t + 2];
-            const int64_t s = x[t] - 1;
-            const double *T = D.T + D.tstride * t;
-            for (int64_t j = 0; j < d2; ++j) {
-                double acc = 0.0;
-                for (int64_t i = 0; i < d0; ++i) acc = acc + v[i] * T[i + d0 * (s + d * j)];
-                w[j] = acc;
-            }
-            double *tmp = v; v = w; w = tmp;
-            chl = d2;
-        }
-        return chl > 0 ? v[0] : 0.0;
-    }
-
     ENG int globalsearch(double abstol)
     {
         const int ns = D.nsearchglobalpivot;
@@ -661,9 +716,8 @@ struct Engine {
         uint8_t *found = D.gpts + (int64_t)ns * L;      // ns × L
         uint8_t *cand = found + (int64_t)ns * L;        // ns·Σd × L
         const int sumd = D.sumd;
-        const int64_t ncand = (int64_t)ns * sumd;
+        const int ncand = ns * sumd;
         double *errs = D.work;                          // ncand
-        double *vec = D.work + ncand;                   // ncand × 2·setcap
         if (tm.tid == 0) {
             for (int q = 0; q < ns; ++q)
                 for (int p = 0; p < L; ++p) {
@@ -672,21 +726,64 @@ struct Engine {
                 }
         }
         tm.sync();
-        for (int64_t c = tm.tid; c < ncand; c += tm.nt) {
-            const int64_t q = c / sumd;
-            int r = (int)(c - q * sumd), p = 0;
+        // candidate points: the point with site p set to v, p-major then v
+        for (int c = tm.tid; c < ncand; c += tm.nt) {
+            const int q = c / sumd;
+            int r = c - q * sumd, p = 0;
             while (r >= D.localdims[p]) { r -= D.localdims[p]; ++p; }
-            uint8_t *x = cand + c * L;
-            for (int t = 0; t < L; ++t) x[t] = pts[q * L + t];
+            uint8_t *x = cand + (int64_t)c * L;
+            for (int t = 0; t < L; ++t) x[t] = pts[(int64_t)q * L + t];
             x[p] = (uint8_t)(r + 1);
-            tci_point pt;
-            pt.a = x; pt.b = x; pt.na = L; pt.L = L;
-            const double fv = tci_functor_eval(D.fkind, D.params, D.iparams, &pt);
-            double *v = vec + c * 2 * D.setcap;
-            const double tv = tt_eval(x, v, v + D.setcap);
-            errs[c] = fabs(fv - tv);
+        }
+        int chi = 1;
+        for (int t = 0; t <= L; ++t) {
+            const int64_t w = t < L ? D.Tdims[3 * t] : 1;
+            chi = w > chi ? (int)w : chi;
+        }
+        // TT values of all candidates, site by site (abstracttensortrain.jl:131,
+        // each product summed in ascending bond order); chunks of candidates
+        // keep the running vectors in LDS
+        int64_t off = 0;
+        int chunk = (int)((tm.lpi_cap > 0 ? tm.lpi_cap : 0) / (2 * (int64_t)chi));
+        double *V = nullptr, *W = nullptr;
+        if (chunk >= 1) {
+            if (chunk > ncand) chunk = ncand;
+            V = lds_take((int64_t)chunk * chi, off);
+            W = lds_take((int64_t)chunk * chi, off);
+        }
+        if (!V || !W) {
+            chunk = ncand;
+            V = D.work + ncand;
+            W = V + (int64_t)ncand * chi;
         }
         tm.sync();
+        for (int c0 = 0; c0 < ncand; c0 += chunk) {
+            const int cn = ncand - c0 < chunk ? ncand - c0 : chunk;
+            for (int c = tm.tid; c < cn; c += tm.nt) V[(int64_t)c * chi] = 1.0;
+            tm.sync();
+            for (int t = 0; t < L; ++t) {
+                const int64_t d0 = D.Tdims[3 * t], d = D.Tdims[3 * t + 1], d2 = D.Tdims[3 * t + 2];
+                const double *T = D.T + D.tstride * t;
+                for (int64_t o = tm.tid; o < (int64_t)cn * d2; o += tm.nt) {
+                    const int64_t c = o / d2, j = o - c * d2;
+                    const int64_t s = cand[(c0 + c) * L + t] - 1;
+                    const double *Tc = T + d0 * (s + d * j);
+                    const double *v = V + c * chi;
+                    double acc = 0.0;
+                    for (int64_t i = 0; i < d0; ++i) acc = acc + v[i] * Tc[i];
+                    W[c * chi + j] = acc;
+                }
+                tm.sync();
+                double *tmp = V; V = W; W = tmp;
+            }
+            for (int c = tm.tid; c < cn; c += tm.nt) {
+                tci_point pt;
+                pt.a = cand + (int64_t)(c0 + c) * L; pt.b = pt.a; pt.na = L; pt.L = L;
+                const double fv = tci_functor_eval(FK ? FK : D.fkind, D.params, D.iparams, &pt);
+                errs[c0 + c] = fabs(fv - V[(int64_t)c * chi]);
+            }
+            tm.sync();
+        }
         if (tm.tid == 0) {
             const double thr = abstol * D.tolmargin;
             int nf = 0;
@@ -737,33 +834,39 @@ struct Engine {
             const int nI = setn(SET_I, b), d = D.localdims[b], nJ = setn(SET_J, b);
             const int m = kron_rows(b);
             if (m > D.mrcap || nJ > D.mrcap) return fail(ST_CAPACITY);
-            load_set(D.cols, SET_J, b, jlen(b));
+            load_set(cols, SET_J, b, jlen(b));
             tm.sync();
-            eval(D.rows, b + 1, m, D.cols, nJ, D.pi, m, false, true);
+            double *A = pibuf(m, nJ);
+            eval(rows, b + 1, m, cols, nJ, A, m, false, true);
             double err;
-            int p = rrlu(m, nJ, D.maxbonddim, 1e-14, abstol, true, &err);
+            int p = rrlu(A, m, nJ, D.maxbonddim, 1e-14, abstol, true, &err, true, Tslot(b));
             if (p < 0) {
                 if (tm.tid == 0) D.ints[INT_ERRSITE] = b + 1;
                 return fail(-p);
             }
             if (p > D.setcap || (int64_t)m * p > D.tstride) return fail(ST_CAPACITY);
-            select_into(SET_I, b + 1, D.rows, D.rowperm, p, b + 1);
-            select_into(SET_J, b, D.cols, D.colperm, p, jlen(b));
+            select_into(SET_I, b + 1, rows, D.rowperm, p, b + 1);
+            select_into(SET_J, b, cols, D.colperm, p, jlen(b));
             // left(luci) = colstimespivotinv (matrixluci.jl:48-57): rows at
             // positions < p are unit vectors, the rest solve x·L11 = L[i,1:p]
+            // (L11 has a unit diagonal: s / 1.0 == s); solved rows go to LDS
+            // when they fit, then are scattered to T in original row order
             double *T = Tslot(b);
-            const double *A = D.pi;
-            for (int64_t q = tm.tid; q < (int64_t)p * p; q += tm.nt) {
-                int64_t j = q / p, i = q - j * p;
-                T[D.rowperm[i] + (int64_t)m * j] = (i == j) ? 1.0 : 0.0;
-            }
+            int64_t off = A == tm.lpi ? (int64_t)m * nJ : 0;
+            double *X = lds_take((int64_t)m * p, off);
             for (int i = p + tm.tid; i < m; i += tm.nt) {
-                double *row = T + D.rowperm[i];
+                double *row = X ? X + i : T + D.rowperm[i];
                 for (int j = p - 1; j >= 0; --j) {
                     double s = A[i + (int64_t)m * j];
                     for (int t = j + 1; t < p; ++t) s = s - row[(int64_t)m * t] * A[t + (int64_t)m * j];
-                    row[(int64_t)m * j] = s;    // L11 has a unit diagonal: s / 1.0 == s
+                    row[(int64_t)m * j] = s;
                 }
+            }
+            if (X) tm.sync();
+            for (int64_t q = tm.tid; q < (int64_t)m * p; q += tm.nt) {
+                const int64_t j = q / m, i = q - j * m;
+                if (i < p) T[D.rowperm[i] + (int64_t)m * j] = (i == j) ? 1.0 : 0.0;
+                else if (X) T[D.rowperm[i] + (int64_t)m * j] = X[q];
             }
             setTdims(b, nI, d, p);
             tm.sync();
@@ -784,10 +887,10 @@ struct Engine {
         const int nI = setn(SET_I, b), d = D.localdims[b], nJ = setn(SET_J, b);
         const int m = kron_rows(b);
         if (m > D.mrcap) return fail(ST_CAPACITY);
-        load_set(D.cols, SET_J, b, jlen(b));
+        load_set(cols, SET_J, b, jlen(b));
         tm.sync();
         if ((int64_t)m * nJ > D.tstride) return fail(ST_CAPACITY);
-        eval(D.rows, b + 1, m, D.cols, nJ, Tslot(b), m, false, false);
+        eval(rows, b + 1, m, cols, nJ, Tslot(b), m, false, false);
         setTdims(b, nI, d, nJ);
         tm.sync();
         return ST_OK;
@@ -806,6 +909,8 @@ struct Engine {
     ENG int run()
     {
         if (tm.tid == 0) {
+            for (int i = 0; i < 8; ++i) S.tph[i] = 0;
+            S.tph[PH_TOTAL] = tm.clock();
             S.status = ST_OK;
             for (int i = 0; i < 6; ++i) S.ws[i] = 0.0;
             for (int q = 0; q < NSETKIND * L; ++q) D.setn[q] = 0;
@@ -825,7 +930,7 @@ struct Engine {
             for (int q = 0; q < D.npivinit; ++q) {
                 tci_point pt;
                 pt.a = D.initpivots + (int64_t)q * L; pt.b = pt.a; pt.na = L; pt.L = L;
-                double v = fabs(tci_functor_eval(D.fkind, D.params, D.iparams, &pt));
+                double v = fabs(tci_functor_eval(FK ? FK : D.fkind, D.params, D.iparams, &pt));
                 mx = q == 0 ? v : jl_max(mx, v);
             }
             S.maxsample = mx;
@@ -849,7 +954,9 @@ struct Engine {
             for (int b = 1; b < L - 1; ++b) perr = jl_max(perr, D.bonderrors[b]);
             int ng = 0;
             if (D.nsearchglobalpivot > 0) {
+                const uint64_t t0 = tm.clock();
                 ng = globalsearch(abstol);
+                if (tm.tid == 0) S.tph[PH_GSEARCH] += tm.clock() - t0;
                 if (ng < 0) return finish(S.status);
             }
             if (tm.tid == 0) {
@@ -870,7 +977,9 @@ struct Engine {
             if (conv) break;
         }
         const double norm = D.normalizeerror ? S.maxsample : 1.0;
+        const uint64_t t1 = tm.clock();
         st = sweep1site_forward(tol * norm);
+        if (tm.tid == 0) S.tph[PH_SWEEP1] += tm.clock() - t1;
         if (st != ST_OK) return finish(st);
         // _sanitycheck (globalsearch.jl:106-111)
         for (int b = 0; b < L - 1; ++b)
@@ -888,6 +997,8 @@ struct Engine {
         tm.sync();
         if (tm.tid == 0) {
             D.ints[INT_STATUS] = st;
+            S.tph[PH_TOTAL] = tm.clock() - S.tph[PH_TOTAL];
+            for (int i = 0; i < 8; ++i) D.scal[SCAL_TPH + i] = (double)S.tph[i];
             D.scal[SCAL_MAXSAMPLE] = S.maxsample;
             for (int i = 0; i < 6; ++i) D.scal[SCAL_WS_PI + i] = S.ws[i];
             D.ints[INT_DONE] = 1;

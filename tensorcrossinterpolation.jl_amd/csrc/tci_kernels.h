@@ -8,7 +8,15 @@
 
 namespace tci {
 
-hipError_t launch_solve(const SolveDesc *descs_device, int count, int max_mrcap, hipStream_t st);
+// on-chip (LDS) placement of one solve workgroup's hot buffers
+struct LdsPlan {
+    int32_t stage_cap;   // rrLU staging vectors: 2 × stage_cap doubles (0: in HBM)
+    int32_t tab_cap;     // bytes per index table (rows, cols) (0: in HBM)
+    int64_t pi_cap;      // doubles available for a Pi matrix (0: in HBM)
+};
+size_t lds_plan_bytes(const LdsPlan &p);
+hipError_t launch_solve(const SolveDesc *descs_device, int count, int team, int fkind,
+                        const LdsPlan &plan, hipStream_t st);
 hipError_t launch_rrlu(tci_lu_job *jobs_device, int64_t count, int64_t max_side, double *gstage,
                        int64_t gstage_stride, hipStream_t st);
 hipError_t launch_lu_unpack(const double *A, int64_t m, int64_t n, int64_t p, int leftorth,

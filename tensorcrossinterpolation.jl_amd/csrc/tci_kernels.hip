@@ -19,12 +19,26 @@
 
 namespace tci {
 
+// register-tile shape per team size (tci_regtile.h): TR×(NT/TR) threads,
+// RA×CB doubles each; capacity (TR·RA)×((NT/TR)·CB)
+template <int NT> struct TileShape;
+template <> struct TileShape<256> { static constexpr int TR = 16, RA = 9, CB = 9; };    // 144×144
+template <> struct TileShape<512> { static constexpr int TR = 32, RA = 5, CB = 9; };    // 160×144
+template <> struct TileShape<1024> { static constexpr int TR = 32, RA = 5, CB = 5; };   // 160×160
+
 template <int NT>
 struct DevTeam {
     static constexpr int NW = NT / 64;
+    static constexpr int TR = TileShape<NT>::TR, RA = TileShape<NT>::RA, CB = TileShape<NT>::CB;
+    static constexpr int PERM_CAP = TR * RA + (NT / TR) * CB;
+    int32_t *lperm;       // LDS, PERM_CAP ints
     int tid;
     int nt;
+    int lane, wsize, wave, nwaves;
     double *xs, *ys;
+    double *lpi;          // LDS area for Pi matrices
+    int64_t lpi_cap;      // its size in doubles (0: none)
+    uint8_t *lrows, *lcols;
     ArgBest *red_a;   // 2 × NW
     double *red_d;    // 2 × NW
     int par_a, par_d;
@@ -66,37 +80,86 @@ struct DevTeam {
         return r;
     }
 
+    __device__ int uniform(int x) const { return __builtin_amdgcn_readfirstlane(x); }
+
+    // team minimum of b.lin carrying b.a and b.orig (b.v is already team-uniform)
+    __device__ ArgBest argmin_key(ArgBest b)
+    {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const int64_t ol = (int64_t)__shfl_xor((long long)b.lin, off);
+            const double oa = __shfl_xor(b.a, off);
+            const int oo = __shfl_xor(b.orig, off);
+            if (ol < b.lin) { b.lin = ol; b.a = oa; b.orig = oo; }
+        }
+        ArgBest *slot = red_a + par_a * NW;
+        if ((tid & 63) == 0) slot[tid >> 6] = b;
+        __syncthreads();
+        ArgBest r = slot[0];
+#pragma unroll
+        for (int w = 1; w < NW; ++w)
+            if (slot[w].lin < r.lin) r = slot[w];
+        par_a ^= 1;
+        return r;
+    }
+
     __device__ int any(int v) { return __syncthreads_or(v); }
+    __device__ uint64_t clock() const { return wall_clock64(); }
+    __device__ void init(int t, double *x, double *y, ArgBest *ra, double *rd)
+    {
+        tid = t;
+        nt = NT;
+        lane = t & 63;
+        wsize = 64;
+        wave = t >> 6;
+        nwaves = NW;
+        xs = x;
+        ys = y;
+        red_a = ra;
+        red_d = rd;
+        par_a = 0;
+        par_d = 0;
+        lpi = nullptr;
+        lpi_cap = 0;
+        lrows = nullptr;
+        lcols = nullptr;
+        lperm = nullptr;
+    }
 };
 
 constexpr int kTeam = 256;
 
 // ---------------------------------------------------------------- solves
-__global__ __launch_bounds__(kTeam) void tci_solve_kernel(const SolveDesc *__restrict__ descs,
-                                                          int count, int lds_stage_cap)
+// Dynamic LDS of a solve workgroup (sizes fixed per launch by LdsPlan):
+//   [stage: 2·stage_cap doubles][rows: tab_cap bytes][cols: tab_cap bytes]
+//   [Pi area: pi_cap doubles]
+// Anything that does not fit stays in the solve's HBM arena.
+template <int NT, int FK>
+__global__ __launch_bounds__(NT) void tci_solve_kernel(const SolveDesc *__restrict__ descs,
+                                                       int count, LdsPlan plan)
 {
     const int s = blockIdx.x;
     if (s >= count) return;
     __shared__ EngShared S;
-    __shared__ ArgBest red_a[2 * (kTeam / 64)];
-    __shared__ double red_d[2 * (kTeam / 64)];
-    extern __shared__ double stage[];
+    __shared__ ArgBest red_a[2 * (NT / 64)];
+    __shared__ double red_d[2 * (NT / 64)];
+    __shared__ int32_t lperm[DevTeam<NT>::PERM_CAP];
+    extern __shared__ double dyn[];
     const SolveDesc &D = descs[s];
-    DevTeam<kTeam> tm;
-    tm.tid = threadIdx.x;
-    tm.nt = kTeam;
-    tm.par_a = 0;
-    tm.par_d = 0;
-    tm.red_a = red_a;
-    tm.red_d = red_d;
-    if (D.mrcap <= lds_stage_cap) {
-        tm.xs = stage;
-        tm.ys = stage + D.mrcap;
-    } else {
-        tm.xs = D.stage;
-        tm.ys = D.stage + D.mrcap;
+    DevTeam<NT> tm;
+    double *stage = dyn;
+    uint8_t *tabs = (uint8_t *)(dyn + 2 * plan.stage_cap);
+    double *pi = (double *)(tabs + 2 * plan.tab_cap);
+    if (D.mrcap <= plan.stage_cap) tm.init(threadIdx.x, stage, stage + D.mrcap, red_a, red_d);
+    else tm.init(threadIdx.x, D.stage, D.stage + D.mrcap, red_a, red_d);
+    if ((int64_t)D.mrcap * D.L <= plan.tab_cap) {
+        tm.lrows = tabs;
+        tm.lcols = tabs + plan.tab_cap;
     }
-    Engine<DevTeam<kTeam>> eng(tm, D, S);
+    tm.lpi = pi;
+    tm.lpi_cap = plan.pi_cap;
+    tm.lperm = lperm;
+    Engine<DevTeam<NT>, FK> eng(tm, D, S);
     eng.run();
 }
 
@@ -115,22 +178,26 @@ __global__ __launch_bounds__(kTeam) void tci_rrlu_kernel(tci_lu_job *__restrict_
     const int m = (int)J.m, n = (int)J.n;
     const int mx = m > n ? m : n;
     DevTeam<kTeam> tm;
-    tm.tid = threadIdx.x;
-    tm.nt = kTeam;
-    tm.par_a = 0;
-    tm.par_d = 0;
-    tm.red_a = red_a;
-    tm.red_d = red_d;
-    if (mx <= lds_stage_cap) {
-        tm.xs = stage;
-        tm.ys = stage + mx;
-    } else {
-        tm.xs = gstage + gstage_stride * s;
-        tm.ys = tm.xs + mx;
-    }
+    if (mx <= lds_stage_cap) tm.init(threadIdx.x, stage, stage + mx, red_a, red_d);
+    else tm.init(threadIdx.x, gstage + gstage_stride * s, gstage + gstage_stride * s + mx, red_a, red_d);
+    __shared__ int32_t lperm[DevTeam<kTeam>::PERM_CAP];
     double err, st[2];
-    int p = rrlu_team(tm, J.A, m, n, J.maxrank, J.reltol, J.abstol, J.leftorth != 0, J.rowperm,
+    int p;
+    using Tile = RegLU<DevTeam<kTeam>, DevTeam<kTeam>::TR, DevTeam<kTeam>::RA, DevTeam<kTeam>::CB>;
+    if (Tile::fits(tm, m, n) && mx <= lds_stage_cap) {
+        Tile R;
+        R.setup(tm, m, n);
+        R.load(J.A, m);
+        p = R.factor(tm, J.maxrank, J.reltol, J.abstol, J.leftorth != 0, lperm, lperm + m, tm.xs,
+                     tm.ys, nullptr, &err, st);
+        R.store_packed(J.A, m);
+        for (int i = tm.tid; i < m; i += kTeam) J.rowperm[i] = lperm[i];
+        for (int j = tm.tid; j < n; j += kTeam) J.colperm[j] = lperm[m + j];
+        __syncthreads();
+    } else {
+        p = rrlu_team(tm, J.A, m, n, J.maxrank, J.reltol, J.abstol, J.leftorth != 0, J.rowperm,
                       J.colperm, nullptr, &err, st);
+    }
     // 1-based permutations (rrLU.rowpermutation / colpermutation)
     for (int i = tm.tid; i < m; i += kTeam) J.rowperm[i] += 1;
     for (int j = tm.tid; j < n; j += kTeam) J.colperm[j] += 1;
@@ -216,13 +283,50 @@ __global__ void tci_eval_kernel(int kind, const double *__restrict__ params,
 }
 
 // ------------------------------------------------------------- launchers
-hipError_t launch_solve(const SolveDesc *descs, int count, int max_mrcap, hipStream_t st)
+size_t lds_plan_bytes(const LdsPlan &p)
 {
-    // staging vectors in LDS when 2·mrcap doubles fit in 48 KiB
-    const int cap = 48 * 1024 / 16;
-    const size_t lds = max_mrcap <= cap ? (size_t)2 * max_mrcap * sizeof(double) : 0;
-    hipLaunchKernelGGL(tci_solve_kernel, dim3(count), dim3(kTeam), lds, st, descs, count,
-                       max_mrcap <= cap ? max_mrcap : 0);
+    return (size_t)2 * p.stage_cap * sizeof(double) + 2 * (size_t)p.tab_cap + (size_t)p.pi_cap * sizeof(double);
+}
+
+template <int NT, int FK>
+static void launch_one(const SolveDesc *descs, int count, const LdsPlan &plan, size_t lds, hipStream_t st)
+{
+    (void)hipFuncSetAttribute((const void *)tci_solve_kernel<NT, FK>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((tci_solve_kernel<NT, FK>), dim3(count), dim3(NT), lds, st, descs, count, plan);
+}
+
+template <int NT>
+static void launch_nt(const SolveDesc *descs, int count, int fkind, const LdsPlan &plan, size_t lds,
+                      hipStream_t st)
+{
+    switch (fkind) {
+    case TCI_F_LORENTZ_INT: launch_one<NT, TCI_F_LORENTZ_INT>(descs, count, plan, lds, st); break;
+    case TCI_F_LORENTZ_WEIGHTED: launch_one<NT, TCI_F_LORENTZ_WEIGHTED>(descs, count, plan, lds, st); break;
+    case TCI_F_QUANTICS_GAUSS_COS: launch_one<NT, TCI_F_QUANTICS_GAUSS_COS>(descs, count, plan, lds, st); break;
+    case TCI_F_GAUSS_SUM: launch_one<NT, TCI_F_GAUSS_SUM>(descs, count, plan, lds, st); break;
+    default: launch_one<NT, 0>(descs, count, plan, lds, st); break;
+    }
+}
+
+// fkind: the functor kind shared by every solve of the batch, or 0 (mixed)
+hipError_t launch_solve(const SolveDesc *descs, int count, int team, int fkind, const LdsPlan &plan,
+                        hipStream_t st)
+{
+    const size_t lds = lds_plan_bytes(plan);
+    // per-functor specializations for the default 512-thread team; the
+    // other team sizes run the generic (run-time dispatched) engine
+    switch (team) {
+    case 1024:
+        if (fkind == TCI_F_LORENTZ_INT) launch_one<1024, TCI_F_LORENTZ_INT>(descs, count, plan, lds, st);
+        else launch_one<1024, 0>(descs, count, plan, lds, st);
+        break;
+    case 256:
+        if (fkind == TCI_F_LORENTZ_INT) launch_one<256, TCI_F_LORENTZ_INT>(descs, count, plan, lds, st);
+        else launch_one<256, 0>(descs, count, plan, lds, st);
+        break;
+    default: launch_nt<512>(descs, count, fkind, plan, lds, st); break;
+    }
     return hipGetLastError();
 }
 

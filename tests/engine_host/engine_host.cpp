@@ -16,12 +16,21 @@
 using namespace tci;
 
 struct HostTeam {
-    int tid = 0, nt = 1;
+    // register-tile shape of the one-thread team: a 1×1 grid holding up to 160×160
+    static constexpr int TR = 1, RA = 160, CB = 160;
+    int32_t lperm[320];
+    int tid = 0, nt = 1, lane = 0, wsize = 1, wave = 0, nwaves = 1;
     double *xs = nullptr, *ys = nullptr;
+    double *lpi = nullptr;
+    int64_t lpi_cap = 0;
+    uint8_t *lrows = nullptr, *lcols = nullptr;
     void sync() {}
     ArgBest argmax(ArgBest b) { return b; }
+    ArgBest argmin_key(ArgBest b) { return b; }
+    int uniform(int x) const { return x; }
     double max_nan(double v) { return v; }
     int any(int v) { return v; }
+    uint64_t clock() const { return 0; }
 };
 
 struct EhSolve {
@@ -49,13 +58,15 @@ extern "C" void *eh_solve(const tci_problem *p, int *status)
     layout_solve(c, &h->D, h->inputs.data(), h->arena.data(), h->report.data(), &ib, &sb, &rb);
     fill_desc_scalars(*p, h->D);
     pack_inputs(*p, c, h->D, h->inputs.data(), h->inputs.data());
-    HostTeam tm;
+    static HostTeam tm;   // large register tile: keep it off the stack
+    tm = HostTeam();
     tm.xs = h->D.stage;
     tm.ys = h->D.stage + c.mrcap;
     EngShared S;
     memset(&S, 0, sizeof S);
-    Engine<HostTeam> e(tm, h->D, S);
-    h->status = e.run();
+    Engine<HostTeam> *e = new Engine<HostTeam>(tm, h->D, S);
+    h->status = e->run();
+    delete e;
     *status = h->status;
     return h;
 }
