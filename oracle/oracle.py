@@ -104,6 +104,10 @@ def lib():
         L.orc_tci_workstats.argtypes = [C.c_void_p, P(C.c_double)]
         L.orc_convergencecriterion.argtypes = [C.c_int32, P(C.c_int64), P(C.c_double), P(C.c_int64),
                                                C.c_double, C.c_int64, C.c_int32, C.c_int32]
+        L.orc_exp.restype = C.c_double
+        L.orc_exp.argtypes = [C.c_double]
+        L.orc_cos.restype = C.c_double
+        L.orc_cos.argtypes = [C.c_double]
         L.orc_rng_seed.argtypes = [P(C.c_uint64), C.c_uint64]
         L.orc_rng_next.restype = C.c_uint64
         L.orc_rng_next.argtypes = [P(C.c_uint64)]

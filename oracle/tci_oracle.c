@@ -803,6 +803,10 @@ double orc_tci_evaluate(const orc_tci *t, const int32_t *x)
     return r;
 }
 
+/* the functor math routines, exported for their accuracy tests */
+double orc_exp(double x) { return tci_exp(x); }
+double orc_cos(double x) { return tci_cos(x); }
+
 /* ---- project PRNG ---- */
 static uint64_t splitmix64(uint64_t *x)
 {

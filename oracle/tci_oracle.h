@@ -139,6 +139,10 @@ int orc_convergencecriterion(int32_t n, const int64_t *ranks, const double *erro
                              const int64_t *nglobal, double tol, int64_t maxbonddim,
                              int32_t ncheckhistory, int32_t checkconvglobalpivot);
 
+/* the project exp/cos of include/tci_functors.h (for accuracy tests) */
+double orc_exp(double x);
+double orc_cos(double x);
+
 /* project PRNG (xoshiro256** seeded by SplitMix64) for fixture generation */
 void orc_rng_seed(uint64_t s[4], uint64_t seed);
 uint64_t orc_rng_next(uint64_t s[4]);

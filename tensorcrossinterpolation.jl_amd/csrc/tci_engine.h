@@ -13,7 +13,8 @@
 //   tm.xs, tm.ys             staging vectors (mrcap doubles each)
 //   tm.lpi, tm.lpi_cap       on-chip (LDS) buffer for Pi matrices, in doubles
 //   tm.lrows, tm.lcols       on-chip index tables (null: use the HBM ones)
-//   tm.lperm                 LDS ints for the register-tile rrLU permutations
+//   tm.txs(i), tm.tys(j), tm.tperm(q)  LDS slots of the register-tile rrLU
+//   Team::NT                 team size as a compile-time constant
 //   tm.uniform(x)            x as a wave-uniform value (all lanes hold the same x)
 //   Team::TR, RA, CB         register-tile shape (tci_regtile.h); RA = 0: none
 // The device implementation lives in tci_kernels.hip.  Every parallel loop
@@ -401,15 +402,14 @@ struct Engine {
             R.load(A, m);
             // L recorded by original row in lscratch, then permuted into A
             double *Lorig = need_left ? lscratch : nullptr;
-            p = R.factor_lean(tm, maxrank, reltol, abstol, leftorth, tm.lperm, tm.lperm + m, tm.xs,
-                              tm.ys, D.pivval, perr, st, Lorig);
-            for (int i = tm.tid; i < m; i += tm.nt) D.rowperm[i] = tm.lperm[i];
-            for (int j = tm.tid; j < n; j += tm.nt) D.colperm[j] = tm.lperm[m + j];
+            p = R.factor_lean(tm, maxrank, reltol, abstol, leftorth, D.pivval, perr, st, Lorig);
+            for (int i = tm.tid; i < m; i += tm.nt) D.rowperm[i] = tm.tperm(i);
+            for (int j = tm.tid; j < n; j += tm.nt) D.colperm[j] = tm.tperm(m + j);
             if (need_left && p > 0) {
                 tm.sync();
                 for (int t = tm.wave; t < p; t += tm.nwaves)
                     for (int pos = t + 1 + tm.lane; pos < m; pos += tm.wsize)
-                        A[pos + (int64_t)m * t] = Lorig[tm.lperm[pos] + (int64_t)m * t];
+                        A[pos + (int64_t)m * t] = Lorig[tm.tperm(pos) + (int64_t)m * t];
             }
             tm.sync();
         } else {
@@ -489,10 +489,9 @@ struct Engine {
         }
         const uint64_t t1 = tm.clock();
         double st[4];
-        int p = R.factor_lean(tm, maxrank, reltol, abstol, leftorth, tm.lperm, tm.lperm + m,
-                              tm.xs, tm.ys, D.pivval, perr, st, nullptr);
-        for (int i = tm.tid; i < m; i += tm.nt) D.rowperm[i] = tm.lperm[i];
-        for (int j = tm.tid; j < n; j += tm.nt) D.colperm[j] = tm.lperm[m + j];
+        int p = R.factor_lean(tm, maxrank, reltol, abstol, leftorth, D.pivval, perr, st, nullptr);
+        for (int i = tm.tid; i < m; i += tm.nt) D.rowperm[i] = tm.tperm(i);
+        for (int j = tm.tid; j < n; j += tm.nt) D.colperm[j] = tm.tperm(m + j);
         tm.sync();
         if (tm.tid == 0) {
             S.tph[PH_RRLU] += tm.clock() - t1;

@@ -26,12 +26,21 @@ template <> struct TileShape<256> { static constexpr int TR = 16, RA = 9, CB = 9
 template <> struct TileShape<512> { static constexpr int TR = 32, RA = 5, CB = 9; };    // 160×144
 template <> struct TileShape<1024> { static constexpr int TR = 32, RA = 5, CB = 5; };   // 160×160
 
-template <int NT>
+// LDS of the register-tile rrLU (largest tile: 160 rows, 160 columns)
+constexpr int kTileRows = 160, kTileCols = 160;
+__shared__ double g_tile_xs[kTileRows];
+__shared__ double g_tile_ys[kTileCols];
+__shared__ int32_t g_tile_perm[kTileRows + kTileCols];
+
+template <int NT_>
 struct DevTeam {
+    static constexpr int NT = NT_;
     static constexpr int NW = NT / 64;
     static constexpr int TR = TileShape<NT>::TR, RA = TileShape<NT>::RA, CB = TileShape<NT>::CB;
-    static constexpr int PERM_CAP = TR * RA + (NT / TR) * CB;
-    int32_t *lperm;       // LDS, PERM_CAP ints
+    static_assert(TR * RA <= kTileRows && (NT / TR) * CB <= kTileCols, "tile exceeds its LDS");
+    __device__ double &txs(int i) const { return g_tile_xs[i]; }
+    __device__ double &tys(int j) const { return g_tile_ys[j]; }
+    __device__ int32_t &tperm(int q) const { return g_tile_perm[q]; }
     int tid;
     int nt;
     int lane, wsize, wave, nwaves;
@@ -123,7 +132,6 @@ struct DevTeam {
         lpi_cap = 0;
         lrows = nullptr;
         lcols = nullptr;
-        lperm = nullptr;
     }
 };
 
@@ -143,7 +151,6 @@ __global__ __launch_bounds__(NT) void tci_solve_kernel(const SolveDesc *__restri
     __shared__ EngShared S;
     __shared__ ArgBest red_a[2 * (NT / 64)];
     __shared__ double red_d[2 * (NT / 64)];
-    __shared__ int32_t lperm[DevTeam<NT>::PERM_CAP];
     extern __shared__ double dyn[];
     const SolveDesc &D = descs[s];
     DevTeam<NT> tm;
@@ -158,7 +165,6 @@ __global__ __launch_bounds__(NT) void tci_solve_kernel(const SolveDesc *__restri
     }
     tm.lpi = pi;
     tm.lpi_cap = plan.pi_cap;
-    tm.lperm = lperm;
     Engine<DevTeam<NT>, FK> eng(tm, D, S);
     eng.run();
 }
@@ -180,24 +186,10 @@ __global__ __launch_bounds__(kTeam) void tci_rrlu_kernel(tci_lu_job *__restrict_
     DevTeam<kTeam> tm;
     if (mx <= lds_stage_cap) tm.init(threadIdx.x, stage, stage + mx, red_a, red_d);
     else tm.init(threadIdx.x, gstage + gstage_stride * s, gstage + gstage_stride * s + mx, red_a, red_d);
-    __shared__ int32_t lperm[DevTeam<kTeam>::PERM_CAP];
     double err, st[2];
-    int p;
-    using Tile = RegLU<DevTeam<kTeam>, DevTeam<kTeam>::TR, DevTeam<kTeam>::RA, DevTeam<kTeam>::CB>;
-    if (Tile::fits(tm, m, n) && mx <= lds_stage_cap) {
-        Tile R;
-        R.setup(tm, m, n);
-        R.load(J.A, m);
-        p = R.factor(tm, J.maxrank, J.reltol, J.abstol, J.leftorth != 0, lperm, lperm + m, tm.xs,
-                     tm.ys, nullptr, &err, st);
-        R.store_packed(J.A, m);
-        for (int i = tm.tid; i < m; i += kTeam) J.rowperm[i] = lperm[i];
-        for (int j = tm.tid; j < n; j += kTeam) J.colperm[j] = lperm[m + j];
-        __syncthreads();
-    } else {
-        p = rrlu_team(tm, J.A, m, n, J.maxrank, J.reltol, J.abstol, J.leftorth != 0, J.rowperm,
-                      J.colperm, nullptr, &err, st);
-    }
+    // the streaming factorization: the packed factors are the output
+    const int p = rrlu_team(tm, J.A, m, n, J.maxrank, J.reltol, J.abstol, J.leftorth != 0,
+                            J.rowperm, J.colperm, nullptr, &err, st);
     // 1-based permutations (rrLU.rowpermutation / colpermutation)
     for (int i = tm.tid; i < m; i += kTeam) J.rowperm[i] += 1;
     for (int j = tm.tid; j < n; j += kTeam) J.colperm[j] += 1;

@@ -22,15 +22,15 @@ namespace tci {
 
 template <class Team, int TR, int RA, int CB>
 struct RegLU {
+    static constexpr int TC = Team::NT / TR;   // thread columns of the grid
     double a[RA][CB];
     int rp[RA];     // current position of my rows (or INT32_MAX when i >= m)
     int cp[CB];     // current position of my columns
-    int tr, tc, TC;
+    int tr, tc;
     int m, n;
 
     ENG_INLINE void setup(Team &tm, int m_, int n_)
     {
-        TC = tm.nt / TR;
         tr = tm.tid % TR;
         tc = tm.tid / TR;
         m = m_;
@@ -47,9 +47,9 @@ struct RegLU {
         }
     }
 
-    ENG_INLINE static bool fits(const Team &tm, int m_, int n_)
+    ENG_INLINE static bool fits(const Team &, int m_, int n_)
     {
-        return m_ <= TR * RA && n_ <= (tm.nt / TR) * CB;
+        return m_ <= TR * RA && n_ <= TC * CB;
     }
 
     ENG_INLINE void load(const double *A, int64_t lda)
@@ -88,16 +88,6 @@ struct RegLU {
         return lmax;
     }
 
-    // packed factors in position order, as rrlu! leaves A (matrixlu.jl:194-204)
-    ENG_INLINE void store_packed(double *A, int64_t lda) const
-    {
-#pragma unroll
-        for (int i = 0; i < RA; ++i)
-#pragma unroll
-            for (int j = 0; j < CB; ++j)
-                if (rp[i] < m && cp[j] < n) A[rp[i] + lda * cp[j]] = a[i][j];
-    }
-
     // position-ordered columns 0..p-1 (the L part and above), m×p
     ENG_INLINE void store_left(double *Lpos, int p) const
     {
@@ -106,19 +96,6 @@ struct RegLU {
 #pragma unroll
             for (int j = 0; j < CB; ++j)
                 if (rp[i] < m && cp[j] < p) Lpos[rp[i] + (int64_t)m * cp[j]] = a[i][j];
-    }
-
-    ENG_INLINE void candidate(ArgBest &best, int i, int j) const
-    {
-        const double x = a[i][j];
-        const double v = x * x;
-        if (v == v) {
-            ArgBest c;
-            c.v = v;
-            c.lin = (int64_t)cp[j] * m + rp[i];
-            c.a = x;
-            if (arg_better(c, best)) best = c;
-        }
     }
 
     // The entry with the largest abs2 and, among equal ones, the smallest
@@ -141,7 +118,7 @@ struct RegLU {
 #pragma unroll
                 for (int j = 0; j < CB; ++j) {
                     const double x = a[i][j];
-                    const int64_t key = (int64_t)cp[j] * m + rp[i];
+                    const int key = cp[j] * m + rp[i];   // < (TC·CB)·(TR·RA) < 2^31
                     const bool hit = rp[i] > lo && rp[i] < m && cp[j] > lo && cp[j] < n &&
                                      x * x == g && key < b.lin;
                     b.lin = hit ? key : b.lin;
@@ -152,181 +129,23 @@ struct RegLU {
         return tm.argmin_key(b);
     }
 
-    // Full-pivot rrLU (matrixlu.jl:141-181).  rowat/colat: LDS arrays of m/n
-    // ints (original index at each position); xs/ys: LDS staging (m/n).
-    // Output: rowat/colat hold rowpermutation/colpermutation (0-based),
-    // pivval[k] (may be null) the pivots, *perr rrLU.error.  Returns p or
-    // -status (NaN in L or U).
-    ENG_INLINE int factor(Team &tm, int64_t maxrank, double reltol, double abstol, bool leftorth,
-                   int32_t *rowat, int32_t *colat, double *xs, double *ys, double *pivval,
-                   double *perr, double *stats)
-    {
-        for (int i = tm.tid; i < m; i += tm.nt) rowat[i] = i;
-        for (int j = tm.tid; j < n; j += tm.nt) colat[j] = j;
-        double vmax0 = -1.0;
-#pragma unroll
-        for (int i = 0; i < RA; ++i)
-#pragma unroll
-            for (int j = 0; j < CB; ++j) {
-                const double v = a[i][j] * a[i][j];
-                vmax0 = (rp[i] < m && cp[j] < n && v > vmax0) ? v : vmax0;
-            }
-        ArgBest best = first_max(tm, vmax0, -1);
-        int64_t mr = maxrank;
-        if (m < mr) mr = m;
-        if (n < mr) mr = n;
-        double maxerr = 0.0, err = 0.0 / 0.0, sarg = 0.0, sschur = 0.0;
-        int p = 0;
-        while (p < mr) {
-            const int k = p;
-            int pr = k, pc = k;
-            if (best.v >= 0.0) {
-                pc = (int)(best.lin / m);
-                pr = (int)(best.lin - (int64_t)pc * m);
-            }
-            sarg += (double)(m - k) * (double)(n - k);
-            // no non-NaN entry left: the reference keeps (k,k), itself NaN
-            const double av = best.v < 0.0 ? 0.0 / 0.0 : best.a;
-            const double e = fabs(av);
-            err = e;
-            if ((e < reltol * maxerr || e < abstol) && p > 0) break;
-            maxerr = jl_max(maxerr, e);
-            // swap positions k <-> pr (rows) and k <-> pc (cols)
-#pragma unroll
-            for (int i = 0; i < RA; ++i) rp[i] = rp[i] == k ? pr : (rp[i] == pr ? k : rp[i]);
-#pragma unroll
-            for (int j = 0; j < CB; ++j) cp[j] = cp[j] == k ? pc : (cp[j] == pc ? k : cp[j]);
-            if (tm.tid == 0) {
-                int32_t t = rowat[k]; rowat[k] = rowat[pr]; rowat[pr] = t;
-                t = colat[k]; colat[k] = colat[pc]; colat[pc] = t;
-                if (pivval) pivval[k] = av;
-            }
-            const double piv = av;
-            // stage the pivot column (x, scaled when leftorth) and row (y):
-            // gather them with selects (static register indices), divide
-            // RA (or CB) values, scatter the scaled ones back with selects
-            double colv[RA], rowv[CB];
-#pragma unroll
-            for (int i = 0; i < RA; ++i) colv[i] = 0.0;
-#pragma unroll
-            for (int j = 0; j < CB; ++j) rowv[j] = 0.0;
-#pragma unroll
-            for (int i = 0; i < RA; ++i)
-#pragma unroll
-                for (int j = 0; j < CB; ++j) {
-                    colv[i] = cp[j] == k ? a[i][j] : colv[i];
-                    rowv[j] = rp[i] == k ? a[i][j] : rowv[j];
-                }
-            bool hascol = false, hasrow = false;
-#pragma unroll
-            for (int j = 0; j < CB; ++j) hascol = hascol || cp[j] == k;
-#pragma unroll
-            for (int i = 0; i < RA; ++i) hasrow = hasrow || rp[i] == k;
-            if (hascol) {
-#pragma unroll
-                for (int i = 0; i < RA; ++i) {
-                    if (rp[i] > k && rp[i] < m) {
-                        double x = colv[i];
-                        if (leftorth) x = x / piv;
-                        colv[i] = x;
-                        xs[rp[i]] = x;
-                    }
-                }
-            }
-            if (hasrow) {
-#pragma unroll
-                for (int j = 0; j < CB; ++j) {
-                    if (cp[j] > k && cp[j] < n) {
-                        double y = rowv[j];
-                        if (!leftorth) y = y / piv;
-                        rowv[j] = y;
-                        ys[cp[j]] = y;
-                    }
-                }
-            }
-            if (leftorth) {
-#pragma unroll
-                for (int i = 0; i < RA; ++i)
-#pragma unroll
-                    for (int j = 0; j < CB; ++j)
-                        a[i][j] = (cp[j] == k && rp[i] > k && rp[i] < m) ? colv[i] : a[i][j];
-            } else {
-#pragma unroll
-                for (int i = 0; i < RA; ++i)
-#pragma unroll
-                    for (int j = 0; j < CB; ++j)
-                        a[i][j] = (rp[i] == k && cp[j] > k && cp[j] < n) ? rowv[j] : a[i][j];
-            }
-            tm.sync();
-            double xr[RA], yc[CB];
-            bool ra[RA], ca[CB];
-#pragma unroll
-            for (int i = 0; i < RA; ++i) {
-                ra[i] = rp[i] > k && rp[i] < m;
-                xr[i] = ra[i] ? xs[rp[i]] : 0.0;
-            }
-#pragma unroll
-            for (int j = 0; j < CB; ++j) {
-                ca[j] = cp[j] > k && cp[j] < n;
-                yc[j] = ca[j] ? ys[cp[j]] : 0.0;
-            }
-            // pass 1: the Schur update and the largest abs2 (NaN never wins)
-            double vmax = -1.0;
-#pragma unroll
-            for (int i = 0; i < RA; ++i)
-#pragma unroll
-                for (int j = 0; j < CB; ++j) {
-                    const bool act = ra[i] && ca[j];
-                    const double t = xr[i] * yc[j];
-                    const double u = a[i][j] - t;
-                    a[i][j] = act ? u : a[i][j];
-                    const double v = u * u;
-                    vmax = (act && v > vmax) ? v : vmax;
-                }
-            best = first_max(tm, vmax, k);
-            p = k + 1;
-            sschur += (double)(m - p) * (double)(n - p);
-        }
-        // NaN in L = tril(A[:,1:p]) or U = triu(A[1:p,:]) (matrixlu.jl:162-169)
-        int nanL = 0, nanU = 0;
-#pragma unroll
-        for (int i = 0; i < RA; ++i)
-#pragma unroll
-            for (int j = 0; j < CB; ++j) {
-                if (rp[i] >= m || cp[j] >= n) continue;
-                const double v = a[i][j];
-                if (v != v) {
-                    if (cp[j] < p && rp[i] >= cp[j]) nanL = 1;
-                    if (rp[i] < p && cp[j] >= rp[i]) nanU = 1;
-                }
-            }
-        nanL = tm.any(nanL);
-        nanU = tm.any(nanU);
-        stats[0] = sarg;
-        stats[1] = sschur;
-        if (p >= (m < n ? m : n)) err = 0.0;
-        *perr = err;
-        if (nanL) return -ST_NAN_L;
-        if (nanU) return -ST_NAN_U;
-        return p;
-    }
-
     // Lean full-pivot rrLU for the solve engine (matrixlu.jl:141-181, same
-    // pivots, pivot values, error and NaN verdicts as factor()).  Entries of
+    // pivots, pivot values, error and NaN verdicts as rrlu_team).  Entries of
     // pivoted rows and columns are turned into NaN by giving them NaN
     // multipliers, so the per-entry work is a[i,j] - x[i]*y[j], its square and
     // a maxNum — NaN never wins.  The pivot column/row values staged at step k
     // are final (matrixlu.jl:126-134 never touches them again); with Lorig
     // non-null the L column k is recorded there by original row
     // (Lorig[orig + m·k], m×p).  Requires invalid entries to hold NaN.
+    // The staging vectors and the position→original maps live in the team's
+    // LDS (tm.txs, tm.tys, tm.tperm: rows at [0, m), columns at [m, m+n)).
     ENG_INLINE int factor_lean(Team &tm, int64_t maxrank, double reltol, double abstol,
-                               bool leftorth, int32_t *rowat, int32_t *colat, double *xs,
-                               double *ys, double *pivval, double *perr, double *stats,
+                               bool leftorth, double *pivval, double *perr, double *stats,
                                double *Lorig)
     {
         const double NaN = 0.0 / 0.0;
-        for (int i = tm.tid; i < m; i += tm.nt) rowat[i] = i;
-        for (int j = tm.tid; j < n; j += tm.nt) colat[j] = j;
+        for (int i = tm.tid; i < m; i += tm.nt) tm.tperm(i) = i;
+        for (int j = tm.tid; j < n; j += tm.nt) tm.tperm(m + j) = j;
         double vmax0 = -1.0;
 #pragma unroll
         for (int i = 0; i < RA; ++i)
@@ -347,8 +166,8 @@ struct RegLU {
                 orow = best.orig & 65535;
                 ocol = best.orig >> 16;
             } else {
-                orow = rowat[k];
-                ocol = colat[k];
+                orow = tm.tperm(k);
+                ocol = tm.tperm(m + k);
             }
             orow = tm.uniform(orow);
             ocol = tm.uniform(ocol);
@@ -363,51 +182,57 @@ struct RegLU {
 #pragma unroll
             for (int j = 0; j < CB; ++j) cp[j] = cp[j] == k ? pc : (cp[j] == pc ? k : cp[j]);
             if (tm.tid == 0) {
-                int32_t t = rowat[k]; rowat[k] = rowat[pr]; rowat[pr] = t;
-                t = colat[k]; colat[k] = colat[pc]; colat[pc] = t;
+                int32_t t = tm.tperm(k); tm.tperm(k) = tm.tperm(pr); tm.tperm(pr) = t;
+                t = tm.tperm(m + k); tm.tperm(m + k) = tm.tperm(m + pc); tm.tperm(m + pc) = t;
                 if (pivval) pivval[k] = av;
             }
             if (av != av) { nanx = 1; nany = 1; }
             const double piv = av;
-            // pivot column (original ocol) and row (original orow)
+            // pivot column (original ocol) and row (original orow): the
+            // register slot (jc, ir) is wave-uniform, so only its branch runs
             const int jc = ocol / TC, ir = orow / TR;
+            double gv[RA > CB ? RA : CB];
             if (tc == ocol - TC * jc) {
 #pragma unroll
-                for (int j = 0; j < CB; ++j) {
-                    if (j != jc) continue;
+                for (int j = 0; j < CB; ++j)
+                    if (j == jc) {
 #pragma unroll
-                    for (int i = 0; i < RA; ++i) {
-                        if (rp[i] > k && rp[i] < m) {
-                            double x = a[i][j];
-                            if (leftorth) x = x / piv;
-                            xs[rp[i]] = x;
-                            if (x != x) nanx = 1;
-                            if (Lorig) Lorig[(tr + TR * i) + (int64_t)m * k] = x;
-                        }
+                        for (int i = 0; i < RA; ++i) gv[i] = a[i][j];
+                    }
+#pragma unroll
+                for (int i = 0; i < RA; ++i) {
+                    if (rp[i] > k && rp[i] < m) {
+                        double x = gv[i];
+                        if (leftorth) x = x / piv;
+                        tm.txs(rp[i]) = x;
+                        if (x != x) nanx = 1;
+                        if (Lorig) Lorig[(tr + TR * i) + (int64_t)m * k] = x;
                     }
                 }
             }
             if (tr == orow - TR * ir) {
 #pragma unroll
-                for (int i = 0; i < RA; ++i) {
-                    if (i != ir) continue;
+                for (int i = 0; i < RA; ++i)
+                    if (i == ir) {
 #pragma unroll
-                    for (int j = 0; j < CB; ++j) {
-                        if (cp[j] > k && cp[j] < n) {
-                            double y = a[i][j];
-                            if (!leftorth) y = y / piv;
-                            ys[cp[j]] = y;
-                            if (y != y) nany = 1;
-                        }
+                        for (int j = 0; j < CB; ++j) gv[j] = a[i][j];
+                    }
+#pragma unroll
+                for (int j = 0; j < CB; ++j) {
+                    if (cp[j] > k && cp[j] < n) {
+                        double y = gv[j];
+                        if (!leftorth) y = y / piv;
+                        tm.tys(cp[j]) = y;
+                        if (y != y) nany = 1;
                     }
                 }
             }
             tm.sync();
             double xr[RA], yc[CB];
 #pragma unroll
-            for (int i = 0; i < RA; ++i) xr[i] = (rp[i] > k && rp[i] < m) ? xs[rp[i]] : NaN;
+            for (int i = 0; i < RA; ++i) xr[i] = (rp[i] > k && rp[i] < m) ? tm.txs(rp[i]) : NaN;
 #pragma unroll
-            for (int j = 0; j < CB; ++j) yc[j] = (cp[j] > k && cp[j] < n) ? ys[cp[j]] : NaN;
+            for (int j = 0; j < CB; ++j) yc[j] = (cp[j] > k && cp[j] < n) ? tm.tys(cp[j]) : NaN;
             double vmax = -1.0;
 #pragma unroll
             for (int i = 0; i < RA; ++i)

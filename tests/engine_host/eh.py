@@ -31,6 +31,12 @@ def lib():
         L.eh_pivoterrors.argtypes = [vp, C.POINTER(C.c_double)]
         L.eh_bonderrors.argtypes = [vp, C.POINTER(C.c_double)]
         L.eh_workstats.argtypes = [vp, C.POINTER(C.c_double)]
+        L.eh_rrlu_mt.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int64, C.c_double,
+                                 C.c_double, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_double)]
+        L.eh_rrlu.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int64, C.c_double,
+                              C.c_double, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                              C.POINTER(C.c_double), C.POINTER(C.c_double)]
         _L = L
     return _L
 
@@ -85,3 +91,31 @@ class EngineSolve:
         out = np.zeros(self.L - 1)
         lib().eh_bonderrors(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
         return out
+
+
+def rrlu(mode, A, maxrank=(1 << 63) - 1, reltol=1e-14, abstol=0.0, leftorthogonal=True):
+    """mode 0: register tile factor, 1: lean tile factor, 2: streaming rrlu_team."""
+    A = np.array(A, dtype=np.float64, order="F", copy=True)
+    m, n = A.shape
+    rp = np.zeros(max(m, 1), np.int32)
+    cp = np.zeros(max(n, 1), np.int32)
+    pv = np.zeros(max(m, n, 1))
+    err = C.c_double()
+    p = lib().eh_rrlu(mode, A.ctypes.data_as(C.POINTER(C.c_double)), m, n, int(maxrank), reltol, abstol,
+                      int(leftorthogonal), rp.ctypes.data_as(C.POINTER(C.c_int32)),
+                      cp.ctypes.data_as(C.POINTER(C.c_int32)), pv.ctypes.data_as(C.POINTER(C.c_double)),
+                      C.byref(err))
+    return dict(p=p, rowperm=rp[:m] + 1, colperm=cp[:n] + 1, pivots=pv[:max(p, 0)], error=err.value, A=A)
+
+
+def rrlu_mt(mode, A, maxrank=(1 << 63) - 1, reltol=1e-14, abstol=0.0, leftorthogonal=True):
+    """Register-tile rrLU on 256 host threads shaped like the device team."""
+    A = np.array(A, dtype=np.float64, order="F", copy=True)
+    m, n = A.shape
+    rp = np.zeros(max(m, 1), np.int32)
+    cp = np.zeros(max(n, 1), np.int32)
+    err = C.c_double()
+    p = lib().eh_rrlu_mt(mode, A.ctypes.data_as(C.POINTER(C.c_double)), m, n, int(maxrank), reltol, abstol,
+                         int(leftorthogonal), rp.ctypes.data_as(C.POINTER(C.c_int32)),
+                         cp.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(err))
+    return dict(p=p, rowperm=rp[:m] + 1, colperm=cp[:n] + 1, error=err.value)

@@ -7,7 +7,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <barrier>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "tci_engine.h"
@@ -17,8 +19,12 @@ using namespace tci;
 
 struct HostTeam {
     // register-tile shape of the one-thread team: a 1×1 grid holding up to 160×160
-    static constexpr int TR = 1, RA = 160, CB = 160;
+    static constexpr int NT = 1, TR = 1, RA = 160, CB = 160;
     int32_t lperm[320];
+    double ltx[160], lty[160];
+    double &txs(int i) { return ltx[i]; }
+    double &tys(int j) { return lty[j]; }
+    int32_t &tperm(int q) { return lperm[q]; }
     int tid = 0, nt = 1, lane = 0, wsize = 1, wave = 0, nwaves = 1;
     double *xs = nullptr, *ys = nullptr;
     double *lpi = nullptr;
@@ -32,6 +38,123 @@ struct HostTeam {
     int any(int v) { return v; }
     uint64_t clock() const { return 0; }
 };
+
+// Multi-threaded host team: NT std::threads emulate one workgroup (one
+// thread per lane), with a std::barrier for __syncthreads and shared slots
+// for the reductions, so the device's cross-thread logic runs on the CPU.
+struct MTShared {
+    std::barrier<> *bar;
+    std::vector<ArgBest> abuf;
+    std::vector<double> dbuf;
+    std::vector<int> ibuf;
+};
+
+struct MTTile {
+    double tx[160], ty[160];
+    int32_t perm[320];
+};
+
+template <int NT_, int TR_, int RA_, int CB_>
+struct HostTeamMT {
+    static constexpr int NT = NT_, TR = TR_, RA = RA_, CB = CB_;
+    MTTile *tile = nullptr;
+    double &txs(int i) { return tile->tx[i]; }
+    double &tys(int j) { return tile->ty[j]; }
+    int32_t &tperm(int q) { return tile->perm[q]; }
+    int tid, nt = NT_, lane = 0, wsize = 1, wave = 0, nwaves = 1;
+    double *xs = nullptr, *ys = nullptr, *lpi = nullptr;
+    int64_t lpi_cap = 0;
+    uint8_t *lrows = nullptr, *lcols = nullptr;
+    int32_t *lperm = nullptr;
+    MTShared *sh = nullptr;
+    void sync() { sh->bar->arrive_and_wait(); }
+    ArgBest argmax(ArgBest b)
+    {
+        sh->abuf[tid] = b;
+        sync();
+        ArgBest r = sh->abuf[0];
+        for (int t = 1; t < nt; ++t) if (arg_better(sh->abuf[t], r)) r = sh->abuf[t];
+        sync();
+        return r;
+    }
+    ArgBest argmin_key(ArgBest b)
+    {
+        sh->abuf[tid] = b;
+        sync();
+        ArgBest r = sh->abuf[0];
+        for (int t = 1; t < nt; ++t) if (sh->abuf[t].lin < r.lin) r = sh->abuf[t];
+        sync();
+        return r;
+    }
+    double max_nan(double v)
+    {
+        sh->dbuf[tid] = v;
+        sync();
+        double r = sh->dbuf[0];
+        for (int t = 1; t < nt; ++t) r = jl_max(r, sh->dbuf[t]);
+        sync();
+        return r;
+    }
+    int any(int v)
+    {
+        sh->ibuf[tid] = v;
+        sync();
+        int r = 0;
+        for (int t = 0; t < nt; ++t) r |= sh->ibuf[t];
+        sync();
+        return r;
+    }
+    int uniform(int x) const { return x; }
+    uint64_t clock() const { return 0; }
+};
+
+// rrlu! through the register tile on NT host threads shaped like the
+// 256-thread device team (16×16 grid, 9×9 tile)
+extern "C" int eh_rrlu_mt(int mode, double *A, int m, int n, int64_t maxrank, double reltol,
+                          double abstol, int leftorth, int32_t *rowperm, int32_t *colperm,
+                          double *err)
+{
+    constexpr int NT = 256;
+    using Team = HostTeamMT<NT, 16, 9, 9>;
+    using Tile = RegLU<Team, 16, 9, 9>;
+    if (m > 144 || n > 144) return -100;
+    std::barrier<> bar(NT);
+    MTShared sh;
+    sh.bar = &bar;
+    sh.abuf.resize(NT);
+    sh.dbuf.resize(NT);
+    sh.ibuf.resize(NT);
+    std::vector<double> xs(m + n + 1), ys(m + n + 1);
+    std::vector<int32_t> perm(m + n);
+    MTTile tile;
+    int pres = 0;
+    double eres = 0;
+    auto body = [&](int tid) {
+        Team tm;
+        tm.tid = tid;
+        tm.xs = xs.data();
+        tm.ys = ys.data();
+        tm.sh = &sh;
+        tm.tile = &tile;
+        Tile R;
+        R.setup(tm, m, n);
+        R.load(A, m);
+        tm.sync();
+        double e, st[2];
+        int p = R.factor_lean(tm, maxrank, reltol, abstol, leftorth != 0, nullptr, &e, st, nullptr);
+        tm.sync();
+        if (tid == 0)
+            for (int q = 0; q < m + n; ++q) perm[q] = tile.perm[q];
+        if (tid == 0) { pres = p; eres = e; }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < NT; ++t) th.emplace_back(body, t);
+    for (auto &t : th) t.join();
+    for (int i = 0; i < m; ++i) rowperm[i] = perm[i];
+    for (int j = 0; j < n; ++j) colperm[j] = perm[m + j];
+    *err = eres;
+    return pres;
+}
 
 struct EhSolve {
     std::vector<uint8_t> inputs, arena, report;
@@ -69,6 +192,35 @@ extern "C" void *eh_solve(const tci_problem *p, int *status)
     delete e;
     *status = h->status;
     return h;
+}
+
+// rrlu! through the register tile (modes 0, 1) or the streaming path (mode 2)
+// on the one-thread team; the streaming path leaves the packed factors in A.
+extern "C" int eh_rrlu(int mode, double *A, int m, int n, int64_t maxrank, double reltol,
+                       double abstol, int leftorth, int32_t *rowperm, int32_t *colperm,
+                       double *pivval, double *err)
+{
+    static HostTeam tm;
+    tm = HostTeam();
+    std::vector<double> xs(m + n + 1), ys(m + n + 1);
+    tm.xs = xs.data();
+    tm.ys = ys.data();
+    double st[2];
+    int p;
+    if (mode == 2) {
+        p = rrlu_team(tm, A, m, n, maxrank, reltol, abstol, leftorth != 0, rowperm, colperm, pivval, err, st);
+    } else {
+        using Tile = RegLU<HostTeam, HostTeam::TR, HostTeam::RA, HostTeam::CB>;
+        if (!Tile::fits(tm, m, n)) return -100;
+        Tile *R = new Tile();
+        R->setup(tm, m, n);
+        R->load(A, m);
+        p = R->factor_lean(tm, maxrank, reltol, abstol, leftorth != 0, pivval, err, st, nullptr);
+        for (int i = 0; i < m; ++i) rowperm[i] = tm.lperm[i];
+        for (int j = 0; j < n; ++j) colperm[j] = tm.lperm[m + j];
+        delete R;
+    }
+    return p;
 }
 
 extern "C" void eh_free(void *hp) { delete (EhSolve *)hp; }

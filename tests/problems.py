@@ -1,0 +1,66 @@
+"""Problem specs shared by the CPU (engine build) and GPU parity tests.
+
+Each spec: (name, kind, params, iparams, localdims, initialpivots, options).
+Functor kinds and parameter layouts: include/tci_functors.h.
+"""
+import numpy as np
+
+LORENTZ, WLORENTZ, QGC, GSUM, SUM, TABLE, TT, QEXP = 1, 2, 3, 4, 5, 6, 7, 8
+
+
+def gauss_sum(K, n, d, seed):
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(0.5, 2.0, K)
+    w = rng.uniform(0.2, 0.6, K)
+    c = rng.uniform(0, 1, (K, n))
+    return np.concatenate([[K, d], a, w, c.ravel()])
+
+
+def quantics_gc(K, seed, A=0.1, F=2.0 ** 6):
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(0.5, 2, K)
+    c = rng.uniform(0, 1, K)
+    w = 10.0 ** rng.uniform(-2, -1, K)
+    trip = np.stack([a, c, w], 1).ravel()
+    return np.concatenate([[K], trip, [A, F]])
+
+
+def random_tt(bd, ld, seed):
+    rng = np.random.default_rng(seed)
+    cores = [rng.random((bd[i], ld[i], bd[i + 1])) for i in range(len(ld))]
+    return np.concatenate([c.ravel(order="F") for c in cores]), list(bd) + list(ld), cores
+
+
+def specs():
+    tt_params, tt_ip, _ = random_tt([1, 2, 3, 2, 1], [2, 3, 3, 2], 5)
+    tab = np.diag([1.0, 1e-5, 0.0])
+    rngw = np.random.default_rng(4)
+    return [
+        ("lorentz8", LORENTZ, [1.0], [], [10] * 8, None, dict(tolerance=1e-8)),
+        ("lorentz8_nosearch", LORENTZ, [1.0], [], [10] * 8, None, dict(tolerance=1e-8, nsearchglobalpivot=0)),
+        ("lorentz5_tight", LORENTZ, [1.0], [], [10] * 5, None, dict(tolerance=1e-12, maxiter=200)),
+        ("lorentz5_pivots", LORENTZ, [1.0], [], [10] * 5,
+         [[1, 1, 1, 1, 1], [10, 8, 10, 4, 4], [5, 4, 8, 9, 3], [7, 7, 10, 5, 9], [7, 7, 10, 5, 9]],
+         dict(tolerance=1e-12, maxiter=200)),
+        ("lorentz5_forward", LORENTZ, [0.5], [], [10] * 5, None,
+         dict(tolerance=1e-8, maxiter=8, sweepstrategy="forward")),
+        ("lorentz6_backward_strict", LORENTZ, [1.0], [], [6] * 6, None,
+         dict(tolerance=1e-9, sweepstrategy="backward", strictlynested=True, nsearchglobalpivot=0)),
+        ("weighted_lorentz", WLORENTZ, np.concatenate([[16.0], rngw.uniform(0.5, 2, 6)]), [], [16] * 6, None,
+         dict(tolerance=1e-10, maxbonddim=40)),
+        ("gauss_sum", GSUM, gauss_sum(3, 5, 8, 1), [], [8] * 5, None, dict(tolerance=1e-8, maxbonddim=32)),
+        ("gauss_sum_d2", GSUM, gauss_sum(2, 9, 2, 2), [], [2] * 9, None, dict(tolerance=1e-10)),
+        ("quantics_gc", QGC, quantics_gc(3, 3), [], [2] * 12, None, dict(tolerance=1e-10, maxbonddim=50)),
+        ("quantics_exp_maxbond1", QEXP, [1.0, -1.0, 0.0, 0.0], [], [2] * 8, [[1] * 8, [1] + [2] * 7],
+         dict(tolerance=1e-4, maxbonddim=1, maxiter=2, normalizeerror=False)),
+        ("quantics_exp_small_maxbond", QEXP, [1.0, -1.0, 1e-4, -2.0], [], [2] * 8, [[1] * 8, [1] + [2] * 7],
+         dict(tolerance=1e-10, maxbonddim=1, maxiter=10, normalizeerror=False, nsearchglobalpivot=10,
+              maxnglobalpivot=5)),
+        ("random_tt", TT, tt_params, tt_ip, [2, 3, 3, 2], None, dict(tolerance=1e-10, maxbonddim=10)),
+        ("diag_table", TABLE, tab.ravel(order="F"), [3, 3], [3, 3], [[1, 1]], dict(tolerance=1e-8)),
+        ("sum_20sites", SUM, [1.0], [], [2] * 20, None, dict(tolerance=1e-8)),
+    ]
+
+
+def oracle_opts(opts):
+    return {k: v for k, v in opts.items() if k != "chi_cap"}

@@ -1,0 +1,101 @@
+"""The device engine's own code, compiled for the CPU, against the oracle.
+
+tests/engine_host builds tci_engine.h (the code the gfx950 solve kernel
+runs) with a one-thread team, and the register-tile rrLU with a 256-thread
+host team shaped like the device team.  Every result here must equal the
+oracle bit for bit.  No GPU.
+"""
+import numpy as np
+import pytest
+
+import eh
+import oracle as O
+from problems import oracle_opts, specs
+
+import importlib.util, os, sys  # noqa: E401
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+from conftest import load_pkg  # noqa: E402
+
+_abi = load_pkg()._abi
+
+
+def _solve_both(spec):
+    name, kind, params, ip, ld, piv, opts = spec
+    h = _abi.ProblemHolder(ld, kind, params, ip, piv, **opts)
+    e = eh.EngineSolve(h)
+    try:
+        t, r, er = O.crossinterpolate2(O.Functor(kind, params, ip), ld, piv, **oracle_opts(opts))
+        ost = 0
+    except O.OracleError as ex:
+        t, ost = None, ex.code
+    return e, t, ost
+
+
+@pytest.mark.parametrize("spec", specs(), ids=lambda s: s[0])
+def test_engine_equals_oracle(spec):
+    e, t, ost = _solve_both(spec)
+    assert e.status == ost
+    if ost:
+        return
+    r, er, ng = e.history()
+    assert list(r) == list(t.ranks)
+    assert np.array_equal(er, t.errors)
+    assert list(ng) == list(t.nglobalpivots)
+    assert e.maxsample == t.maxsamplevalue
+    L = len(spec[4])
+    for b in range(L):
+        assert e.set(0, b) == t.Iset(b)
+        assert e.set(1, b) == t.Jset(b)
+        assert np.array_equal(e.sitetensor(b), t.sitetensor(b))
+    assert np.array_equal(e.pivoterrors(), t.pivoterrors)
+    assert np.array_equal(e.bonderrors(), t.bonderrors)
+
+
+def test_engine_errors():
+    h = _abi.ProblemHolder([2] * 5, 1, [0.0], tolerance=1e-8)          # f == 0 at the pivot
+    assert eh.EngineSolve(h).status == 4
+    h = _abi.ProblemHolder([2] * 5, 1, [1.0], tolerance=0.0)           # tolerance 0, unlimited χ
+    assert eh.EngineSolve(h).status == 3
+    h = _abi.ProblemHolder([10] * 8, 1, [1.0], tolerance=1e-12, chi_cap=4, maxiter=3)
+    assert eh.EngineSolve(h).status == 10                               # capacity → loud error
+
+
+def _rrlu_cases(seed, maxside):
+    rng = np.random.default_rng(seed)
+    for t in range(6):
+        m, n = (int(x) for x in rng.integers(1, maxside + 1, 2))
+        kind = t % 3
+        if kind == 0:
+            A = rng.standard_normal((m, n))
+        elif kind == 1:
+            r = int(rng.integers(1, 12))
+            A = rng.standard_normal((m, r)) @ rng.standard_normal((r, n)) + 1e-12 * rng.standard_normal((m, n))
+        else:
+            A = rng.integers(-1, 2, (m, n)).astype(float)
+        yield A, bool(rng.integers(0, 2)), [0.0, 1e-10, 1e-3][t % 3]
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_tile_and_stream_rrlu_equal_oracle(seed):
+    for A, lo, rt in _rrlu_cases(seed, 160):
+        b = O.rrlu(A, reltol=rt, leftorthogonal=lo) if rt > 0 or True else None
+        for mode in (0, 1, 2):
+            a = eh.rrlu(mode, A, reltol=rt, leftorthogonal=lo)
+            if a["p"] == -100:
+                continue
+            assert a["p"] == b.npivot, (mode, A.shape)
+            assert np.array_equal(a["rowperm"], b.rowpermutation)
+            assert np.array_equal(a["colperm"], b.colpermutation)
+            assert a["error"] == b.error or (np.isnan(a["error"]) and np.isnan(b.error))
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_tile_rrlu_multithreaded_equals_oracle(seed):
+    """256 host threads in the device team's 16×16 grid with 9×9 tiles."""
+    for A, lo, rt in _rrlu_cases(100 + seed, 144):
+        b = O.rrlu(A, reltol=rt, leftorthogonal=lo)
+        for mode in (0, 1):
+            a = eh.rrlu_mt(mode, A, reltol=rt, leftorthogonal=lo)
+            assert a["p"] == b.npivot, (mode, A.shape)
+            assert np.array_equal(a["rowperm"], b.rowpermutation)
+            assert np.array_equal(a["colperm"], b.colpermutation)
