@@ -159,12 +159,22 @@ tci_status tci_ctx_create(tci_ctx **out, int32_t device, uint64_t hbm_budget_byt
         hbm_budget_bytes = (uint64_t)(0.8 * (double)fr);
     }
     c->budget = hbm_budget_bytes;
-    int lds = 0;
-    HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
-    c->lds_max = lds - 4096;   // static LDS of the solve kernel stays below 4 KiB
-    const char *ev = getenv("TCI_TEAM");
+    // dynamic LDS per solve workgroup: the CU's LDS shared by the workgroups
+    // that should be resident together (TCI_WG_PER_CU, default 2), less the
+    // solve kernel's static LDS (below 16 KiB)
+    int lds_blk = 0, lds_cu = 0;
+    HIPCHK(hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    HIPCHK(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device));
+    const char *ev = getenv("TCI_WG_PER_CU");
+    int wgs = ev ? atoi(ev) : 1;
+    if (wgs < 1) wgs = 1;
+    int per_wg = (lds_cu > 0 ? lds_cu : lds_blk) / wgs;
+    if (per_wg > lds_blk) per_wg = lds_blk;
+    c->lds_max = per_wg - 16384;
+    if (c->lds_max < 0) c->lds_max = 0;
+    ev = getenv("TCI_TEAM");
     c->team = ev ? atoi(ev) : 512;
-    if (c->team != 256 && c->team != 512 && c->team != 1024) c->team = 512;
+    if (c->team != 512 && c->team != 1024 && c->team != 5122) c->team = 512;
     ev = getenv("TCI_PI_LDS_KB");
     c->pi_lds_kb = ev ? atoi(ev) : 1 << 20;
     *out = c;

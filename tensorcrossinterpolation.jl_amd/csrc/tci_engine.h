@@ -3,9 +3,9 @@
 // memory: pivot sets, Pi matrices, rrLU, site tensors, global pivot search.
 //
 // The engine is written once against a "Team" interface:
-//   tm.tid, tm.nt            thread id / team size
-//   tm.lane, tm.wsize        lane within a wave / wave width (64 on gfx950)
-//   tm.wave, tm.nwaves       wave id / waves per team
+//   tm.tid(), tm.nt()            thread id / team size
+//   tm.lane(), tm.wsize()        lane within a wave / wave width (64 on gfx950)
+//   tm.wave(), tm.nwaves()       wave id / waves per team
 //   tm.sync()                team barrier
 //   tm.argmax(ArgBest)       team-wide argmax, result in every thread
 //   tm.max_nan(double)       team-wide NaN-propagating max
@@ -13,10 +13,12 @@
 //   tm.xs, tm.ys             staging vectors (mrcap doubles each)
 //   tm.lpi, tm.lpi_cap       on-chip (LDS) buffer for Pi matrices, in doubles
 //   tm.lrows, tm.lcols       on-chip index tables (null: use the HBM ones)
-//   tm.txs(i), tm.tys(j), tm.tperm(q)  LDS slots of the register-tile rrLU
-//   Team::NT                 team size as a compile-time constant
+//   Team::NT, WS, NW         team size, wave width, waves (compile-time)
 //   tm.uniform(x)            x as a wave-uniform value (all lanes hold the same x)
-//   Team::TR, RA, CB         register-tile shape (tci_regtile.h); RA = 0: none
+//   tm.wave_max/min/any/first, tm.wave_sync()   wave-level reductions, ballot
+//                            and an LDS ordering point inside one wave
+//   tm.tA(w), tB(w), tX(w, r), tY(w, j)  LDS slots of the register-tile rrLU
+//   Team::RA, CB             register-tile shape (tci_regtile.h); RA = 0: none
 // The device implementation lives in tci_kernels.hip.  Every parallel loop
 // writes disjoint elements and every reduction is over a total order, so the
 // result does not depend on the team size: a one-thread team on the CPU
@@ -48,16 +50,16 @@ ENG int rrlu_team(Team &tm, double *A, int m, int n, int64_t maxrank, double rel
               double *pivval, double *perr, double *stats)
 {
     const int64_t lda = m;
-    for (int i = tm.tid; i < m; i += tm.nt) rowperm[i] = i;
-    for (int j = tm.tid; j < n; j += tm.nt) colperm[j] = j;
+    for (int i = tm.tid(); i < m; i += tm.nt()) rowperm[i] = i;
+    for (int j = tm.tid(); j < n; j += tm.nt()) colperm[j] = j;
     // first argmax over the whole matrix (submatrixargmax, matrixlu.jl:1-32)
     ArgBest best;
     best.v = -1.0;
     best.lin = INT64_MAX;
     best.a = 0.0;
-    for (int j = tm.wave; j < n; j += tm.nwaves) {
+    for (int j = tm.wave(); j < n; j += tm.nwaves()) {
         const double *col = A + lda * j;
-        for (int i = tm.lane; i < m; i += tm.wsize) {
+        for (int i = tm.lane(); i < m; i += tm.wsize()) {
             double a = col[i];
             double v = a * a;
             if (v == v) {
@@ -90,20 +92,20 @@ ENG int rrlu_team(Team &tm, double *A, int m, int n, int64_t maxrank, double rel
         // swaprow!(k, r) then swapcol!(k, c) (matrixlu.jl:98-117), one pass:
         // A'[i,j] = A[σ(i), τ(j)] with σ = (k r), τ = (k c)
         if (r != k)
-            for (int j = tm.tid; j < n; j += tm.nt) {
+            for (int j = tm.tid(); j < n; j += tm.nt()) {
                 if (j == k || j == c) continue;
                 double t = A[k + lda * j];
                 A[k + lda * j] = A[r + lda * j];
                 A[r + lda * j] = t;
             }
         if (c != k)
-            for (int i = tm.tid; i < m; i += tm.nt) {
+            for (int i = tm.tid(); i < m; i += tm.nt()) {
                 if (i == k || i == r) continue;
                 double t = A[i + lda * k];
                 A[i + lda * k] = A[i + lda * c];
                 A[i + lda * c] = t;
             }
-        if (tm.tid == 0) {
+        if (tm.tid() == 0) {
             double akk = A[k + lda * k], akc = A[k + lda * c];
             double ark = A[r + lda * k], arc = A[r + lda * c];
             A[k + lda * k] = arc;
@@ -115,22 +117,22 @@ ENG int rrlu_team(Team &tm, double *A, int m, int n, int64_t maxrank, double rel
         }
         tm.sync();
         const double piv = A[k + lda * k];
-        if (tm.tid == 0 && pivval) pivval[k] = piv;
+        if (tm.tid() == 0 && pivval) pivval[k] = piv;
         // scale the pivot column (leftorth) or row by true division, stage x, y
         if (leftorth) {
-            for (int i = k + 1 + tm.tid; i < m; i += tm.nt) {
+            for (int i = k + 1 + tm.tid(); i < m; i += tm.nt()) {
                 double x = A[i + lda * k] / piv;
                 A[i + lda * k] = x;
                 tm.xs[i] = x;
             }
-            for (int j = k + 1 + tm.tid; j < n; j += tm.nt) tm.ys[j] = A[k + lda * j];
+            for (int j = k + 1 + tm.tid(); j < n; j += tm.nt()) tm.ys[j] = A[k + lda * j];
         } else {
-            for (int j = k + 1 + tm.tid; j < n; j += tm.nt) {
+            for (int j = k + 1 + tm.tid(); j < n; j += tm.nt()) {
                 double y = A[k + lda * j] / piv;
                 A[k + lda * j] = y;
                 tm.ys[j] = y;
             }
-            for (int i = k + 1 + tm.tid; i < m; i += tm.nt) tm.xs[i] = A[i + lda * k];
+            for (int i = k + 1 + tm.tid(); i < m; i += tm.nt()) tm.xs[i] = A[i + lda * k];
         }
         tm.sync();
         // A[k+1:,k+1:] -= x·yᵀ, unfused, fused with the next argmax
@@ -138,10 +140,10 @@ ENG int rrlu_team(Team &tm, double *A, int m, int n, int64_t maxrank, double rel
         best.lin = INT64_MAX;
         best.a = 0.0;
         // columns over waves, rows over lanes: coalesced, no index division
-        for (int j = k + 1 + tm.wave; j < n; j += tm.nwaves) {
+        for (int j = k + 1 + tm.wave(); j < n; j += tm.nwaves()) {
             double *col = A + lda * j;
             const double yj = tm.ys[j];
-            for (int i = k + 1 + tm.lane; i < m; i += tm.wsize) {
+            for (int i = k + 1 + tm.lane(); i < m; i += tm.wsize()) {
                 double t = tm.xs[i] * yj;
                 double a = col[i] - t;
                 col[i] = a;
@@ -161,14 +163,14 @@ ENG int rrlu_team(Team &tm, double *A, int m, int n, int64_t maxrank, double rel
     }
     // L = tril(A[:,1:p]), U = triu(A[1:p,:]); NaN checks (matrixlu.jl:162-169)
     int nanL = 0, nanU = 0;
-    for (int j = tm.wave; j < p; j += tm.nwaves)
-        for (int i = j + tm.lane; i < m; i += tm.wsize) {
+    for (int j = tm.wave(); j < p; j += tm.nwaves())
+        for (int i = j + tm.lane(); i < m; i += tm.wsize()) {
             double v = A[i + lda * j];
             if (v != v) nanL = 1;
         }
-    for (int j = tm.wave; j < n; j += tm.nwaves) {
+    for (int j = tm.wave(); j < n; j += tm.nwaves()) {
         const int imax = j < p - 1 ? j : p - 1;
-        for (int i = tm.lane; i <= imax; i += tm.wsize) {
+        for (int i = tm.lane(); i <= imax; i += tm.wsize()) {
             double v = A[i + lda * j];
             if (v != v) nanU = 1;
         }
@@ -193,6 +195,7 @@ enum { PH_EVAL = 0, PH_RRLU = 1, PH_SETS = 2, PH_FILL = 3, PH_GSEARCH = 4, PH_SW
 // FK = 0 dispatches on the descriptor's kind at run time.
 template <class Team, int FK = 0>
 struct Engine {
+    using Tile = TileLU<Team, (Team::RA > 0 ? Team::RA : 1), (Team::CB > 0 ? Team::CB : 1)>;
     Team &tm;
     const SolveDesc &D;
     EngShared &S;
@@ -241,7 +244,7 @@ struct Engine {
     // addglobalpivots! (tensorci2.jl:193-213); thread 0, then barrier
     ENG int addglobalpivots(const uint8_t *piv, int npiv)
     {
-        if (tm.tid == 0) {
+        if (tm.tid() == 0) {
             int st = ST_OK;
             for (int q = 0; q < npiv && st == ST_OK; ++q) {
                 const uint8_t *p = piv + (int64_t)q * L;
@@ -264,7 +267,7 @@ struct Engine {
         const uint8_t *I = setp(SET_I, b);
         const int m = nI * d;
         if (m > D.mrcap) return m;   // caller reports ST_CAPACITY
-        for (int q = tm.tid; q < m; q += tm.nt) {
+        for (int q = tm.tid(); q < m; q += tm.nt()) {
             int s = q / nI, i = q - s * nI;
             uint8_t *r = rows + (int64_t)q * L;
             for (int t = 0; t < b; ++t) r[t] = I[(int64_t)i * L + t];
@@ -281,7 +284,7 @@ struct Engine {
         const int len = jlen(b + 1);
         const int n = nJ * d;
         if (n > D.mrcap) return n;   // caller reports ST_CAPACITY
-        for (int q = tm.tid; q < n; q += tm.nt) {
+        for (int q = tm.tid(); q < n; q += tm.nt()) {
             int j = q / d, s = q - j * d;
             uint8_t *c = cols + (int64_t)q * L;
             c[0] = (uint8_t)(s + 1);
@@ -306,14 +309,14 @@ struct Engine {
         const int cmplen = rowside ? b : jlen(b + 1);
         const int off = rowside ? 0 : 1;
         const int elen = rowside ? b + 1 : jlen(b);
-        for (int e = tm.tid; e < ne; e += tm.nt) {
+        for (int e = tm.tid(); e < ne; e += tm.nt()) {
             const uint8_t *x = E + (int64_t)e * L + off;
             bool found = false;
             for (int i = 0; i < nref && !found; ++i) found = eq(R + (int64_t)i * L, x, cmplen);
             D.flags[e] = found ? 0 : 1;
         }
         tm.sync();
-        if (tm.tid == 0) {
+        if (tm.tid() == 0) {
             int m = m0;
             uint8_t *dst = rowside ? rows : cols;
             for (int e = 0; e < ne; ++e) {
@@ -333,7 +336,7 @@ struct Engine {
     {
         const int n = setn(kind, b);
         const uint8_t *src = setp(kind, b);
-        for (int64_t q = tm.tid; q < (int64_t)n * len; q += tm.nt) {
+        for (int64_t q = tm.tid(); q < (int64_t)n * len; q += tm.nt()) {
             int64_t e = q / len, t = q - e * len;
             dst[e * L + t] = src[e * L + t];
         }
@@ -343,11 +346,11 @@ struct Engine {
     ENG void select_into(int kind, int b, const uint8_t *table, const int32_t *perm, int p, int len)
     {
         uint8_t *dst = setp(kind, b);
-        for (int64_t q = tm.tid; q < (int64_t)p * len; q += tm.nt) {
+        for (int64_t q = tm.tid(); q < (int64_t)p * len; q += tm.nt()) {
             int64_t k = q / len, t = q - k * len;
             dst[k * L + t] = table[(int64_t)perm[k] * L + t];
         }
-        if (tm.tid == 0) setn(kind, b) = p;
+        if (tm.tid() == 0) setn(kind, b) = p;
     }
 
     // ---------------------------------------------------------- evaluation
@@ -360,8 +363,8 @@ struct Engine {
         const uint64_t t0 = tm.clock();
         double lmax = 0.0;
         const int64_t cnt = (int64_t)m * n;
-        for (int j = tm.wave; j < n; j += tm.nwaves)
-            for (int i = tm.lane; i < m; i += tm.wsize) {
+        for (int j = tm.wave(); j < n; j += tm.nwaves())
+            for (int i = tm.lane(); i < m; i += tm.wsize()) {
                 tci_point x;
                 x.a = rows + (int64_t)i * L;
                 x.b = cols + (int64_t)j * L;
@@ -373,14 +376,14 @@ struct Engine {
             }
         if (fold) {
             double g = tm.max_nan(lmax);
-            if (tm.tid == 0) {
+            if (tm.tid() == 0) {
                 S.maxsample = jl_max(fabs(S.maxsample), g);
                 S.ws[0] += (double)cnt;
                 S.ws[5] += 1.0;
             }
         }
         tm.sync();
-        if (tm.tid == 0) S.tph[PH_EVAL] += tm.clock() - t0;
+        if (tm.tid() == 0) S.tph[PH_EVAL] += tm.clock() - t0;
     }
 
     // --------------------------------------------------------------- rrLU
@@ -396,27 +399,26 @@ struct Engine {
         double st[4];
         const uint64_t t0 = tm.clock();
         int p;
-        if (Team::RA > 0 && RegLU<Team, Team::TR, Team::RA, Team::CB>::fits(tm, m, n)) {
-            RegLU<Team, Team::TR, Team::RA, Team::CB> R;
+        if (Team::RA > 0 && Tile::fits(m, n)) {
+            Tile R;
             R.setup(tm, m, n);
-            R.load(A, m);
+            R.load(tm, A, m);
             // L recorded by original row in lscratch, then permuted into A
             double *Lorig = need_left ? lscratch : nullptr;
-            p = R.factor_lean(tm, maxrank, reltol, abstol, leftorth, D.pivval, perr, st, Lorig);
-            for (int i = tm.tid; i < m; i += tm.nt) D.rowperm[i] = tm.tperm(i);
-            for (int j = tm.tid; j < n; j += tm.nt) D.colperm[j] = tm.tperm(m + j);
+            p = R.factor(tm, maxrank, reltol, abstol, leftorth, D.pivval, perr, st, Lorig,
+                         D.rowperm, D.colperm);
             if (need_left && p > 0) {
                 tm.sync();
-                for (int t = tm.wave; t < p; t += tm.nwaves)
-                    for (int pos = t + 1 + tm.lane; pos < m; pos += tm.wsize)
-                        A[pos + (int64_t)m * t] = Lorig[tm.tperm(pos) + (int64_t)m * t];
+                for (int t = tm.wave(); t < p; t += tm.nwaves())
+                    for (int pos = t + 1 + tm.lane(); pos < m; pos += tm.wsize())
+                        A[pos + (int64_t)m * t] = Lorig[D.rowperm[pos] + (int64_t)m * t];
             }
             tm.sync();
         } else {
             p = rrlu_team(tm, A, m, n, maxrank, reltol, abstol, leftorth, D.rowperm, D.colperm,
                           D.pivval, perr, st);
         }
-        if (tm.tid == 0) {
+        if (tm.tid() == 0) {
             S.tph[PH_RRLU] += tm.clock() - t0;
             S.ws[1] += 8.0 * st[0];
             S.ws[2] += 8.0 * (double)m * (double)n + 16.0 * st[1] +
@@ -430,7 +432,7 @@ struct Engine {
     // updateerrors! (tensorci2.jl:161-169); thread 0
     ENG void updateerrors(int b, int p, double err)
     {
-        if (tm.tid == 0) {
+        if (tm.tid() == 0) {
             D.bonderrors[b] = err;
             int npe = D.ints[INT_NPE];
             int n = npe > p + 1 ? npe : p + 1;
@@ -445,7 +447,7 @@ struct Engine {
 
     ENG int fail(int st)
     {
-        if (tm.tid == 0 && S.status == ST_OK) S.status = st;
+        if (tm.tid() == 0 && S.status == ST_OK) S.status = st;
         tm.sync();
         return S.status;
     }
@@ -456,7 +458,7 @@ struct Engine {
     ENG_INLINE bool lorentz_tile(int m, int n) const
     {
         return (FK == TCI_F_LORENTZ_INT || (FK == 0 && D.fkind == TCI_F_LORENTZ_INT)) &&
-               Team::RA > 0 && RegLU<Team, Team::TR, Team::RA, Team::CB>::fits(tm, m, n) &&
+               Team::RA > 0 && Tile::fits(m, n) &&
                m <= D.mrcap && n <= D.mrcap;
     }
 
@@ -466,22 +468,22 @@ struct Engine {
         const uint64_t t0 = tm.clock();
         // integer sums of squares of the row part and the column part (exact
         // in double: at most 250·255² < 2^53)
-        for (int q = tm.tid; q < m; q += tm.nt) {
+        for (int q = tm.tid(); q < m; q += tm.nt()) {
             int64_t acc = 0;
             for (int t = 0; t < nl; ++t) { int64_t v = rows[(int64_t)q * L + t]; acc += v * v; }
             tm.xs[q] = (double)acc;
         }
-        for (int q = tm.tid; q < n; q += tm.nt) {
+        for (int q = tm.tid(); q < n; q += tm.nt()) {
             int64_t acc = 0;
             for (int t = 0; t < L - nl; ++t) { int64_t v = cols[(int64_t)q * L + t]; acc += v * v; }
             tm.ys[q] = (double)acc;
         }
         tm.sync();
-        RegLU<Team, Team::TR, Team::RA, Team::CB> R;
+        Tile R;
         R.setup(tm, m, n);
-        const double lmax = R.fill_lorentz(D.params[0], tm.xs, tm.ys);
+        const double lmax = R.fill_lorentz(tm, D.params[0], tm.xs, tm.ys);
         const double g = tm.max_nan(lmax);
-        if (tm.tid == 0) {
+        if (tm.tid() == 0) {
             S.maxsample = jl_max(fabs(S.maxsample), g);
             S.ws[0] += (double)m * (double)n;
             S.ws[5] += 1.0;
@@ -489,11 +491,10 @@ struct Engine {
         }
         const uint64_t t1 = tm.clock();
         double st[4];
-        int p = R.factor_lean(tm, maxrank, reltol, abstol, leftorth, D.pivval, perr, st, nullptr);
-        for (int i = tm.tid; i < m; i += tm.nt) D.rowperm[i] = tm.tperm(i);
-        for (int j = tm.tid; j < n; j += tm.nt) D.colperm[j] = tm.tperm(m + j);
+        int p = R.factor(tm, maxrank, reltol, abstol, leftorth, D.pivval, perr, st, nullptr,
+                         D.rowperm, D.colperm);
         tm.sync();
-        if (tm.tid == 0) {
+        if (tm.tid() == 0) {
             S.tph[PH_RRLU] += tm.clock() - t1;
             S.ws[1] += 8.0 * st[0];
             S.ws[2] += 8.0 * (double)m * (double)n + 16.0 * st[1] +
@@ -544,7 +545,7 @@ struct Engine {
     // and X may live in LDS; X (r×c) is the result.
     ENG int solve_right(const double *Pi1, int r, double *A, int c, double *X)
     {
-        if (tm.tid == 0) {
+        if (tm.tid() == 0) {
             int singular = 0;
             for (int k = 0; k < c; ++k) {
                 int pr = k;
@@ -572,7 +573,7 @@ struct Engine {
         }
         tm.sync();
         if (S.cnt) return ST_SINGULAR;
-        for (int q = tm.tid; q < r; q += tm.nt) {
+        for (int q = tm.tid(); q < r; q += tm.nt()) {
             for (int i = 0; i < c; ++i) X[q + (int64_t)r * i] = Pi1[q + (int64_t)r * i];
             for (int k = 0; k < c; ++k) {
                 int pk = D.ipiv[k];
@@ -609,7 +610,7 @@ struct Engine {
     ENG double *Tslot(int b) const { return D.T + D.tstride * b; }
     ENG void setTdims(int b, int64_t a, int64_t d, int64_t c)
     {
-        if (tm.tid == 0) { D.Tdims[3 * b] = a; D.Tdims[3 * b + 1] = d; D.Tdims[3 * b + 2] = c; }
+        if (tm.tid() == 0) { D.Tdims[3 * b] = a; D.Tdims[3 * b + 1] = d; D.Tdims[3 * b + 2] = c; }
     }
 
     // fillsitetensors! (globalsearch.jl:97-104) → setsitetensor!(tci, f, b)
@@ -642,19 +643,19 @@ struct Engine {
             if (!Pi1 || !At || !X) { Pi1 = D.pi; At = D.pmat; X = Tslot(b); }
             eval(rows, b + 1, m, cols, nJ, Pi1, m, false, true);
             if (np != nJ) {
-                if (tm.tid == 0) D.ints[INT_ERRSITE] = b + 1;
+                if (tm.tid() == 0) D.ints[INT_ERRSITE] = b + 1;
                 return fail(ST_NONSQUARE);
             }
             // Pᵀ[jj, ii] = f(Iset[b+1][ii] ⊕ Jset[b][jj])
             eval(setp(SET_I, b + 1), b + 1, np, cols, nJ, At, np, true, false);
             int st = solve_right(Pi1, m, At, np, X);
             if (st != ST_OK) {
-                if (tm.tid == 0) D.ints[INT_ERRSITE] = b + 1;
+                if (tm.tid() == 0) D.ints[INT_ERRSITE] = b + 1;
                 return fail(st);
             }
             double *T = Tslot(b);
             if (X != T)
-                for (int64_t q = tm.tid; q < (int64_t)m * np; q += tm.nt) T[q] = X[q];
+                for (int64_t q = tm.tid(); q < (int64_t)m * np; q += tm.nt()) T[q] = X[q];
             setTdims(b, nI, d, np);
             tm.sync();
         }
@@ -677,11 +678,11 @@ struct Engine {
                 const int nI = setn(SET_I, b), nJ = setn(SET_J, b);
                 const uint8_t *si = setp(SET_I, b), *sj = setp(SET_J, b);
                 uint8_t *di = setp(nh_i, b), *dj = setp(nh_j, b);
-                for (int64_t q = tm.tid; q < (int64_t)nI * L; q += tm.nt) di[q] = si[q];
-                for (int64_t q = tm.tid; q < (int64_t)nJ * L; q += tm.nt) dj[q] = sj[q];
+                for (int64_t q = tm.tid(); q < (int64_t)nI * L; q += tm.nt()) di[q] = si[q];
+                for (int64_t q = tm.tid(); q < (int64_t)nJ * L; q += tm.nt()) dj[q] = sj[q];
             }
             tm.sync();
-            if (tm.tid == 0) {
+            if (tm.tid() == 0) {
                 for (int b = 0; b < L; ++b) {
                     setn(nh_i, b) = setn(SET_I, b);
                     setn(nh_j, b) = setn(SET_J, b);
@@ -695,14 +696,14 @@ struct Engine {
                 const int b = fwd ? s : L - 2 - s;
                 int st = updatepivots(b, fwd, abstol, extras ? ek_i : -1, extras ? ek_j : -1);
                 if (st != ST_OK) {
-                    if (tm.tid == 0) D.ints[INT_ERRSITE] = b + 1;
+                    if (tm.tid() == 0) D.ints[INT_ERRSITE] = b + 1;
                     return st;
                 }
             }
         }
         const uint64_t t0 = tm.clock();
         int st = fillsitetensors(D.nsearchglobalpivot > 0);
-        if (tm.tid == 0) S.tph[PH_FILL] += tm.clock() - t0;
+        if (tm.tid() == 0) S.tph[PH_FILL] += tm.clock() - t0;
         return st;
     }
 
@@ -717,7 +718,7 @@ struct Engine {
         const int sumd = D.sumd;
         const int ncand = ns * sumd;
         double *errs = D.work;                          // ncand
-        if (tm.tid == 0) {
+        if (tm.tid() == 0) {
             for (int q = 0; q < ns; ++q)
                 for (int p = 0; p < L; ++p) {
                     uint64_t r = eng_rng_next(S.rng) >> 32;
@@ -726,7 +727,7 @@ struct Engine {
         }
         tm.sync();
         // candidate points: the point with site p set to v, p-major then v
-        for (int c = tm.tid; c < ncand; c += tm.nt) {
+        for (int c = tm.tid(); c < ncand; c += tm.nt()) {
             const int q = c / sumd;
             int r = c - q * sumd, p = 0;
             while (r >= D.localdims[p]) { r -= D.localdims[p]; ++p; }
@@ -758,12 +759,12 @@ struct Engine {
         tm.sync();
         for (int c0 = 0; c0 < ncand; c0 += chunk) {
             const int cn = ncand - c0 < chunk ? ncand - c0 : chunk;
-            for (int c = tm.tid; c < cn; c += tm.nt) V[(int64_t)c * chi] = 1.0;
+            for (int c = tm.tid(); c < cn; c += tm.nt()) V[(int64_t)c * chi] = 1.0;
             tm.sync();
             for (int t = 0; t < L; ++t) {
                 const int64_t d0 = D.Tdims[3 * t], d = D.Tdims[3 * t + 1], d2 = D.Tdims[3 * t + 2];
                 const double *T = D.T + D.tstride * t;
-                for (int64_t o = tm.tid; o < (int64_t)cn * d2; o += tm.nt) {
+                for (int64_t o = tm.tid(); o < (int64_t)cn * d2; o += tm.nt()) {
                     const int64_t c = o / d2, j = o - c * d2;
                     const int64_t s = cand[(c0 + c) * L + t] - 1;
                     const double *Tc = T + d0 * (s + d * j);
@@ -775,7 +776,7 @@ struct Engine {
                 tm.sync();
                 double *tmp = V; V = W; W = tmp;
             }
-            for (int c = tm.tid; c < cn; c += tm.nt) {
+            for (int c = tm.tid(); c < cn; c += tm.nt()) {
                 tci_point pt;
                 pt.a = cand + (int64_t)(c0 + c) * L; pt.b = pt.a; pt.na = L; pt.L = L;
                 const double fv = tci_functor_eval(FK ? FK : D.fkind, D.params, D.iparams, &pt);
@@ -783,7 +784,7 @@ struct Engine {
             }
             tm.sync();
         }
-        if (tm.tid == 0) {
+        if (tm.tid() == 0) {
             const double thr = abstol * D.tolmargin;
             int nf = 0;
             for (int q = 0; q < ns; ++q) {
@@ -828,7 +829,7 @@ struct Engine {
     // tensorci2.jl:402-461, forward, updatetensors = true
     ENG int sweep1site_forward(double abstol)
     {
-        if (tm.tid == 0) D.ints[INT_NPE] = 0;
+        if (tm.tid() == 0) D.ints[INT_NPE] = 0;
         for (int b = 0; b < L - 1; ++b) {
             const int nI = setn(SET_I, b), d = D.localdims[b], nJ = setn(SET_J, b);
             const int m = kron_rows(b);
@@ -840,7 +841,7 @@ struct Engine {
             double err;
             int p = rrlu(A, m, nJ, D.maxbonddim, 1e-14, abstol, true, &err, true, Tslot(b));
             if (p < 0) {
-                if (tm.tid == 0) D.ints[INT_ERRSITE] = b + 1;
+                if (tm.tid() == 0) D.ints[INT_ERRSITE] = b + 1;
                 return fail(-p);
             }
             if (p > D.setcap || (int64_t)m * p > D.tstride) return fail(ST_CAPACITY);
@@ -853,7 +854,7 @@ struct Engine {
             double *T = Tslot(b);
             int64_t off = A == tm.lpi ? (int64_t)m * nJ : 0;
             double *X = lds_take((int64_t)m * p, off);
-            for (int i = p + tm.tid; i < m; i += tm.nt) {
+            for (int i = p + tm.tid(); i < m; i += tm.nt()) {
                 double *row = X ? X + i : T + D.rowperm[i];
                 for (int j = p - 1; j >= 0; --j) {
                     double s = A[i + (int64_t)m * j];
@@ -862,7 +863,7 @@ struct Engine {
                 }
             }
             if (X) tm.sync();
-            for (int64_t q = tm.tid; q < (int64_t)m * p; q += tm.nt) {
+            for (int64_t q = tm.tid(); q < (int64_t)m * p; q += tm.nt()) {
                 const int64_t j = q / m, i = q - j * m;
                 if (i < p) T[D.rowperm[i] + (int64_t)m * j] = (i == j) ? 1.0 : 0.0;
                 else if (X) T[D.rowperm[i] + (int64_t)m * j] = X[q];
@@ -870,12 +871,12 @@ struct Engine {
             setTdims(b, nI, d, p);
             tm.sync();
             int nan = 0;
-            for (int64_t q = tm.tid; q < (int64_t)m * p; q += tm.nt) {
+            for (int64_t q = tm.tid(); q < (int64_t)m * p; q += tm.nt()) {
                 double v = T[q];
                 if (v != v) nan = 1;
             }
             if (tm.any(nan)) {
-                if (tm.tid == 0) D.ints[INT_ERRSITE] = b + 1;
+                if (tm.tid() == 0) D.ints[INT_ERRSITE] = b + 1;
                 return fail(ST_NAN_TENSOR);
             }
             updateerrors(b, p, err);
@@ -907,7 +908,7 @@ struct Engine {
     // (42-53) → optimize! (700-850)
     ENG int run()
     {
-        if (tm.tid == 0) {
+        if (tm.tid() == 0) {
             for (int i = 0; i < 8; ++i) S.tph[i] = 0;
             S.tph[PH_TOTAL] = tm.clock();
             S.status = ST_OK;
@@ -924,7 +925,7 @@ struct Engine {
         int st = addglobalpivots(D.initpivots, D.npivinit);
         if (st != ST_OK) return finish(st);
         // maxsamplevalue = maximum(abs, f(x) for x in initialpivots) (49-50)
-        if (tm.tid == 0) {
+        if (tm.tid() == 0) {
             double mx = 0.0;
             for (int q = 0; q < D.npivinit; ++q) {
                 tci_point pt;
@@ -941,7 +942,7 @@ struct Engine {
         if (D.nsearchglobalpivot > 0 && D.nsearchglobalpivot < D.maxnglobalpivot) return finish(ST_GENERIC);
         const double tol = D.tolerance;
         if (D.maxbonddim == INT64_MAX && tol <= 0) return finish(ST_ARG);
-        if (tm.tid == 0) eng_rng_seed(S.rng, D.seed);
+        if (tm.tid() == 0) eng_rng_seed(S.rng, D.seed);
         tm.sync();
         int niter = 0;
         for (int iter = 1; iter <= D.maxiter; ++iter) {
@@ -955,10 +956,10 @@ struct Engine {
             if (D.nsearchglobalpivot > 0) {
                 const uint64_t t0 = tm.clock();
                 ng = globalsearch(abstol);
-                if (tm.tid == 0) S.tph[PH_GSEARCH] += tm.clock() - t0;
+                if (tm.tid() == 0) S.tph[PH_GSEARCH] += tm.clock() - t0;
                 if (ng < 0) return finish(S.status);
             }
-            if (tm.tid == 0) {
+            if (tm.tid() == 0) {
                 if (niter < D.maxiter_cap) {
                     D.ranks[niter] = rank();
                     D.errors[niter] = perr;
@@ -969,7 +970,7 @@ struct Engine {
             ++niter;
             tm.sync();
             bool conv = false;
-            if (tm.tid == 0) S.cnt = converged(niter, abstol) ? 1 : 0;
+            if (tm.tid() == 0) S.cnt = converged(niter, abstol) ? 1 : 0;
             tm.sync();
             conv = S.cnt != 0;
             tm.sync();
@@ -978,15 +979,15 @@ struct Engine {
         const double norm = D.normalizeerror ? S.maxsample : 1.0;
         const uint64_t t1 = tm.clock();
         st = sweep1site_forward(tol * norm);
-        if (tm.tid == 0) S.tph[PH_SWEEP1] += tm.clock() - t1;
+        if (tm.tid() == 0) S.tph[PH_SWEEP1] += tm.clock() - t1;
         if (st != ST_OK) return finish(st);
         // _sanitycheck (globalsearch.jl:106-111)
         for (int b = 0; b < L - 1; ++b)
             if (setn(SET_I, b + 1) != setn(SET_J, b)) {
-                if (tm.tid == 0) D.ints[INT_ERRSITE] = b + 1;
+                if (tm.tid() == 0) D.ints[INT_ERRSITE] = b + 1;
                 return finish(ST_NONSQUARE);
             }
-        if (tm.tid == 0)
+        if (tm.tid() == 0)
             for (int i = 0; i < niter; ++i) D.errors[i] = D.errors[i] / norm;
         return finish(ST_OK);
     }
@@ -994,7 +995,7 @@ struct Engine {
     ENG int finish(int st)
     {
         tm.sync();
-        if (tm.tid == 0) {
+        if (tm.tid() == 0) {
             D.ints[INT_STATUS] = st;
             S.tph[PH_TOTAL] = tm.clock() - S.tph[PH_TOTAL];
             for (int i = 0; i < 8; ++i) D.scal[SCAL_TPH + i] = (double)S.tph[i];

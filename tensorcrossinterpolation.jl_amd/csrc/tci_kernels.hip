@@ -14,126 +14,12 @@
 // reduction costs one barrier.  Compiled with -ffp-contract=off.
 #include <hip/hip_runtime.h>
 
+#include "tci_devteam.h"
 #include "tci_engine.h"
 #include "tci_kernels.h"
 
 namespace tci {
 
-// register-tile shape per team size (tci_regtile.h): TR×(NT/TR) threads,
-// RA×CB doubles each; capacity (TR·RA)×((NT/TR)·CB)
-template <int NT> struct TileShape;
-template <> struct TileShape<256> { static constexpr int TR = 16, RA = 9, CB = 9; };    // 144×144
-template <> struct TileShape<512> { static constexpr int TR = 32, RA = 5, CB = 9; };    // 160×144
-template <> struct TileShape<1024> { static constexpr int TR = 32, RA = 5, CB = 5; };   // 160×160
-
-// LDS of the register-tile rrLU (largest tile: 160 rows, 160 columns)
-constexpr int kTileRows = 160, kTileCols = 160;
-__shared__ double g_tile_xs[kTileRows];
-__shared__ double g_tile_ys[kTileCols];
-__shared__ int32_t g_tile_perm[kTileRows + kTileCols];
-
-template <int NT_>
-struct DevTeam {
-    static constexpr int NT = NT_;
-    static constexpr int NW = NT / 64;
-    static constexpr int TR = TileShape<NT>::TR, RA = TileShape<NT>::RA, CB = TileShape<NT>::CB;
-    static_assert(TR * RA <= kTileRows && (NT / TR) * CB <= kTileCols, "tile exceeds its LDS");
-    __device__ double &txs(int i) const { return g_tile_xs[i]; }
-    __device__ double &tys(int j) const { return g_tile_ys[j]; }
-    __device__ int32_t &tperm(int q) const { return g_tile_perm[q]; }
-    int tid;
-    int nt;
-    int lane, wsize, wave, nwaves;
-    double *xs, *ys;
-    double *lpi;          // LDS area for Pi matrices
-    int64_t lpi_cap;      // its size in doubles (0: none)
-    uint8_t *lrows, *lcols;
-    ArgBest *red_a;   // 2 × NW
-    double *red_d;    // 2 × NW
-    int par_a, par_d;
-
-    __device__ void sync() { __syncthreads(); }
-
-    __device__ ArgBest argmax(ArgBest b)
-    {
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            ArgBest o;
-            o.v = __shfl_xor(b.v, off);
-            o.lin = (int64_t)__shfl_xor((long long)b.lin, off);
-            o.a = __shfl_xor(b.a, off);
-            if (arg_better(o, b)) b = o;
-        }
-        ArgBest *slot = red_a + par_a * NW;
-        if ((tid & 63) == 0) slot[tid >> 6] = b;
-        __syncthreads();
-        ArgBest r = slot[0];
-#pragma unroll
-        for (int w = 1; w < NW; ++w)
-            if (arg_better(slot[w], r)) r = slot[w];
-        par_a ^= 1;
-        return r;
-    }
-
-    __device__ double max_nan(double v)
-    {
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) v = jl_max(v, __shfl_xor(v, off));
-        double *slot = red_d + par_d * NW;
-        if ((tid & 63) == 0) slot[tid >> 6] = v;
-        __syncthreads();
-        double r = slot[0];
-#pragma unroll
-        for (int w = 1; w < NW; ++w) r = jl_max(r, slot[w]);
-        par_d ^= 1;
-        return r;
-    }
-
-    __device__ int uniform(int x) const { return __builtin_amdgcn_readfirstlane(x); }
-
-    // team minimum of b.lin carrying b.a and b.orig (b.v is already team-uniform)
-    __device__ ArgBest argmin_key(ArgBest b)
-    {
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            const int64_t ol = (int64_t)__shfl_xor((long long)b.lin, off);
-            const double oa = __shfl_xor(b.a, off);
-            const int oo = __shfl_xor(b.orig, off);
-            if (ol < b.lin) { b.lin = ol; b.a = oa; b.orig = oo; }
-        }
-        ArgBest *slot = red_a + par_a * NW;
-        if ((tid & 63) == 0) slot[tid >> 6] = b;
-        __syncthreads();
-        ArgBest r = slot[0];
-#pragma unroll
-        for (int w = 1; w < NW; ++w)
-            if (slot[w].lin < r.lin) r = slot[w];
-        par_a ^= 1;
-        return r;
-    }
-
-    __device__ int any(int v) { return __syncthreads_or(v); }
-    __device__ uint64_t clock() const { return wall_clock64(); }
-    __device__ void init(int t, double *x, double *y, ArgBest *ra, double *rd)
-    {
-        tid = t;
-        nt = NT;
-        lane = t & 63;
-        wsize = 64;
-        wave = t >> 6;
-        nwaves = NW;
-        xs = x;
-        ys = y;
-        red_a = ra;
-        red_d = rd;
-        par_a = 0;
-        par_d = 0;
-        lpi = nullptr;
-        lpi_cap = 0;
-        lrows = nullptr;
-        lcols = nullptr;
-    }
-};
 
 constexpr int kTeam = 256;
 
@@ -142,30 +28,28 @@ constexpr int kTeam = 256;
 //   [stage: 2·stage_cap doubles][rows: tab_cap bytes][cols: tab_cap bytes]
 //   [Pi area: pi_cap doubles]
 // Anything that does not fit stays in the solve's HBM arena.
-template <int NT, int FK>
-__global__ __launch_bounds__(NT) void tci_solve_kernel(const SolveDesc *__restrict__ descs,
+template <int NT, int FK, int WPE>
+__global__ __launch_bounds__(NT, WPE) void tci_solve_kernel(const SolveDesc *__restrict__ descs,
                                                        int count, LdsPlan plan)
 {
     const int s = blockIdx.x;
     if (s >= count) return;
     __shared__ EngShared S;
-    __shared__ ArgBest red_a[2 * (NT / 64)];
-    __shared__ double red_d[2 * (NT / 64)];
     extern __shared__ double dyn[];
     const SolveDesc &D = descs[s];
-    DevTeam<NT> tm;
+    DevTeam<NT, WPE> tm;
     double *stage = dyn;
     uint8_t *tabs = (uint8_t *)(dyn + 2 * plan.stage_cap);
     double *pi = (double *)(tabs + 2 * plan.tab_cap);
-    if (D.mrcap <= plan.stage_cap) tm.init(threadIdx.x, stage, stage + D.mrcap, red_a, red_d);
-    else tm.init(threadIdx.x, D.stage, D.stage + D.mrcap, red_a, red_d);
+    if (D.mrcap <= plan.stage_cap) tm.init(stage, stage + D.mrcap);
+    else tm.init(D.stage, D.stage + D.mrcap);
     if ((int64_t)D.mrcap * D.L <= plan.tab_cap) {
         tm.lrows = tabs;
         tm.lcols = tabs + plan.tab_cap;
     }
     tm.lpi = pi;
     tm.lpi_cap = plan.pi_cap;
-    Engine<DevTeam<NT>, FK> eng(tm, D, S);
+    Engine<DevTeam<NT, WPE>, FK> eng(tm, D, S);
     eng.run();
 }
 
@@ -177,23 +61,21 @@ __global__ __launch_bounds__(kTeam) void tci_rrlu_kernel(tci_lu_job *__restrict_
 {
     const int64_t s = blockIdx.x;
     if (s >= count) return;
-    __shared__ ArgBest red_a[2 * (kTeam / 64)];
-    __shared__ double red_d[2 * (kTeam / 64)];
     extern __shared__ double stage[];
     tci_lu_job &J = jobs[s];
     const int m = (int)J.m, n = (int)J.n;
     const int mx = m > n ? m : n;
     DevTeam<kTeam> tm;
-    if (mx <= lds_stage_cap) tm.init(threadIdx.x, stage, stage + mx, red_a, red_d);
-    else tm.init(threadIdx.x, gstage + gstage_stride * s, gstage + gstage_stride * s + mx, red_a, red_d);
+    if (mx <= lds_stage_cap) tm.init(stage, stage + mx);
+    else tm.init(gstage + gstage_stride * s, gstage + gstage_stride * s + mx);
     double err, st[2];
     // the streaming factorization: the packed factors are the output
     const int p = rrlu_team(tm, J.A, m, n, J.maxrank, J.reltol, J.abstol, J.leftorth != 0,
                             J.rowperm, J.colperm, nullptr, &err, st);
     // 1-based permutations (rrLU.rowpermutation / colpermutation)
-    for (int i = tm.tid; i < m; i += kTeam) J.rowperm[i] += 1;
-    for (int j = tm.tid; j < n; j += kTeam) J.colperm[j] += 1;
-    if (tm.tid == 0) {
+    for (int i = tm.tid(); i < m; i += kTeam) J.rowperm[i] += 1;
+    for (int j = tm.tid(); j < n; j += kTeam) J.colperm[j] += 1;
+    if (tm.tid() == 0) {
         J.status = p < 0 ? -p : 0;
         J.npivot = p < 0 ? 0 : p;
         J.error = err;
@@ -280,24 +162,24 @@ size_t lds_plan_bytes(const LdsPlan &p)
     return (size_t)2 * p.stage_cap * sizeof(double) + 2 * (size_t)p.tab_cap + (size_t)p.pi_cap * sizeof(double);
 }
 
-template <int NT, int FK>
+template <int NT, int FK, int WPE>
 static void launch_one(const SolveDesc *descs, int count, const LdsPlan &plan, size_t lds, hipStream_t st)
 {
-    (void)hipFuncSetAttribute((const void *)tci_solve_kernel<NT, FK>,
+    (void)hipFuncSetAttribute((const void *)tci_solve_kernel<NT, FK, WPE>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((tci_solve_kernel<NT, FK>), dim3(count), dim3(NT), lds, st, descs, count, plan);
+    hipLaunchKernelGGL((tci_solve_kernel<NT, FK, WPE>), dim3(count), dim3(NT), lds, st, descs, count, plan);
 }
 
-template <int NT>
+template <int NT, int WPE>
 static void launch_nt(const SolveDesc *descs, int count, int fkind, const LdsPlan &plan, size_t lds,
                       hipStream_t st)
 {
     switch (fkind) {
-    case TCI_F_LORENTZ_INT: launch_one<NT, TCI_F_LORENTZ_INT>(descs, count, plan, lds, st); break;
-    case TCI_F_LORENTZ_WEIGHTED: launch_one<NT, TCI_F_LORENTZ_WEIGHTED>(descs, count, plan, lds, st); break;
-    case TCI_F_QUANTICS_GAUSS_COS: launch_one<NT, TCI_F_QUANTICS_GAUSS_COS>(descs, count, plan, lds, st); break;
-    case TCI_F_GAUSS_SUM: launch_one<NT, TCI_F_GAUSS_SUM>(descs, count, plan, lds, st); break;
-    default: launch_one<NT, 0>(descs, count, plan, lds, st); break;
+    case TCI_F_LORENTZ_INT: launch_one<NT, TCI_F_LORENTZ_INT, WPE>(descs, count, plan, lds, st); break;
+    case TCI_F_LORENTZ_WEIGHTED: launch_one<NT, TCI_F_LORENTZ_WEIGHTED, WPE>(descs, count, plan, lds, st); break;
+    case TCI_F_QUANTICS_GAUSS_COS: launch_one<NT, TCI_F_QUANTICS_GAUSS_COS, WPE>(descs, count, plan, lds, st); break;
+    case TCI_F_GAUSS_SUM: launch_one<NT, TCI_F_GAUSS_SUM, WPE>(descs, count, plan, lds, st); break;
+    default: launch_one<NT, 0, WPE>(descs, count, plan, lds, st); break;
     }
 }
 
@@ -306,18 +188,19 @@ hipError_t launch_solve(const SolveDesc *descs, int count, int team, int fkind, 
                         hipStream_t st)
 {
     const size_t lds = lds_plan_bytes(plan);
-    // per-functor specializations for the default 512-thread team; the
-    // other team sizes run the generic (run-time dispatched) engine
+    // team shapes: 512 threads with up to 256 VGPRs (one solve per CU, the
+    // default, every functor specialized); 1024 threads, or 512 threads with
+    // two workgroups per CU, at 128 VGPRs (Lorentzian specialized)
     switch (team) {
     case 1024:
-        if (fkind == TCI_F_LORENTZ_INT) launch_one<1024, TCI_F_LORENTZ_INT>(descs, count, plan, lds, st);
-        else launch_one<1024, 0>(descs, count, plan, lds, st);
+        if (fkind == TCI_F_LORENTZ_INT) launch_one<1024, TCI_F_LORENTZ_INT, 4>(descs, count, plan, lds, st);
+        else launch_one<1024, 0, 4>(descs, count, plan, lds, st);
         break;
-    case 256:
-        if (fkind == TCI_F_LORENTZ_INT) launch_one<256, TCI_F_LORENTZ_INT>(descs, count, plan, lds, st);
-        else launch_one<256, 0>(descs, count, plan, lds, st);
+    case 5122:
+        if (fkind == TCI_F_LORENTZ_INT) launch_one<512, TCI_F_LORENTZ_INT, 4>(descs, count, plan, lds, st);
+        else launch_one<512, 0, 4>(descs, count, plan, lds, st);
         break;
-    default: launch_nt<512>(descs, count, fkind, plan, lds, st); break;
+    default: launch_nt<512, 2>(descs, count, fkind, plan, lds, st); break;
     }
     return hipGetLastError();
 }

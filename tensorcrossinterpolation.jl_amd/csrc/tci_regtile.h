@@ -1,157 +1,177 @@
-// tci_regtile.h — rrLU with the whole matrix in registers.
+// tci_regtile.h — full-pivot rrLU with the whole matrix in registers.
 //
-// The team is a TR×TC grid of threads; thread (tr, tc) keeps A[i, j] for
-// i = tr + TR·a (a < RA) and j = tc + TC·b (b < CB) in a register array, so a
-// matrix of up to (TR·RA)×(TC·CB) never touches memory while it is factored.
-// Rows and columns are not moved: each thread tracks the current position of
-// its rows and columns and applies the swap of step k (positions k ↔ pivot)
-// to those positions locally.  The argmax key of an entry is its current
-// column-major position colpos·m + rowpos, so ties resolve exactly as the
-// reference's column-major scan of the physically swapped matrix does
-// (matrixlu.jl:1-32), and every entry sees the same operation sequence as in
-// addpivot! (matrixlu.jl:114-136): true division of the pivot column (or
-// row), then a[i,j] - x[i]*y[j] with separate roundings.
+// Layout.  The team is NW waves of WS lanes.  Lane l of wave w keeps
+// A[r, c] for r = l + WS·i (i < RA) and c = w + NW·j (j < CB), so every wave
+// holds complete columns and every row is present in every wave.  A matrix
+// of up to (WS·RA)×(NW·CB) is factored without touching memory.
 //
-// Per pivot step: one team argmax (1 barrier) and one staging barrier; LDS
-// traffic per thread is O(RA + CB), not O(RA·CB).
+// Rows and columns are never moved.  Each thread tracks the current position
+// of its rows (rp) and of its wave's columns (cp, the same in every lane) and
+// applies the swap of step k (positions k ↔ pivot) to them.  The argmax key
+// of an entry is its current position, colpos·256 + rowpos, so the smallest
+// key among equal abs2 values is the first maximum of the reference's
+// column-major scan of the physically swapped matrix (matrixlu.jl:1-32).
+// Every entry sees the operation sequence of addpivot! (matrixlu.jl:114-136):
+// a true division of the pivot column (or row), then a - x*y with two
+// roundings.  Rows and columns that have been pivoted get NaN multipliers, so
+// their entries turn into NaN and drop out of every later search (a NaN is
+// never the maximum, as in submatrixargmax).
+//
+// One pivot step costs two team barriers:
+//   A  the team maximum g of |entry| over the trailing block; the maximum
+//      abs2 is then G = g·g (squaring is monotone).
+//   B  the team minimum key among entries with x·x == G.  Before B each wave
+//      that holds a candidate writes the column of its best candidate to its
+//      LDS slot, so after B the winning column is already staged.
+// The pivot row lies in every wave (one lane holds it) and is broadcast
+// through a wave-private LDS slot without a team barrier.
 #pragma once
 
 #include "tci_common.h"
 
 namespace tci {
 
-template <class Team, int TR, int RA, int CB>
-struct RegLU {
-    static constexpr int TC = Team::NT / TR;   // thread columns of the grid
+template <class Team, int RA, int CB>
+struct TileLU {
+    static constexpr int WS = Team::WS, NW = Team::NW;
+    static constexpr int MR = WS * RA, MC = NW * CB;   // capacity: MR rows, MC columns
+    static_assert(MR <= 256, "row positions must fit the 8-bit part of the key");
     double a[RA][CB];
-    int rp[RA];     // current position of my rows (or INT32_MAX when i >= m)
-    int cp[CB];     // current position of my columns
-    int tr, tc;
+    int rp[RA];     // current position of my rows (large for rows ≥ m)
+    int cp[CB];     // current position of my wave's columns (large for cols ≥ n)
     int m, n;
 
-    ENG_INLINE void setup(Team &tm, int m_, int n_)
+    ENG_INLINE static bool fits(int m_, int n_) { return m_ <= MR && n_ <= MC; }
+
+    ENG_INLINE void setup(const Team &tm, int m_, int n_)
     {
-        tr = tm.tid % TR;
-        tc = tm.tid / TR;
         m = m_;
         n = n_;
 #pragma unroll
         for (int i = 0; i < RA; ++i) {
-            int r = tr + TR * i;
-            rp[i] = r < m ? r : 0x7fffffff;
+            const int r = tm.lane() + WS * i;
+            rp[i] = r < m ? r : 0x3fffffff;
         }
 #pragma unroll
         for (int j = 0; j < CB; ++j) {
-            int c = tc + TC * j;
-            cp[j] = c < n ? c : 0x7fffffff;
+            const int c = tm.wave() + NW * j;
+            cp[j] = c < n ? c : 0x3fffffff;
         }
     }
 
-    ENG_INLINE static bool fits(const Team &, int m_, int n_)
-    {
-        return m_ <= TR * RA && n_ <= TC * CB;
-    }
-
-    ENG_INLINE void load(const double *A, int64_t lda)
+    // entries outside the matrix hold NaN
+    ENG_INLINE void load(const Team &tm, const double *A, int64_t lda)
     {
 #pragma unroll
-        for (int i = 0; i < RA; ++i)
+        for (int j = 0; j < CB; ++j)
 #pragma unroll
-            for (int j = 0; j < CB; ++j) {
-                const int r = tr + TR * i, c = tc + TC * j;
+            for (int i = 0; i < RA; ++i) {
+                const int r = tm.lane() + WS * i, c = tm.wave() + NW * j;
                 a[i][j] = (r < m && c < n) ? A[r + lda * c] : 0.0 / 0.0;
             }
     }
 
-    // f = c / (1 + rs[i] + cs[j]) with exact integer partial sums of v² over the
-    // row and column parts of the point (TCI_F_LORENTZ_INT, whose value is
-    // c / (double)(Σ v² + 1) with an exact integer sum); returns this
-    // thread's NaN-propagating max|A|
-    ENG_INLINE double fill_lorentz(double c, const double *rs, const double *cs)
+    // f = c / (1 + rs[r] + cs[c]) for TCI_F_LORENTZ_INT, whose value is
+    // c / (double)(Σ v² + 1) with an exact integer sum split into the row
+    // part rs and the column part cs.  Returns this thread's NaN-propagating
+    // max|f| (the maxsamplevalue contribution).
+    ENG_INLINE double fill_lorentz(const Team &tm, double c, const double *rs, const double *cs)
     {
         double rsv[RA], csv[CB];
 #pragma unroll
-        for (int i = 0; i < RA; ++i) rsv[i] = rp[i] < m ? rs[rp[i]] : 0.0;
+        for (int i = 0; i < RA; ++i) {
+            const int r = tm.lane() + WS * i;
+            rsv[i] = r < m ? rs[r] : 0.0;
+        }
 #pragma unroll
-        for (int j = 0; j < CB; ++j) csv[j] = cp[j] < n ? cs[cp[j]] : 0.0;
+        for (int j = 0; j < CB; ++j) {
+            const int cc = tm.wave() + NW * j;
+            csv[j] = cc < n ? cs[cc] : 0.0;
+        }
         double lmax = 0.0;
 #pragma unroll
-        for (int i = 0; i < RA; ++i)
+        for (int j = 0; j < CB; ++j)
 #pragma unroll
-            for (int j = 0; j < CB; ++j) {
+            for (int i = 0; i < RA; ++i) {
                 const bool ok = rp[i] < m && cp[j] < n;
                 const double v = c / (rsv[i] + csv[j] + 1.0);
                 a[i][j] = ok ? v : 0.0 / 0.0;
-                const double av = fabs(v);
-                lmax = ok ? jl_max(lmax, av) : lmax;
+                lmax = ok ? jl_max(lmax, fabs(v)) : lmax;
             }
         return lmax;
     }
 
-    // position-ordered columns 0..p-1 (the L part and above), m×p
-    ENG_INLINE void store_left(double *Lpos, int p) const
+    // Barriers A and B of a pivot step (see the header).  lmax is this
+    // thread's max |entry| (-1: none).  Returns the winning key (UINT32_MAX
+    // when every trailing entry is NaN) and the wave whose LDS slot holds the
+    // winning column.
+    ENG_INLINE uint32_t search(Team &tm, double lmax, int &wstar)
     {
+        const double wm = tm.wave_max(lmax);
+        if (tm.lane() == 0) tm.tA(tm.wave()) = wm;
+        tm.sync();                                            // A
+        double g = tm.tA(0);
 #pragma unroll
-        for (int i = 0; i < RA; ++i)
+        for (int q = 1; q < NW; ++q) g = fmax(g, tm.tA(q));
+        uint32_t kw = 0xffffffffu;
+        if (g >= 0.0) {
+            const double G = g * g;
+            const bool mine = lmax * lmax == G;
+            if (tm.wave_any(mine)) {
+                uint32_t kl = 0xffffffffu;
+                if (mine) {
 #pragma unroll
-            for (int j = 0; j < CB; ++j)
-                if (rp[i] < m && cp[j] < p) Lpos[rp[i] + (int64_t)m * cp[j]] = a[i][j];
-    }
-
-    // The entry with the largest abs2 and, among equal ones, the smallest
-    // current column-major position — the first maximum of the reference's
-    // column-major scan over the trailing block (positions > lo).  vmax is
-    // this thread's largest abs2 over that block (-1: none).  Only threads
-    // holding the team maximum search their entries for its smallest position.
-    ENG_INLINE ArgBest first_max(Team &tm, double vmax, int lo) const
-    {
-        const double g = tm.max_nan(vmax);
-        ArgBest b;
-        b.v = g;
-        b.lin = INT64_MAX;
-        b.a = 0.0;
-        b.orig = 0;
-        b.pad = 0;
-        if (g >= 0.0 && vmax == g) {
+                    for (int j = 0; j < CB; ++j) {
+                        const uint32_t ck = (uint32_t)cp[j] << 8;
 #pragma unroll
-            for (int i = 0; i < RA; ++i)
-#pragma unroll
-                for (int j = 0; j < CB; ++j) {
-                    const double x = a[i][j];
-                    const int key = cp[j] * m + rp[i];   // < (TC·CB)·(TR·RA) < 2^31
-                    const bool hit = rp[i] > lo && rp[i] < m && cp[j] > lo && cp[j] < n &&
-                                     x * x == g && key < b.lin;
-                    b.lin = hit ? key : b.lin;
-                    b.a = hit ? x : b.a;
-                    b.orig = hit ? (tr + TR * i) + 65536 * (tc + TC * j) : b.orig;
+                        for (int i = 0; i < RA; ++i) {
+                            const double v = a[i][j];
+                            const uint32_t key = ck | (uint32_t)rp[i];
+                            kl = (v * v == G && key < kl) ? key : kl;
+                        }
+                    }
                 }
+                kw = tm.wave_min(kl);
+                // stage the column of this wave's candidate (raw values)
+                const int cpos = (int)(kw >> 8);
+#pragma unroll
+                for (int j = 0; j < CB; ++j)
+                    if (cp[j] == cpos) {
+#pragma unroll
+                        for (int i = 0; i < RA; ++i) tm.tX(tm.wave(), tm.lane() + WS * i) = a[i][j];
+                    }
+            }
         }
-        return tm.argmin_key(b);
+        if (tm.lane() == 0) tm.tB(tm.wave()) = kw;
+        tm.sync();                                            // B
+        uint32_t K = tm.tB(0);
+        wstar = 0;
+#pragma unroll
+        for (int q = 1; q < NW; ++q) {
+            const uint32_t v = tm.tB(q);
+            if (v < K) { K = v; wstar = q; }
+        }
+        return K;
     }
 
-    // Lean full-pivot rrLU for the solve engine (matrixlu.jl:141-181, same
-    // pivots, pivot values, error and NaN verdicts as rrlu_team).  Entries of
-    // pivoted rows and columns are turned into NaN by giving them NaN
-    // multipliers, so the per-entry work is a[i,j] - x[i]*y[j], its square and
-    // a maxNum — NaN never wins.  The pivot column/row values staged at step k
-    // are final (matrixlu.jl:126-134 never touches them again); with Lorig
-    // non-null the L column k is recorded there by original row
-    // (Lorig[orig + m·k], m×p).  Requires invalid entries to hold NaN.
-    // The staging vectors and the position→original maps live in the team's
-    // LDS (tm.txs, tm.tys, tm.tperm: rows at [0, m), columns at [m, m+n)).
-    ENG_INLINE int factor_lean(Team &tm, int64_t maxrank, double reltol, double abstol,
-                               bool leftorth, double *pivval, double *perr, double *stats,
-                               double *Lorig)
+    // Full-pivot rrLU (matrixlu.jl:141-181): same pivots, pivot values, error
+    // and NaN verdicts as rrlu_team.  Requires entries outside the matrix to
+    // hold NaN.  Outputs: rowperm/colperm (original index at each position,
+    // written when non-null), pivval[k], *perr, stats[0..1] (argmax and Schur
+    // element counts), and with Lorig non-null the L column k by original row
+    // (Lorig[orig + m·k]).  Returns p, or -status for the NaN errors.
+    ENG_INLINE int factor(Team &tm, int64_t maxrank, double reltol, double abstol, bool leftorth,
+                          double *pivval, double *perr, double *stats, double *Lorig,
+                          int32_t *rowperm, int32_t *colperm)
     {
         const double NaN = 0.0 / 0.0;
-        for (int i = tm.tid; i < m; i += tm.nt) tm.tperm(i) = i;
-        for (int j = tm.tid; j < n; j += tm.nt) tm.tperm(m + j) = j;
-        double vmax0 = -1.0;
+        double lmax = -1.0;
 #pragma unroll
-        for (int i = 0; i < RA; ++i)
+        for (int j = 0; j < CB; ++j)
 #pragma unroll
-            for (int j = 0; j < CB; ++j) vmax0 = fmax(vmax0, a[i][j] * a[i][j]);
-        ArgBest best = first_max(tm, vmax0, -1);
+            for (int i = 0; i < RA; ++i) lmax = fmax(lmax, fabs(a[i][j]));
+        int wstar;
+        uint32_t K = search(tm, lmax, wstar);
         int64_t mr = maxrank;
         if (m < mr) mr = m;
         if (n < mr) mr = n;
@@ -159,20 +179,18 @@ struct RegLU {
         int p = 0, nanx = 0, nany = 0;
         while (p < mr) {
             const int k = p;
-            int pr = k, pc = k, orow, ocol;
-            if (best.v >= 0.0) {
-                pc = (int)(best.lin / m);
-                pr = (int)(best.lin - (int64_t)pc * m);
-                orow = best.orig & 65535;
-                ocol = best.orig >> 16;
-            } else {
-                orow = tm.tperm(k);
-                ocol = tm.tperm(m + k);
+            const bool cand = K != 0xffffffffu;
+            const int pr = cand ? (int)(K & 255u) : k;
+            const int pc = cand ? (int)(K >> 8) : k;
+            // original row of position pr: held in every wave by one lane
+            int orow = 0;
+#pragma unroll
+            for (int i = 0; i < RA; ++i) {
+                const int l = tm.wave_first(rp[i] == pr);
+                if (l >= 0) orow = l + WS * i;
             }
-            orow = tm.uniform(orow);
-            ocol = tm.uniform(ocol);
+            const double av = cand ? tm.tX(wstar, orow) : NaN;
             sarg += (double)(m - k) * (double)(n - k);
-            const double av = best.v < 0.0 ? NaN : best.a;
             const double e = fabs(av);
             err = e;
             if ((e < reltol * maxerr || e < abstol) && p > 0) break;
@@ -181,76 +199,75 @@ struct RegLU {
             for (int i = 0; i < RA; ++i) rp[i] = rp[i] == k ? pr : (rp[i] == pr ? k : rp[i]);
 #pragma unroll
             for (int j = 0; j < CB; ++j) cp[j] = cp[j] == k ? pc : (cp[j] == pc ? k : cp[j]);
-            if (tm.tid == 0) {
-                int32_t t = tm.tperm(k); tm.tperm(k) = tm.tperm(pr); tm.tperm(pr) = t;
-                t = tm.tperm(m + k); tm.tperm(m + k) = tm.tperm(m + pc); tm.tperm(m + pc) = t;
-                if (pivval) pivval[k] = av;
-            }
+            if (tm.tid() == 0 && pivval) pivval[k] = av;
             if (av != av) { nanx = 1; nany = 1; }
-            const double piv = av;
-            // pivot column (original ocol) and row (original orow): the
-            // register slot (jc, ir) is wave-uniform, so only its branch runs
-            const int jc = ocol / TC, ir = orow / TR;
-            double gv[RA > CB ? RA : CB];
-            if (tc == ocol - TC * jc) {
+            // x: the staged pivot column, divided by the pivot when leftorth
+            double x[RA];
 #pragma unroll
-                for (int j = 0; j < CB; ++j)
-                    if (j == jc) {
+            for (int i = 0; i < RA; ++i) {
+                const bool live = rp[i] > k && rp[i] < m;
+                double v = NaN;
+                if (cand && live) {
+                    v = tm.tX(wstar, tm.lane() + WS * i);
+                    if (leftorth) v = v / av;
+                }
+                if (live && v != v) nanx = 1;
+                if (Lorig && live && tm.wave() == 0) Lorig[(tm.lane() + WS * i) + (int64_t)m * k] = v;
+                x[i] = live ? v : NaN;
+            }
+            // y: the pivot row, broadcast from its holder lane in this wave
+            // through the wave's LDS slot, read column by column in the update
+            if (cand) {
+                const int lo = orow % WS, io = orow / WS;
+                if (tm.lane() == lo) {
 #pragma unroll
-                        for (int i = 0; i < RA; ++i) gv[i] = a[i][j];
-                    }
+                    for (int i = 0; i < RA; ++i)
+                        if (i == io) {
+#pragma unroll
+                            for (int j = 0; j < CB; ++j) tm.tY(tm.wave(), j) = a[i][j];
+                        }
+                }
+                tm.wave_sync();
+                if (!leftorth) {
+                    for (int j = tm.lane(); j < CB; j += WS) tm.tY(tm.wave(), j) = tm.tY(tm.wave(), j) / av;
+                    tm.wave_sync();
+                }
+            }
+            // trailing update and the next search
+            lmax = -1.0;
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
+                const bool live = cp[j] > k && cp[j] < n;
+                double yj = NaN;
+                if (live) {
+                    yj = cand ? tm.tY(tm.wave(), j) : NaN;
+                    if (yj != yj) nany = 1;
+                }
 #pragma unroll
                 for (int i = 0; i < RA; ++i) {
-                    if (rp[i] > k && rp[i] < m) {
-                        double x = gv[i];
-                        if (leftorth) x = x / piv;
-                        tm.txs(rp[i]) = x;
-                        if (x != x) nanx = 1;
-                        if (Lorig) Lorig[(tr + TR * i) + (int64_t)m * k] = x;
-                    }
-                }
-            }
-            if (tr == orow - TR * ir) {
-#pragma unroll
-                for (int i = 0; i < RA; ++i)
-                    if (i == ir) {
-#pragma unroll
-                        for (int j = 0; j < CB; ++j) gv[j] = a[i][j];
-                    }
-#pragma unroll
-                for (int j = 0; j < CB; ++j) {
-                    if (cp[j] > k && cp[j] < n) {
-                        double y = gv[j];
-                        if (!leftorth) y = y / piv;
-                        tm.tys(cp[j]) = y;
-                        if (y != y) nany = 1;
-                    }
-                }
-            }
-            tm.sync();
-            double xr[RA], yc[CB];
-#pragma unroll
-            for (int i = 0; i < RA; ++i) xr[i] = (rp[i] > k && rp[i] < m) ? tm.txs(rp[i]) : NaN;
-#pragma unroll
-            for (int j = 0; j < CB; ++j) yc[j] = (cp[j] > k && cp[j] < n) ? tm.tys(cp[j]) : NaN;
-            double vmax = -1.0;
-#pragma unroll
-            for (int i = 0; i < RA; ++i)
-#pragma unroll
-                for (int j = 0; j < CB; ++j) {
-                    const double t = xr[i] * yc[j];
+                    const double t = x[i] * yj;
                     const double u = a[i][j] - t;
                     a[i][j] = u;
-                    vmax = fmax(vmax, u * u);
+                    lmax = fmax(lmax, fabs(u));
                 }
-            best = first_max(tm, vmax, k);
+            }
             p = k + 1;
             sschur += (double)(m - p) * (double)(n - p);
+            K = search(tm, lmax, wstar);
         }
-        // for leftorth L holds the staged x (and the pivots), U the staged y;
-        // for !leftorth the other way round (matrixlu.jl:162-169)
-        const int nanL = tm.any(leftorth ? nanx : nanx);
-        const int nanU = tm.any(leftorth ? nany : nany);
+        // position → original index
+        if (rowperm && tm.wave() == 0) {
+#pragma unroll
+            for (int i = 0; i < RA; ++i)
+                if (rp[i] < m) rowperm[rp[i]] = tm.lane() + WS * i;
+        }
+        if (colperm && tm.lane() == 0) {
+#pragma unroll
+            for (int j = 0; j < CB; ++j)
+                if (cp[j] < n) colperm[cp[j]] = tm.wave() + NW * j;
+        }
+        const int nanL = tm.any(nanx);
+        const int nanU = tm.any(nany);
         stats[0] = sarg;
         stats[1] = sschur;
         if (p >= (m < n ? m : n)) err = 0.0;

@@ -1,0 +1,284 @@
+# TCIAMD.jl — Julia binding of libtci_hip.so for TensorCrossInterpolation.jl.
+#
+# Adds methods (no new names a user has to learn) so that the TCI2 hot path
+# runs on an MI355X:
+#
+#   TCI.rrlu(A::Matrix{Float64}; maxrank, reltol, abstol, leftorthogonal)
+#       → tci_rrlu_f64                                 (matrixlu.jl:217-225)
+#   DeviceFunctor{Float64} <: TCI.BatchEvaluator{Float64}
+#       (f)(Iset, Jset, Val(M)) → tci_pi_eval_f64      (batcheval.jl:67-83)
+#   TCI.crossinterpolate2(Float64, f::DeviceFunctor, localdims, initialpivots; kw...)
+#       → tci_solve_submit / tci_batch_wait            (tensorci2.jl:943-953)
+#
+# Every entry point is a plain ccall on the C ABI of include/tci.h; arrays
+# are passed as pointers under GC.@preserve.  Nonzero status codes become
+# the reference's exceptions (ErrorException with the reference's message,
+# ArgumentError, DimensionMismatch, SingularException).
+#
+# This file is not exercised in this repository's test suite (the build
+# image has no Julia); the Python package mirrors the same calls and is
+# what the tests run.
+module TCIAMD
+
+import TensorCrossInterpolation as TCI
+using LinearAlgebra: SingularException
+
+const libtci = get(ENV, "TCI_HIP_LIB", joinpath(@__DIR__, "..", "libtci_hip.so"))
+const UNLIMITED = typemax(Int64)
+
+# ---------------------------------------------------------------- errors
+function throwstatus(st::Integer, what::AbstractString="")
+    st == 0 && return nothing
+    msg = unsafe_string(ccall((:tci_last_error, libtci), Cstring, ()))
+    st == 1 && error("lu.L contains NaNs")
+    st == 2 && error("lu.U contains NaNs")
+    st == 3 && throw(ArgumentError(msg))
+    st == 4 && error("maxsamplevalue is zero!")
+    st == 5 && error(msg)                      # "Error: NaN in tensor T[b]"
+    st == 6 && error(msg)                      # "Pivot matrix at bond b is not square!"
+    st == 7 && throw(DimensionMismatch(msg))
+    st == 9 && throw(SingularException(0))
+    error("libtci_hip ($what): status $st: $msg")
+end
+
+# ---------------------------------------------------------------- context
+mutable struct Context
+    handle::Ptr{Cvoid}
+    function Context(device::Integer=0; hbm_budget::Integer=0)
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        throwstatus(ccall((:tci_ctx_create, libtci), Cint, (Ref{Ptr{Cvoid}}, Int32, UInt64),
+                          h, device, hbm_budget), "tci_ctx_create")
+        c = new(h[])
+        finalizer(c) do c
+            ccall((:tci_ctx_destroy, libtci), Cint, (Ptr{Cvoid},), c.handle)
+        end
+        return c
+    end
+end
+
+const _ctx = Ref{Union{Nothing,Context}}(nothing)
+defaultcontext() = something(_ctx[], (_ctx[] = Context(0)))
+
+# ---------------------------------------------------------------- rrLU
+"""
+    TCI.rrlu(A::Matrix{Float64}; maxrank, reltol, abstol, leftorthogonal)
+
+Full-pivot rank-revealing LU on the GPU; the pivots, L, U and error are
+bit-identical to the reference `rrlu` (matrixlu.jl:217-225).
+"""
+function TCI.rrlu(A::Matrix{Float64}; maxrank::Int=UNLIMITED, reltol::Number=1e-14,
+                  abstol::Number=0.0, leftorthogonal::Bool=true)
+    m, n = size(A)
+    cap = max(min(m, n, maxrank), 1)
+    rp = Vector{Int64}(undef, m)
+    cp = Vector{Int64}(undef, n)
+    L = zeros(Float64, m, cap)
+    U = zeros(Float64, cap, n)
+    p = Ref{Int64}(0)
+    err = Ref{Float64}(0.0)
+    GC.@preserve A rp cp L U begin
+        st = ccall((:tci_rrlu_f64, libtci), Cint,
+                   (Ptr{Cvoid}, Ptr{Float64}, Int64, Int64, Int64, Int64, Float64, Float64, Int32,
+                    Ptr{Int64}, Ptr{Int64}, Ptr{Float64}, Int64, Ptr{Float64}, Int64,
+                    Ref{Int64}, Ref{Float64}),
+                   defaultcontext().handle, A, m, m, n, maxrank, reltol, abstol, leftorthogonal,
+                   rp, cp, L, m, U, cap, p, err)
+    end
+    throwstatus(st, "tci_rrlu_f64")
+    np = Int(p[])
+    return TCI.rrLU(Vector{Int}(rp), Vector{Int}(cp), L[:, 1:np], U[1:np, :],
+                    leftorthogonal, np, err[])
+end
+
+# ---------------------------------------------------------------- functors
+# Mirrors tci_functor of include/tci.h; the kinds and parameter layouts are
+# those of include/tci_functors.h.
+struct CFunctor
+    kind::Int32
+    nparams::Int32
+    params::Ptr{Float64}
+    niparams::Int32
+    iparams::Ptr{Int32}
+end
+
+"""
+    DeviceFunctor{Float64}(kind, params, iparams=Int32[])
+
+A function given by a device functor kind and its parameters.  Scalar calls
+and batch calls both evaluate on the GPU.
+"""
+struct DeviceFunctor{T} <: TCI.BatchEvaluator{T}
+    kind::Int32
+    params::Vector{Float64}
+    iparams::Vector{Int32}
+    localdims::Vector{Int}
+end
+
+DeviceFunctor{Float64}(kind::Integer, params, localdims; iparams=Int32[]) =
+    DeviceFunctor{Float64}(Int32(kind), Vector{Float64}(params), Vector{Int32}(iparams),
+                           collect(localdims))
+
+lorentzian(localdims; a=1.0) = DeviceFunctor{Float64}(1, [a], localdims)
+
+_cf(f::DeviceFunctor) = CFunctor(f.kind, length(f.params), pointer(f.params),
+                                 length(f.iparams), pointer(f.iparams))
+
+function _packindices(set::Vector{Vector{Int}}, len::Int)
+    out = Vector{UInt8}(undef, length(set) * len)
+    for (i, x) in enumerate(set), t in 1:len
+        out[(i-1)*len+t] = UInt8(x[t])
+    end
+    return out
+end
+
+# (f)(Iset, Jset, Val(M)) of the BatchEvaluator contract (batcheval.jl:67-83)
+function (f::DeviceFunctor{Float64})(leftindexset::Vector{Vector{Int}},
+                                     rightindexset::Vector{Vector{Int}}, ::Val{M}) where {M}
+    nI, nJ = length(leftindexset), length(rightindexset)
+    L = length(f.localdims)
+    nl = nI > 0 ? length(first(leftindexset)) : 0
+    nr = nJ > 0 ? length(first(rightindexset)) : L - nl - M
+    dims = (nI, f.localdims[nl+1:nl+M]..., nJ)
+    out = zeros(Float64, dims...)
+    (nI == 0 || nJ == 0) && return out
+    left = _packindices(leftindexset, nl)
+    right = _packindices(rightindexset, nr)
+    ld = Int32.(f.localdims)
+    mx = Ref{Float64}(0.0)
+    GC.@preserve f left right ld out begin
+        cf = Ref(_cf(f))
+        st = ccall((:tci_pi_eval_f64, libtci), Cint,
+                   (Ptr{Cvoid}, Ref{CFunctor}, Int32, Ptr{Int32}, Ptr{UInt8}, Int64, Int32,
+                    Ptr{UInt8}, Int64, Int32, Ptr{Float64}, Ref{Float64}),
+                   defaultcontext().handle, cf, L, ld, left, nI, nl, right, nJ, nr, out, mx)
+    end
+    throwstatus(st, "tci_pi_eval_f64")
+    return out
+end
+
+(f::DeviceFunctor{Float64})(x::Vector{Int}) =
+    f([x[1:1]], [x[2:end]], Val(0))[1, 1]
+
+# ---------------------------------------------------------------- solves
+struct COptions
+    tolerance::Float64
+    maxbonddim::Int64
+    maxiter::Int32
+    sweepstrategy::Int32
+    normalizeerror::Int32
+    ncheckhistory::Int32
+    maxnglobalpivot::Int32
+    nsearchglobalpivot::Int32
+    tolmarginglobalsearch::Float64
+    strictlynested::Int32
+    checkconvglobalpivot::Int32
+    seed::UInt64
+    chi_cap::Int32
+    reserved::Int32
+end
+
+struct CProblem
+    L::Int32
+    localdims::Ptr{Int32}
+    f::CFunctor
+    npivots::Int32
+    initialpivots::Ptr{Int32}
+    opts::COptions
+end
+
+struct CResult
+    status::Int32
+    niter::Int32
+    L::Int32
+    npivoterrors::Int32
+    maxsamplevalue::Float64
+    workstats::NTuple{6,Float64}
+end
+
+const _sweep = Dict(:backandforth => 0, :forward => 1, :backward => 2)
+
+"""
+    TCI.crossinterpolate2(Float64, f::DeviceFunctor, localdims, initialpivots; kwargs...)
+
+The whole TCI2 solve (optimize! and the final sweep1site!) on the GPU, one
+workgroup per solve.  Returns `(tci::TensorCI2{Float64}, ranks, errors)` like
+the reference (tensorci2.jl:943-953).  The global pivot search draws from the
+library's PRNG (seed keyword) instead of Julia's default RNG.
+"""
+function TCI.crossinterpolate2(::Type{Float64}, f::DeviceFunctor{Float64},
+                               localdims::Union{Vector{Int},NTuple{N,Int}},
+                               initialpivots::Vector{Vector{Int}}=[ones(Int, length(localdims))];
+                               tolerance::Float64=1e-8, maxbonddim::Int=typemax(Int),
+                               maxiter::Int=20, sweepstrategy::Symbol=:backandforth,
+                               normalizeerror::Bool=true, ncheckhistory::Int=3,
+                               maxnglobalpivot::Int=5, nsearchglobalpivot::Int=5,
+                               tolmarginglobalsearch::Float64=10.0, strictlynested::Bool=false,
+                               checkconvglobalpivot::Bool=true, seed::Integer=0,
+                               chi_cap::Integer=0) where {N}
+    L = length(localdims)
+    ld = Int32.(collect(localdims))
+    piv = Int32.(reduce(vcat, initialpivots))
+    opts = COptions(tolerance, maxbonddim, maxiter, _sweep[sweepstrategy], normalizeerror,
+                    ncheckhistory, maxnglobalpivot, nsearchglobalpivot, tolmarginglobalsearch,
+                    strictlynested, checkconvglobalpivot, UInt64(seed), chi_cap, 0)
+    batch = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve f ld piv begin
+        prob = Ref(CProblem(L, pointer(ld), _cf(f), length(initialpivots), pointer(piv), opts))
+        throwstatus(ccall((:tci_solve_submit, libtci), Cint,
+                          (Ptr{Cvoid}, Int32, Ref{CProblem}, Ref{Ptr{Cvoid}}),
+                          defaultcontext().handle, 1, prob, batch), "tci_solve_submit")
+    end
+    b = batch[]
+    try
+        throwstatus(ccall((:tci_batch_wait, libtci), Cint, (Ptr{Cvoid},), b), "tci_batch_wait")
+        r = Ref{CResult}()
+        ccall((:tci_batch_result, libtci), Cint, (Ptr{Cvoid}, Int32, Ref{CResult}), b, 0, r)
+        throwstatus(r[].status, "solve")
+        return _totci(b, r[], collect(localdims))
+    finally
+        ccall((:tci_batch_free, libtci), Cint, (Ptr{Cvoid},), b)
+    end
+end
+
+function _totci(b::Ptr{Cvoid}, r::CResult, localdims::Vector{Int})
+    L = length(localdims)
+    tci = TCI.TensorCI2{Float64}(localdims)
+    for site in 1:L, which in (0, 1)
+        len = which == 0 ? site - 1 : L - site
+        cnt = Ref{Int32}(0)
+        # first call reports the count (and a too-small-buffer status)
+        ccall((:tci_batch_indexset, libtci), Cint,
+              (Ptr{Cvoid}, Int32, Int32, Int32, Ref{Int32}, Ptr{UInt8}, Int64),
+              b, 0, which, site, cnt, C_NULL, 0)
+        buf = Vector{UInt8}(undef, max(Int(cnt[]) * len, 1))
+        throwstatus(ccall((:tci_batch_indexset, libtci), Cint,
+                          (Ptr{Cvoid}, Int32, Int32, Int32, Ref{Int32}, Ptr{UInt8}, Int64),
+                          b, 0, which, site, cnt, buf, length(buf)), "tci_batch_indexset")
+        set = [Int.(buf[(i-1)*len+1:i*len]) for i in 1:cnt[]]
+        which == 0 ? (tci.Iset[site] = set) : (tci.Jset[site] = set)
+    end
+    for site in 1:L
+        d3 = zeros(Int64, 3)
+        ccall((:tci_batch_sitetensor, libtci), Cint,
+              (Ptr{Cvoid}, Int32, Int32, Ptr{Int64}, Ptr{Float64}, Int64), b, 0, site, d3, C_NULL, 0)
+        T = zeros(Float64, d3...)
+        ccall((:tci_batch_sitetensor, libtci), Cint,
+              (Ptr{Cvoid}, Int32, Int32, Ptr{Int64}, Ptr{Float64}, Int64), b, 0, site, d3, T, length(T))
+        tci.sitetensors[site] = T
+    end
+    pe = zeros(Float64, r.npivoterrors)
+    ccall((:tci_batch_pivoterrors, libtci), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}, Int64),
+          b, 0, pe, length(pe))
+    tci.pivoterrors = pe
+    be = zeros(Float64, L - 1)
+    ccall((:tci_batch_bonderrors, libtci), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}), b, 0, be)
+    tci.bonderrors = be
+    tci.maxsamplevalue = r.maxsamplevalue
+    ranks = zeros(Int64, r.niter)
+    errors = zeros(Float64, r.niter)
+    ccall((:tci_batch_history, libtci), Cint,
+          (Ptr{Cvoid}, Int32, Ptr{Int64}, Ptr{Float64}, Ptr{Int64}), b, 0, ranks, errors, C_NULL)
+    return tci, Vector{Int}(ranks), errors
+end
+
+end # module

@@ -18,89 +18,121 @@
 using namespace tci;
 
 struct HostTeam {
-    // register-tile shape of the one-thread team: a 1×1 grid holding up to 160×160
-    static constexpr int NT = 1, TR = 1, RA = 160, CB = 160;
-    int32_t lperm[320];
-    double ltx[160], lty[160];
-    double &txs(int i) { return ltx[i]; }
-    double &tys(int j) { return lty[j]; }
-    int32_t &tperm(int q) { return lperm[q]; }
-    int tid = 0, nt = 1, lane = 0, wsize = 1, wave = 0, nwaves = 1;
+    // register tile of the one-thread team: one wave of one lane holding up to 160×160
+    static constexpr int NT = 1, WS = 1, NW = 1, RA = 160, CB = 160;
+    double lA[1], lX[160], lY[160];
+    uint32_t lB[1];
+    double &tA(int) { return lA[0]; }
+    uint32_t &tB(int) { return lB[0]; }
+    double &tX(int, int r) { return lX[r]; }
+    double &tY(int, int j) { return lY[j]; }
+    int tid() const { return 0; }
+    static constexpr int nt() { return 1; }
+    int lane() const { return 0; }
+    static constexpr int wsize() { return 1; }
+    int wave() const { return 0; }
+    static constexpr int nwaves() { return 1; }
     double *xs = nullptr, *ys = nullptr;
     double *lpi = nullptr;
     int64_t lpi_cap = 0;
     uint8_t *lrows = nullptr, *lcols = nullptr;
     void sync() {}
+    void wave_sync() {}
     ArgBest argmax(ArgBest b) { return b; }
-    ArgBest argmin_key(ArgBest b) { return b; }
     int uniform(int x) const { return x; }
     double max_nan(double v) { return v; }
+    double wave_max(double v) const { return v; }
+    uint32_t wave_min(uint32_t v) const { return v; }
+    bool wave_any(bool p) const { return p; }
+    int wave_first(bool p) const { return p ? 0 : -1; }
     int any(int v) { return v; }
     uint64_t clock() const { return 0; }
 };
 
-// Multi-threaded host team: NT std::threads emulate one workgroup (one
-// thread per lane), with a std::barrier for __syncthreads and shared slots
-// for the reductions, so the device's cross-thread logic runs on the CPU.
+// Multi-threaded host team: NT std::threads emulate one workgroup of NT/64
+// waves (one thread per lane), with std::barriers for __syncthreads and for
+// the wave-level operations and shared slots for the reductions, so the
+// device's cross-lane logic runs on the CPU.
 struct MTShared {
     std::barrier<> *bar;
+    std::vector<std::barrier<> *> wbar;
     std::vector<ArgBest> abuf;
     std::vector<double> dbuf;
     std::vector<int> ibuf;
+    std::vector<uint32_t> ubuf;
 };
 
+template <int NT_, int RA_, int CB_>
 struct MTTile {
-    double tx[160], ty[160];
-    int32_t perm[320];
+    double A[NT_ / 64], X[(NT_ / 64) * 64 * RA_], Y[(NT_ / 64) * CB_];
+    uint32_t B[NT_ / 64];
 };
 
-template <int NT_, int TR_, int RA_, int CB_>
+template <int NT_, int RA_, int CB_>
 struct HostTeamMT {
-    static constexpr int NT = NT_, TR = TR_, RA = RA_, CB = CB_;
-    MTTile *tile = nullptr;
-    double &txs(int i) { return tile->tx[i]; }
-    double &tys(int j) { return tile->ty[j]; }
-    int32_t &tperm(int q) { return tile->perm[q]; }
-    int tid, nt = NT_, lane = 0, wsize = 1, wave = 0, nwaves = 1;
+    static constexpr int NT = NT_, WS = 64, NW = NT_ / 64, RA = RA_, CB = CB_;
+    MTTile<NT_, RA_, CB_> *tile = nullptr;
+    double &tA(int w) { return tile->A[w]; }
+    uint32_t &tB(int w) { return tile->B[w]; }
+    double &tX(int w, int r) { return tile->X[w * 64 * RA + r]; }
+    double &tY(int w, int j) { return tile->Y[w * CB + j]; }
+    int t_tid = 0;
+    int tid() const { return t_tid; }
+    static constexpr int nt() { return NT_; }
+    int lane() const { return t_tid % 64; }
+    static constexpr int wsize() { return 64; }
+    int wave() const { return t_tid / 64; }
+    static constexpr int nwaves() { return NW; }
     double *xs = nullptr, *ys = nullptr, *lpi = nullptr;
     int64_t lpi_cap = 0;
     uint8_t *lrows = nullptr, *lcols = nullptr;
-    int32_t *lperm = nullptr;
     MTShared *sh = nullptr;
     void sync() { sh->bar->arrive_and_wait(); }
-    ArgBest argmax(ArgBest b)
-    {
-        sh->abuf[tid] = b;
-        sync();
-        ArgBest r = sh->abuf[0];
-        for (int t = 1; t < nt; ++t) if (arg_better(sh->abuf[t], r)) r = sh->abuf[t];
-        sync();
-        return r;
-    }
-    ArgBest argmin_key(ArgBest b)
-    {
-        sh->abuf[tid] = b;
-        sync();
-        ArgBest r = sh->abuf[0];
-        for (int t = 1; t < nt; ++t) if (sh->abuf[t].lin < r.lin) r = sh->abuf[t];
-        sync();
-        return r;
-    }
+    void wsync() { sh->wbar[wave()]->arrive_and_wait(); }
+    void wave_sync() { wsync(); }
     double max_nan(double v)
     {
-        sh->dbuf[tid] = v;
+        sh->dbuf[t_tid] = v;
         sync();
         double r = sh->dbuf[0];
-        for (int t = 1; t < nt; ++t) r = jl_max(r, sh->dbuf[t]);
+        for (int t = 1; t < NT_; ++t) r = jl_max(r, sh->dbuf[t]);
         sync();
         return r;
     }
+    double wave_max(double v)
+    {
+        sh->dbuf[t_tid] = v;
+        wsync();
+        double r = sh->dbuf[wave() * 64];
+        for (int t = 1; t < 64; ++t) r = fmax(r, sh->dbuf[wave() * 64 + t]);
+        wsync();
+        return r;
+    }
+    uint32_t wave_min(uint32_t v)
+    {
+        sh->ubuf[t_tid] = v;
+        wsync();
+        uint32_t r = sh->ubuf[wave() * 64];
+        for (int t = 1; t < 64; ++t) r = std::min(r, sh->ubuf[wave() * 64 + t]);
+        wsync();
+        return r;
+    }
+    int wave_first(bool p)
+    {
+        sh->ibuf[t_tid] = p ? 1 : 0;
+        wsync();
+        int r = -1;
+        for (int t = 0; t < 64 && r < 0; ++t) if (sh->ibuf[wave() * 64 + t]) r = t;
+        wsync();
+        return r;
+    }
+    bool wave_any(bool p) { return wave_first(p) >= 0; }
     int any(int v)
     {
-        sh->ibuf[tid] = v;
+        sh->ibuf[t_tid] = v;
         sync();
         int r = 0;
-        for (int t = 0; t < nt; ++t) r |= sh->ibuf[t];
+        for (int t = 0; t < NT_; ++t) r |= sh->ibuf[t];
         sync();
         return r;
     }
@@ -109,49 +141,50 @@ struct HostTeamMT {
 };
 
 // rrlu! through the register tile on NT host threads shaped like the
-// 256-thread device team (16×16 grid, 9×9 tile)
+// 256-thread device team (4 waves of 64 lanes, 2×32 doubles per lane)
 extern "C" int eh_rrlu_mt(int mode, double *A, int m, int n, int64_t maxrank, double reltol,
                           double abstol, int leftorth, int32_t *rowperm, int32_t *colperm,
                           double *err)
 {
-    constexpr int NT = 256;
-    using Team = HostTeamMT<NT, 16, 9, 9>;
-    using Tile = RegLU<Team, 16, 9, 9>;
-    if (m > 144 || n > 144) return -100;
+    constexpr int NT = 256, RA = 2, CB = 32;
+    using Team = HostTeamMT<NT, RA, CB>;
+    using Tile = TileLU<Team, RA, CB>;
+    if (!Tile::fits(m, n)) return -100;
     std::barrier<> bar(NT);
+    std::vector<std::barrier<> *> wb;
+    for (int w = 0; w < NT / 64; ++w) wb.push_back(new std::barrier<>(64));
     MTShared sh;
     sh.bar = &bar;
+    sh.wbar = wb;
     sh.abuf.resize(NT);
     sh.dbuf.resize(NT);
     sh.ibuf.resize(NT);
-    std::vector<double> xs(m + n + 1), ys(m + n + 1);
-    std::vector<int32_t> perm(m + n);
-    MTTile tile;
+    sh.ubuf.resize(NT);
+    MTTile<NT, RA, CB> tile;
+    std::vector<int32_t> rp(m), cp(n);
     int pres = 0;
     double eres = 0;
     auto body = [&](int tid) {
         Team tm;
-        tm.tid = tid;
-        tm.xs = xs.data();
-        tm.ys = ys.data();
+        tm.t_tid = tid;
         tm.sh = &sh;
         tm.tile = &tile;
         Tile R;
         R.setup(tm, m, n);
-        R.load(A, m);
+        R.load(tm, A, m);
         tm.sync();
         double e, st[2];
-        int p = R.factor_lean(tm, maxrank, reltol, abstol, leftorth != 0, nullptr, &e, st, nullptr);
+        int p = R.factor(tm, maxrank, reltol, abstol, leftorth != 0, nullptr, &e, st, nullptr,
+                         rp.data(), cp.data());
         tm.sync();
-        if (tid == 0)
-            for (int q = 0; q < m + n; ++q) perm[q] = tile.perm[q];
         if (tid == 0) { pres = p; eres = e; }
     };
     std::vector<std::thread> th;
     for (int t = 0; t < NT; ++t) th.emplace_back(body, t);
     for (auto &t : th) t.join();
-    for (int i = 0; i < m; ++i) rowperm[i] = perm[i];
-    for (int j = 0; j < n; ++j) colperm[j] = perm[m + j];
+    for (auto *b : wb) delete b;
+    for (int i = 0; i < m; ++i) rowperm[i] = rp[i];
+    for (int j = 0; j < n; ++j) colperm[j] = cp[j];
     *err = eres;
     return pres;
 }
@@ -210,14 +243,13 @@ extern "C" int eh_rrlu(int mode, double *A, int m, int n, int64_t maxrank, doubl
     if (mode == 2) {
         p = rrlu_team(tm, A, m, n, maxrank, reltol, abstol, leftorth != 0, rowperm, colperm, pivval, err, st);
     } else {
-        using Tile = RegLU<HostTeam, HostTeam::TR, HostTeam::RA, HostTeam::CB>;
-        if (!Tile::fits(tm, m, n)) return -100;
+        using Tile = TileLU<HostTeam, HostTeam::RA, HostTeam::CB>;
+        if (!Tile::fits(m, n)) return -100;
         Tile *R = new Tile();
         R->setup(tm, m, n);
-        R->load(A, m);
-        p = R->factor_lean(tm, maxrank, reltol, abstol, leftorth != 0, pivval, err, st, nullptr);
-        for (int i = 0; i < m; ++i) rowperm[i] = tm.lperm[i];
-        for (int j = 0; j < n; ++j) colperm[j] = tm.lperm[m + j];
+        R->load(tm, A, m);
+        p = R->factor(tm, maxrank, reltol, abstol, leftorth != 0, pivval, err, st, nullptr,
+                      rowperm, colperm);
         delete R;
     }
     return p;

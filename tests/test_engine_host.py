@@ -91,11 +91,51 @@ def test_tile_and_stream_rrlu_equal_oracle(seed):
 
 @pytest.mark.parametrize("seed", range(2))
 def test_tile_rrlu_multithreaded_equals_oracle(seed):
-    """256 host threads in the device team's 16×16 grid with 9×9 tiles."""
-    for A, lo, rt in _rrlu_cases(100 + seed, 144):
+    """256 host threads shaped like a 4-wave device team (2×32 doubles per lane, 128×128)."""
+    for A, lo, rt in _rrlu_cases(100 + seed, 128):
         b = O.rrlu(A, reltol=rt, leftorthogonal=lo)
         for mode in (0, 1):
             a = eh.rrlu_mt(mode, A, reltol=rt, leftorthogonal=lo)
             assert a["p"] == b.npivot, (mode, A.shape)
             assert np.array_equal(a["rowperm"], b.rowpermutation)
             assert np.array_equal(a["colperm"], b.colpermutation)
+
+
+def _nan_cases(seed):
+    rng = np.random.default_rng(seed)
+    for t in range(8):
+        m, n = (int(x) for x in rng.integers(1, 41, 2))
+        A = rng.integers(-2, 3, (m, n)).astype(float)
+        if t % 4 == 0:
+            A[rng.integers(0, m), :] = np.nan                   # a NaN row
+        elif t % 4 == 1:
+            A[rng.random((m, n)) < 0.1] = np.nan                 # scattered NaNs
+        elif t % 4 == 2:
+            A[:] = np.nan                                        # nothing to pick
+        else:
+            A[:, rng.integers(0, n)] = np.nan                    # a NaN column
+        yield A, bool(t & 1)
+
+
+def _oracle_status(A, lo):
+    try:
+        b = O.rrlu(A, leftorthogonal=lo)
+        return 0, b
+    except O.OracleError as e:
+        return e.code, None
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_tile_rrlu_nan_verdicts_equal_oracle(seed):
+    """NaN entries: the same error (lu.L / lu.U contains NaNs) or the same
+    pivots as the reference restatement, on both host teams."""
+    for A, lo in _nan_cases(300 + seed):
+        st, b = _oracle_status(A, lo)
+        for a in (eh.rrlu(0, A, leftorthogonal=lo), eh.rrlu_mt(0, A, leftorthogonal=lo),
+                  eh.rrlu(2, A, leftorthogonal=lo)):
+            if st:
+                assert a["p"] == -st, (A.shape, lo, st, a["p"])
+            else:
+                assert a["p"] == b.npivot
+                assert np.array_equal(a["rowperm"], b.rowpermutation)
+                assert np.array_equal(a["colperm"], b.colpermutation)
