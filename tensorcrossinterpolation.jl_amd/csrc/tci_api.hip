@@ -59,7 +59,7 @@ struct tci_ctx {
     uint64_t budget;
     uint64_t used;
     std::vector<PoolBuf> pool;
-    int lds_max;        // dynamic LDS bytes a workgroup may use
+    int lds_wg;         // LDS bytes (static + dynamic) a solve workgroup may use
     int team;           // threads per solve workgroup
     int pi_lds_kb;      // LDS budget for Pi matrices (KiB)
 };
@@ -67,11 +67,11 @@ struct tci_ctx {
 // Launch shape of the solve kernel: a team of `team` threads per solve; LDS
 // holds the rrLU staging vectors, the index tables and as much of a Pi
 // matrix as the budget allows (one resident workgroup per CU when large).
-static LdsPlan make_plan(const tci_ctx *c, int max_mrcap, int max_tab)
+static LdsPlan make_plan(const tci_ctx *c, int64_t left, int max_mrcap, int max_tab)
 {
     LdsPlan p;
     memset(&p, 0, sizeof p);
-    int64_t left = c->lds_max;
+    if (left < 0) left = 0;
     if ((int64_t)2 * max_mrcap * 8 <= left) {
         p.stage_cap = max_mrcap;
         left -= (int64_t)2 * max_mrcap * 8;
@@ -159,9 +159,9 @@ tci_status tci_ctx_create(tci_ctx **out, int32_t device, uint64_t hbm_budget_byt
         hbm_budget_bytes = (uint64_t)(0.8 * (double)fr);
     }
     c->budget = hbm_budget_bytes;
-    // dynamic LDS per solve workgroup: the CU's LDS shared by the workgroups
-    // that should be resident together (TCI_WG_PER_CU, default 2), less the
-    // solve kernel's static LDS (below 16 KiB)
+    // LDS per solve workgroup: the CU's LDS shared by the workgroups that
+    // should be resident together (TCI_WG_PER_CU, default 1), at most the
+    // per-workgroup limit; the kernel's static LDS is subtracted at launch
     int lds_blk = 0, lds_cu = 0;
     HIPCHK(hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     HIPCHK(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device));
@@ -170,8 +170,7 @@ tci_status tci_ctx_create(tci_ctx **out, int32_t device, uint64_t hbm_budget_byt
     if (wgs < 1) wgs = 1;
     int per_wg = (lds_cu > 0 ? lds_cu : lds_blk) / wgs;
     if (per_wg > lds_blk) per_wg = lds_blk;
-    c->lds_max = per_wg - 16384;
-    if (c->lds_max < 0) c->lds_max = 0;
+    c->lds_wg = per_wg;
     ev = getenv("TCI_TEAM");
     c->team = ev ? atoi(ev) : 512;
     if (c->team != 512 && c->team != 1024 && c->team != 5122) c->team = 512;
@@ -483,7 +482,9 @@ tci_status tci_solve_submit(tci_ctx *ctx, int32_t count, const tci_problem *prob
     if ((e = hipEventCreate(&b->ev0)) != hipSuccess) return fail(e, "hipEventCreate");
     if ((e = hipEventCreate(&b->ev1)) != hipSuccess) return fail(e, "hipEventCreate");
     if ((e = hipEventRecord(b->ev0, st)) != hipSuccess) return fail(e, "hipEventRecord");
-    const LdsPlan plan = make_plan(ctx, max_mr, max_tab);
+    const int stat = solve_static_lds(ctx->team, fkind);
+    if (stat < 0) return fail(hipErrorInvalidValue, "solve kernel attributes");
+    const LdsPlan plan = make_plan(ctx, (int64_t)ctx->lds_wg - stat, max_mr, max_tab);
     if ((e = launch_solve(b->d_desc, count, ctx->team, fkind, plan, st)) != hipSuccess) return fail(e, "solve launch");
     if ((e = hipEventRecord(b->ev1, st)) != hipSuccess) return fail(e, "hipEventRecord");
     *out = b;

@@ -83,12 +83,13 @@ struct DevTeam {
         const unsigned long long b = __ballot(p);
         return b ? (int)__builtin_ctzll(b) : -1;
     }
-    // orders this wave's LDS accesses (a wave's LDS operations complete in order)
+    // makes this wave's earlier LDS and global writes visible to its later
+    // reads in any lane (no barrier with the other waves)
     __device__ void wave_sync() const
     {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
     // thread coordinates come from the hardware registers, so a team passed
     // by reference costs no memory traffic to read them

@@ -545,31 +545,48 @@ struct Engine {
     // and X may live in LDS; X (r×c) is the result.
     ENG int solve_right(const double *Pi1, int r, double *A, int c, double *X)
     {
-        if (tm.tid() == 0) {
+        // LU with partial pivoting of A = Pᵀ by wave 0: lanes stride over rows
+        // for the pivot search, the scaling and the update (each element sees
+        // A[i,j] - A[i,k]·A[k,j] once per k, as in the serial elimination)
+        // and over columns for the row swap
+        if (tm.wave() == 0) {
             int singular = 0;
             for (int k = 0; k < c; ++k) {
+                // first maximum |A[i,k]| over i >= k (strictly greater wins, a
+                // NaN never does; a NaN diagonal keeps row k)
+                const double akk = fabs(A[k + (int64_t)c * k]);
                 int pr = k;
-                double bestv = fabs(A[k + (int64_t)c * k]);
-                for (int i = k + 1; i < c; ++i) {
-                    double v = fabs(A[i + (int64_t)c * k]);
-                    if (v > bestv) { bestv = v; pr = i; }
+                if (akk == akk) {
+                    double lv = -1.0;
+                    uint32_t li = 0xffffffffu;
+                    for (int i = k + tm.lane(); i < c; i += tm.wsize()) {
+                        const double v = fabs(A[i + (int64_t)c * k]);
+                        if (v > lv) { lv = v; li = (uint32_t)i; }
+                    }
+                    const double g = tm.wave_max(lv);
+                    pr = (int)tm.wave_min(lv == g ? li : 0xffffffffu);
                 }
-                D.ipiv[k] = pr;
+                if (tm.lane() == 0) D.ipiv[k] = pr;
                 if (pr != k)
-                    for (int j = 0; j < c; ++j) {
-                        double x = A[k + (int64_t)c * j];
+                    for (int j = tm.lane(); j < c; j += tm.wsize()) {
+                        const double x = A[k + (int64_t)c * j];
                         A[k + (int64_t)c * j] = A[pr + (int64_t)c * j];
                         A[pr + (int64_t)c * j] = x;
                     }
+                tm.wave_sync();
                 const double d = A[k + (int64_t)c * k];
                 if (d == 0.0) singular = 1;
-                for (int i = k + 1; i < c; ++i) A[i + (int64_t)c * k] = A[i + (int64_t)c * k] / d;
-                for (int j = k + 1; j < c; ++j) {
-                    const double u = A[k + (int64_t)c * j];
-                    for (int i = k + 1; i < c; ++i) A[i + (int64_t)c * j] = A[i + (int64_t)c * j] - A[i + (int64_t)c * k] * u;
+                for (int i = k + 1 + tm.lane(); i < c; i += tm.wsize())
+                    A[i + (int64_t)c * k] = A[i + (int64_t)c * k] / d;
+                tm.wave_sync();
+                for (int i = k + 1 + tm.lane(); i < c; i += tm.wsize()) {
+                    const double l = A[i + (int64_t)c * k];
+                    for (int j = k + 1; j < c; ++j)
+                        A[i + (int64_t)c * j] = A[i + (int64_t)c * j] - l * A[k + (int64_t)c * j];
                 }
+                tm.wave_sync();
             }
-            S.cnt = singular;
+            if (tm.lane() == 0) S.cnt = singular;
         }
         tm.sync();
         if (S.cnt) return ST_SINGULAR;

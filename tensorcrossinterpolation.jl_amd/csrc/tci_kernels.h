@@ -15,6 +15,8 @@ struct LdsPlan {
     int64_t pi_cap;      // doubles available for a Pi matrix (0: in HBM)
 };
 size_t lds_plan_bytes(const LdsPlan &p);
+// static LDS bytes of the solve kernel variant for (team, functor kind), -1 on error
+int solve_static_lds(int team, int fkind);
 hipError_t launch_solve(const SolveDesc *descs_device, int count, int team, int fkind,
                         const LdsPlan &plan, hipStream_t st);
 hipError_t launch_rrlu(tci_lu_job *jobs_device, int64_t count, int64_t max_side, double *gstage,

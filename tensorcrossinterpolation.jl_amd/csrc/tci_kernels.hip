@@ -162,6 +162,34 @@ size_t lds_plan_bytes(const LdsPlan &p)
     return (size_t)2 * p.stage_cap * sizeof(double) + 2 * (size_t)p.tab_cap + (size_t)p.pi_cap * sizeof(double);
 }
 
+// the kernel variant a (team, functor kind) pair runs
+static const void *solve_kernel_ptr(int team, int fkind)
+{
+    switch (team) {
+    case 1024:
+        return fkind == TCI_F_LORENTZ_INT ? (const void *)tci_solve_kernel<1024, TCI_F_LORENTZ_INT, 4>
+                                          : (const void *)tci_solve_kernel<1024, 0, 4>;
+    case 5122:
+        return fkind == TCI_F_LORENTZ_INT ? (const void *)tci_solve_kernel<512, TCI_F_LORENTZ_INT, 4>
+                                          : (const void *)tci_solve_kernel<512, 0, 4>;
+    default:
+        switch (fkind) {
+        case TCI_F_LORENTZ_INT: return (const void *)tci_solve_kernel<512, TCI_F_LORENTZ_INT, 2>;
+        case TCI_F_LORENTZ_WEIGHTED: return (const void *)tci_solve_kernel<512, TCI_F_LORENTZ_WEIGHTED, 2>;
+        case TCI_F_QUANTICS_GAUSS_COS: return (const void *)tci_solve_kernel<512, TCI_F_QUANTICS_GAUSS_COS, 2>;
+        case TCI_F_GAUSS_SUM: return (const void *)tci_solve_kernel<512, TCI_F_GAUSS_SUM, 2>;
+        default: return (const void *)tci_solve_kernel<512, 0, 2>;
+        }
+    }
+}
+
+int solve_static_lds(int team, int fkind)
+{
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, solve_kernel_ptr(team, fkind)) != hipSuccess) return -1;
+    return (int)a.sharedSizeBytes;
+}
+
 template <int NT, int FK, int WPE>
 static void launch_one(const SolveDesc *descs, int count, const LdsPlan &plan, size_t lds, hipStream_t st)
 {
@@ -188,6 +216,14 @@ hipError_t launch_solve(const SolveDesc *descs, int count, int team, int fkind, 
                         hipStream_t st)
 {
     const size_t lds = lds_plan_bytes(plan);
+    // a launch whose LDS (static + dynamic) exceeds the workgroup limit is
+    // refused here, never sent to the device
+    int lds_blk = 0, dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e != hipSuccess) return e;
+    const int stat = solve_static_lds(team, fkind);
+    if (stat < 0 || (size_t)stat + lds > (size_t)lds_blk) return hipErrorInvalidValue;
     // team shapes: 512 threads with up to 256 VGPRs (one solve per CU, the
     // default, every functor specialized); 1024 threads, or 512 threads with
     // two workgroups per CU, at 128 VGPRs (Lorentzian specialized)

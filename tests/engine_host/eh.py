@@ -15,6 +15,8 @@ def lib():
         vp = C.c_void_p
         L.eh_solve.restype = vp
         L.eh_solve.argtypes = [vp, C.POINTER(C.c_int)]
+        L.eh_solve_mt.restype = vp
+        L.eh_solve_mt.argtypes = [vp, C.POINTER(C.c_int)]
         L.eh_last_error.restype = C.c_char_p
         for nm in ("eh_free",):
             getattr(L, nm).argtypes = [vp]
@@ -42,10 +44,11 @@ def lib():
 
 
 class EngineSolve:
-    def __init__(self, holder):
+    def __init__(self, holder, threads=1):
         st = C.c_int()
         self.L = holder.L
-        self.h = lib().eh_solve(C.byref(holder.st), C.byref(st))
+        fn = lib().eh_solve_mt if threads > 1 else lib().eh_solve
+        self.h = fn(C.byref(holder.st), C.byref(st))
         self.status = st.value
 
     def __del__(self):

@@ -90,6 +90,15 @@ struct HostTeamMT {
     void sync() { sh->bar->arrive_and_wait(); }
     void wsync() { sh->wbar[wave()]->arrive_and_wait(); }
     void wave_sync() { wsync(); }
+    ArgBest argmax(ArgBest b)
+    {
+        sh->abuf[t_tid] = b;
+        sync();
+        ArgBest r = sh->abuf[0];
+        for (int t = 1; t < NT_; ++t) if (arg_better(sh->abuf[t], r)) r = sh->abuf[t];
+        sync();
+        return r;
+    }
     double max_nan(double v)
     {
         sh->dbuf[t_tid] = v;
@@ -224,6 +233,60 @@ extern "C" void *eh_solve(const tci_problem *p, int *status)
     h->status = e->run();
     delete e;
     *status = h->status;
+    return h;
+}
+
+// The whole solve on 256 host threads shaped like a 4-wave device team, so
+// the engine's cross-thread logic (wave reductions, staging, the parallel
+// LU of Pᵀ) is exercised on the CPU
+extern "C" void *eh_solve_mt(const tci_problem *p, int *status)
+{
+    constexpr int NT = 256, RA = 2, CB = 32;
+    using Team = HostTeamMT<NT, RA, CB>;
+    SolveCaps c;
+    int st = check_problem(*p, c, &g_msg);
+    if (st != TCI_OK) { *status = st; return nullptr; }
+    EhSolve *h = new EhSolve();
+    size_t ib, sb, rb;
+    layout_solve(c, nullptr, nullptr, nullptr, nullptr, &ib, &sb, &rb);
+    h->inputs.assign(ib, 0);
+    h->arena.assign(sb, 0);
+    h->report.assign(rb, 0);
+    memset(&h->D, 0, sizeof(SolveDesc));
+    layout_solve(c, &h->D, h->inputs.data(), h->arena.data(), h->report.data(), &ib, &sb, &rb);
+    fill_desc_scalars(*p, h->D);
+    pack_inputs(*p, c, h->D, h->inputs.data(), h->inputs.data());
+    std::barrier<> bar(NT);
+    std::vector<std::barrier<> *> wb;
+    for (int w = 0; w < NT / 64; ++w) wb.push_back(new std::barrier<>(64));
+    MTShared sh;
+    sh.bar = &bar;
+    sh.wbar = wb;
+    sh.abuf.resize(NT);
+    sh.dbuf.resize(NT);
+    sh.ibuf.resize(NT);
+    sh.ubuf.resize(NT);
+    MTTile<NT, RA, CB> tile;
+    EngShared S;
+    memset(&S, 0, sizeof S);
+    int res = 0;
+    auto body = [&](int tid) {
+        Team tm;
+        tm.t_tid = tid;
+        tm.sh = &sh;
+        tm.tile = &tile;
+        tm.xs = h->D.stage;
+        tm.ys = h->D.stage + c.mrcap;
+        Engine<Team> e(tm, h->D, S);
+        int r = e.run();
+        if (tid == 0) res = r;
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < NT; ++t) th.emplace_back(body, t);
+    for (auto &t : th) t.join();
+    for (auto *b : wb) delete b;
+    h->status = res;
+    *status = res;
     return h;
 }
 

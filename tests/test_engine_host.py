@@ -19,10 +19,10 @@ from conftest import load_pkg  # noqa: E402
 _abi = load_pkg()._abi
 
 
-def _solve_both(spec):
+def _solve_both(spec, threads=1):
     name, kind, params, ip, ld, piv, opts = spec
     h = _abi.ProblemHolder(ld, kind, params, ip, piv, **opts)
-    e = eh.EngineSolve(h)
+    e = eh.EngineSolve(h, threads)
     try:
         t, r, er = O.crossinterpolate2(O.Functor(kind, params, ip), ld, piv, **oracle_opts(opts))
         ost = 0
@@ -31,9 +31,18 @@ def _solve_both(spec):
     return e, t, ost
 
 
-@pytest.mark.parametrize("spec", specs(), ids=lambda s: s[0])
-def test_engine_equals_oracle(spec):
-    e, t, ost = _solve_both(spec)
+# the 256-thread emulation is slow (std::barrier per reduction): a subset
+_MT = {"lorentz5_pivots", "lorentz6_backward_strict", "weighted_lorentz", "gauss_sum",
+       "quantics_exp_small_maxbond", "random_tt", "diag_table"}
+_CASES = [pytest.param(s, 1, id=s[0] + "-1") for s in specs()] + \
+         [pytest.param(s, 256, id=s[0] + "-256") for s in specs() if s[0] in _MT]
+
+
+@pytest.mark.parametrize("spec,threads", _CASES)
+def test_engine_equals_oracle(spec, threads):
+    """The engine on a 1-thread team and on 256 threads shaped like a
+    4-wave device team equals the oracle bit for bit."""
+    e, t, ost = _solve_both(spec, threads)
     assert e.status == ost
     if ost:
         return
