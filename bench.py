@@ -106,6 +106,9 @@ def main():
     ap.add_argument("--plain", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--served", type=int, default=8192, help="requests in the served (Poisson) run; 0: skip")
+    ap.add_argument("--served-load", type=float, default=0.5, help="arrival rate as a fraction of batched capacity")
+    ap.add_argument("--streams", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -173,6 +176,23 @@ def main():
                      "kernel": "tci_solve_kernel", "kernel_ms": round(kms_avg, 3),
                      "bytes_per_solve": bytes_per_solve},
     }
+    if args.served > 0:
+        # served: Poisson arrivals of the same request at a fraction of the
+        # batched capacity, admitted by continuous batching over several streams
+        from importlib import import_module
+        serving = import_module("tci_amd.serving")
+        rate = args.served_load * value
+        f = tci.Lorentz(1.0)
+        reqs = [(f, [10] * 8, None, dict(tolerance=1e-8, chi_cap=32))] * args.served
+        arrivals = serving.poisson_arrivals(args.served, rate, seed=5)
+        srv = serving.TCIServer(ctx, streams=args.streams, max_batch=B)
+        res, st = srv.run(reqs, arrivals)
+        bad = sum(r.status != 0 for r in res)
+        out["served"] = {"requests": st.n, "arrival_rate_per_s": round(rate, 1),
+                         "solves_per_s": round(st.solves_per_s, 2), "p50_ms": round(st.p50_ms, 3),
+                         "p99_ms": round(st.p99_ms, 3), "batches": st.batches,
+                         "mean_batch": round(st.mean_batch, 1), "streams": args.streams, "failed": bad,
+                         "note": "latency from arrival to results on the host"}
     if args.cpu_seconds > 0:
         thr = args.cpu_threads or min(16, os.cpu_count() or 1)
         cps, n, dt, lat1 = cpu_baseline(thr, args.cpu_seconds)

@@ -50,6 +50,11 @@ const char *tci_version(void);
 tci_status tci_ctx_create(tci_ctx **out, int32_t device, uint64_t hbm_budget_bytes);
 tci_status tci_ctx_destroy(tci_ctx *ctx);
 tci_status tci_ctx_synchronize(tci_ctx *ctx);
+/* Serving (no counterpart in the reference, whose solves run one at a time):
+ * give the context n HIP streams (default-priority) so that batches submitted
+ * while others run execute concurrently.  tci_solve_submit puts a batch on
+ * an idle stream when there is one.  The HBM budget is shared by all. */
+tci_status tci_ctx_set_streams(tci_ctx *ctx, int32_t n);
 
 /* rrLU ----------------------------------------------------------------- */
 /* Replaces rrlu(A; maxrank, reltol, abstol, leftorthogonal)::rrLU{Float64}

@@ -55,7 +55,9 @@ struct PoolBuf {
 
 struct tci_ctx {
     int device;
-    hipStream_t stream;
+    hipStream_t stream;                 // synchronous calls and the first solve stream
+    std::vector<hipStream_t> extra;     // more solve streams (tci_ctx_set_streams)
+    size_t next = 0;
     uint64_t budget;
     uint64_t used;
     std::vector<PoolBuf> pool;
@@ -124,6 +126,7 @@ struct tci_batch {
     std::vector<uint8_t> h_rep;
     bool fetched = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipStream_t stream = nullptr;
 };
 
 extern "C" {
@@ -185,6 +188,10 @@ tci_status tci_ctx_destroy(tci_ctx *ctx)
     if (!ctx) return TCI_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    for (hipStream_t s : ctx->extra) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+    }
     for (PoolBuf &b : ctx->pool) (void)hipFree(b.ptr);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -195,7 +202,43 @@ tci_status tci_ctx_synchronize(tci_ctx *ctx)
 {
     if (!ctx) return set_err(TCI_ERR_ARG, "ctx is NULL");
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (hipStream_t s : ctx->extra) HIPCHK(hipStreamSynchronize(s));
     return TCI_OK;
+}
+
+tci_status tci_ctx_set_streams(tci_ctx *ctx, int32_t n)
+{
+    if (!ctx || n < 1 || n > 64) return set_err(TCI_ERR_ARG, "need 1 <= n <= 64 streams");
+    HIPCHK(hipSetDevice(ctx->device));
+    while ((int)ctx->extra.size() > n - 1) {
+        HIPCHK(hipStreamSynchronize(ctx->extra.back()));
+        HIPCHK(hipStreamDestroy(ctx->extra.back()));
+        ctx->extra.pop_back();
+    }
+    while ((int)ctx->extra.size() < n - 1) {
+        hipStream_t s;
+        HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        ctx->extra.push_back(s);
+    }
+    ctx->next = 0;
+    return TCI_OK;
+}
+
+// the stream a new batch goes to: an idle one if any, else round robin
+static hipStream_t pick_stream(tci_ctx *ctx)
+{
+    const size_t ns = 1 + ctx->extra.size();
+    for (size_t t = 0; t < ns; ++t) {
+        const size_t q = (ctx->next + t) % ns;
+        hipStream_t s = q == 0 ? ctx->stream : ctx->extra[q - 1];
+        if (hipStreamQuery(s) == hipSuccess) {
+            ctx->next = (q + 1) % ns;
+            return s;
+        }
+    }
+    const size_t q = ctx->next % ns;
+    ctx->next = (q + 1) % ns;
+    return q == 0 ? ctx->stream : ctx->extra[q - 1];
 }
 
 // ------------------------------------------------------------------- rrLU
@@ -475,7 +518,8 @@ tci_status tci_solve_submit(tci_ctx *ctx, int32_t count, const tci_problem *prob
         ost += sb[(size_t)i];
         orp += rb[(size_t)i];
     }
-    hipStream_t st = ctx->stream;
+    hipStream_t st = pick_stream(ctx);
+    b->stream = st;
     if ((e = hipMemcpyAsync(b->d_in, himg, tin, hipMemcpyHostToDevice, st)) != hipSuccess) return fail(e, "upload inputs");
     if ((e = hipMemcpyAsync(b->d_desc, b->desc.data(), sizeof(SolveDesc) * (size_t)count, hipMemcpyHostToDevice, st)) != hipSuccess)
         return fail(e, "upload descriptors");

@@ -37,6 +37,22 @@ ENG inline bool arg_better(const ArgBest &a, const ArgBest &b)
     return a.v > b.v || (a.v == b.v && a.lin < b.lin);
 }
 
+// bit casts usable on host and device
+ENG_INLINE int64_t eng_as_i64(double x)
+{
+    int64_t i;
+    __builtin_memcpy(&i, &x, sizeof i);
+    return i;
+}
+ENG_INLINE double eng_as_f64(int64_t i)
+{
+    double x;
+    __builtin_memcpy(&x, &i, sizeof x);
+    return x;
+}
+
+ENG_INLINE uint32_t eng_umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
+
 // Base.max on Float64: NaN-propagating
 ENG inline double jl_max(double a, double b)
 {

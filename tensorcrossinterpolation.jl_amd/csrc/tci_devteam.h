@@ -83,6 +83,13 @@ struct DevTeam {
         const unsigned long long b = __ballot(p);
         return b ? (int)__builtin_ctzll(b) : -1;
     }
+    // as wave_sync, for LDS only (does not wait for global memory operations)
+    __device__ void wave_sync_lds() const
+    {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    }
     // makes this wave's earlier LDS and global writes visible to its later
     // reads in any lane (no barrier with the other waves)
     __device__ void wave_sync() const

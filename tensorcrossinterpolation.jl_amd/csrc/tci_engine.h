@@ -15,8 +15,9 @@
 //   tm.lrows, tm.lcols       on-chip index tables (null: use the HBM ones)
 //   Team::NT, WS, NW         team size, wave width, waves (compile-time)
 //   tm.uniform(x)            x as a wave-uniform value (all lanes hold the same x)
-//   tm.wave_max/min/any/first, tm.wave_sync()   wave-level reductions, ballot
-//                            and an LDS ordering point inside one wave
+//   tm.wave_max/min/any/first, tm.wave_sync(), tm.wave_sync_lds()
+//                            wave-level reductions, ballot, and ordering points
+//                            (LDS and global, or LDS only) inside one wave
 //   tm.tA(w), tB(w), tX(w, r), tY(w, j)  LDS slots of the register-tile rrLU
 //   Team::RA, CB             register-tile shape (tci_regtile.h); RA = 0: none
 // The device implementation lives in tci_kernels.hip.  Every parallel loop

@@ -116,18 +116,47 @@ struct TileLU {
         uint32_t kw = 0xffffffffu;
         if (g >= 0.0) {
             const double G = g * g;
-            const bool mine = lmax * lmax == G;
+            // {x >= 0 : x·x == G} is an interval of doubles [lo, g] (rounding
+            // keeps squaring monotone): find lo by stepping down from g, so
+            // the per-entry test is |v| >= lo.  G == 0 means every entry is 0
+            // in magnitude or squares to 0; G subnormal may span many doubles,
+            // then the exact test v·v == G is used.
+            double lo = 0.0;
+            bool exact = false;
+            if (G > 0.0) {
+                lo = g;
+                int t = 0;
+                for (; t < 8; ++t) {
+                    const double q = eng_as_f64(eng_as_i64(lo) - 1);
+                    if (q * q == G) lo = q;
+                    else break;
+                }
+                exact = t == 8;
+            }
+            const bool mine = exact ? lmax * lmax == G : lmax >= lo;
             if (tm.wave_any(mine)) {
                 uint32_t kl = 0xffffffffu;
                 if (mine) {
+                    if (!exact) {
 #pragma unroll
-                    for (int j = 0; j < CB; ++j) {
-                        const uint32_t ck = (uint32_t)cp[j] << 8;
+                        for (int j = 0; j < CB; ++j) {
+                            const uint32_t ck = (uint32_t)cp[j] << 8;
 #pragma unroll
-                        for (int i = 0; i < RA; ++i) {
-                            const double v = a[i][j];
-                            const uint32_t key = ck | (uint32_t)rp[i];
-                            kl = (v * v == G && key < kl) ? key : kl;
+                            for (int i = 0; i < RA; ++i) {
+                                const uint32_t key = ck | (uint32_t)rp[i];
+                                kl = eng_umin(kl, fabs(a[i][j]) >= lo ? key : 0xffffffffu);
+                            }
+                        }
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < CB; ++j) {
+                            const uint32_t ck = (uint32_t)cp[j] << 8;
+#pragma unroll
+                            for (int i = 0; i < RA; ++i) {
+                                const double v = a[i][j];
+                                const uint32_t key = ck | (uint32_t)rp[i];
+                                kl = eng_umin(kl, v * v == G ? key : 0xffffffffu);
+                            }
                         }
                     }
                 }
@@ -216,7 +245,7 @@ struct TileLU {
                 x[i] = live ? v : NaN;
             }
             // y: the pivot row, broadcast from its holder lane in this wave
-            // through the wave's LDS slot, read column by column in the update
+            // through the wave's LDS slot
             if (cand) {
                 const int lo = orow % WS, io = orow / WS;
                 if (tm.lane() == lo) {
@@ -227,22 +256,24 @@ struct TileLU {
                             for (int j = 0; j < CB; ++j) tm.tY(tm.wave(), j) = a[i][j];
                         }
                 }
-                tm.wave_sync();
+                tm.wave_sync_lds();
                 if (!leftorth) {
                     for (int j = tm.lane(); j < CB; j += WS) tm.tY(tm.wave(), j) = tm.tY(tm.wave(), j) / av;
-                    tm.wave_sync();
+                    tm.wave_sync_lds();
                 }
             }
-            // trailing update and the next search
+            // trailing update and the next search; the multipliers are read
+            // from LDS up front (unconditional, in-bounds reads) so their
+            // latency overlaps
+            double yv[CB];
+#pragma unroll
+            for (int j = 0; j < CB; ++j) yv[j] = tm.tY(tm.wave(), j);
             lmax = -1.0;
 #pragma unroll
             for (int j = 0; j < CB; ++j) {
                 const bool live = cp[j] > k && cp[j] < n;
-                double yj = NaN;
-                if (live) {
-                    yj = cand ? tm.tY(tm.wave(), j) : NaN;
-                    if (yj != yj) nany = 1;
-                }
+                const double yj = (cand && live) ? yv[j] : NaN;
+                if (live && yj != yj) nany = 1;
 #pragma unroll
                 for (int i = 0; i < RA; ++i) {
                     const double t = x[i] * yj;

@@ -1,0 +1,147 @@
+"""Served TCI: continuous batching of independent crossinterpolate2 requests.
+
+Requests arrive over time (an arrival time per request).  The loop admits
+every request that has arrived into a new batch as soon as one of the
+context's streams is free and the HBM budget allows, so requests that arrive
+while other batches are on the GPU start without waiting for them; a batch's
+solves run one per workgroup and each solve keeps its pivot sets, Pi buffers
+and site tensors resident on the device until it finishes.  Latency is
+measured from a request's arrival to its results being on the host.
+
+The reference runs one solve per call (tensorci2.jl:943-953); this module has
+no counterpart there.  A solve's result does not depend on what else is in its
+batch or on the GPU at the same time (each solve is one workgroup with its own
+state), which tests/test_gpu_serving.py checks.
+"""
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+
+
+@dataclass
+class ServedResult:
+    status: int
+    ranks: np.ndarray
+    errors: np.ndarray
+    maxsamplevalue: float
+    arrival: float
+    done: float
+    tci: object = None          # TensorCI2 when keep_tci
+
+    @property
+    def latency(self):
+        return self.done - self.arrival
+
+
+@dataclass
+class ServeStats:
+    n: int
+    wall_s: float
+    solves_per_s: float
+    p50_ms: float
+    p99_ms: float
+    batches: int
+    mean_batch: float
+    oom_waits: int = 0
+    latencies_ms: list = field(default_factory=list)
+
+
+class TCIServer:
+    """Continuous-batching front end over one context.
+
+    streams: batches that may run on the GPU at the same time.
+    max_batch: most requests admitted into one batch."""
+
+    def __init__(self, ctx, streams=4, max_batch=4096, idle_sleep_s=20e-6):
+        self.ctx = ctx
+        self.streams = int(streams)
+        self.max_batch = int(max_batch)
+        self.idle_sleep_s = idle_sleep_s
+        st = _abi.lib().tci_ctx_set_streams(ctx.handle, self.streams)
+        if st:
+            raise RuntimeError(_abi.lib().tci_last_error().decode())
+
+    def run(self, requests, arrivals, keep_tci=False):
+        """requests[i] = (f, localdims, initialpivots_or_None, kwargs); arrivals
+        sorted, seconds from the start.  Returns (results, ServeStats)."""
+        from . import TensorCI2, _holder
+        lib = _abi.lib()
+        n = len(requests)
+        holders = [_holder(f, ld, piv, kw) for f, ld, piv, kw in requests]
+        results = [None] * n
+        inflight = []                       # (batch handle, [request ids], arrays kept alive)
+        nxt = 0
+        nbatches = 0
+        oom_waits = 0
+        t0 = time.perf_counter()
+        while nxt < n or inflight:
+            now = time.perf_counter() - t0
+            progressed = False
+            # admit everything that has arrived, when a stream is free
+            if nxt < n and arrivals[nxt] <= now and len(inflight) < self.streams:
+                hi = nxt
+                while hi < n and arrivals[hi] <= now and hi - nxt < self.max_batch:
+                    hi += 1
+                cnt = hi - nxt
+                while cnt > 0:
+                    arr = (_abi.tci_problem * cnt)(*[h.st for h in holders[nxt:nxt + cnt]])
+                    b = C.c_void_p()
+                    st = lib.tci_solve_submit(self.ctx.handle, cnt, arr, C.byref(b))
+                    if st == 0:
+                        inflight.append((b, list(range(nxt, nxt + cnt)), arr))
+                        nxt += cnt
+                        nbatches += 1
+                        progressed = True
+                        break
+                    if st == 11 and inflight:       # HBM budget: wait for a batch to finish
+                        oom_waits += 1
+                        break
+                    if st == 11 and cnt > 1:        # nothing running: admit fewer
+                        cnt //= 2
+                        continue
+                    raise RuntimeError(lib.tci_last_error().decode())
+            # retire finished batches
+            still = []
+            for b, ids, arr in inflight:
+                if lib.tci_batch_query(b) == 0:
+                    lib.tci_batch_wait(b)                    # reports on the host
+                    done = time.perf_counter() - t0
+                    r = _abi.tci_result()
+                    for k, i in enumerate(ids):
+                        lib.tci_batch_result(b, k, C.byref(r))
+                        m = max(r.niter, 1)
+                        ranks = np.zeros(m, np.int64)
+                        errs = np.zeros(m)
+                        lib.tci_batch_history(b, k, _abi.i64ptr(ranks), _abi.dptr(errs), None)
+                        t = TensorCI2(b, k, requests[i][1]) if keep_tci else None
+                        results[i] = ServedResult(r.status, ranks[:r.niter], errs[:r.niter],
+                                                  r.maxsamplevalue, float(arrivals[i]), done, t)
+                    lib.tci_batch_free(b)
+                    progressed = True
+                else:
+                    still.append((b, ids, arr))
+            inflight = still
+            if not progressed:
+                if nxt < n and not inflight:
+                    wait = arrivals[nxt] - (time.perf_counter() - t0)
+                    if wait > 0:
+                        time.sleep(min(wait, 1e-3))
+                else:
+                    time.sleep(self.idle_sleep_s)
+        wall = time.perf_counter() - t0
+        lat = np.array([r.latency for r in results]) * 1e3
+        stats = ServeStats(n=n, wall_s=wall, solves_per_s=n / wall if wall > 0 else 0.0,
+                           p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                           batches=nbatches, mean_batch=n / max(nbatches, 1), oom_waits=oom_waits,
+                           latencies_ms=lat.tolist())
+        return results, stats
+
+
+def poisson_arrivals(n, rate_per_s, seed=5):
+    """n arrival times of a Poisson stream (exponential gaps), seconds."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate_per_s, n))

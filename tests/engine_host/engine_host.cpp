@@ -38,6 +38,7 @@ struct HostTeam {
     uint8_t *lrows = nullptr, *lcols = nullptr;
     void sync() {}
     void wave_sync() {}
+    void wave_sync_lds() {}
     ArgBest argmax(ArgBest b) { return b; }
     int uniform(int x) const { return x; }
     double max_nan(double v) { return v; }
@@ -90,6 +91,7 @@ struct HostTeamMT {
     void sync() { sh->bar->arrive_and_wait(); }
     void wsync() { sh->wbar[wave()]->arrive_and_wait(); }
     void wave_sync() { wsync(); }
+    void wave_sync_lds() { wsync(); }
     ArgBest argmax(ArgBest b)
     {
         sh->abuf[t_tid] = b;

@@ -64,3 +64,21 @@ def specs():
 
 def oracle_opts(opts):
     return {k: v for k, v in opts.items() if k != "chi_cap"}
+
+
+def served_mix(count, seed=5, chi_cap=16):
+    """Requests of the served workload: each a sum of K ~ U{1..8} separable
+    Gaussians in n ~ U{4..12} dims, one local dimension d ∈ {2,4,8,16,32} for
+    all sites, tolerance 10^U(-10,-6), maxbonddim 128.  Returns specs in the
+    format of specs()."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(count):
+        K = int(rng.integers(1, 9))
+        n = int(rng.integers(4, 13))
+        d = int(rng.choice([2, 4, 8, 16, 32]))
+        tol = float(10.0 ** rng.uniform(-10, -6))
+        params = gauss_sum(K, n, d, int(rng.integers(0, 2 ** 31)))
+        out.append((f"served{q}", GSUM, params, [], [d] * n, None,
+                    dict(tolerance=tol, maxbonddim=128, chi_cap=chi_cap)))
+    return out

@@ -148,3 +148,26 @@ def test_tile_rrlu_nan_verdicts_equal_oracle(seed):
                 assert a["p"] == b.npivot
                 assert np.array_equal(a["rowperm"], b.rowpermutation)
                 assert np.array_equal(a["colperm"], b.colpermutation)
+
+
+@pytest.mark.parametrize("scale", [1e-150, 1e-158, 1e-162, 1e-170, 1e-200])
+def test_tile_rrlu_tiny_magnitudes_equal_oracle(scale):
+    """Entries whose squares are subnormal or underflow to zero: ties in abs2
+    among different magnitudes must resolve exactly as the reference's
+    column-major first maximum does."""
+    rng = np.random.default_rng(int(-np.log10(scale)))
+    for t in range(4):
+        m, n = (int(x) for x in rng.integers(2, 41, 2))
+        A = rng.standard_normal((m, n)) * scale
+        if t % 2:
+            A[rng.random((m, n)) < 0.3] = 0.0
+        lo = bool(t & 1)
+        st, b = _oracle_status(A, lo)
+        for a in (eh.rrlu(0, A, leftorthogonal=lo), eh.rrlu_mt(0, A, leftorthogonal=lo)):
+            if st:
+                assert a["p"] == -st
+                continue
+            assert a["p"] == b.npivot, (scale, t)
+            assert np.array_equal(a["rowperm"], b.rowpermutation)
+            assert np.array_equal(a["colperm"], b.colpermutation)
+            assert a["error"] == b.error or (np.isnan(a["error"]) and np.isnan(b.error))
