@@ -231,10 +231,12 @@ static hipStream_t pick_stream(tci_ctx *ctx)
     for (size_t t = 0; t < ns; ++t) {
         const size_t q = (ctx->next + t) % ns;
         hipStream_t s = q == 0 ? ctx->stream : ctx->extra[q - 1];
-        if (hipStreamQuery(s) == hipSuccess) {
+        const hipError_t e = hipStreamQuery(s);
+        if (e == hipSuccess) {
             ctx->next = (q + 1) % ns;
             return s;
         }
+        (void)hipGetLastError();   // a busy stream is not an error: clear it
     }
     const size_t q = ctx->next % ns;
     ctx->next = (q + 1) % ns;
@@ -550,7 +552,10 @@ tci_status tci_batch_query(tci_batch *b)
 {
     if (!b) return set_err(TCI_ERR_ARG, "batch is NULL");
     hipError_t e = hipEventQuery(b->ev1);
-    if (e == hipErrorNotReady) return TCI_ERR_PENDING;
+    if (e == hipErrorNotReady) {
+        (void)hipGetLastError();   // not finished is not an error: clear it
+        return TCI_ERR_PENDING;
+    }
     if (e != hipSuccess) return set_err(TCI_ERR_DEVICE, "hipEventQuery: %s", hipGetErrorString(e));
     return TCI_OK;
 }

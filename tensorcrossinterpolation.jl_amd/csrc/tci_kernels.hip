@@ -193,8 +193,9 @@ int solve_static_lds(int team, int fkind)
 template <int NT, int FK, int WPE>
 static void launch_one(const SolveDesc *descs, int count, const LdsPlan &plan, size_t lds, hipStream_t st)
 {
-    (void)hipFuncSetAttribute((const void *)tci_solve_kernel<NT, FK, WPE>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (hipFuncSetAttribute((const void *)tci_solve_kernel<NT, FK, WPE>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return;   // the launch guard in launch_solve reports the error
     hipLaunchKernelGGL((tci_solve_kernel<NT, FK, WPE>), dim3(count), dim3(NT), lds, st, descs, count, plan);
 }
 

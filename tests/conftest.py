@@ -40,4 +40,9 @@ def tci():
 
 @pytest.fixture(scope="session")
 def ctx(tci):
+    # torch is only the plumbing for device buffers in a few tests; let it
+    # find the device before the library's own HIP calls
+    import torch
+    if torch.cuda.is_available():
+        torch.cuda.init()
     return tci.default_context()

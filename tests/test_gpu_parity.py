@@ -113,3 +113,24 @@ def test_pivoterrors_diag(tci, ctx):
     tab = np.diag([1.0, 1e-5, 0.0])
     t, _, _ = tci.crossinterpolate2(float, tci.TableFunctor(tab), [3, 3], [[1, 1]], tolerance=1e-8)
     assert t.pivoterrors.tolist() == [1.0, 1e-5, 0.0]
+
+
+def test_rrlu_batched_c2_shapes_vs_oracle(tci, ctx):
+    """Matrices of the batched-rrLU workload (rows/cols up to 1024, A = X·Y +
+    1e-12·E, rank up to 128, reltol 1e-10), uneven sizes in one launch: the
+    pivot sequences, pivot counts and errors equal the oracle's."""
+    rng = np.random.default_rng(7)
+    shapes = [(1024, 1024, 128), (1000, 37, 20), (33, 900, 64), (517, 611, 8), (256, 256, 100)]
+    mats = []
+    for m, n, r in shapes:
+        X = rng.standard_normal((m, r))
+        Y = rng.standard_normal((r, n))
+        mats.append(X @ Y + 1e-12 * rng.standard_normal((m, n)))
+    res = tci.rrlu_batched(mats, reltol=1e-10, ctx=ctx)
+    for A, g in zip(mats, res):
+        o = O.rrlu(A, reltol=1e-10)
+        assert g["status"] == 0
+        assert g["npivot"] == o.npivot, A.shape
+        assert np.array_equal(g["rowpermutation"], o.rowpermutation), A.shape
+        assert np.array_equal(g["colpermutation"], o.colpermutation), A.shape
+        assert g["error"] == o.error, A.shape
