@@ -77,6 +77,8 @@ struct DevTeam {
         return min(min(a, b), min(c, d));
     }
     __device__ bool wave_any(bool p) const { return __ballot(p) != 0; }
+    // lane l's v in every lane (l wave-uniform)
+    __device__ int wave_bcast(int v, int l) const { return __builtin_amdgcn_readlane(v, l); }
     // lowest lane where p holds, -1 if none (wave-uniform)
     __device__ int wave_first(bool p) const
     {

@@ -15,7 +15,7 @@
 //   tm.lrows, tm.lcols       on-chip index tables (null: use the HBM ones)
 //   Team::NT, WS, NW         team size, wave width, waves (compile-time)
 //   tm.uniform(x)            x as a wave-uniform value (all lanes hold the same x)
-//   tm.wave_max/min/any/first, tm.wave_sync(), tm.wave_sync_lds()
+//   tm.wave_max/min/any/first/bcast, tm.wave_sync(), tm.wave_sync_lds()
 //                            wave-level reductions, ballot, and ordering points
 //                            (LDS and global, or LDS only) inside one wave
 //   tm.tA(w), tB(w), tX(w, r), tY(w, j)  LDS slots of the register-tile rrLU

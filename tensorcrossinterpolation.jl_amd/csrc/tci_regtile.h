@@ -6,8 +6,9 @@
 // of up to (WS·RA)×(NW·CB) is factored without touching memory.
 //
 // Rows and columns are never moved.  Each thread tracks the current position
-// of its rows (rp) and of its wave's columns (cp, the same in every lane) and
-// applies the swap of step k (positions k ↔ pivot) to them.  The argmax key
+// of its rows (rp); the positions of a wave's CB register columns are spread
+// over its lanes (lane l holds that of register column l, in cpl), so the
+// swap of step k (positions k ↔ pivot) costs a few vector operations.  The argmax key
 // of an entry is its current position, colpos·256 + rowpos, so the smallest
 // key among equal abs2 values is the first maximum of the reference's
 // column-major scan of the physically swapped matrix (matrixlu.jl:1-32).
@@ -15,7 +16,8 @@
 // a true division of the pivot column (or row), then a - x*y with two
 // roundings.  Rows and columns that have been pivoted get NaN multipliers, so
 // their entries turn into NaN and drop out of every later search (a NaN is
-// never the maximum, as in submatrixargmax).
+// never the maximum, as in submatrixargmax); a dead column's entries being
+// NaN, its staged multiplier is NaN without any per-column test.
 //
 // One pivot step costs two team barriers:
 //   A  the team maximum g of |entry| over the trailing block; the maximum
@@ -36,10 +38,15 @@ struct TileLU {
     static constexpr int WS = Team::WS, NW = Team::NW;
     static constexpr int MR = WS * RA, MC = NW * CB;   // capacity: MR rows, MC columns
     static_assert(MR <= 256, "row positions must fit the 8-bit part of the key");
+    static constexpr int CL = (CB + WS - 1) / WS;   // column-position slots per lane
     double a[RA][CB];
     int rp[RA];     // current position of my rows (large for rows ≥ m)
-    int cp[CB];     // current position of my wave's columns (large for cols ≥ n)
+    int cpl[CL];    // slot c: position of register column lane + WS·c (large if none / ≥ n)
+    int jw;         // register column this wave staged in its last search
     int m, n;
+
+    // position of register column j of this wave (wave-uniform)
+    ENG_INLINE int colpos(Team &tm, int j) const { return tm.wave_bcast(cpl[j / WS], j % WS); }
 
     ENG_INLINE static bool fits(int m_, int n_) { return m_ <= MR && n_ <= MC; }
 
@@ -53,10 +60,12 @@ struct TileLU {
             rp[i] = r < m ? r : 0x3fffffff;
         }
 #pragma unroll
-        for (int j = 0; j < CB; ++j) {
-            const int c = tm.wave() + NW * j;
-            cp[j] = c < n ? c : 0x3fffffff;
+        for (int c = 0; c < CL; ++c) {
+            const int j = tm.lane() + WS * c;
+            const int col = tm.wave() + NW * j;
+            cpl[c] = (j < CB && col < n) ? col : 0x3fffffff;
         }
+        jw = 0;
     }
 
     // entries outside the matrix hold NaN
@@ -93,7 +102,7 @@ struct TileLU {
         for (int j = 0; j < CB; ++j)
 #pragma unroll
             for (int i = 0; i < RA; ++i) {
-                const bool ok = rp[i] < m && cp[j] < n;
+                const bool ok = rp[i] < m && tm.wave() + NW * j < n;
                 const double v = c / (rsv[i] + csv[j] + 1.0);
                 a[i][j] = ok ? v : 0.0 / 0.0;
                 lmax = ok ? jl_max(lmax, fabs(v)) : lmax;
@@ -135,26 +144,28 @@ struct TileLU {
             }
             const bool mine = exact ? lmax * lmax == G : lmax >= lo;
             if (tm.wave_any(mine)) {
+                // column parts of the keys, read by the whole wave
+                uint32_t ck[CB];
+#pragma unroll
+                for (int j = 0; j < CB; ++j) ck[j] = (uint32_t)colpos(tm, j) << 8;
                 uint32_t kl = 0xffffffffu;
                 if (mine) {
                     if (!exact) {
 #pragma unroll
                         for (int j = 0; j < CB; ++j) {
-                            const uint32_t ck = (uint32_t)cp[j] << 8;
 #pragma unroll
                             for (int i = 0; i < RA; ++i) {
-                                const uint32_t key = ck | (uint32_t)rp[i];
+                                const uint32_t key = ck[j] | (uint32_t)rp[i];
                                 kl = eng_umin(kl, fabs(a[i][j]) >= lo ? key : 0xffffffffu);
                             }
                         }
                     } else {
 #pragma unroll
                         for (int j = 0; j < CB; ++j) {
-                            const uint32_t ck = (uint32_t)cp[j] << 8;
 #pragma unroll
                             for (int i = 0; i < RA; ++i) {
                                 const double v = a[i][j];
-                                const uint32_t key = ck | (uint32_t)rp[i];
+                                const uint32_t key = ck[j] | (uint32_t)rp[i];
                                 kl = eng_umin(kl, v * v == G ? key : 0xffffffffu);
                             }
                         }
@@ -164,8 +175,15 @@ struct TileLU {
                 // stage the column of this wave's candidate (raw values)
                 const int cpos = (int)(kw >> 8);
 #pragma unroll
+                for (int c = 0; c < CL; ++c) {
+                    const int l = tm.wave_first(tm.lane() + WS * c < CB && cpl[c] == cpos);
+                    if (l >= 0) jw = l + WS * c;
+                }
+                // (selected by position, not by index, so the tile stays in
+                // registers rather than becoming an indexed array)
+#pragma unroll
                 for (int j = 0; j < CB; ++j)
-                    if (cp[j] == cpos) {
+                    if ((ck[j] >> 8) == (uint32_t)cpos) {
 #pragma unroll
                         for (int i = 0; i < RA; ++i) tm.tX(tm.wave(), tm.lane() + WS * i) = a[i][j];
                     }
@@ -227,7 +245,7 @@ struct TileLU {
 #pragma unroll
             for (int i = 0; i < RA; ++i) rp[i] = rp[i] == k ? pr : (rp[i] == pr ? k : rp[i]);
 #pragma unroll
-            for (int j = 0; j < CB; ++j) cp[j] = cp[j] == k ? pc : (cp[j] == pc ? k : cp[j]);
+            for (int c = 0; c < CL; ++c) cpl[c] = cpl[c] == k ? pc : (cpl[c] == pc ? k : cpl[c]);
             if (tm.tid() == 0 && pivval) pivval[k] = av;
             if (av != av) { nanx = 1; nany = 1; }
             // x: the staged pivot column, divided by the pivot when leftorth
@@ -245,7 +263,9 @@ struct TileLU {
                 x[i] = live ? v : NaN;
             }
             // y: the pivot row, broadcast from its holder lane in this wave
-            // through the wave's LDS slot
+            // through the wave's LDS slot.  Dead columns hold NaN already; the
+            // pivot column (staged by wave wstar as its register column jw)
+            // dies now.
             if (cand) {
                 const int lo = orow % WS, io = orow / WS;
                 if (tm.lane() == lo) {
@@ -255,33 +275,41 @@ struct TileLU {
 #pragma unroll
                             for (int j = 0; j < CB; ++j) tm.tY(tm.wave(), j) = a[i][j];
                         }
+                    if (tm.wave() == wstar) tm.tY(tm.wave(), jw) = NaN;
                 }
                 tm.wave_sync_lds();
                 if (!leftorth) {
                     for (int j = tm.lane(); j < CB; j += WS) tm.tY(tm.wave(), j) = tm.tY(tm.wave(), j) / av;
                     tm.wave_sync_lds();
                 }
+            } else {
+                for (int j = tm.lane(); j < CB; j += WS) tm.tY(tm.wave(), j) = NaN;
+                tm.wave_sync_lds();
+            }
+            // U row: NaN on a live column (lane l checks register column l)
+#pragma unroll
+            for (int c = 0; c < CL; ++c) {
+                const int j = tm.lane() + WS * c;
+                if (j < CB && cpl[c] > k && cpl[c] < n) {
+                    const double v = tm.tY(tm.wave(), j);
+                    if (v != v) nany = 1;
+                }
             }
             // trailing update and the next search; the multipliers are read
-            // from LDS up front (unconditional, in-bounds reads) so their
-            // latency overlaps
+            // from LDS up front so their latency overlaps
             double yv[CB];
 #pragma unroll
             for (int j = 0; j < CB; ++j) yv[j] = tm.tY(tm.wave(), j);
             lmax = -1.0;
 #pragma unroll
-            for (int j = 0; j < CB; ++j) {
-                const bool live = cp[j] > k && cp[j] < n;
-                const double yj = (cand && live) ? yv[j] : NaN;
-                if (live && yj != yj) nany = 1;
+            for (int j = 0; j < CB; ++j)
 #pragma unroll
                 for (int i = 0; i < RA; ++i) {
-                    const double t = x[i] * yj;
+                    const double t = x[i] * yv[j];
                     const double u = a[i][j] - t;
                     a[i][j] = u;
                     lmax = fmax(lmax, fabs(u));
                 }
-            }
             p = k + 1;
             sschur += (double)(m - p) * (double)(n - p);
             K = search(tm, lmax, wstar);
@@ -292,10 +320,12 @@ struct TileLU {
             for (int i = 0; i < RA; ++i)
                 if (rp[i] < m) rowperm[rp[i]] = tm.lane() + WS * i;
         }
-        if (colperm && tm.lane() == 0) {
+        if (colperm) {
 #pragma unroll
-            for (int j = 0; j < CB; ++j)
-                if (cp[j] < n) colperm[cp[j]] = tm.wave() + NW * j;
+            for (int c = 0; c < CL; ++c) {
+                const int j = tm.lane() + WS * c;
+                if (j < CB && cpl[c] < n) colperm[cpl[c]] = tm.wave() + NW * j;
+            }
         }
         const int nanL = tm.any(nanx);
         const int nanU = tm.any(nany);

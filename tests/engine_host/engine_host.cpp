@@ -46,6 +46,7 @@ struct HostTeam {
     uint32_t wave_min(uint32_t v) const { return v; }
     bool wave_any(bool p) const { return p; }
     int wave_first(bool p) const { return p ? 0 : -1; }
+    int wave_bcast(int v, int) const { return v; }
     int any(int v) { return v; }
     uint64_t clock() const { return 0; }
 };
@@ -138,6 +139,14 @@ struct HostTeamMT {
         return r;
     }
     bool wave_any(bool p) { return wave_first(p) >= 0; }
+    int wave_bcast(int v, int l)
+    {
+        sh->ibuf[t_tid] = v;
+        wsync();
+        const int r = sh->ibuf[wave() * 64 + l];
+        wsync();
+        return r;
+    }
     int any(int v)
     {
         sh->ibuf[t_tid] = v;
