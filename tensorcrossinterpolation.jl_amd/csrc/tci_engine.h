@@ -229,34 +229,60 @@ struct Engine {
         return true;
     }
 
-    // pushunique! (util.jl:16-20), called by thread 0 only
-    ENG int pushunique(int kind, int b, const uint8_t *x, int len)
-    {
-        uint8_t *base = setp(kind, b);
-        int32_t &n = setn(kind, b);
-        for (int i = 0; i < n; ++i)
-            if (eq(base + (int64_t)i * L, x, len)) return ST_OK;
-        if (n >= D.setcap) return ST_CAPACITY;
-        for (int t = 0; t < len; ++t) base[(int64_t)n * L + t] = x[t];
-        n += 1;
-        return ST_OK;
-    }
-
-    // addglobalpivots! (tensorci2.jl:193-213); thread 0, then barrier
+    // addglobalpivots! (tensorci2.jl:193-213): for each pivot q in order and
+    // each site b, pushunique!(Iset[b], q[1:b-1]) and pushunique!(Jset[b],
+    // q[b+1:end]) (util.jl:16-20).  The 2L sets are independent, and a set
+    // receives pivot q's part iff that part is neither in the set as it was
+    // nor an earlier pivot's part; so the membership tests run in parallel
+    // (one thread per set and pivot, a chunk of pivots at a time) and then
+    // one thread per set appends in pivot order.
     ENG int addglobalpivots(const uint8_t *piv, int npiv)
     {
-        if (tm.tid() == 0) {
-            int st = ST_OK;
-            for (int q = 0; q < npiv && st == ST_OK; ++q) {
-                const uint8_t *p = piv + (int64_t)q * L;
-                for (int b = 0; b < L && st == ST_OK; ++b) {
-                    st = pushunique(SET_I, b, p, ilen(b));
-                    if (st == ST_OK) st = pushunique(SET_J, b, p + b + 1, jlen(b));
-                }
+        const int nsets = 2 * L;
+        int chunk = D.mrcap / nsets;          // flag bytes available per pivot chunk
+        if (chunk < 1) chunk = 1;
+        int caperr = 0;
+        for (int q0 = 0; q0 < npiv; q0 += chunk) {
+            const int nq = npiv - q0 < chunk ? npiv - q0 : chunk;
+            for (int t = tm.tid(); t < nsets * nq; t += tm.nt()) {
+                const int sidx = t / nq, q = q0 + (t - sidx * nq);
+                const bool iside = sidx < L;
+                const int b = iside ? sidx : sidx - L;
+                const int kind = iside ? SET_I : SET_J;
+                const int len = iside ? ilen(b) : jlen(b);
+                const int off = iside ? 0 : b + 1;
+                const uint8_t *x = piv + (int64_t)q * L + off;
+                const uint8_t *base = setp(kind, b);
+                const int n = setn(kind, b);
+                bool dup = false;
+                for (int e = 0; e < n && !dup; ++e) dup = eq(base + (int64_t)e * L, x, len);
+                for (int q2 = 0; q2 < q && !dup; ++q2) dup = eq(piv + (int64_t)q2 * L + off, x, len);
+                D.flags[t] = dup ? 0 : 1;
             }
-            if (st != ST_OK) S.status = st;
+            tm.sync();
+            for (int sidx = tm.tid(); sidx < nsets; sidx += tm.nt()) {
+                const bool iside = sidx < L;
+                const int b = iside ? sidx : sidx - L;
+                const int kind = iside ? SET_I : SET_J;
+                const int len = iside ? ilen(b) : jlen(b);
+                const int off = iside ? 0 : b + 1;
+                uint8_t *base = setp(kind, b);
+                int n = setn(kind, b);
+                for (int q = 0; q < nq; ++q) {
+                    if (!D.flags[sidx * nq + q]) continue;
+                    if (n >= D.setcap) { caperr = 1; break; }
+                    const uint8_t *x = piv + (int64_t)(q0 + q) * L + off;
+                    for (int t = 0; t < len; ++t) base[(int64_t)n * L + t] = x[t];
+                    ++n;
+                }
+                setn(kind, b) = n;
+            }
+            if (tm.any(caperr)) {
+                if (tm.tid() == 0 && S.status == ST_OK) S.status = ST_CAPACITY;
+                tm.sync();
+                return S.status;
+            }
         }
-        tm.sync();
         return S.status;
     }
 
@@ -317,19 +343,20 @@ struct Engine {
             D.flags[e] = found ? 0 : 1;
         }
         tm.sync();
-        if (tm.tid() == 0) {
-            int m = m0;
-            uint8_t *dst = rowside ? rows : cols;
-            for (int e = 0; e < ne; ++e) {
-                if (!D.flags[e]) continue;
-                if (m >= D.mrcap) { S.status = ST_CAPACITY; break; }
-                for (int t = 0; t < elen; ++t) dst[(int64_t)m * L + t] = E[(int64_t)e * L + t];
-                ++m;
-            }
-            S.cnt = m;
+        // kept extras go after the kron part, in their order: each thread
+        // counts the kept ones before its own (ne is at most a set capacity)
+        uint8_t *dst = rowside ? rows : cols;
+        int total = 0;
+        for (int e = 0; e < ne; ++e) total += D.flags[e];
+        if (m0 + total > D.mrcap) return fail(ST_CAPACITY);
+        for (int e = tm.tid(); e < ne; e += tm.nt()) {
+            if (!D.flags[e]) continue;
+            int pos = m0;
+            for (int e2 = 0; e2 < e; ++e2) pos += D.flags[e2];
+            for (int t = 0; t < elen; ++t) dst[(int64_t)pos * L + t] = E[(int64_t)e * L + t];
         }
         tm.sync();
-        return S.cnt;
+        return m0 + total;
     }
 
     // copy a set into the rows or cols table
@@ -430,18 +457,21 @@ struct Engine {
         return p;
     }
 
-    // updateerrors! (tensorci2.jl:161-169); thread 0
+    // updateerrors! (tensorci2.jl:161-169): pivoterrors[i] = max(pivoterrors[i],
+    // |pivot i| or the last error), zero-padded (util.jl:12-14), element-wise
+    // in parallel; ends with a barrier
     ENG void updateerrors(int b, int p, double err)
     {
+        const int npe = D.ints[INT_NPE];
+        const int n = npe > p + 1 ? npe : p + 1;
+        for (int i = tm.tid(); i < n; i += tm.nt()) {
+            const double a = i < npe ? D.pivoterrors[i] : 0.0;
+            const double e = i < p ? fabs(D.pivval[i]) : (i == p ? err : 0.0);
+            D.pivoterrors[i] = jl_max(a, e);
+        }
+        tm.sync();
         if (tm.tid() == 0) {
             D.bonderrors[b] = err;
-            int npe = D.ints[INT_NPE];
-            int n = npe > p + 1 ? npe : p + 1;
-            for (int i = 0; i < n; ++i) {
-                double a = i < npe ? D.pivoterrors[i] : 0.0;
-                double e = i < p ? fabs(D.pivval[i]) : (i == p ? err : 0.0);
-                D.pivoterrors[i] = jl_max(a, e);
-            }
             D.ints[INT_NPE] = n;
         }
     }
@@ -802,18 +832,32 @@ struct Engine {
             }
             tm.sync();
         }
+        // per point, the first candidate with the largest error (a strict
+        // maximum over errors > 0, as in the reference's scan), one wave per
+        // point; then thread 0 keeps the points above the threshold in order
+        const double thr = abstol * D.tolmargin;
+        double *bv = errs + ncand;                    // ns best errors
+        int64_t *bix = (int64_t *)(bv + ns);          // ns candidate indices (-1: none)
+        for (int q = tm.wave(); q < ns; q += tm.nwaves()) {
+            double lb = 0.0;
+            uint32_t li = 0xffffffffu;
+            for (int r = tm.lane(); r < sumd; r += tm.wsize()) {
+                const double e = errs[(int64_t)q * sumd + r];
+                if (e > lb) { lb = e; li = (uint32_t)r; }
+            }
+            const double g = tm.wave_max(lb);
+            const uint32_t ri = tm.wave_min((lb == g && g > 0.0) ? li : 0xffffffffu);
+            if (tm.lane() == 0) {
+                bv[q] = g;
+                bix[q] = ri == 0xffffffffu ? -1 : (int64_t)q * sumd + ri;
+            }
+        }
+        tm.sync();
         if (tm.tid() == 0) {
-            const double thr = abstol * D.tolmargin;
             int nf = 0;
             for (int q = 0; q < ns; ++q) {
-                double best_err = 0.0;
-                int64_t bi = -1;
-                for (int r = 0; r < sumd; ++r) {
-                    double e = errs[(int64_t)q * sumd + r];
-                    if (e > best_err) { best_err = e; bi = (int64_t)q * sumd + r; }
-                }
-                if (best_err > thr) {
-                    const uint8_t *src = bi >= 0 ? cand + bi * L : pts + (int64_t)q * L;
+                if (bv[q] > thr) {
+                    const uint8_t *src = bix[q] >= 0 ? cand + bix[q] * L : pts + (int64_t)q * L;
                     for (int t = 0; t < L; ++t) found[(int64_t)nf * L + t] = src[t];
                     ++nf;
                 }
