@@ -251,7 +251,7 @@ def main():
     # algorithmic bytes per launch: Pi evaluation writes 8 B per element, the
     # fused rrLU moves 8mn + 16Σ(m-k)(n-k) + 8p(m+n) per factorization
     bytes_per_solve = 8.0 * ws[0] + ws[2]
-    phases = dict(zip(["eval", "rrlu", "sets", "fill", "gsearch", "sweep1", "total"], ws[6][:7]))
+    phases = dict(zip(["eval", "rrlu", "sets", "fill", "gsearch", "sweep1", "total", "pinv"], ws[6][:8]))
     achieved = B * bytes_per_solve / (kms_avg * 1e-3) / 1e9
     out = {
         "metric": "TCI solves/s + p50/p99 latency at tol 1e-8, batched/served vs plain on MI355X",

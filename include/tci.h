@@ -190,9 +190,10 @@ tci_status tci_batch_free(tci_batch *b);
 tci_status tci_batch_elapsed_ms(tci_batch *b, float *ms);
 /* Diagnostic (no counterpart in the reference, whose only timing split is the
  * verbosity>2 print at tensorci2.jl:547-550): microseconds solve i spent in
- * each phase — [0] Pi evaluation, [1] rrLU, [2] pivot-set updates (unused),
- * [3] fillsitetensors!, [4] global pivot search, [5] final sweep1site!,
- * [6] whole solve, [7] unused. */
+ * each phase — [0] Pi evaluation, [1] rrLU, [2] pivot-set updates in the
+ * 2-site sweeps, [3] fillsitetensors!, [4] global pivot search, [5] final
+ * sweep1site!, [6] whole solve, [7] the Pi1·P⁻¹ solves inside [3].  [0] and
+ * [1] also count the evaluations and rrLUs inside [3] and [5]. */
 tci_status tci_batch_profile(tci_batch *b, int32_t i, double *us8);
 
 #ifdef __cplusplus

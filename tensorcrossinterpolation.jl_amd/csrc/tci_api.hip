@@ -69,7 +69,7 @@ struct tci_ctx {
 // Launch shape of the solve kernel: a team of `team` threads per solve; LDS
 // holds the rrLU staging vectors, the index tables and as much of a Pi
 // matrix as the budget allows (one resident workgroup per CU when large).
-static LdsPlan make_plan(const tci_ctx *c, int64_t left, int max_mrcap, int max_tab)
+static LdsPlan make_plan(const tci_ctx *c, int64_t left, int max_mrcap, int max_tab, int64_t max_sets)
 {
     LdsPlan p;
     memset(&p, 0, sizeof p);
@@ -82,6 +82,13 @@ static LdsPlan make_plan(const tci_ctx *c, int64_t left, int max_mrcap, int max_
     if (2 * tab <= left) {
         p.tab_cap = (int32_t)tab;
         left -= 2 * tab;
+    }
+    // the pivot sets and counts (16-byte aligned parts) when they fit in a
+    // quarter of what is left, so the Pi area keeps most of the LDS
+    const int64_t sb = (max_sets + 15) & ~(int64_t)15;
+    if (max_sets > 0 && sb <= left / 4) {
+        p.sets_cap = (int32_t)sb;
+        left -= sb;
     }
     int64_t pib = (int64_t)c->pi_lds_kb * 1024;
     if (pib > left) pib = left;
@@ -465,6 +472,7 @@ tci_status tci_solve_submit(tci_ctx *ctx, int32_t count, const tci_problem *prob
     std::vector<size_t> ib((size_t)count), sb((size_t)count), rb((size_t)count);
     size_t tin = 0, tstate = 0, trep = 0;
     int max_mr = 0, max_tab = 0;
+    int64_t max_sets = 0;
     int fkind = problems[0].f.kind;
     for (int i = 0; i < count; ++i) {
         if (problems[i].f.kind != fkind) fkind = 0;
@@ -478,6 +486,10 @@ tci_status tci_solve_submit(tci_ctx *ctx, int32_t count, const tci_problem *prob
         if (caps[(size_t)i].mrcap > max_mr) max_mr = caps[(size_t)i].mrcap;
         const int tab = caps[(size_t)i].mrcap * caps[(size_t)i].L;
         if (tab > max_tab) max_tab = tab;
+        const SolveCaps &cp = caps[(size_t)i];
+        const int64_t sbytes = ((((int64_t)NSETKIND * cp.L * cp.setcap * cp.L) + 15) & ~(int64_t)15) +
+                               4 * (int64_t)NSETKIND * cp.L;
+        if (sbytes > max_sets) max_sets = sbytes;
     }
     const size_t total = tin + tstate + trep + sizeof(SolveDesc) * (size_t)count;
     if (ctx->used + total > ctx->budget)
@@ -530,7 +542,7 @@ tci_status tci_solve_submit(tci_ctx *ctx, int32_t count, const tci_problem *prob
     if ((e = hipEventRecord(b->ev0, st)) != hipSuccess) return fail(e, "hipEventRecord");
     const int stat = solve_static_lds(ctx->team, fkind);
     if (stat < 0) return fail(hipErrorInvalidValue, "solve kernel attributes");
-    const LdsPlan plan = make_plan(ctx, (int64_t)ctx->lds_wg - stat, max_mr, max_tab);
+    const LdsPlan plan = make_plan(ctx, (int64_t)ctx->lds_wg - stat, max_mr, max_tab, max_sets);
     if ((e = launch_solve(b->d_desc, count, ctx->team, fkind, plan, st)) != hipSuccess) return fail(e, "solve launch");
     if ((e = hipEventRecord(b->ev1, st)) != hipSuccess) return fail(e, "hipEventRecord");
     *out = b;

@@ -75,7 +75,7 @@ struct SolveDesc {
     double *pmat;            // setcap × setcap (pivot matrix P for Pi1·P⁻¹)
     int32_t *rowperm;        // mrcap
     int32_t *colperm;        // mrcap
-    int32_t *ipiv;           // setcap (partial pivoting of Pᵀ)
+    int32_t *ipiv;           // 2·setcap (partial pivoting of Pᵀ, then its permutation)
     double *pivval;          // mrcap
     double *T;               // L site-tensor slots of tstride doubles
     int64_t tstride;

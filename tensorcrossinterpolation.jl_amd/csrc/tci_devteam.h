@@ -79,6 +79,7 @@ struct DevTeam {
     __device__ bool wave_any(bool p) const { return __ballot(p) != 0; }
     // lane l's v in every lane (l wave-uniform)
     __device__ int wave_bcast(int v, int l) const { return __builtin_amdgcn_readlane(v, l); }
+    __device__ double wave_bcast_f64(double v, int l) const { return readlane_f64(v, l); }
     // lowest lane where p holds, -1 if none (wave-uniform)
     __device__ int wave_first(bool p) const
     {
@@ -112,6 +113,8 @@ struct DevTeam {
     double *lpi;          // LDS area for Pi matrices
     int64_t lpi_cap;      // its size in doubles (0: none)
     uint8_t *lrows, *lcols;
+    uint8_t *lsets;       // LDS pivot sets (null: in HBM)
+    int32_t *lsetn;       // LDS set counts
     // team-reduction slots, double-buffered by par_a / par_d
     __device__ static ArgBest *lds_ra() { __shared__ ArgBest b[2 * NW]; return b; }
     __device__ static double *lds_rd() { __shared__ double b[2 * NW]; return b; }
@@ -168,6 +171,8 @@ struct DevTeam {
         lpi_cap = 0;
         lrows = nullptr;
         lcols = nullptr;
+        lsets = nullptr;
+        lsetn = nullptr;
     }
 };
 

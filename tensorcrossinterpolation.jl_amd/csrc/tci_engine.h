@@ -190,7 +190,8 @@ ENG int rrlu_team(Team &tm, double *A, int m, int n, int64_t maxrank, double rel
 
 
 // phases timed by the engine (ticks of tm.clock(), accumulated by thread 0)
-enum { PH_EVAL = 0, PH_RRLU = 1, PH_SETS = 2, PH_FILL = 3, PH_GSEARCH = 4, PH_SWEEP1 = 5, PH_TOTAL = 6 };
+enum { PH_EVAL = 0, PH_RRLU = 1, PH_SETS = 2, PH_FILL = 3, PH_GSEARCH = 4, PH_SWEEP1 = 5, PH_TOTAL = 6,
+       PH_SOLVE = 7 };
 
 // FK > 0 compiles the engine for one functor kind (the switch folds away);
 // FK = 0 dispatches on the descriptor's kind at run time.
@@ -203,9 +204,12 @@ struct Engine {
     int L;
 
     uint8_t *rows, *cols;   // combined index tables (LDS when the team has them)
+    uint8_t *sets;          // pivot sets (LDS when the team has room, else the HBM arena)
+    int32_t *setcnt;        // their counts
 
     ENG Engine(Team &t, const SolveDesc &d, EngShared &s)
-        : tm(t), D(d), S(s), L(d.L), rows(t.lrows ? t.lrows : d.rows), cols(t.lcols ? t.lcols : d.cols) {}
+        : tm(t), D(d), S(s), L(d.L), rows(t.lrows ? t.lrows : d.rows), cols(t.lcols ? t.lcols : d.cols),
+          sets(t.lsets ? t.lsets : d.sets), setcnt(t.lsets ? t.lsetn : d.setn) {}
 
     // a Pi matrix of m×n doubles lives in LDS when it fits, else in HBM
     ENG double *pibuf(int m, int n) const
@@ -216,9 +220,9 @@ struct Engine {
     // ---------------------------------------------------------------- sets
     ENG uint8_t *setp(int kind, int b) const
     {
-        return D.sets + ((int64_t)(kind * L + b) * D.setcap) * L;
+        return sets + ((int64_t)(kind * L + b) * D.setcap) * L;
     }
-    ENG int32_t &setn(int kind, int b) const { return D.setn[kind * L + b]; }
+    ENG int32_t &setn(int kind, int b) const { return setcnt[kind * L + b]; }
     ENG static int ilen(int b) { return b; }                 // |Iset[b]| entries (0-based site b)
     ENG int jlen(int b) const { return L - 1 - b; }          // |Jset[b]| entries
 
@@ -540,6 +544,7 @@ struct Engine {
     // tensorci2.jl:510-551, 599-607 (pivotsearch = :full).  ekind < 0: no extras.
     ENG int updatepivots(int b, bool leftorth, double abstol, int ekind_i, int ekind_j)
     {
+        const uint64_t t0 = tm.clock();
         int m = kron_rows(b);
         int n = kron_cols(b);
         if (m > D.mrcap || n > D.mrcap) return fail(ST_CAPACITY);
@@ -552,6 +557,7 @@ struct Engine {
         }
         double err;
         int p;
+        const uint64_t t1 = tm.clock();
         if (lorentz_tile(m, n)) {
             p = eval_rrlu_lorentz(m, b + 1, n, D.maxbonddim, 1e-14, abstol, leftorth, &err);
         } else {
@@ -561,10 +567,12 @@ struct Engine {
         }
         if (p < 0) return fail(-p);
         if (p > D.setcap) return fail(ST_CAPACITY);
+        const uint64_t t2 = tm.clock();
         select_into(SET_I, b + 1, rows, D.rowperm, p, b + 1);
         select_into(SET_J, b, cols, D.colperm, p, jlen(b));
         updateerrors(b, p, err);
         tm.sync();
+        if (tm.tid() == 0) S.tph[PH_SETS] += (t1 - t0) + (tm.clock() - t2);
         return ST_OK;
     }
 
@@ -574,7 +582,8 @@ struct Engine {
     // done by one thread (c is a bond dimension, small), then forward and
     // back substitution for each of the r rows of Pi1 in parallel.  A, Pi1
     // and X may live in LDS; X (r×c) is the result.
-    ENG int solve_right(const double *Pi1, int r, double *A, int c, double *X)
+    // ipiv: 2c ints (the pivots, then the permutation they compose)
+    ENG int solve_right(const double *Pi1, int r, double *A, int c, double *X, int32_t *ipiv, bool onchip)
     {
         // LU with partial pivoting of A = Pᵀ by wave 0: lanes stride over rows
         // for the pivot search, the scaling and the update (each element sees
@@ -597,40 +606,44 @@ struct Engine {
                     const double g = tm.wave_max(lv);
                     pr = (int)tm.wave_min(lv == g ? li : 0xffffffffu);
                 }
-                if (tm.lane() == 0) D.ipiv[k] = pr;
+                if (tm.lane() == 0) ipiv[k] = pr;
                 if (pr != k)
                     for (int j = tm.lane(); j < c; j += tm.wsize()) {
                         const double x = A[k + (int64_t)c * j];
                         A[k + (int64_t)c * j] = A[pr + (int64_t)c * j];
                         A[pr + (int64_t)c * j] = x;
                     }
-                tm.wave_sync();
+                if (onchip) tm.wave_sync_lds(); else tm.wave_sync();
                 const double d = A[k + (int64_t)c * k];
                 if (d == 0.0) singular = 1;
                 for (int i = k + 1 + tm.lane(); i < c; i += tm.wsize())
                     A[i + (int64_t)c * k] = A[i + (int64_t)c * k] / d;
-                tm.wave_sync();
+                if (onchip) tm.wave_sync_lds(); else tm.wave_sync();
                 for (int i = k + 1 + tm.lane(); i < c; i += tm.wsize()) {
                     const double l = A[i + (int64_t)c * k];
                     for (int j = k + 1; j < c; ++j)
                         A[i + (int64_t)c * j] = A[i + (int64_t)c * j] - l * A[k + (int64_t)c * j];
                 }
-                tm.wave_sync();
+                if (onchip) tm.wave_sync_lds(); else tm.wave_sync();
             }
-            if (tm.lane() == 0) S.cnt = singular;
+            if (tm.lane() == 0) {
+                S.cnt = singular;
+                // the row interchanges as one permutation: column i of the
+                // permuted right-hand side is column perm[i] of Pi1
+                int32_t *perm = ipiv + c;
+                for (int i = 0; i < c; ++i) perm[i] = i;
+                for (int k = 0; k < c; ++k) {
+                    const int pk = ipiv[k], t = perm[k];
+                    perm[k] = perm[pk];
+                    perm[pk] = t;
+                }
+            }
         }
         tm.sync();
         if (S.cnt) return ST_SINGULAR;
+        const int32_t *perm = ipiv + c;
         for (int q = tm.tid(); q < r; q += tm.nt()) {
-            for (int i = 0; i < c; ++i) X[q + (int64_t)r * i] = Pi1[q + (int64_t)r * i];
-            for (int k = 0; k < c; ++k) {
-                int pk = D.ipiv[k];
-                if (pk != k) {
-                    double x = X[q + (int64_t)r * k];
-                    X[q + (int64_t)r * k] = X[q + (int64_t)r * pk];
-                    X[q + (int64_t)r * pk] = x;
-                }
-            }
+            for (int i = 0; i < c; ++i) X[q + (int64_t)r * i] = Pi1[q + (int64_t)r * perm[i]];
             for (int i = 0; i < c; ++i) {
                 double s = X[q + (int64_t)r * i];
                 for (int t = 0; t < i; ++t) s = s - A[i + (int64_t)c * t] * X[q + (int64_t)r * t];
@@ -688,7 +701,9 @@ struct Engine {
             double *Pi1 = lds_take((int64_t)m * nJ, off);
             double *At = lds_take((int64_t)np * np, off);
             double *X = lds_take((int64_t)m * np, off);
-            if (!Pi1 || !At || !X) { Pi1 = D.pi; At = D.pmat; X = Tslot(b); }
+            int32_t *ipv = (int32_t *)lds_take((int64_t)np, off);
+            const bool onchip = Pi1 && At && X && ipv;
+            if (!onchip) { Pi1 = D.pi; At = D.pmat; X = Tslot(b); ipv = D.ipiv; }
             eval(rows, b + 1, m, cols, nJ, Pi1, m, false, true);
             if (np != nJ) {
                 if (tm.tid() == 0) D.ints[INT_ERRSITE] = b + 1;
@@ -696,7 +711,9 @@ struct Engine {
             }
             // Pᵀ[jj, ii] = f(Iset[b+1][ii] ⊕ Jset[b][jj])
             eval(setp(SET_I, b + 1), b + 1, np, cols, nJ, At, np, true, false);
-            int st = solve_right(Pi1, m, At, np, X);
+            const uint64_t ts = tm.clock();
+            int st = solve_right(Pi1, m, At, np, X, ipv, onchip);
+            if (tm.tid() == 0) S.tph[PH_SOLVE] += tm.clock() - ts;
             if (st != ST_OK) {
                 if (tm.tid() == 0) D.ints[INT_ERRSITE] = b + 1;
                 return fail(st);
@@ -789,10 +806,16 @@ struct Engine {
             chi = w > chi ? (int)w : chi;
         }
         // TT values of all candidates, site by site (abstracttensortrain.jl:131,
-        // each product summed in ascending bond order); chunks of candidates
-        // keep the running vectors in LDS
-        int64_t off = 0;
-        int chunk = (int)((tm.lpi_cap > 0 ? tm.lpi_cap : 0) / (2 * (int64_t)chi));
+        // each product summed in ascending bond order), chunks of candidates;
+        // keep the running vectors in LDS, and each site tensor too when it fits
+        int64_t off = 0, tmax = 0;
+        for (int t = 0; t < L; ++t) {
+            const int64_t sz = D.Tdims[3 * t] * D.Tdims[3 * t + 1] * D.Tdims[3 * t + 2];
+            tmax = sz > tmax ? sz : tmax;
+        }
+        double *Ts = lds_take(tmax, off);
+        const int64_t room = tm.lpi_cap > off ? tm.lpi_cap - off : 0;
+        int chunk = (int)(room / (2 * (int64_t)chi));
         double *V = nullptr, *W = nullptr;
         if (chunk >= 1) {
             if (chunk > ncand) chunk = ncand;
@@ -812,6 +835,11 @@ struct Engine {
             for (int t = 0; t < L; ++t) {
                 const int64_t d0 = D.Tdims[3 * t], d = D.Tdims[3 * t + 1], d2 = D.Tdims[3 * t + 2];
                 const double *T = D.T + D.tstride * t;
+                if (Ts) {
+                    for (int64_t q = tm.tid(); q < d0 * d * d2; q += tm.nt()) Ts[q] = T[q];
+                    tm.sync();
+                    T = Ts;
+                }
                 for (int64_t o = tm.tid(); o < (int64_t)cn * d2; o += tm.nt()) {
                     const int64_t c = o / d2, j = o - c * d2;
                     const int64_t s = cand[(c0 + c) * L + t] - 1;
@@ -975,7 +1003,7 @@ struct Engine {
             S.tph[PH_TOTAL] = tm.clock();
             S.status = ST_OK;
             for (int i = 0; i < 6; ++i) S.ws[i] = 0.0;
-            for (int q = 0; q < NSETKIND * L; ++q) D.setn[q] = 0;
+            for (int q = 0; q < NSETKIND * L; ++q) setcnt[q] = 0;
             D.ints[INT_NITER] = 0;
             D.ints[INT_NPE] = 0;
             D.ints[INT_HIST] = 0;
@@ -1057,6 +1085,15 @@ struct Engine {
     ENG int finish(int st)
     {
         tm.sync();
+        if (sets != D.sets) {   // sets held in LDS: write them to the solve's arena
+            for (int q = tm.tid(); q < NSETKIND * L; q += tm.nt()) D.setn[q] = setcnt[q];
+            const int64_t per = (int64_t)D.setcap * L;
+            for (int64_t x = tm.tid(); x < (int64_t)NSETKIND * L * per; x += tm.nt()) {
+                const int64_t q = x / per;
+                if ((x - q * per) < (int64_t)setcnt[q] * L) D.sets[x] = sets[x];
+            }
+            tm.sync();
+        }
         if (tm.tid() == 0) {
             D.ints[INT_STATUS] = st;
             S.tph[PH_TOTAL] = tm.clock() - S.tph[PH_TOTAL];

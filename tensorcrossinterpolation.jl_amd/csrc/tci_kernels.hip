@@ -26,7 +26,7 @@ constexpr int kTeam = 256;
 // ---------------------------------------------------------------- solves
 // Dynamic LDS of a solve workgroup (sizes fixed per launch by LdsPlan):
 //   [stage: 2·stage_cap doubles][rows: tab_cap bytes][cols: tab_cap bytes]
-//   [Pi area: pi_cap doubles]
+//   [sets: sets_cap bytes (the pivot sets, then their counts)][Pi area: pi_cap doubles]
 // Anything that does not fit stays in the solve's HBM arena.
 template <int NT, int FK, int WPE>
 __global__ __launch_bounds__(NT, WPE) void tci_solve_kernel(const SolveDesc *__restrict__ descs,
@@ -40,12 +40,18 @@ __global__ __launch_bounds__(NT, WPE) void tci_solve_kernel(const SolveDesc *__r
     DevTeam<NT, WPE> tm;
     double *stage = dyn;
     uint8_t *tabs = (uint8_t *)(dyn + 2 * plan.stage_cap);
-    double *pi = (double *)(tabs + 2 * plan.tab_cap);
+    uint8_t *lsets = tabs + 2 * plan.tab_cap;
+    double *pi = (double *)(lsets + plan.sets_cap);
     if (D.mrcap <= plan.stage_cap) tm.init(stage, stage + D.mrcap);
     else tm.init(D.stage, D.stage + D.mrcap);
     if ((int64_t)D.mrcap * D.L <= plan.tab_cap) {
         tm.lrows = tabs;
         tm.lcols = tabs + plan.tab_cap;
+    }
+    const int64_t setbytes = (int64_t)NSETKIND * D.L * D.setcap * D.L;
+    if (((setbytes + 15) & ~(int64_t)15) + 4 * NSETKIND * D.L <= plan.sets_cap) {
+        tm.lsets = lsets;
+        tm.lsetn = (int32_t *)(lsets + ((setbytes + 15) & ~(int64_t)15));
     }
     tm.lpi = pi;
     tm.lpi_cap = plan.pi_cap;
@@ -159,7 +165,8 @@ __global__ void tci_eval_kernel(int kind, const double *__restrict__ params,
 // ------------------------------------------------------------- launchers
 size_t lds_plan_bytes(const LdsPlan &p)
 {
-    return (size_t)2 * p.stage_cap * sizeof(double) + 2 * (size_t)p.tab_cap + (size_t)p.pi_cap * sizeof(double);
+    return (size_t)2 * p.stage_cap * sizeof(double) + 2 * (size_t)p.tab_cap + (size_t)p.sets_cap +
+           (size_t)p.pi_cap * sizeof(double);
 }
 
 // the kernel variant a (team, functor kind) pair runs

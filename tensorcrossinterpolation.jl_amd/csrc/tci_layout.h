@@ -58,7 +58,7 @@ inline void layout_solve(const SolveCaps &c, SolveDesc *D, uint8_t *in, uint8_t 
     size_t o_pmat = take(sc * sc * 8);
     size_t o_rp = take(mr * 4);
     size_t o_cp = take(mr * 4);
-    size_t o_ipiv = take(sc * 4);
+    size_t o_ipiv = take(2 * sc * 4);
     size_t o_pv = take(mr * 8);
     size_t o_T = take(L * tstride * 8);
     size_t o_Td = take(3 * L * 8);

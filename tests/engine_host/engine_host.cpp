@@ -36,6 +36,8 @@ struct HostTeam {
     double *lpi = nullptr;
     int64_t lpi_cap = 0;
     uint8_t *lrows = nullptr, *lcols = nullptr;
+    uint8_t *lsets = nullptr;
+    int32_t *lsetn = nullptr;
     void sync() {}
     void wave_sync() {}
     void wave_sync_lds() {}
@@ -47,6 +49,7 @@ struct HostTeam {
     bool wave_any(bool p) const { return p; }
     int wave_first(bool p) const { return p ? 0 : -1; }
     int wave_bcast(int v, int) const { return v; }
+    double wave_bcast_f64(double v, int) const { return v; }
     int any(int v) { return v; }
     uint64_t clock() const { return 0; }
 };
@@ -88,6 +91,8 @@ struct HostTeamMT {
     double *xs = nullptr, *ys = nullptr, *lpi = nullptr;
     int64_t lpi_cap = 0;
     uint8_t *lrows = nullptr, *lcols = nullptr;
+    uint8_t *lsets = nullptr;
+    int32_t *lsetn = nullptr;
     MTShared *sh = nullptr;
     void sync() { sh->bar->arrive_and_wait(); }
     void wsync() { sh->wbar[wave()]->arrive_and_wait(); }
@@ -144,6 +149,14 @@ struct HostTeamMT {
         sh->ibuf[t_tid] = v;
         wsync();
         const int r = sh->ibuf[wave() * 64 + l];
+        wsync();
+        return r;
+    }
+    double wave_bcast_f64(double v, int l)
+    {
+        sh->dbuf[t_tid] = v;
+        wsync();
+        const double r = sh->dbuf[wave() * 64 + l];
         wsync();
         return r;
     }
@@ -211,9 +224,30 @@ extern "C" int eh_rrlu_mt(int mode, double *A, int m, int n, int64_t maxrank, do
 
 struct EhSolve {
     std::vector<uint8_t> inputs, arena, report;
+    std::vector<double> lpi;          // stands in for the workgroup's LDS areas
+    std::vector<uint8_t> ltab, lsets;
     SolveDesc D;
     int status;
 };
+
+// give a host team the on-chip areas the device kernel has (Pi area, index
+// tables, pivot sets), so the engine's LDS code paths run on the CPU too
+template <class Team>
+static void attach_lds(Team &tm, EhSolve *h, const SolveCaps &c)
+{
+    const int64_t tab = (int64_t)c.mrcap * c.L;
+    h->ltab.assign((size_t)(2 * tab), 0);
+    tm.lrows = h->ltab.data();
+    tm.lcols = h->ltab.data() + tab;
+    const int64_t sb = (int64_t)NSETKIND * c.L * c.setcap * c.L;
+    const int64_t sba = (sb + 15) & ~(int64_t)15;
+    h->lsets.assign((size_t)(sba + 4 * NSETKIND * c.L), 0);
+    tm.lsets = h->lsets.data();
+    tm.lsetn = (int32_t *)(h->lsets.data() + sba);
+    h->lpi.assign(16384, 0.0);        // 128 KiB
+    tm.lpi = h->lpi.data();
+    tm.lpi_cap = (int64_t)h->lpi.size();
+}
 
 static std::string g_msg;
 
@@ -238,6 +272,7 @@ extern "C" void *eh_solve(const tci_problem *p, int *status)
     tm = HostTeam();
     tm.xs = h->D.stage;
     tm.ys = h->D.stage + c.mrcap;
+    if (getenv("EH_LDS")) attach_lds(tm, h, c);
     EngShared S;
     memset(&S, 0, sizeof S);
     Engine<HostTeam> *e = new Engine<HostTeam>(tm, h->D, S);
@@ -281,6 +316,9 @@ extern "C" void *eh_solve_mt(const tci_problem *p, int *status)
     EngShared S;
     memset(&S, 0, sizeof S);
     int res = 0;
+    Team proto;
+    const bool lds = getenv("EH_LDS") != nullptr;
+    if (lds) attach_lds(proto, h, c);
     auto body = [&](int tid) {
         Team tm;
         tm.t_tid = tid;
@@ -288,6 +326,10 @@ extern "C" void *eh_solve_mt(const tci_problem *p, int *status)
         tm.tile = &tile;
         tm.xs = h->D.stage;
         tm.ys = h->D.stage + c.mrcap;
+        if (lds) {
+            tm.lrows = proto.lrows; tm.lcols = proto.lcols; tm.lsets = proto.lsets;
+            tm.lsetn = proto.lsetn; tm.lpi = proto.lpi; tm.lpi_cap = proto.lpi_cap;
+        }
         Engine<Team> e(tm, h->D, S);
         int r = e.run();
         if (tid == 0) res = r;

@@ -38,10 +38,29 @@ _CASES = [pytest.param(s, 1, id=s[0] + "-1") for s in specs()] + \
          [pytest.param(s, 256, id=s[0] + "-256") for s in specs() if s[0] in _MT]
 
 
+_CASES_LDS = [pytest.param(s, 1, id=s[0] + "-lds") for s in specs()]
+
+
+@pytest.fixture
+def host_lds(monkeypatch):
+    monkeypatch.setenv("EH_LDS", "1")
+
+
+@pytest.mark.parametrize("spec,threads", _CASES_LDS)
+def test_engine_with_onchip_areas_equals_oracle(spec, threads, host_lds):
+    """The engine with the on-chip areas the device has (Pi area, index
+    tables, pivot sets held apart from the HBM arena) equals the oracle."""
+    _compare(spec, threads)
+
+
 @pytest.mark.parametrize("spec,threads", _CASES)
 def test_engine_equals_oracle(spec, threads):
-    """The engine on a 1-thread team and on 256 threads shaped like a
-    4-wave device team equals the oracle bit for bit."""
+    _compare(spec, threads)
+
+
+def _compare(spec, threads):
+    """The engine (on a 1-thread team, or on 256 threads shaped like a 4-wave
+    device team) equals the oracle bit for bit."""
     e, t, ost = _solve_both(spec, threads)
     assert e.status == ost
     if ost:
