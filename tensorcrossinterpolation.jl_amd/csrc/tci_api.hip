@@ -170,20 +170,21 @@ tci_status tci_ctx_create(tci_ctx **out, int32_t device, uint64_t hbm_budget_byt
     }
     c->budget = hbm_budget_bytes;
     // LDS per solve workgroup: the CU's LDS shared by the workgroups that
-    // should be resident together (TCI_WG_PER_CU, default 1), at most the
-    // per-workgroup limit; the kernel's static LDS is subtracted at launch
+    // should be resident together (TCI_WG_PER_CU; 2 for the default team),
+    // at most the per-workgroup limit; the kernel's static LDS is subtracted
+    // at launch
     int lds_blk = 0, lds_cu = 0;
     HIPCHK(hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     HIPCHK(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device));
-    const char *ev = getenv("TCI_WG_PER_CU");
-    int wgs = ev ? atoi(ev) : 1;
+    const char *ev = getenv("TCI_TEAM");
+    c->team = ev ? atoi(ev) : 512;
+    if (c->team != 512 && c->team != 1024 && c->team != 5121) c->team = 512;
+    ev = getenv("TCI_WG_PER_CU");
+    int wgs = ev ? atoi(ev) : (c->team == 512 ? 2 : 1);
     if (wgs < 1) wgs = 1;
     int per_wg = (lds_cu > 0 ? lds_cu : lds_blk) / wgs;
     if (per_wg > lds_blk) per_wg = lds_blk;
     c->lds_wg = per_wg;
-    ev = getenv("TCI_TEAM");
-    c->team = ev ? atoi(ev) : 512;
-    if (c->team != 512 && c->team != 1024 && c->team != 5122) c->team = 512;
     ev = getenv("TCI_PI_LDS_KB");
     c->pi_lds_kb = ev ? atoi(ev) : 1 << 20;
     *out = c;

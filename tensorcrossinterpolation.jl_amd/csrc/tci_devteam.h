@@ -41,7 +41,7 @@ constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirr
 // register-allocated for that variant's budget
 template <int NT_, int WPE_ = 2>
 struct DevTeam {
-    static constexpr int NT = NT_, WS = 64, NW = NT / 64;
+    static constexpr int NT = NT_, WS = 64, NW = NT / 64, WPE = WPE_;
     static constexpr int RA = TileShape<NT>::RA, CB = TileShape<NT>::CB;
     static constexpr int TMR = WS * RA;   // rows of one wave's staged column
     // LDS of the register-tile rrLU (one set per kernel instantiation)

@@ -222,7 +222,8 @@ struct TileLU {
         int64_t mr = maxrank;
         if (m < mr) mr = m;
         if (n < mr) mr = n;
-        double maxerr = 0.0, err = NaN, sarg = 0.0, sschur = 0.0;
+        double maxerr = 0.0, err = NaN;
+        int64_t sarg = 0, sschur = 0;   // element counts (integers stay in scalar registers)
         int p = 0, nanx = 0, nany = 0;
         while (p < mr) {
             const int k = p;
@@ -237,7 +238,7 @@ struct TileLU {
                 if (l >= 0) orow = l + WS * i;
             }
             const double av = cand ? tm.tX(wstar, orow) : NaN;
-            sarg += (double)(m - k) * (double)(n - k);
+            sarg += (int64_t)(m - k) * (n - k);
             const double e = fabs(av);
             err = e;
             if ((e < reltol * maxerr || e < abstol) && p > 0) break;
@@ -295,23 +296,39 @@ struct TileLU {
                     if (v != v) nany = 1;
                 }
             }
-            // trailing update and the next search; the multipliers are read
-            // from LDS up front so their latency overlaps
-            double yv[CB];
-#pragma unroll
-            for (int j = 0; j < CB; ++j) yv[j] = tm.tY(tm.wave(), j);
+            // trailing update and the next search.  With a full register
+            // budget the multipliers are read from LDS up front so their
+            // latency overlaps; with half of it (two workgroups per CU) each
+            // is read as its column is updated, to keep the tile in registers
             lmax = -1.0;
+            if (Team::WPE <= 2) {
+                double yv[CB];
 #pragma unroll
-            for (int j = 0; j < CB; ++j)
+                for (int j = 0; j < CB; ++j) yv[j] = tm.tY(tm.wave(), j);
 #pragma unroll
-                for (int i = 0; i < RA; ++i) {
-                    const double t = x[i] * yv[j];
-                    const double u = a[i][j] - t;
-                    a[i][j] = u;
-                    lmax = fmax(lmax, fabs(u));
+                for (int j = 0; j < CB; ++j)
+#pragma unroll
+                    for (int i = 0; i < RA; ++i) {
+                        const double t = x[i] * yv[j];
+                        const double u = a[i][j] - t;
+                        a[i][j] = u;
+                        lmax = fmax(lmax, fabs(u));
+                    }
+            } else {
+#pragma unroll
+                for (int j = 0; j < CB; ++j) {
+                    const double yj = tm.tY(tm.wave(), j);
+#pragma unroll
+                    for (int i = 0; i < RA; ++i) {
+                        const double t = x[i] * yj;
+                        const double u = a[i][j] - t;
+                        a[i][j] = u;
+                        lmax = fmax(lmax, fabs(u));
+                    }
                 }
+            }
             p = k + 1;
-            sschur += (double)(m - p) * (double)(n - p);
+            sschur += (int64_t)(m - p) * (n - p);
             K = search(tm, lmax, wstar);
         }
         // position → original index
@@ -329,8 +346,8 @@ struct TileLU {
         }
         const int nanL = tm.any(nanx);
         const int nanU = tm.any(nany);
-        stats[0] = sarg;
-        stats[1] = sschur;
+        stats[0] = (double)sarg;
+        stats[1] = (double)sschur;
         if (p >= (m < n ? m : n)) err = 0.0;
         *perr = err;
         if (nanL) return -ST_NAN_L;

@@ -19,7 +19,7 @@ using namespace tci;
 
 struct HostTeam {
     // register tile of the one-thread team: one wave of one lane holding up to 160×160
-    static constexpr int NT = 1, WS = 1, NW = 1, RA = 160, CB = 160;
+    static constexpr int NT = 1, WS = 1, NW = 1, RA = 160, CB = 160, WPE = 1;
     double lA[1], lX[160], lY[160];
     uint32_t lB[1];
     double &tA(int) { return lA[0]; }
@@ -75,7 +75,7 @@ struct MTTile {
 
 template <int NT_, int RA_, int CB_>
 struct HostTeamMT {
-    static constexpr int NT = NT_, WS = 64, NW = NT_ / 64, RA = RA_, CB = CB_;
+    static constexpr int NT = NT_, WS = 64, NW = NT_ / 64, RA = RA_, CB = CB_, WPE = 1;
     MTTile<NT_, RA_, CB_> *tile = nullptr;
     double &tA(int w) { return tile->A[w]; }
     uint32_t &tB(int w) { return tile->B[w]; }
