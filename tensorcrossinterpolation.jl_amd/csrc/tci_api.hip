@@ -177,10 +177,10 @@ tci_status tci_ctx_create(tci_ctx **out, int32_t device, uint64_t hbm_budget_byt
     HIPCHK(hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     HIPCHK(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device));
     const char *ev = getenv("TCI_TEAM");
-    c->team = ev ? atoi(ev) : 512;
-    if (c->team != 512 && c->team != 1024 && c->team != 5121) c->team = 512;
+    c->team = ev ? atoi(ev) : 256;
+    if (c->team != 256 && c->team != 512 && c->team != 1024) c->team = 256;
     ev = getenv("TCI_WG_PER_CU");
-    int wgs = ev ? atoi(ev) : (c->team == 512 ? 2 : 1);
+    int wgs = ev ? atoi(ev) : (c->team == 256 ? 2 : 1);
     if (wgs < 1) wgs = 1;
     int per_wg = (lds_cu > 0 ? lds_cu : lds_blk) / wgs;
     if (per_wg > lds_blk) per_wg = lds_blk;

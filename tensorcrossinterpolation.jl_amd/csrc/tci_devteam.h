@@ -158,6 +158,8 @@ struct DevTeam {
     }
 
     __device__ int uniform(int x) const { return __builtin_amdgcn_readfirstlane(x); }
+    __device__ uint32_t uniform_u32(uint32_t x) const { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
+    __device__ double uniform_f64(double x) const { return readlane_f64(x, 0); }
 
     __device__ int any(int v) { return __syncthreads_or(v); }
     __device__ uint64_t clock() const { return wall_clock64(); }

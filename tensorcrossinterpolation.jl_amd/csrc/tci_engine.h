@@ -206,10 +206,32 @@ struct Engine {
     uint8_t *rows, *cols;   // combined index tables (LDS when the team has them)
     uint8_t *sets;          // pivot sets (LDS when the team has room, else the HBM arena)
     int32_t *setcnt;        // their counts
+    int32_t *rperm, *cperm; // rrLU permutations and pivots of the current bond
+    double *pvals;          // (LDS when there is room, see perm_area)
 
     ENG Engine(Team &t, const SolveDesc &d, EngShared &s)
         : tm(t), D(d), S(s), L(d.L), rows(t.lrows ? t.lrows : d.rows), cols(t.lcols ? t.lcols : d.cols),
-          sets(t.lsets ? t.lsets : d.sets), setcnt(t.lsets ? t.lsetn : d.setn) {}
+          sets(t.lsets ? t.lsets : d.sets), setcnt(t.lsets ? t.lsetn : d.setn),
+          rperm(d.rowperm), cperm(d.colperm), pvals(d.pivval) {}
+
+    // place the permutations and pivots of an m×n rrLU in the LDS Pi area at
+    // offset off (doubles) when they fit, else in the arena; returns the
+    // first free offset after them
+    ENG int64_t perm_area(int64_t off, int m, int n)
+    {
+        const int64_t ints = ((int64_t)m + n + 1) / 2;
+        const int64_t need = (ints + (m < n ? m : n) + 2) & ~(int64_t)1;
+        if (tm.lpi && off + need <= tm.lpi_cap) {
+            rperm = (int32_t *)(tm.lpi + off);
+            cperm = rperm + m;
+            pvals = tm.lpi + off + ints;
+            return off + need;
+        }
+        rperm = D.rowperm;
+        cperm = D.colperm;
+        pvals = D.pivval;
+        return off;
+    }
 
     // a Pi matrix of m×n doubles lives in LDS when it fits, else in HBM
     ENG double *pibuf(int m, int n) const
@@ -340,23 +362,24 @@ struct Engine {
         const int cmplen = rowside ? b : jlen(b + 1);
         const int off = rowside ? 0 : 1;
         const int elen = rowside ? b + 1 : jlen(b);
-        for (int e = tm.tid(); e < ne; e += tm.nt()) {
-            const uint8_t *x = E + (int64_t)e * L + off;
-            bool found = false;
-            for (int i = 0; i < nref && !found; ++i) found = eq(R + (int64_t)i * L, x, cmplen);
-            D.flags[e] = found ? 0 : 1;
+        // membership of every (extra, reference entry) pair in parallel; the
+        // flags live in the (idle) LDS Pi area when there is one
+        uint8_t *fl = (tm.lpi && tm.lpi_cap * 8 >= ne) ? (uint8_t *)tm.lpi : D.flags;
+        for (int e = tm.tid(); e < ne; e += tm.nt()) fl[e] = 1;
+        tm.sync();
+        for (int64_t t = tm.tid(); t < (int64_t)ne * nref; t += tm.nt()) {
+            const int e = (int)(t / nref), i = (int)(t - (int64_t)e * nref);
+            if (eq(R + (int64_t)i * L, E + (int64_t)e * L + off, cmplen)) fl[e] = 0;
         }
         tm.sync();
-        // kept extras go after the kron part, in their order: each thread
-        // counts the kept ones before its own (ne is at most a set capacity)
         uint8_t *dst = rowside ? rows : cols;
         int total = 0;
-        for (int e = 0; e < ne; ++e) total += D.flags[e];
+        for (int e = 0; e < ne; ++e) total += fl[e];
         if (m0 + total > D.mrcap) return fail(ST_CAPACITY);
         for (int e = tm.tid(); e < ne; e += tm.nt()) {
-            if (!D.flags[e]) continue;
+            if (!fl[e]) continue;
             int pos = m0;
-            for (int e2 = 0; e2 < e; ++e2) pos += D.flags[e2];
+            for (int e2 = 0; e2 < e; ++e2) pos += fl[e2];
             for (int t = 0; t < elen; ++t) dst[(int64_t)pos * L + t] = E[(int64_t)e * L + t];
         }
         tm.sync();
@@ -437,18 +460,18 @@ struct Engine {
             R.load(tm, A, m);
             // L recorded by original row in lscratch, then permuted into A
             double *Lorig = need_left ? lscratch : nullptr;
-            p = R.factor(tm, maxrank, reltol, abstol, leftorth, D.pivval, perr, st, Lorig,
-                         D.rowperm, D.colperm);
+            p = R.factor(tm, maxrank, reltol, abstol, leftorth, pvals, perr, st, Lorig,
+                         rperm, cperm);
             if (need_left && p > 0) {
                 tm.sync();
                 for (int t = tm.wave(); t < p; t += tm.nwaves())
                     for (int pos = t + 1 + tm.lane(); pos < m; pos += tm.wsize())
-                        A[pos + (int64_t)m * t] = Lorig[D.rowperm[pos] + (int64_t)m * t];
+                        A[pos + (int64_t)m * t] = Lorig[rperm[pos] + (int64_t)m * t];
             }
             tm.sync();
         } else {
-            p = rrlu_team(tm, A, m, n, maxrank, reltol, abstol, leftorth, D.rowperm, D.colperm,
-                          D.pivval, perr, st);
+            p = rrlu_team(tm, A, m, n, maxrank, reltol, abstol, leftorth, rperm, cperm,
+                          pvals, perr, st);
         }
         if (tm.tid() == 0) {
             S.tph[PH_RRLU] += tm.clock() - t0;
@@ -470,7 +493,7 @@ struct Engine {
         const int n = npe > p + 1 ? npe : p + 1;
         for (int i = tm.tid(); i < n; i += tm.nt()) {
             const double a = i < npe ? D.pivoterrors[i] : 0.0;
-            const double e = i < p ? fabs(D.pivval[i]) : (i == p ? err : 0.0);
+            const double e = i < p ? fabs(pvals[i]) : (i == p ? err : 0.0);
             D.pivoterrors[i] = jl_max(a, e);
         }
         tm.sync();
@@ -526,8 +549,8 @@ struct Engine {
         }
         const uint64_t t1 = tm.clock();
         double st[4];
-        int p = R.factor(tm, maxrank, reltol, abstol, leftorth, D.pivval, perr, st, nullptr,
-                         D.rowperm, D.colperm);
+        int p = R.factor(tm, maxrank, reltol, abstol, leftorth, pvals, perr, st, nullptr,
+                         rperm, cperm);
         tm.sync();
         if (tm.tid() == 0) {
             S.tph[PH_RRLU] += tm.clock() - t1;
@@ -559,17 +582,19 @@ struct Engine {
         int p;
         const uint64_t t1 = tm.clock();
         if (lorentz_tile(m, n)) {
+            perm_area(0, m, n);
             p = eval_rrlu_lorentz(m, b + 1, n, D.maxbonddim, 1e-14, abstol, leftorth, &err);
         } else {
             double *A = pibuf(m, n);
+            perm_area(A == tm.lpi ? (((int64_t)m * n + 1) & ~(int64_t)1) : 0, m, n);
             eval(rows, b + 1, m, cols, n, A, m, false, true);
             p = rrlu(A, m, n, D.maxbonddim, 1e-14, abstol, leftorth, &err);
         }
         if (p < 0) return fail(-p);
         if (p > D.setcap) return fail(ST_CAPACITY);
         const uint64_t t2 = tm.clock();
-        select_into(SET_I, b + 1, rows, D.rowperm, p, b + 1);
-        select_into(SET_J, b, cols, D.colperm, p, jlen(b));
+        select_into(SET_I, b + 1, rows, rperm, p, b + 1);
+        select_into(SET_J, b, cols, cperm, p, jlen(b));
         updateerrors(b, p, err);
         tm.sync();
         if (tm.tid() == 0) S.tph[PH_SETS] += (t1 - t0) + (tm.clock() - t2);
@@ -927,6 +952,7 @@ struct Engine {
             load_set(cols, SET_J, b, jlen(b));
             tm.sync();
             double *A = pibuf(m, nJ);
+            int64_t off = perm_area(A == tm.lpi ? (((int64_t)m * nJ + 1) & ~(int64_t)1) : 0, m, nJ);
             eval(rows, b + 1, m, cols, nJ, A, m, false, true);
             double err;
             int p = rrlu(A, m, nJ, D.maxbonddim, 1e-14, abstol, true, &err, true, Tslot(b));
@@ -935,17 +961,16 @@ struct Engine {
                 return fail(-p);
             }
             if (p > D.setcap || (int64_t)m * p > D.tstride) return fail(ST_CAPACITY);
-            select_into(SET_I, b + 1, rows, D.rowperm, p, b + 1);
-            select_into(SET_J, b, cols, D.colperm, p, jlen(b));
+            select_into(SET_I, b + 1, rows, rperm, p, b + 1);
+            select_into(SET_J, b, cols, cperm, p, jlen(b));
             // left(luci) = colstimespivotinv (matrixluci.jl:48-57): rows at
             // positions < p are unit vectors, the rest solve x·L11 = L[i,1:p]
             // (L11 has a unit diagonal: s / 1.0 == s); solved rows go to LDS
             // when they fit, then are scattered to T in original row order
             double *T = Tslot(b);
-            int64_t off = A == tm.lpi ? (int64_t)m * nJ : 0;
             double *X = lds_take((int64_t)m * p, off);
             for (int i = p + tm.tid(); i < m; i += tm.nt()) {
-                double *row = X ? X + i : T + D.rowperm[i];
+                double *row = X ? X + i : T + rperm[i];
                 for (int j = p - 1; j >= 0; --j) {
                     double s = A[i + (int64_t)m * j];
                     for (int t = j + 1; t < p; ++t) s = s - row[(int64_t)m * t] * A[t + (int64_t)m * j];
@@ -955,8 +980,8 @@ struct Engine {
             if (X) tm.sync();
             for (int64_t q = tm.tid(); q < (int64_t)m * p; q += tm.nt()) {
                 const int64_t j = q / m, i = q - j * m;
-                if (i < p) T[D.rowperm[i] + (int64_t)m * j] = (i == j) ? 1.0 : 0.0;
-                else if (X) T[D.rowperm[i] + (int64_t)m * j] = X[q];
+                if (i < p) T[rperm[i] + (int64_t)m * j] = (i == j) ? 1.0 : 0.0;
+                else if (X) T[rperm[i] + (int64_t)m * j] = X[q];
             }
             setTdims(b, nI, d, p);
             tm.sync();

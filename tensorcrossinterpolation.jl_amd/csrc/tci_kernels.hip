@@ -176,16 +176,16 @@ static const void *solve_kernel_ptr(int team, int fkind)
     case 1024:
         return fkind == TCI_F_LORENTZ_INT ? (const void *)tci_solve_kernel<1024, TCI_F_LORENTZ_INT, 4>
                                           : (const void *)tci_solve_kernel<1024, 0, 4>;
-    case 5121:
+    case 512:
         return fkind == TCI_F_LORENTZ_INT ? (const void *)tci_solve_kernel<512, TCI_F_LORENTZ_INT, 2>
                                           : (const void *)tci_solve_kernel<512, 0, 2>;
     default:
         switch (fkind) {
-        case TCI_F_LORENTZ_INT: return (const void *)tci_solve_kernel<512, TCI_F_LORENTZ_INT, 4>;
-        case TCI_F_LORENTZ_WEIGHTED: return (const void *)tci_solve_kernel<512, TCI_F_LORENTZ_WEIGHTED, 4>;
-        case TCI_F_QUANTICS_GAUSS_COS: return (const void *)tci_solve_kernel<512, TCI_F_QUANTICS_GAUSS_COS, 4>;
-        case TCI_F_GAUSS_SUM: return (const void *)tci_solve_kernel<512, TCI_F_GAUSS_SUM, 4>;
-        default: return (const void *)tci_solve_kernel<512, 0, 4>;
+        case TCI_F_LORENTZ_INT: return (const void *)tci_solve_kernel<256, TCI_F_LORENTZ_INT, 2>;
+        case TCI_F_LORENTZ_WEIGHTED: return (const void *)tci_solve_kernel<256, TCI_F_LORENTZ_WEIGHTED, 2>;
+        case TCI_F_QUANTICS_GAUSS_COS: return (const void *)tci_solve_kernel<256, TCI_F_QUANTICS_GAUSS_COS, 2>;
+        case TCI_F_GAUSS_SUM: return (const void *)tci_solve_kernel<256, TCI_F_GAUSS_SUM, 2>;
+        default: return (const void *)tci_solve_kernel<256, 0, 2>;
         }
     }
 }
@@ -232,7 +232,7 @@ hipError_t launch_solve(const SolveDesc *descs, int count, int team, int fkind, 
     if (e != hipSuccess) return e;
     const int stat = solve_static_lds(team, fkind);
     if (stat < 0 || (size_t)stat + lds > (size_t)lds_blk) return hipErrorInvalidValue;
-    // team shapes: 512 threads at 128 VGPRs, two solves per CU (the default,
+    // team shapes: 256 threads at 256 VGPRs, two solves per CU (the default,
     // every functor specialized); 512 threads at 256 VGPRs, one solve per CU;
     // 1024 threads at 128 VGPRs
     switch (team) {
@@ -240,11 +240,11 @@ hipError_t launch_solve(const SolveDesc *descs, int count, int team, int fkind, 
         if (fkind == TCI_F_LORENTZ_INT) launch_one<1024, TCI_F_LORENTZ_INT, 4>(descs, count, plan, lds, st);
         else launch_one<1024, 0, 4>(descs, count, plan, lds, st);
         break;
-    case 5121:
+    case 512:
         if (fkind == TCI_F_LORENTZ_INT) launch_one<512, TCI_F_LORENTZ_INT, 2>(descs, count, plan, lds, st);
         else launch_one<512, 0, 2>(descs, count, plan, lds, st);
         break;
-    default: launch_nt<512, 4>(descs, count, fkind, plan, lds, st); break;
+    default: launch_nt<256, 2>(descs, count, fkind, plan, lds, st); break;
     }
     return hipGetLastError();
 }

@@ -122,6 +122,7 @@ struct TileLU {
         double g = tm.tA(0);
 #pragma unroll
         for (int q = 1; q < NW; ++q) g = fmax(g, tm.tA(q));
+        g = tm.uniform_f64(g);   // the same in every lane: keep it scalar
         uint32_t kw = 0xffffffffu;
         if (g >= 0.0) {
             const double G = g * g;
@@ -191,11 +192,11 @@ struct TileLU {
         }
         if (tm.lane() == 0) tm.tB(tm.wave()) = kw;
         tm.sync();                                            // B
-        uint32_t K = tm.tB(0);
+        uint32_t K = tm.uniform_u32(tm.tB(0));
         wstar = 0;
 #pragma unroll
         for (int q = 1; q < NW; ++q) {
-            const uint32_t v = tm.tB(q);
+            const uint32_t v = tm.uniform_u32(tm.tB(q));
             if (v < K) { K = v; wstar = q; }
         }
         return K;
@@ -237,7 +238,7 @@ struct TileLU {
                 const int l = tm.wave_first(rp[i] == pr);
                 if (l >= 0) orow = l + WS * i;
             }
-            const double av = cand ? tm.tX(wstar, orow) : NaN;
+            const double av = cand ? tm.uniform_f64(tm.tX(wstar, orow)) : NaN;
             sarg += (int64_t)(m - k) * (n - k);
             const double e = fabs(av);
             err = e;

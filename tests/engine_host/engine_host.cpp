@@ -43,6 +43,8 @@ struct HostTeam {
     void wave_sync_lds() {}
     ArgBest argmax(ArgBest b) { return b; }
     int uniform(int x) const { return x; }
+    uint32_t uniform_u32(uint32_t x) const { return x; }
+    double uniform_f64(double x) const { return x; }
     double max_nan(double v) { return v; }
     double wave_max(double v) const { return v; }
     uint32_t wave_min(uint32_t v) const { return v; }
@@ -170,6 +172,8 @@ struct HostTeamMT {
         return r;
     }
     int uniform(int x) const { return x; }
+    uint32_t uniform_u32(uint32_t x) const { return x; }
+    double uniform_f64(double x) const { return x; }
     uint64_t clock() const { return 0; }
 };
 
