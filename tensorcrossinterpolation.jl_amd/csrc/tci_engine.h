@@ -799,6 +799,7 @@ struct Engine {
 
     // ------------------------------------------------------ global search
     // DefaultGlobalPivotFinder (globalpivotfinder.jl:143-195), project PRNG.
+    static constexpr int kChainReg = 16, kChainCand = 2;
     ENG int globalsearch(double abstol)
     {
         const int ns = D.nsearchglobalpivot;
@@ -830,60 +831,115 @@ struct Engine {
             const int64_t w = t < L ? D.Tdims[3 * t] : 1;
             chi = w > chi ? (int)w : chi;
         }
-        // TT values of all candidates, site by site (abstracttensortrain.jl:131,
-        // each product summed in ascending bond order), chunks of candidates;
-        // keep the running vectors in LDS, and each site tensor too when it fits
-        int64_t off = 0, tmax = 0;
+        int64_t toff = 0, tmax = 0;
         for (int t = 0; t < L; ++t) {
             const int64_t sz = D.Tdims[3 * t] * D.Tdims[3 * t + 1] * D.Tdims[3 * t + 2];
             tmax = sz > tmax ? sz : tmax;
         }
-        double *Ts = lds_take(tmax, off);
-        const int64_t room = tm.lpi_cap > off ? tm.lpi_cap - off : 0;
-        int chunk = (int)(room / (2 * (int64_t)chi));
-        double *V = nullptr, *W = nullptr;
-        if (chunk >= 1) {
-            if (chunk > ncand) chunk = ncand;
-            V = lds_take((int64_t)chunk * chi, off);
-            W = lds_take((int64_t)chunk * chi, off);
-        }
-        if (!V || !W) {
-            chunk = ncand;
-            V = D.work + ncand;
-            W = V + (int64_t)ncand * chi;
-        }
-        tm.sync();
-        for (int c0 = 0; c0 < ncand; c0 += chunk) {
-            const int cn = ncand - c0 < chunk ? ncand - c0 : chunk;
-            for (int c = tm.tid(); c < cn; c += tm.nt()) V[(int64_t)c * chi] = 1.0;
-            tm.sync();
+        if (chi <= kChainReg && ncand <= kChainCand * tm.nt()) {
+            // bond dimensions up to 16: each thread carries its (at most two)
+            // candidates' running vectors in registers; one site tensor at a
+            // time is staged in LDS (the same products and summation order
+            // as below)
+            double *Ts = lds_take(tmax, toff);
+            double v[kChainCand][kChainReg];
+#pragma unroll
+            for (int q = 0; q < kChainCand; ++q)
+#pragma unroll
+                for (int i = 0; i < kChainReg; ++i) v[q][i] = i == 0 ? 1.0 : 0.0;
             for (int t = 0; t < L; ++t) {
                 const int64_t d0 = D.Tdims[3 * t], d = D.Tdims[3 * t + 1], d2 = D.Tdims[3 * t + 2];
                 const double *T = D.T + D.tstride * t;
                 if (Ts) {
+                    tm.sync();
                     for (int64_t q = tm.tid(); q < d0 * d * d2; q += tm.nt()) Ts[q] = T[q];
                     tm.sync();
                     T = Ts;
                 }
-                for (int64_t o = tm.tid(); o < (int64_t)cn * d2; o += tm.nt()) {
-                    const int64_t c = o / d2, j = o - c * d2;
-                    const int64_t s = cand[(c0 + c) * L + t] - 1;
-                    const double *Tc = T + d0 * (s + d * j);
-                    const double *v = V + c * chi;
-                    double acc = 0.0;
-                    for (int64_t i = 0; i < d0; ++i) acc = acc + v[i] * Tc[i];
-                    W[c * chi + j] = acc;
+#pragma unroll
+                for (int q = 0; q < kChainCand; ++q) {
+                    const int c = tm.tid() + q * tm.nt();
+                    if (c < ncand) {
+                        const double *Tc = T + d0 * (cand[(int64_t)c * L + t] - 1);
+                        double w[kChainReg];
+#pragma unroll
+                        for (int j = 0; j < kChainReg; ++j) {
+                            double acc = 0.0;
+                            if (j < d2) {
+#pragma unroll
+                                for (int i = 0; i < kChainReg; ++i)
+                                    if (i < d0) acc = acc + v[q][i] * Tc[i + d0 * d * j];
+                            }
+                            w[j] = acc;
+                        }
+#pragma unroll
+                        for (int j = 0; j < kChainReg; ++j) v[q][j] = w[j];
+                    }
                 }
-                tm.sync();
-                double *tmp = V; V = W; W = tmp;
             }
-            for (int c = tm.tid(); c < cn; c += tm.nt()) {
-                tci_point pt;
-                pt.a = cand + (int64_t)(c0 + c) * L; pt.b = pt.a; pt.na = L; pt.L = L;
-                const double fv = tci_functor_eval(FK ? FK : D.fkind, D.params, D.iparams, &pt);
-                errs[c0 + c] = fabs(fv - V[(int64_t)c * chi]);
+#pragma unroll
+            for (int q = 0; q < kChainCand; ++q) {
+                const int c = tm.tid() + q * tm.nt();
+                if (c < ncand) {
+                    tci_point pt;
+                    pt.a = cand + (int64_t)c * L; pt.b = pt.a; pt.na = L; pt.L = L;
+                    const double fv = tci_functor_eval(FK ? FK : D.fkind, D.params, D.iparams, &pt);
+                    errs[c] = fabs(fv - v[q][0]);
+                }
             }
             tm.sync();
+        } else {
+            // TT values of all candidates, site by site (abstracttensortrain.jl:131,
+            // each product summed in ascending bond order), chunks of candidates;
+            // keep the running vectors in LDS, and each site tensor too when it fits
+            int64_t off = 0;
+            double *Ts = lds_take(tmax, off);
+            const int64_t room = tm.lpi_cap > off ? tm.lpi_cap - off : 0;
+            int chunk = (int)(room / (2 * (int64_t)chi));
+            double *V = nullptr, *W = nullptr;
+            if (chunk >= 1) {
+                if (chunk > ncand) chunk = ncand;
+                V = lds_take((int64_t)chunk * chi, off);
+                W = lds_take((int64_t)chunk * chi, off);
+            }
+            if (!V || !W) {
+                chunk = ncand;
+                V = D.work + ncand;
+                W = V + (int64_t)ncand * chi;
+            }
+            tm.sync();
+            for (int c0 = 0; c0 < ncand; c0 += chunk) {
+                const int cn = ncand - c0 < chunk ? ncand - c0 : chunk;
+                for (int c = tm.tid(); c < cn; c += tm.nt()) V[(int64_t)c * chi] = 1.0;
+                tm.sync();
+                for (int t = 0; t < L; ++t) {
+                    const int64_t d0 = D.Tdims[3 * t], d = D.Tdims[3 * t + 1], d2 = D.Tdims[3 * t + 2];
+                    const double *T = D.T + D.tstride * t;
+                    if (Ts) {
+                        for (int64_t q = tm.tid(); q < d0 * d * d2; q += tm.nt()) Ts[q] = T[q];
+                        tm.sync();
+                        T = Ts;
+                    }
+                    for (int64_t o = tm.tid(); o < (int64_t)cn * d2; o += tm.nt()) {
+                        const int64_t c = o / d2, j = o - c * d2;
+                        const int64_t s = cand[(c0 + c) * L + t] - 1;
+                        const double *Tc = T + d0 * (s + d * j);
+                        const double *v = V + c * chi;
+                        double acc = 0.0;
+                        for (int64_t i = 0; i < d0; ++i) acc = acc + v[i] * Tc[i];
+                        W[c * chi + j] = acc;
+                    }
+                    tm.sync();
+                    double *tmp = V; V = W; W = tmp;
+                }
+                for (int c = tm.tid(); c < cn; c += tm.nt()) {
+                    tci_point pt;
+                    pt.a = cand + (int64_t)(c0 + c) * L; pt.b = pt.a; pt.na = L; pt.L = L;
+                    const double fv = tci_functor_eval(FK ? FK : D.fkind, D.params, D.iparams, &pt);
+                    errs[c0 + c] = fabs(fv - V[(int64_t)c * chi]);
+                }
+                tm.sync();
+            }
         }
         // per point, the first candidate with the largest error (a strict
         // maximum over errors > 0, as in the reference's scan), one wave per
