@@ -69,7 +69,8 @@ struct tci_ctx {
 // Launch shape of the solve kernel: a team of `team` threads per solve; LDS
 // holds the rrLU staging vectors, the index tables and as much of a Pi
 // matrix as the budget allows (one resident workgroup per CU when large).
-static LdsPlan make_plan(const tci_ctx *c, int64_t left, int max_mrcap, int max_tab, int64_t max_sets)
+static LdsPlan make_plan(const tci_ctx *c, int64_t left, int max_mrcap, int max_tab, int64_t max_sets,
+                         int64_t max_lut)
 {
     LdsPlan p;
     memset(&p, 0, sizeof p);
@@ -89,6 +90,11 @@ static LdsPlan make_plan(const tci_ctx *c, int64_t left, int max_mrcap, int max_
     if (max_sets > 0 && sb <= left / 4) {
         p.sets_cap = (int32_t)sb;
         left -= sb;
+    }
+    // the Lorentzian value table (a few KiB) when it fits in a quarter
+    if (max_lut > 0 && max_lut * 8 <= left / 4) {
+        p.lut_cap = (int32_t)((max_lut + 1) & ~(int64_t)1);
+        left -= (int64_t)p.lut_cap * 8;
     }
     int64_t pib = (int64_t)c->pi_lds_kb * 1024;
     if (pib > left) pib = left;
@@ -473,7 +479,7 @@ tci_status tci_solve_submit(tci_ctx *ctx, int32_t count, const tci_problem *prob
     std::vector<size_t> ib((size_t)count), sb((size_t)count), rb((size_t)count);
     size_t tin = 0, tstate = 0, trep = 0;
     int max_mr = 0, max_tab = 0;
-    int64_t max_sets = 0;
+    int64_t max_sets = 0, max_lut = 0;
     int fkind = problems[0].f.kind;
     for (int i = 0; i < count; ++i) {
         if (problems[i].f.kind != fkind) fkind = 0;
@@ -491,6 +497,12 @@ tci_status tci_solve_submit(tci_ctx *ctx, int32_t count, const tci_problem *prob
         const int64_t sbytes = ((((int64_t)NSETKIND * cp.L * cp.setcap * cp.L) + 15) & ~(int64_t)15) +
                                4 * (int64_t)NSETKIND * cp.L;
         if (sbytes > max_sets) max_sets = sbytes;
+        if (problems[i].f.kind == TCI_F_LORENTZ_INT) {
+            int64_t smax = 2;
+            for (int t = 0; t < problems[i].L; ++t)
+                smax += (int64_t)problems[i].localdims[t] * problems[i].localdims[t];
+            if (smax > max_lut) max_lut = smax;
+        }
     }
     const size_t total = tin + tstate + trep + sizeof(SolveDesc) * (size_t)count;
     if (ctx->used + total > ctx->budget)
@@ -543,7 +555,7 @@ tci_status tci_solve_submit(tci_ctx *ctx, int32_t count, const tci_problem *prob
     if ((e = hipEventRecord(b->ev0, st)) != hipSuccess) return fail(e, "hipEventRecord");
     const int stat = solve_static_lds(ctx->team, fkind);
     if (stat < 0) return fail(hipErrorInvalidValue, "solve kernel attributes");
-    const LdsPlan plan = make_plan(ctx, (int64_t)ctx->lds_wg - stat, max_mr, max_tab, max_sets);
+    const LdsPlan plan = make_plan(ctx, (int64_t)ctx->lds_wg - stat, max_mr, max_tab, max_sets, max_lut);
     if ((e = launch_solve(b->d_desc, count, ctx->team, fkind, plan, st)) != hipSuccess) return fail(e, "solve launch");
     if ((e = hipEventRecord(b->ev1, st)) != hipSuccess) return fail(e, "hipEventRecord");
     *out = b;

@@ -115,6 +115,8 @@ struct DevTeam {
     uint8_t *lrows, *lcols;
     uint8_t *lsets;       // LDS pivot sets (null: in HBM)
     int32_t *lsetn;       // LDS set counts
+    double *llut;         // LDS table of Lorentzian values (null: none)
+    int64_t llut_cap;
     // team-reduction slots, double-buffered by par_a / par_d
     __device__ static ArgBest *lds_ra() { __shared__ ArgBest b[2 * NW]; return b; }
     __device__ static double *lds_rd() { __shared__ double b[2 * NW]; return b; }
@@ -175,6 +177,8 @@ struct DevTeam {
         lcols = nullptr;
         lsets = nullptr;
         lsetn = nullptr;
+        llut = nullptr;
+        llut_cap = 0;
     }
 };
 

@@ -208,11 +208,26 @@ struct Engine {
     int32_t *setcnt;        // their counts
     int32_t *rperm, *cperm; // rrLU permutations and pivots of the current bond
     double *pvals;          // (LDS when there is room, see perm_area)
+    const double *lut;      // Lorentzian values by integer sum (LDS), or null
 
     ENG Engine(Team &t, const SolveDesc &d, EngShared &s)
         : tm(t), D(d), S(s), L(d.L), rows(t.lrows ? t.lrows : d.rows), cols(t.lcols ? t.lcols : d.cols),
           sets(t.lsets ? t.lsets : d.sets), setcnt(t.lsets ? t.lsetn : d.setn),
-          rperm(d.rowperm), cperm(d.colperm), pvals(d.pivval) {}
+          rperm(d.rowperm), cperm(d.colperm), pvals(d.pivval), lut(nullptr) {}
+
+    // lut[s] = c / (double)s, s = 1 .. Σ d², for the Lorentzian functor when
+    // the team has room for it (the same division the functor performs)
+    ENG void build_lut()
+    {
+        if (!(FK == TCI_F_LORENTZ_INT || (FK == 0 && D.fkind == TCI_F_LORENTZ_INT)) || !tm.llut) return;
+        int64_t smax = 0;
+        for (int t = 0; t < L; ++t) smax += (int64_t)D.localdims[t] * D.localdims[t];
+        if (smax + 2 > tm.llut_cap) return;
+        const double c = D.params[0];
+        for (int64_t q = 1 + tm.tid(); q <= smax + 1; q += tm.nt()) tm.llut[q] = c / (double)q;
+        tm.sync();
+        lut = tm.llut;
+    }
 
     // place the permutations and pivots of an m×n rrLU in the LDS Pi area at
     // offset off (doubles) when they fit, else in the arena; returns the
@@ -539,7 +554,7 @@ struct Engine {
         tm.sync();
         Tile R;
         R.setup(tm, m, n);
-        const double lmax = R.fill_lorentz(tm, D.params[0], tm.xs, tm.ys);
+        const double lmax = R.fill_lorentz(tm, D.params[0], tm.xs, tm.ys, lut);
         const double g = tm.max_nan(lmax);
         if (tm.tid() == 0) {
             S.maxsample = jl_max(fabs(S.maxsample), g);
@@ -1093,6 +1108,7 @@ struct Engine {
             for (int b = 0; b < L; ++b) { D.Tdims[3 * b] = 0; D.Tdims[3 * b + 1] = 0; D.Tdims[3 * b + 2] = 0; }
         }
         tm.sync();
+        build_lut();
         int st = addglobalpivots(D.initpivots, D.npivinit);
         if (st != ST_OK) return finish(st);
         // maxsamplevalue = maximum(abs, f(x) for x in initialpivots) (49-50)

@@ -13,7 +13,7 @@ struct LdsPlan {
     int32_t stage_cap;   // rrLU staging vectors: 2 × stage_cap doubles (0: in HBM)
     int32_t tab_cap;     // bytes per index table (rows, cols) (0: in HBM)
     int32_t sets_cap;    // bytes for the pivot sets and their counts (0: in HBM)
-    int32_t pad;
+    int32_t lut_cap;     // doubles for the Lorentzian value table (0: none)
     int64_t pi_cap;      // doubles available for a Pi matrix (0: in HBM)
 };
 size_t lds_plan_bytes(const LdsPlan &p);

@@ -26,7 +26,8 @@ constexpr int kTeam = 256;
 // ---------------------------------------------------------------- solves
 // Dynamic LDS of a solve workgroup (sizes fixed per launch by LdsPlan):
 //   [stage: 2·stage_cap doubles][rows: tab_cap bytes][cols: tab_cap bytes]
-//   [sets: sets_cap bytes (the pivot sets, then their counts)][Pi area: pi_cap doubles]
+//   [sets: sets_cap bytes (the pivot sets, then their counts)]
+//   [Lorentzian value table: lut_cap doubles][Pi area: pi_cap doubles]
 // Anything that does not fit stays in the solve's HBM arena.
 template <int NT, int FK, int WPE>
 __global__ __launch_bounds__(NT, WPE) void tci_solve_kernel(const SolveDesc *__restrict__ descs,
@@ -41,7 +42,8 @@ __global__ __launch_bounds__(NT, WPE) void tci_solve_kernel(const SolveDesc *__r
     double *stage = dyn;
     uint8_t *tabs = (uint8_t *)(dyn + 2 * plan.stage_cap);
     uint8_t *lsets = tabs + 2 * plan.tab_cap;
-    double *pi = (double *)(lsets + plan.sets_cap);
+    double *lut = (double *)(lsets + plan.sets_cap);
+    double *pi = lut + plan.lut_cap;
     if (D.mrcap <= plan.stage_cap) tm.init(stage, stage + D.mrcap);
     else tm.init(D.stage, D.stage + D.mrcap);
     if ((int64_t)D.mrcap * D.L <= plan.tab_cap) {
@@ -53,6 +55,8 @@ __global__ __launch_bounds__(NT, WPE) void tci_solve_kernel(const SolveDesc *__r
         tm.lsets = lsets;
         tm.lsetn = (int32_t *)(lsets + ((setbytes + 15) & ~(int64_t)15));
     }
+    tm.llut = plan.lut_cap ? lut : nullptr;
+    tm.llut_cap = plan.lut_cap;
     tm.lpi = pi;
     tm.lpi_cap = plan.pi_cap;
     Engine<DevTeam<NT, WPE>, FK> eng(tm, D, S);
@@ -166,7 +170,7 @@ __global__ void tci_eval_kernel(int kind, const double *__restrict__ params,
 size_t lds_plan_bytes(const LdsPlan &p)
 {
     return (size_t)2 * p.stage_cap * sizeof(double) + 2 * (size_t)p.tab_cap + (size_t)p.sets_cap +
-           (size_t)p.pi_cap * sizeof(double);
+           (size_t)p.lut_cap * sizeof(double) + (size_t)p.pi_cap * sizeof(double);
 }
 
 // the kernel variant a (team, functor kind) pair runs

@@ -82,9 +82,12 @@ struct TileLU {
 
     // f = c / (1 + rs[r] + cs[c]) for TCI_F_LORENTZ_INT, whose value is
     // c / (double)(Σ v² + 1) with an exact integer sum split into the row
-    // part rs and the column part cs.  Returns this thread's NaN-propagating
-    // max|f| (the maxsamplevalue contribution).
-    ENG_INLINE double fill_lorentz(const Team &tm, double c, const double *rs, const double *cs)
+    // part rs and the column part cs.  With lut (lut[s] = c / (double)s for
+    // every sum that occurs) the division is a lookup with the same bits.
+    // Returns this thread's NaN-propagating max|f| (the maxsamplevalue
+    // contribution).
+    ENG_INLINE double fill_lorentz(const Team &tm, double c, const double *rs, const double *cs,
+                                   const double *lut)
     {
         double rsv[RA], csv[CB];
 #pragma unroll
@@ -98,6 +101,21 @@ struct TileLU {
             csv[j] = cc < n ? cs[cc] : 0.0;
         }
         double lmax = 0.0;
+        if (lut) {
+            bool any = false;
+#pragma unroll
+            for (int j = 0; j < CB; ++j)
+#pragma unroll
+                for (int i = 0; i < RA; ++i) {
+                    const bool ok = rp[i] < m && tm.wave() + NW * j < n;
+                    const double v = lut[(int)(rsv[i] + csv[j]) + 1];
+                    a[i][j] = ok ? v : 0.0 / 0.0;
+                    lmax = ok ? fmax(lmax, fabs(v)) : lmax;
+                    any = any || ok;
+                }
+            if (any && c != c) lmax = c;   // every value is NaN: max propagates it
+            return lmax;
+        }
 #pragma unroll
         for (int j = 0; j < CB; ++j)
 #pragma unroll

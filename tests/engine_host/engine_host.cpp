@@ -38,6 +38,8 @@ struct HostTeam {
     uint8_t *lrows = nullptr, *lcols = nullptr;
     uint8_t *lsets = nullptr;
     int32_t *lsetn = nullptr;
+    double *llut = nullptr;
+    int64_t llut_cap = 0;
     void sync() {}
     void wave_sync() {}
     void wave_sync_lds() {}
@@ -95,6 +97,8 @@ struct HostTeamMT {
     uint8_t *lrows = nullptr, *lcols = nullptr;
     uint8_t *lsets = nullptr;
     int32_t *lsetn = nullptr;
+    double *llut = nullptr;
+    int64_t llut_cap = 0;
     MTShared *sh = nullptr;
     void sync() { sh->bar->arrive_and_wait(); }
     void wsync() { sh->wbar[wave()]->arrive_and_wait(); }
@@ -228,7 +232,7 @@ extern "C" int eh_rrlu_mt(int mode, double *A, int m, int n, int64_t maxrank, do
 
 struct EhSolve {
     std::vector<uint8_t> inputs, arena, report;
-    std::vector<double> lpi;          // stands in for the workgroup's LDS areas
+    std::vector<double> lpi, llut;    // stand in for the workgroup's LDS areas
     std::vector<uint8_t> ltab, lsets;
     SolveDesc D;
     int status;
@@ -251,6 +255,9 @@ static void attach_lds(Team &tm, EhSolve *h, const SolveCaps &c)
     h->lpi.assign(16384, 0.0);        // 128 KiB
     tm.lpi = h->lpi.data();
     tm.lpi_cap = (int64_t)h->lpi.size();
+    h->llut.assign(8192, 0.0);
+    tm.llut = h->llut.data();
+    tm.llut_cap = (int64_t)h->llut.size();
 }
 
 static std::string g_msg;
@@ -333,6 +340,7 @@ extern "C" void *eh_solve_mt(const tci_problem *p, int *status)
         if (lds) {
             tm.lrows = proto.lrows; tm.lcols = proto.lcols; tm.lsets = proto.lsets;
             tm.lsetn = proto.lsetn; tm.lpi = proto.lpi; tm.lpi_cap = proto.lpi_cap;
+            tm.llut = proto.llut; tm.llut_cap = proto.llut_cap;
         }
         Engine<Team> e(tm, h->D, S);
         int r = e.run();
