@@ -616,89 +616,6 @@ struct Engine {
         return ST_OK;
     }
 
-    // ------------------------------------------------- Pi1·P⁻¹ (site tensor)
-    // T_b = transpose(transpose(P) \ transpose(Pi1)) (tensorci2.jl:391):
-    // LU with partial pivoting of A = Pᵀ (c×c, first maximum |.| per column),
-    // done by one thread (c is a bond dimension, small), then forward and
-    // back substitution for each of the r rows of Pi1 in parallel.  A, Pi1
-    // and X may live in LDS; X (r×c) is the result.
-    // ipiv: 2c ints (the pivots, then the permutation they compose)
-    ENG int solve_right(const double *Pi1, int r, double *A, int c, double *X, int32_t *ipiv, bool onchip)
-    {
-        // LU with partial pivoting of A = Pᵀ by wave 0: lanes stride over rows
-        // for the pivot search, the scaling and the update (each element sees
-        // A[i,j] - A[i,k]·A[k,j] once per k, as in the serial elimination)
-        // and over columns for the row swap
-        if (tm.wave() == 0) {
-            int singular = 0;
-            for (int k = 0; k < c; ++k) {
-                // first maximum |A[i,k]| over i >= k (strictly greater wins, a
-                // NaN never does; a NaN diagonal keeps row k)
-                const double akk = fabs(A[k + (int64_t)c * k]);
-                int pr = k;
-                if (akk == akk) {
-                    double lv = -1.0;
-                    uint32_t li = 0xffffffffu;
-                    for (int i = k + tm.lane(); i < c; i += tm.wsize()) {
-                        const double v = fabs(A[i + (int64_t)c * k]);
-                        if (v > lv) { lv = v; li = (uint32_t)i; }
-                    }
-                    const double g = tm.wave_max(lv);
-                    pr = (int)tm.wave_min(lv == g ? li : 0xffffffffu);
-                }
-                if (tm.lane() == 0) ipiv[k] = pr;
-                if (pr != k)
-                    for (int j = tm.lane(); j < c; j += tm.wsize()) {
-                        const double x = A[k + (int64_t)c * j];
-                        A[k + (int64_t)c * j] = A[pr + (int64_t)c * j];
-                        A[pr + (int64_t)c * j] = x;
-                    }
-                if (onchip) tm.wave_sync_lds(); else tm.wave_sync();
-                const double d = A[k + (int64_t)c * k];
-                if (d == 0.0) singular = 1;
-                for (int i = k + 1 + tm.lane(); i < c; i += tm.wsize())
-                    A[i + (int64_t)c * k] = A[i + (int64_t)c * k] / d;
-                if (onchip) tm.wave_sync_lds(); else tm.wave_sync();
-                for (int i = k + 1 + tm.lane(); i < c; i += tm.wsize()) {
-                    const double l = A[i + (int64_t)c * k];
-                    for (int j = k + 1; j < c; ++j)
-                        A[i + (int64_t)c * j] = A[i + (int64_t)c * j] - l * A[k + (int64_t)c * j];
-                }
-                if (onchip) tm.wave_sync_lds(); else tm.wave_sync();
-            }
-            if (tm.lane() == 0) {
-                S.cnt = singular;
-                // the row interchanges as one permutation: column i of the
-                // permuted right-hand side is column perm[i] of Pi1
-                int32_t *perm = ipiv + c;
-                for (int i = 0; i < c; ++i) perm[i] = i;
-                for (int k = 0; k < c; ++k) {
-                    const int pk = ipiv[k], t = perm[k];
-                    perm[k] = perm[pk];
-                    perm[pk] = t;
-                }
-            }
-        }
-        tm.sync();
-        if (S.cnt) return ST_SINGULAR;
-        const int32_t *perm = ipiv + c;
-        for (int q = tm.tid(); q < r; q += tm.nt()) {
-            for (int i = 0; i < c; ++i) X[q + (int64_t)r * i] = Pi1[q + (int64_t)r * perm[i]];
-            for (int i = 0; i < c; ++i) {
-                double s = X[q + (int64_t)r * i];
-                for (int t = 0; t < i; ++t) s = s - A[i + (int64_t)c * t] * X[q + (int64_t)r * t];
-                X[q + (int64_t)r * i] = s;
-            }
-            for (int i = c - 1; i >= 0; --i) {
-                double s = X[q + (int64_t)r * i];
-                for (int t = i + 1; t < c; ++t) s = s - A[i + (int64_t)c * t] * X[q + (int64_t)r * t];
-                X[q + (int64_t)r * i] = s / A[i + (int64_t)c * i];
-            }
-        }
-        tm.sync();
-        return ST_OK;
-    }
-
     // LDS scratch carved from the team's Pi area; null when it does not fit
     ENG double *lds_take(int64_t n, int64_t &off) const
     {
@@ -714,12 +631,118 @@ struct Engine {
         if (tm.tid() == 0) { D.Tdims[3 * b] = a; D.Tdims[3 * b + 1] = d; D.Tdims[3 * b + 2] = c; }
     }
 
+    // ------------------------------------------------- Pi1·P⁻¹ (site tensor)
+    // T_b = transpose(transpose(P) \ transpose(Pi1)) (tensorci2.jl:391): LU
+    // with partial pivoting of A = Pᵀ (c×c, first maximum |.| per column) by
+    // one wave, then forward and back substitution of each of the r rows of
+    // Pi1 by one lane.
+    //
+    // lu_wave: lanes stride over rows for the pivot search, the scaling and
+    // the update (each element sees A[i,j] - A[i,k]·A[k,j] once per k, as in
+    // the serial elimination) and over columns for the row swap.  ipiv: 2c
+    // ints, the pivots and then the permutation they compose.  Returns 1 when
+    // a pivot is zero (LAPACK's SingularException).
+    ENG int lu_wave(double *A, int c, int32_t *ipiv, bool onchip)
+    {
+        int singular = 0;
+        for (int k = 0; k < c; ++k) {
+            // first maximum |A[i,k]| over i >= k (strictly greater wins, a
+            // NaN never does; a NaN diagonal keeps row k)
+            const double akk = fabs(A[k + (int64_t)c * k]);
+            int pr = k;
+            if (akk == akk) {
+                double lv = -1.0;
+                uint32_t li = 0xffffffffu;
+                for (int i = k + tm.lane(); i < c; i += tm.wsize()) {
+                    const double v = fabs(A[i + (int64_t)c * k]);
+                    if (v > lv) { lv = v; li = (uint32_t)i; }
+                }
+                const double g = tm.wave_max(lv);
+                pr = (int)tm.wave_min(lv == g ? li : 0xffffffffu);
+            }
+            if (tm.lane() == 0) ipiv[k] = pr;
+            if (pr != k)
+                for (int j = tm.lane(); j < c; j += tm.wsize()) {
+                    const double x = A[k + (int64_t)c * j];
+                    A[k + (int64_t)c * j] = A[pr + (int64_t)c * j];
+                    A[pr + (int64_t)c * j] = x;
+                }
+            if (onchip) tm.wave_sync_lds(); else tm.wave_sync();
+            const double d = A[k + (int64_t)c * k];
+            if (d == 0.0) singular = 1;
+            for (int i = k + 1 + tm.lane(); i < c; i += tm.wsize())
+                A[i + (int64_t)c * k] = A[i + (int64_t)c * k] / d;
+            if (onchip) tm.wave_sync_lds(); else tm.wave_sync();
+            for (int i = k + 1 + tm.lane(); i < c; i += tm.wsize()) {
+                const double l = A[i + (int64_t)c * k];
+                for (int j = k + 1; j < c; ++j)
+                    A[i + (int64_t)c * j] = A[i + (int64_t)c * j] - l * A[k + (int64_t)c * j];
+            }
+            if (onchip) tm.wave_sync_lds(); else tm.wave_sync();
+        }
+        if (tm.lane() == 0) {
+            // the row interchanges as one permutation: column i of the
+            // permuted right-hand side is column perm[i] of Pi1
+            int32_t *perm = ipiv + c;
+            for (int i = 0; i < c; ++i) perm[i] = i;
+            for (int k = 0; k < c; ++k) {
+                const int pk = ipiv[k], t = perm[k];
+                perm[k] = perm[pk];
+                perm[pk] = t;
+            }
+        }
+        if (onchip) tm.wave_sync_lds(); else tm.wave_sync();
+        return singular;
+    }
+
+    // Row q of X = Pi1·A⁻¹ in place over Pi1 (r rows, c columns), with the
+    // LU and pivots of lu_wave.  x: scratch for the row (stride sx) or null,
+    // in which case the row interchanges are applied to the row itself in
+    // pivot order (the same permutation as perm).
+    ENG void subst_row(double *Pi1, int r, int q, const double *A, int c, const int32_t *ipiv,
+                       double *x, int64_t sx) const
+    {
+        const bool scratch = x != nullptr;
+        if (scratch) {
+            const int32_t *perm = ipiv + c;
+            for (int i = 0; i < c; ++i) x[sx * i] = Pi1[q + (int64_t)r * perm[i]];
+        } else {
+            x = Pi1 + q;
+            sx = r;
+            for (int k = 0; k < c; ++k) {
+                const int pk = ipiv[k];
+                if (pk != k) {
+                    const double t = x[sx * k];
+                    x[sx * k] = x[sx * pk];
+                    x[sx * pk] = t;
+                }
+            }
+        }
+        for (int i = 0; i < c; ++i) {
+            double s = x[sx * i];
+            for (int t = 0; t < i; ++t) s = s - A[i + (int64_t)c * t] * x[sx * t];
+            x[sx * i] = s;
+        }
+        for (int i = c - 1; i >= 0; --i) {
+            double s = x[sx * i];
+            for (int t = i + 1; t < c; ++t) s = s - A[i + (int64_t)c * t] * x[sx * t];
+            x[sx * i] = s / A[i + (int64_t)c * i];
+        }
+        if (scratch)
+            for (int i = 0; i < c; ++i) Pi1[q + (int64_t)r * i] = x[sx * i];
+    }
+
     // fillsitetensors! (globalsearch.jl:97-104) → setsitetensor!(tci, f, b)
     // (tensorci2.jl:367-394).  When no global search follows (nsearch == 0)
     // the tensors are never read before sweep1site! rebuilds all of them, so
-    // only max|Pi1| (which feeds maxsamplevalue) is computed.
+    // only max|Pi1| (which feeds maxsamplevalue) is computed.  Otherwise each
+    // site's Pi1 is evaluated into its tensor slot, and the independent solves
+    // T_b = Pi1·P⁻¹ run in groups of sites whose Pᵀ fit in LDS together, one
+    // wave per site.  Errors keep the reference's order: a singular P at a
+    // site before the first non-square one wins.
     ENG int fillsitetensors(bool need_tensors)
     {
+        int nsq = -1;   // first site whose P is not square
         for (int b = 0; b < L; ++b) {
             const int nI = setn(SET_I, b), d = D.localdims[b], nJ = setn(SET_J, b);
             const int m = kron_rows(b);
@@ -731,38 +754,72 @@ struct Engine {
                 continue;
             }
             if ((int64_t)m * nJ > D.tstride) return fail(ST_CAPACITY);
-            if (b == L - 1) {
-                eval(rows, b + 1, m, cols, nJ, Tslot(b), m, false, true);
-                setTdims(b, nI, d, nJ);
-                continue;
+            eval(rows, b + 1, m, cols, nJ, Tslot(b), m, false, true);
+            setTdims(b, nI, d, nJ);
+            if (b + 1 < L && setn(SET_I, b + 1) != nJ) { nsq = b; break; }
+        }
+        if (!need_tensors) return ST_OK;
+        const int nsolve = nsq >= 0 ? nsq : L - 1;
+        // per-wave row scratch for the substitutions (null: in place in HBM)
+        int cmax = 1;
+        for (int b = 0; b < nsolve; ++b) cmax = setn(SET_I, b + 1) > cmax ? setn(SET_I, b + 1) : cmax;
+        int64_t off0 = 0;
+        double *xsl = lds_take((int64_t)tm.nwaves() * tm.wsize() * cmax, off0);
+        for (int g0 = 0; g0 < nsolve;) {
+            // a group: consecutive sites whose Pᵀ and pivots fit in LDS
+            // together; a site that does not fit alone uses the HBM buffers
+            int64_t off = off0;
+            int g1 = g0;
+            while (g1 < nsolve) {
+                const int64_t np = setn(SET_I, g1 + 1);
+                if (!lds_take(np * np + np, off)) break;
+                ++g1;
             }
-            const int np = setn(SET_I, b + 1);
-            int64_t off = 0;
-            double *Pi1 = lds_take((int64_t)m * nJ, off);
-            double *At = lds_take((int64_t)np * np, off);
-            double *X = lds_take((int64_t)m * np, off);
-            int32_t *ipv = (int32_t *)lds_take((int64_t)np, off);
-            const bool onchip = Pi1 && At && X && ipv;
-            if (!onchip) { Pi1 = D.pi; At = D.pmat; X = Tslot(b); ipv = D.ipiv; }
-            eval(rows, b + 1, m, cols, nJ, Pi1, m, false, true);
-            if (np != nJ) {
-                if (tm.tid() == 0) D.ints[INT_ERRSITE] = b + 1;
-                return fail(ST_NONSQUARE);
+            const bool onchip = g1 > g0;
+            if (!onchip) g1 = g0 + 1;
+            off = off0;
+            for (int b = g0; b < g1; ++b) {
+                const int np = setn(SET_I, b + 1);
+                double *At = onchip ? lds_take((int64_t)np * np + np, off) : D.pmat;
+                load_set(cols, SET_J, b, jlen(b));
+                tm.sync();
+                // Pᵀ[jj, ii] = f(Iset[b+1][ii] ⊕ Jset[b][jj])
+                eval(setp(SET_I, b + 1), b + 1, np, cols, np, At, np, true, false);
             }
-            // Pᵀ[jj, ii] = f(Iset[b+1][ii] ⊕ Jset[b][jj])
-            eval(setp(SET_I, b + 1), b + 1, np, cols, nJ, At, np, true, false);
-            const uint64_t ts = tm.clock();
-            int st = solve_right(Pi1, m, At, np, X, ipv, onchip);
-            if (tm.tid() == 0) S.tph[PH_SOLVE] += tm.clock() - ts;
-            if (st != ST_OK) {
-                if (tm.tid() == 0) D.ints[INT_ERRSITE] = b + 1;
-                return fail(st);
-            }
-            double *T = Tslot(b);
-            if (X != T)
-                for (int64_t q = tm.tid(); q < (int64_t)m * np; q += tm.nt()) T[q] = X[q];
-            setTdims(b, nI, d, np);
             tm.sync();
+            const uint64_t ts = tm.clock();
+            int singular_site = 0;   // lowest singular site + 1 of this wave
+            off = off0;
+            for (int b = g0; b < g1; ++b) {
+                const int m = setn(SET_I, b) * D.localdims[b], np = setn(SET_I, b + 1);
+                double *At = onchip ? lds_take((int64_t)np * np + np, off) : D.pmat;
+                int32_t *ipv = onchip ? (int32_t *)(At + (int64_t)np * np) : D.ipiv;
+                if ((b - g0) % tm.nwaves() != tm.wave()) continue;
+                if (lu_wave(At, np, ipv, onchip) && !singular_site) singular_site = b + 1;
+                double *T = Tslot(b);
+                double *xw = xsl ? xsl + (int64_t)tm.wave() * tm.wsize() * cmax + tm.lane() : nullptr;
+                for (int q = tm.lane(); q < m; q += tm.wsize())
+                    subst_row(T, m, q, At, np, ipv, xw, tm.wsize());
+            }
+            const int any_sing = tm.any(tm.lane() == 0 && singular_site != 0);
+            if (tm.tid() == 0) S.tph[PH_SOLVE] += tm.clock() - ts;
+            if (any_sing) {
+                // the lowest singular site, as the site-ordered loop raises it
+                if (tm.tid() == 0) S.cnt = 0x7fffffff;
+                tm.sync();
+                for (int w = 0; w < tm.nwaves(); ++w) {
+                    if (tm.wave() == w && tm.lane() == 0 && singular_site && singular_site < S.cnt)
+                        S.cnt = singular_site;
+                    tm.sync();
+                }
+                if (tm.tid() == 0) D.ints[INT_ERRSITE] = S.cnt;
+                return fail(ST_SINGULAR);
+            }
+            g0 = g1;
+        }
+        if (nsq >= 0) {
+            if (tm.tid() == 0) D.ints[INT_ERRSITE] = nsq + 1;
+            return fail(ST_NONSQUARE);
         }
         return ST_OK;
     }
