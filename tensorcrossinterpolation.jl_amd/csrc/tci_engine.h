@@ -197,7 +197,15 @@ enum { PH_EVAL = 0, PH_RRLU = 1, PH_SETS = 2, PH_FILL = 3, PH_GSEARCH = 4, PH_SW
 // FK = 0 dispatches on the descriptor's kind at run time.
 template <class Team, int FK = 0>
 struct Engine {
-    using Tile = TileLU<Team, (Team::RA > 0 ? Team::RA : 1), (Team::CB > 0 ? Team::CB : 1)>;
+    static constexpr int TRA = Team::RA > 0 ? Team::RA : 1, TCB = Team::CB > 0 ? Team::CB : 1;
+    static constexpr int TCBN = TCB >= 4 ? TCB / 4 : TCB;
+    // the team's register tile, and narrower ones for thin matrices (one
+    // block of rows per lane, a quarter of the columns, or both), whose
+    // pivot steps cost proportionally less
+    using Tile = TileLU<Team, TRA, TCB>;
+    using TileR = TileLU<Team, 1, TCB>;
+    using TileC = TileLU<Team, TRA, TCBN>;
+    using TileRC = TileLU<Team, 1, TCBN>;
     Team &tm;
     const SolveDesc &D;
     EngShared &S;
@@ -457,6 +465,17 @@ struct Engine {
     }
 
     // --------------------------------------------------------------- rrLU
+    // the register-tile factorization of A (lda = m) with tile shape T
+    template <class T>
+    ENG_NOINLINE int tile_factor(const double *A, int m, int n, int64_t maxrank, double reltol,
+                                 double abstol, bool leftorth, double *perr, double *st, double *Lorig)
+    {
+        T R;
+        R.setup(tm, m, n);
+        R.load(tm, A, m);
+        return R.factor(tm, maxrank, reltol, abstol, leftorth, pvals, perr, st, Lorig, rperm, cperm);
+    }
+
     // rrlu!(A) on the m×n matrix A (lda = m).  With need_left, A's first p
     // columns hold the factor L in position order afterwards (used by
     // colstimespivotinv); otherwise only the permutations, pivots and error
@@ -470,13 +489,12 @@ struct Engine {
         const uint64_t t0 = tm.clock();
         int p;
         if (Team::RA > 0 && Tile::fits(m, n)) {
-            Tile R;
-            R.setup(tm, m, n);
-            R.load(tm, A, m);
             // L recorded by original row in lscratch, then permuted into A
             double *Lorig = need_left ? lscratch : nullptr;
-            p = R.factor(tm, maxrank, reltol, abstol, leftorth, pvals, perr, st, Lorig,
-                         rperm, cperm);
+            if (TileRC::fits(m, n)) p = tile_factor<TileRC>(A, m, n, maxrank, reltol, abstol, leftorth, perr, st, Lorig);
+            else if (TileR::fits(m, n)) p = tile_factor<TileR>(A, m, n, maxrank, reltol, abstol, leftorth, perr, st, Lorig);
+            else if (TileC::fits(m, n)) p = tile_factor<TileC>(A, m, n, maxrank, reltol, abstol, leftorth, perr, st, Lorig);
+            else p = tile_factor<Tile>(A, m, n, maxrank, reltol, abstol, leftorth, perr, st, Lorig);
             if (need_left && p > 0) {
                 tm.sync();
                 for (int t = tm.wave(); t < p; t += tm.nwaves())
@@ -552,7 +570,17 @@ struct Engine {
             tm.ys[q] = (double)acc;
         }
         tm.sync();
-        Tile R;
+        if (TileRC::fits(m, n)) return lorentz_factor<TileRC>(m, n, maxrank, reltol, abstol, leftorth, perr, t0);
+        if (TileR::fits(m, n)) return lorentz_factor<TileR>(m, n, maxrank, reltol, abstol, leftorth, perr, t0);
+        if (TileC::fits(m, n)) return lorentz_factor<TileC>(m, n, maxrank, reltol, abstol, leftorth, perr, t0);
+        return lorentz_factor<Tile>(m, n, maxrank, reltol, abstol, leftorth, perr, t0);
+    }
+
+    template <class T>
+    ENG_NOINLINE int lorentz_factor(int m, int n, int64_t maxrank, double reltol, double abstol,
+                                    bool leftorth, double *perr, uint64_t t0)
+    {
+        T R;
         R.setup(tm, m, n);
         const double lmax = R.fill_lorentz(tm, D.params[0], tm.xs, tm.ys, lut);
         const double g = tm.max_nan(lmax);
