@@ -110,6 +110,7 @@ struct DevTeam {
     __device__ int wave() const { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
     __device__ static constexpr int nwaves() { return NW; }
     double *xs, *ys;
+    bool stage_lds;       // xs and ys are in LDS
     double *lpi;          // LDS area for Pi matrices
     int64_t lpi_cap;      // its size in doubles (0: none)
     uint8_t *lrows, *lcols;
@@ -145,10 +146,15 @@ struct DevTeam {
         return r;
     }
 
+    // NaN-propagating max over the team (the wave part by DPP, as wave_max)
     __device__ double max_nan(double v)
     {
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) v = jl_max(v, __shfl_xor(v, off));
+        v = jl_max(v, dpp_f64<kDppXor1>(v));
+        v = jl_max(v, dpp_f64<kDppXor2>(v));
+        v = jl_max(v, dpp_f64<kDppHalfMirror>(v));
+        v = jl_max(v, dpp_f64<kDppMirror>(v));
+        v = jl_max(jl_max(readlane_f64(v, 0), readlane_f64(v, 16)),
+                   jl_max(readlane_f64(v, 32), readlane_f64(v, 48)));
         double *slot = lds_rd() + par_d * NW;
         if (lane() == 0) slot[wave()] = v;
         __syncthreads();
@@ -164,11 +170,21 @@ struct DevTeam {
     __device__ double uniform_f64(double x) const { return readlane_f64(x, 0); }
 
     __device__ int any(int v) { return __syncthreads_or(v); }
+    // a promise that p points into LDS (the compiler then emits LDS
+    // instructions instead of generic ones for accesses through it); it
+    // must hold
+    __device__ static void assume_lds(const void *p)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        __builtin_assume(__builtin_amdgcn_is_shared(p));
+#endif
+    }
     __device__ uint64_t clock() const { return wall_clock64(); }
     __device__ void init(double *x, double *y)
     {
         xs = x;
         ys = y;
+        stage_lds = false;
         par_a = 0;
         par_d = 0;
         lpi = nullptr;

@@ -20,6 +20,9 @@
 //                            (LDS and global, or LDS only) inside one wave
 //   tm.tA(w), tB(w), tX(w, r), tY(w, j)  LDS slots of the register-tile rrLU
 //   Team::RA, CB             register-tile shape (tci_regtile.h); RA = 0: none
+//   tm.stage_lds             xs and ys are in LDS
+//   tm.assume_lds(p)         a promise that p points into LDS (lets the device
+//                            use LDS instructions; a no-op on the CPU)
 // The device implementation lives in tci_kernels.hip.  Every parallel loop
 // writes disjoint elements and every reduction is over a total order, so the
 // result does not depend on the team size: a one-thread team on the CPU
@@ -441,6 +444,24 @@ struct Engine {
         const uint64_t t0 = tm.clock();
         double lmax = 0.0;
         const int64_t cnt = (int64_t)m * n;
+        if (lut && m <= D.mrcap && n <= D.mrcap) {
+            // the Lorentzian through its row and column sums and the table
+            // (the same bits as the functor's division)
+            lorentz_sums(rows, nl, m, cols, n);
+            if (tm.stage_lds) {
+                tm.assume_lds(tm.xs);
+                tm.assume_lds(tm.ys);
+            }
+            tm.assume_lds(lut);
+            for (int j = tm.wave(); j < n; j += tm.nwaves()) {
+                const double cs = tm.ys[j];
+                for (int i = tm.lane(); i < m; i += tm.wsize()) {
+                    const double v = lut[(int)(tm.xs[i] + cs) + 1];
+                    if (out) out[transpose ? (j + ld * i) : (i + ld * j)] = v;
+                    lmax = jl_max(lmax, fabs(v));
+                }
+            }
+        } else
         for (int j = tm.wave(); j < n; j += tm.nwaves())
             for (int i = tm.lane(); i < m; i += tm.wsize()) {
                 tci_point x;
@@ -553,23 +574,30 @@ struct Engine {
                m <= D.mrcap && n <= D.mrcap;
     }
 
-    ENG_NOINLINE int eval_rrlu_lorentz(int m, int nl, int n, int64_t maxrank, double reltol,
-                                       double abstol, bool leftorth, double *perr)
+    // integer sums of squares of the row parts (first nl entries of each
+    // row of r) into xs and of the column parts (L - nl entries of each row
+    // of c) into ys (exact in double: at most 250·255² < 2^53); ends with a
+    // barrier
+    ENG void lorentz_sums(const uint8_t *r, int nl, int m, const uint8_t *c, int n)
     {
-        const uint64_t t0 = tm.clock();
-        // integer sums of squares of the row part and the column part (exact
-        // in double: at most 250·255² < 2^53)
         for (int q = tm.tid(); q < m; q += tm.nt()) {
             int64_t acc = 0;
-            for (int t = 0; t < nl; ++t) { int64_t v = rows[(int64_t)q * L + t]; acc += v * v; }
+            for (int t = 0; t < nl; ++t) { int64_t v = r[(int64_t)q * L + t]; acc += v * v; }
             tm.xs[q] = (double)acc;
         }
         for (int q = tm.tid(); q < n; q += tm.nt()) {
             int64_t acc = 0;
-            for (int t = 0; t < L - nl; ++t) { int64_t v = cols[(int64_t)q * L + t]; acc += v * v; }
+            for (int t = 0; t < L - nl; ++t) { int64_t v = c[(int64_t)q * L + t]; acc += v * v; }
             tm.ys[q] = (double)acc;
         }
         tm.sync();
+    }
+
+    ENG_NOINLINE int eval_rrlu_lorentz(int m, int nl, int n, int64_t maxrank, double reltol,
+                                       double abstol, bool leftorth, double *perr)
+    {
+        const uint64_t t0 = tm.clock();
+        lorentz_sums(rows, nl, m, cols, n);
         if (TileRC::fits(m, n)) return lorentz_factor<TileRC>(m, n, maxrank, reltol, abstol, leftorth, perr, t0);
         if (TileR::fits(m, n)) return lorentz_factor<TileR>(m, n, maxrank, reltol, abstol, leftorth, perr, t0);
         if (TileC::fits(m, n)) return lorentz_factor<TileC>(m, n, maxrank, reltol, abstol, leftorth, perr, t0);

@@ -44,8 +44,12 @@ __global__ __launch_bounds__(NT, WPE) void tci_solve_kernel(const SolveDesc *__r
     uint8_t *lsets = tabs + 2 * plan.tab_cap;
     double *lut = (double *)(lsets + plan.sets_cap);
     double *pi = lut + plan.lut_cap;
-    if (D.mrcap <= plan.stage_cap) tm.init(stage, stage + D.mrcap);
-    else tm.init(D.stage, D.stage + D.mrcap);
+    if (D.mrcap <= plan.stage_cap) {
+        tm.init(stage, stage + D.mrcap);
+        tm.stage_lds = true;
+    } else {
+        tm.init(D.stage, D.stage + D.mrcap);
+    }
     if ((int64_t)D.mrcap * D.L <= plan.tab_cap) {
         tm.lrows = tabs;
         tm.lcols = tabs + plan.tab_cap;

@@ -89,19 +89,37 @@ struct TileLU {
     ENG_INLINE double fill_lorentz(const Team &tm, double c, const double *rs, const double *cs,
                                    const double *lut)
     {
+        // with the sums in LDS (the table always is) the loads are LDS
+        // loads, not generic ones
+        if (tm.stage_lds) return fill_lorentz_t<true>(tm, c, rs, cs, lut);
+        return fill_lorentz_t<false>(tm, c, rs, cs, lut);
+    }
+
+    template <bool LDS>
+    ENG_INLINE double fill_lorentz_t(const Team &tm, double c, const double *rs, const double *cs,
+                                     const double *lut)
+    {
+        if (LDS) {
+            tm.assume_lds(rs);
+            tm.assume_lds(cs);
+        }
+        // (indices clamped into the arrays so every load is unconditional)
         double rsv[RA], csv[CB];
 #pragma unroll
         for (int i = 0; i < RA; ++i) {
             const int r = tm.lane() + WS * i;
-            rsv[i] = r < m ? rs[r] : 0.0;
+            const double v = rs[r < m ? r : 0];
+            rsv[i] = r < m ? v : 0.0;
         }
 #pragma unroll
         for (int j = 0; j < CB; ++j) {
             const int cc = tm.wave() + NW * j;
-            csv[j] = cc < n ? cs[cc] : 0.0;
+            const double v = cs[cc < n ? cc : 0];
+            csv[j] = cc < n ? v : 0.0;
         }
         double lmax = 0.0;
         if (lut) {
+            tm.assume_lds(lut);
             bool any = false;
 #pragma unroll
             for (int j = 0; j < CB; ++j)

@@ -56,6 +56,8 @@ struct HostTeam {
     double wave_bcast_f64(double v, int) const { return v; }
     int any(int v) { return v; }
     uint64_t clock() const { return 0; }
+    static constexpr bool stage_lds = false;
+    static void assume_lds(const void *) {}
 };
 
 // Multi-threaded host team: NT std::threads emulate one workgroup of NT/64
@@ -179,6 +181,8 @@ struct HostTeamMT {
     uint32_t uniform_u32(uint32_t x) const { return x; }
     double uniform_f64(double x) const { return x; }
     uint64_t clock() const { return 0; }
+    static constexpr bool stage_lds = false;
+    static void assume_lds(const void *) {}
 };
 
 // rrlu! through the register tile on NT host threads shaped like the
