@@ -61,6 +61,7 @@ ENG inline double jl_max(double a, double b)
 }
 
 // Scalars shared by the team (written by thread 0 between barriers).
+constexpr int kPivErrCap = 256;
 struct EngShared {
     double maxsample;
     double abstol;
@@ -70,7 +71,8 @@ struct EngShared {
     int32_t status;
     int32_t cnt;       // scratch count (union compaction)
     int32_t nf;        // global pivots found
-    int32_t pad;
+    int32_t npe;       // length of pivoterrors
+    double pe[kPivErrCap];   // pivoterrors while the solve runs, when they fit
 };
 
 // ---- project PRNG: SplitMix64-seeded xoshiro256** --------------------------

@@ -77,6 +77,8 @@ struct DevTeam {
         return min(min(a, b), min(c, d));
     }
     __device__ bool wave_any(bool p) const { return __ballot(p) != 0; }
+    // lanes where p holds, as a wave-uniform bit mask
+    __device__ uint64_t wave_ballot(bool p) const { return __ballot(p); }
     // lane l's v in every lane (l wave-uniform)
     __device__ int wave_bcast(int v, int l) const { return __builtin_amdgcn_readlane(v, l); }
     __device__ double wave_bcast_f64(double v, int l) const { return readlane_f64(v, l); }

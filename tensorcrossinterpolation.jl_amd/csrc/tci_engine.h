@@ -400,13 +400,27 @@ struct Engine {
         tm.sync();
         uint8_t *dst = rowside ? rows : cols;
         int total = 0;
-        for (int e = 0; e < ne; ++e) total += fl[e];
-        if (m0 + total > D.mrcap) return fail(ST_CAPACITY);
-        for (int e = tm.tid(); e < ne; e += tm.nt()) {
-            if (!fl[e]) continue;
-            int pos = m0;
-            for (int e2 = 0; e2 < e; ++e2) pos += fl[e2];
-            for (int t = 0; t < elen; ++t) dst[(int64_t)pos * L + t] = E[(int64_t)e * L + t];
+        if (ne <= tm.wsize()) {
+            // the kept extras as a lane mask (every wave forms it); wave 0
+            // appends them, each at the count of kept extras before it
+            const bool keep = tm.lane() < ne && fl[tm.lane()];
+            const uint64_t mask = tm.wave_ballot(keep);
+            total = __builtin_popcountll(mask);
+            if (m0 + total > D.mrcap) return fail(ST_CAPACITY);
+            if (tm.wave() == 0 && keep) {
+                const int e = tm.lane();
+                const int pos = m0 + __builtin_popcountll(mask & (((uint64_t)1 << e) - 1));
+                for (int t = 0; t < elen; ++t) dst[(int64_t)pos * L + t] = E[(int64_t)e * L + t];
+            }
+        } else {
+            for (int e = 0; e < ne; ++e) total += fl[e];
+            if (m0 + total > D.mrcap) return fail(ST_CAPACITY);
+            for (int e = tm.tid(); e < ne; e += tm.nt()) {
+                if (!fl[e]) continue;
+                int pos = m0;
+                for (int e2 = 0; e2 < e; ++e2) pos += fl[e2];
+                for (int t = 0; t < elen; ++t) dst[(int64_t)pos * L + t] = E[(int64_t)e * L + t];
+            }
         }
         tm.sync();
         return m0 + total;
@@ -543,19 +557,25 @@ struct Engine {
     // in parallel; ends with a barrier
     ENG void updateerrors(int b, int p, double err)
     {
-        const int npe = D.ints[INT_NPE];
+        const int npe = S.npe;
         const int n = npe > p + 1 ? npe : p + 1;
+        double *pe = piverrs();
         for (int i = tm.tid(); i < n; i += tm.nt()) {
-            const double a = i < npe ? D.pivoterrors[i] : 0.0;
+            const double a = i < npe ? pe[i] : 0.0;
             const double e = i < p ? fabs(pvals[i]) : (i == p ? err : 0.0);
-            D.pivoterrors[i] = jl_max(a, e);
+            pe[i] = jl_max(a, e);
         }
         tm.sync();
         if (tm.tid() == 0) {
             D.bonderrors[b] = err;
-            D.ints[INT_NPE] = n;
+            S.npe = n;
         }
     }
+
+    // pivoterrors: in the team's shared block while the solve runs when
+    // setcap + 1 entries fit there (copied to the arena by finish), else in
+    // the arena
+    ENG double *piverrs() const { return D.setcap + 1 <= kPivErrCap ? S.pe : D.pivoterrors; }
 
     ENG int fail(int st)
     {
@@ -906,7 +926,7 @@ struct Engine {
                     setn(nh_j, b) = setn(SET_J, b);
                 }
                 D.ints[INT_HIST] = hist == 1 ? 2 : 1;
-                D.ints[INT_NPE] = 0;   // flushpivoterror!
+                S.npe = 0;   // flushpivoterror!
             }
             tm.sync();
             const bool fwd = D.sweepstrategy == 1 || (D.sweepstrategy == 0 && (iter & 1));
@@ -1128,7 +1148,7 @@ struct Engine {
     // tensorci2.jl:402-461, forward, updatetensors = true
     ENG int sweep1site_forward(double abstol)
     {
-        if (tm.tid() == 0) D.ints[INT_NPE] = 0;
+        if (tm.tid() == 0) S.npe = 0;
         for (int b = 0; b < L - 1; ++b) {
             const int nI = setn(SET_I, b), d = D.localdims[b], nJ = setn(SET_J, b);
             const int m = kron_rows(b);
@@ -1215,6 +1235,7 @@ struct Engine {
             for (int q = 0; q < NSETKIND * L; ++q) setcnt[q] = 0;
             D.ints[INT_NITER] = 0;
             D.ints[INT_NPE] = 0;
+            S.npe = 0;
             D.ints[INT_HIST] = 0;
             D.ints[INT_ERRSITE] = 0;
             for (int b = 0; b < L - 1; ++b) D.bonderrors[b] = 0.0;
@@ -1304,7 +1325,10 @@ struct Engine {
             }
             tm.sync();
         }
+        if (piverrs() != D.pivoterrors)
+            for (int i = tm.tid(); i < S.npe; i += tm.nt()) D.pivoterrors[i] = S.pe[i];
         if (tm.tid() == 0) {
+            D.ints[INT_NPE] = S.npe;
             D.ints[INT_STATUS] = st;
             S.tph[PH_TOTAL] = tm.clock() - S.tph[PH_TOTAL];
             for (int i = 0; i < 8; ++i) D.scal[SCAL_TPH + i] = (double)S.tph[i];

@@ -52,6 +52,7 @@ struct HostTeam {
     uint32_t wave_min(uint32_t v) const { return v; }
     bool wave_any(bool p) const { return p; }
     int wave_first(bool p) const { return p ? 0 : -1; }
+    uint64_t wave_ballot(bool p) const { return p ? 1u : 0u; }
     int wave_bcast(int v, int) const { return v; }
     double wave_bcast_f64(double v, int) const { return v; }
     int any(int v) { return v; }
@@ -152,6 +153,15 @@ struct HostTeamMT {
         return r;
     }
     bool wave_any(bool p) { return wave_first(p) >= 0; }
+    uint64_t wave_ballot(bool p)
+    {
+        sh->ibuf[t_tid] = p ? 1 : 0;
+        wsync();
+        uint64_t r = 0;
+        for (int t = 0; t < 64; ++t) if (sh->ibuf[wave() * 64 + t]) r |= (uint64_t)1 << t;
+        wsync();
+        return r;
+    }
     int wave_bcast(int v, int l)
     {
         sh->ibuf[t_tid] = v;
