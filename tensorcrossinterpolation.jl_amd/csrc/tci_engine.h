@@ -502,7 +502,7 @@ struct Engine {
     // --------------------------------------------------------------- rrLU
     // the register-tile factorization of A (lda = m) with tile shape T
     template <class T>
-    ENG_NOINLINE int tile_factor(const double *A, int m, int n, int64_t maxrank, double reltol,
+    ENG_INLINE int tile_factor(const double *A, int m, int n, int64_t maxrank, double reltol,
                                  double abstol, bool leftorth, double *perr, double *st, double *Lorig)
     {
         T R;
@@ -613,7 +613,7 @@ struct Engine {
         tm.sync();
     }
 
-    ENG_NOINLINE int eval_rrlu_lorentz(int m, int nl, int n, int64_t maxrank, double reltol,
+    ENG_INLINE int eval_rrlu_lorentz(int m, int nl, int n, int64_t maxrank, double reltol,
                                        double abstol, bool leftorth, double *perr)
     {
         const uint64_t t0 = tm.clock();
