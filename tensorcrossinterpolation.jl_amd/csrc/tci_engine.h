@@ -243,7 +243,7 @@ struct Engine {
     // place the permutations and pivots of an m×n rrLU in the LDS Pi area at
     // offset off (doubles) when they fit, else in the arena; returns the
     // first free offset after them
-    ENG int64_t perm_area(int64_t off, int m, int n)
+    ENG_INLINE int64_t perm_area(int64_t off, int m, int n)
     {
         const int64_t ints = ((int64_t)m + n + 1) / 2;
         const int64_t need = (ints + (m < n ? m : n) + 2) & ~(int64_t)1;
@@ -340,7 +340,7 @@ struct Engine {
 
     // ------------------------------------------------------ index tables
     // rows = kronecker(Iset[b], d_b) (tensorci2.jl:315-320): entry q = i + nI·s
-    ENG int kron_rows(int b)
+    ENG_INLINE int kron_rows(int b)
     {
         const int nI = setn(SET_I, b), d = D.localdims[b];
         const uint8_t *I = setp(SET_I, b);
@@ -356,7 +356,7 @@ struct Engine {
     }
 
     // cols = kronecker(d_{b+1}, Jset[b+1]) (tensorci2.jl:322-327): q = s + d·j
-    ENG int kron_cols(int b)
+    ENG_INLINE int kron_cols(int b)
     {
         const int nJ = setn(SET_J, b + 1), d = D.localdims[b + 1];
         const uint8_t *J = setp(SET_J, b + 1);
@@ -376,7 +376,7 @@ struct Engine {
     // (length b+1) lies in kronecker(Iset[b], d_b) iff e[1:b] ∈ Iset[b]; an
     // extra suffix lies in kronecker(d, Jset[b+1]) iff e[2:end] ∈ Jset[b+1].
     // Extras come from an earlier Iset/Jset and are unique already.
-    ENG int union_extras(bool rowside, int b, int m0, int ekind)
+    ENG_INLINE int union_extras(bool rowside, int b, int m0, int ekind)
     {
         const int ne = setn(ekind, rowside ? b + 1 : b);
         if (ne == 0) return m0;
@@ -438,7 +438,7 @@ struct Engine {
     }
 
     // set[kind][b] = table[perm[0..p-1]] (pivot order kept, tensorci2.jl:599-600)
-    ENG void select_into(int kind, int b, const uint8_t *table, const int32_t *perm, int p, int len)
+    ENG_INLINE void select_into(int kind, int b, const uint8_t *table, const int32_t *perm, int p, int len)
     {
         uint8_t *dst = setp(kind, b);
         for (int64_t q = tm.tid(); q < (int64_t)p * len; q += tm.nt()) {
@@ -555,7 +555,7 @@ struct Engine {
     // updateerrors! (tensorci2.jl:161-169): pivoterrors[i] = max(pivoterrors[i],
     // |pivot i| or the last error), zero-padded (util.jl:12-14), element-wise
     // in parallel; ends with a barrier
-    ENG void updateerrors(int b, int p, double err)
+    ENG_INLINE void updateerrors(int b, int p, double err)
     {
         const int npe = S.npe;
         const int n = npe > p + 1 ? npe : p + 1;
@@ -577,7 +577,7 @@ struct Engine {
     // the arena
     ENG double *piverrs() const { return D.setcap + 1 <= kPivErrCap ? S.pe : D.pivoterrors; }
 
-    ENG int fail(int st)
+    ENG_INLINE int fail(int st)
     {
         if (tm.tid() == 0 && S.status == ST_OK) S.status = st;
         tm.sync();
@@ -598,7 +598,7 @@ struct Engine {
     // row of r) into xs and of the column parts (L - nl entries of each row
     // of c) into ys (exact in double: at most 250·255² < 2^53); ends with a
     // barrier
-    ENG void lorentz_sums(const uint8_t *r, int nl, int m, const uint8_t *c, int n)
+    ENG_INLINE void lorentz_sums(const uint8_t *r, int nl, int m, const uint8_t *c, int n)
     {
         for (int q = tm.tid(); q < m; q += tm.nt()) {
             int64_t acc = 0;
