@@ -160,17 +160,26 @@ def bench_rrlu(args, tci, ctx):
         j.reltol, j.abstol, j.leftorth = 1e-10, 0.0, 1
         po += m + n
     jb = torch.frombuffer(bytearray(jobs), dtype=torch.uint8).to("cuda")
-    times = []
+    # HIP events on the library's own stream time the launch
+    sp = C.c_void_p()
+    if lib.tci_ctx_stream(ctx.handle, C.byref(sp)):
+        raise RuntimeError(lib.tci_last_error().decode())
+    lstream = torch.cuda.ExternalStream(sp.value)
+    times, kms = [], []
     for it in range(args.warmup + args.steps):
         A.copy_(pristine)
         torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         t0 = time.perf_counter()
+        e0.record(lstream)
         st = lib.tci_rrlu_batched_f64(ctx.handle, C.c_void_p(jb.data_ptr()), cnt)
         if st:
             raise RuntimeError(lib.tci_last_error().decode())
+        e1.record(lstream)
         ctx.synchronize()
         if it >= args.warmup:
             times.append(time.perf_counter() - t0)
+            kms.append(e0.elapsed_time(e1))
     C.memmove(jobs, bytes(jb.cpu().numpy().tobytes()), C.sizeof(jobs))
     ps = np.array([jobs[i].npivot for i in range(cnt)])
     bad = sum(jobs[i].status != 0 for i in range(cnt))
@@ -180,7 +189,9 @@ def bench_rrlu(args, tci, ctx):
     for (m, n), p in zip(shapes, ps):
         k = np.arange(1, p + 1, dtype=np.float64)
         tot_bytes += 8.0 * m * n + 16.0 * float(np.sum((m - k) * (n - k))) + 8.0 * p * (m + n)
-    achieved = tot_bytes / (elapsed / len(times)) / 1e9
+    kms_avg = float(np.mean(kms))
+    achieved = tot_bytes / (kms_avg * 1e-3) / 1e9
+    traffic, tsrc = profiled_traffic("tci_rrlu_kernel_c2_4096", cnt, 4096)
     out = {"metric": "batched standalone rrLU matrices/s (FP64, full pivoting, reltol 1e-10)",
            "value": round(value, 2), "unit": "matrices/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(elapsed / len(times) * 1e3, 3),
@@ -190,8 +201,12 @@ def bench_rrlu(args, tci, ctx):
                                   "rank U{8..128}, one launch" % cnt,
                       "mean_pivots": round(float(ps.mean()), 2), "failed": int(bad)},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                        "kernel": "tci_rrlu_kernel", "bytes_per_launch": tot_bytes}}
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                        "traffic_source": tsrc, "kernel": "tci_rrlu_kernel",
+                        "kernel_ms": round(kms_avg, 3), "bytes_per_launch": tot_bytes,
+                        "note": "achieved = algorithmic bytes (8mn + 16 sum (m-k)(n-k) + 8p(m+n) "
+                                "per matrix) over the launch time from HIP events on the "
+                                "library's stream"}}
     if args.cpu_seconds > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
@@ -222,7 +237,19 @@ def main():
     ap.add_argument("--served", type=int, default=8192, help="requests in the served (Poisson) run; 0: skip")
     ap.add_argument("--served-load", type=float, default=0.5, help="arrival rate as a fraction of batched capacity")
     ap.add_argument("--streams", type=int, default=4)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 3 plain device copies of 1 GiB (the reference kernel for "
+                         "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 2 * src.numel() * 8, "dispatches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(solves are independent: shard requests across GPUs)"}))

@@ -55,6 +55,10 @@ tci_status tci_ctx_synchronize(tci_ctx *ctx);
  * while others run execute concurrently.  tci_solve_submit puts a batch on
  * an idle stream when there is one.  The HBM budget is shared by all. */
 tci_status tci_ctx_set_streams(tci_ctx *ctx, int32_t n);
+/* The context's first HIP stream (a hipStream_t), on which the synchronous
+ * calls and tci_rrlu_batched_f64 run, so that a caller can order its own
+ * work or timing events with them (no counterpart in the reference). */
+tci_status tci_ctx_stream(tci_ctx *ctx, void **stream);
 
 /* rrLU ----------------------------------------------------------------- */
 /* Replaces rrlu(A; maxrank, reltol, abstol, leftorthogonal)::rrLU{Float64}

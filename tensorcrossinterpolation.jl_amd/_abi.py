@@ -137,6 +137,7 @@ def lib():
     L.tci_ctx_destroy.argtypes = [vp]
     L.tci_ctx_synchronize.argtypes = [vp]
     L.tci_ctx_set_streams.argtypes = [vp, C.c_int32]
+    L.tci_ctx_stream.argtypes = [vp, C.POINTER(vp)]
     L.tci_rrlu_f64.argtypes = [vp, P(C.c_double), C.c_int64, C.c_int64, C.c_int64, C.c_int64,
                                C.c_double, C.c_double, C.c_int32, P(C.c_int64), P(C.c_int64),
                                P(C.c_double), C.c_int64, P(C.c_double), C.c_int64, P(C.c_int64),
@@ -168,7 +169,7 @@ def lib():
 
 EXPORTED_SYMBOLS = [
     "tci_last_error", "tci_version", "tci_ctx_create", "tci_ctx_destroy", "tci_ctx_synchronize",
-    "tci_ctx_set_streams",
+    "tci_ctx_set_streams", "tci_ctx_stream",
     "tci_rrlu_f64", "tci_rrlu_batched_f64", "tci_pi_eval_f64", "tci_options_default",
     "tci_solve_submit", "tci_batch_query", "tci_batch_wait", "tci_batch_result",
     "tci_batch_history", "tci_batch_linkdims", "tci_batch_indexset", "tci_batch_sitetensor",

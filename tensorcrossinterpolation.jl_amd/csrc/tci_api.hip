@@ -220,6 +220,13 @@ tci_status tci_ctx_synchronize(tci_ctx *ctx)
     return TCI_OK;
 }
 
+tci_status tci_ctx_stream(tci_ctx *ctx, void **stream)
+{
+    if (!ctx || !stream) return set_err(TCI_ERR_ARG, "bad arguments");
+    *stream = (void *)ctx->stream;
+    return TCI_OK;
+}
+
 tci_status tci_ctx_set_streams(tci_ctx *ctx, int32_t n)
 {
     if (!ctx || n < 1 || n > 64) return set_err(TCI_ERR_ARG, "need 1 <= n <= 64 streams");
