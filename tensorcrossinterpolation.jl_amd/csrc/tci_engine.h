@@ -951,17 +951,25 @@ struct Engine {
     ENG int globalsearch(double abstol)
     {
         const int ns = D.nsearchglobalpivot;
-        uint8_t *pts = D.gpts;                          // ns × L
-        uint8_t *found = D.gpts + (int64_t)ns * L;      // ns × L
-        uint8_t *cand = found + (int64_t)ns * L;        // ns·Σd × L
         const int sumd = D.sumd;
         const int ncand = ns * sumd;
-        double *errs = D.work;                          // ncand
+        uint8_t *found = D.gpts + (int64_t)ns * L;      // ns × L
+        // the start points (ns × L), the candidates (ns·Σd × L) and their
+        // errors (ncand, then ns best errors and ns indices) in LDS when
+        // they fit, else in the arena
+        int64_t loff = 0;
+        uint8_t *pts = (uint8_t *)lds_take(((int64_t)ns * L + 7) / 8, loff);
+        uint8_t *cand = (uint8_t *)lds_take(((int64_t)ncand * L + 7) / 8, loff);
+        double *errs = lds_take((int64_t)ncand + 2 * ns, loff);
+        if (!pts) pts = D.gpts;
+        if (!cand) cand = found + (int64_t)ns * L;
+        if (!errs) errs = D.work;
         if (tm.tid() == 0) {
             for (int q = 0; q < ns; ++q)
                 for (int p = 0; p < L; ++p) {
-                    uint64_t r = eng_rng_next(S.rng) >> 32;
-                    pts[(int64_t)q * L + p] = (uint8_t)(1 + (int)(r % (uint64_t)D.localdims[p]));
+                    // (a 32-bit draw, so a 32-bit remainder)
+                    const uint32_t r = (uint32_t)(eng_rng_next(S.rng) >> 32);
+                    pts[(int64_t)q * L + p] = (uint8_t)(1 + (int)(r % (uint32_t)D.localdims[p]));
                 }
         }
         tm.sync();
@@ -989,6 +997,7 @@ struct Engine {
             // candidates' running vectors in registers; one site tensor at a
             // time is staged in LDS (the same products and summation order
             // as below)
+            toff = loff;
             double *Ts = lds_take(tmax, toff);
             double v[kChainCand][kChainReg];
 #pragma unroll
@@ -1040,7 +1049,7 @@ struct Engine {
             // TT values of all candidates, site by site (abstracttensortrain.jl:131,
             // each product summed in ascending bond order), chunks of candidates;
             // keep the running vectors in LDS, and each site tensor too when it fits
-            int64_t off = 0;
+            int64_t off = loff;
             double *Ts = lds_take(tmax, off);
             const int64_t room = tm.lpi_cap > off ? tm.lpi_cap - off : 0;
             int chunk = (int)(room / (2 * (int64_t)chi));
