@@ -945,6 +945,71 @@ struct Engine {
         return st;
     }
 
+    // TT values of the candidates (abstracttensortrain.jl:131: the running
+    // row vector times each site's slice, each product summed in ascending
+    // bond order) with each thread's (at most kChainCand) running vectors in
+    // registers, CR >= every bond dimension.  Each site tensor is staged in
+    // LDS zero-padded to CR×d×CR, so the products run over a fixed CR×CR
+    // block: a padded term adds 0·0 to a sum that is never -0, which leaves
+    // it unchanged, and the entries past the bond dimension are reset to 0
+    // after each site.  errs[c] = |f(x_c) - TT(x_c)|.  Returns false (and
+    // does nothing) when the padded tensor does not fit in LDS.
+    template <int CR>
+    ENG_NOINLINE bool chain_reg(const uint8_t *cand, int ncand, double *errs, int64_t loff)
+    {
+        int dmax = 1;
+        for (int t = 0; t < L; ++t) dmax = D.localdims[t] > dmax ? D.localdims[t] : dmax;
+        int64_t off = loff;
+        double *Tp = lds_take((int64_t)CR * dmax * CR, off);
+        if (!Tp) return false;
+        double v[kChainCand][CR];
+#pragma unroll
+        for (int q = 0; q < kChainCand; ++q)
+#pragma unroll
+            for (int i = 0; i < CR; ++i) v[q][i] = i == 0 ? 1.0 : 0.0;
+        for (int t = 0; t < L; ++t) {
+            const int d0 = tm.uniform((int)D.Tdims[3 * t]), d = tm.uniform((int)D.Tdims[3 * t + 1]);
+            const int d2 = tm.uniform((int)D.Tdims[3 * t + 2]);
+            const double *T = D.T + D.tstride * t;
+            tm.sync();
+            for (int e = tm.tid(); e < CR * d * CR; e += tm.nt()) {
+                const int i = e % CR, r = e / CR, sl = r % d, j = r / d;
+                Tp[e] = (i < d0 && j < d2) ? T[i + (int64_t)d0 * (sl + (int64_t)d * j)] : 0.0;
+            }
+            tm.sync();
+#pragma unroll
+            for (int q = 0; q < kChainCand; ++q) {
+                const int c = tm.tid() + q * tm.nt();
+                if (c < ncand) {
+                    const double *Tc = Tp + CR * (cand[(int64_t)c * L + t] - 1);
+                    double w[CR];
+#pragma unroll
+                    for (int j = 0; j < CR; ++j) {
+                        const double *Tj = Tc + (int64_t)CR * d * j;
+                        double acc = 0.0;
+#pragma unroll
+                        for (int i = 0; i < CR; ++i) acc = acc + v[q][i] * Tj[i];
+                        w[j] = j < d2 ? acc : 0.0;
+                    }
+#pragma unroll
+                    for (int j = 0; j < CR; ++j) v[q][j] = w[j];
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kChainCand; ++q) {
+            const int c = tm.tid() + q * tm.nt();
+            if (c < ncand) {
+                tci_point pt;
+                pt.a = cand + (int64_t)c * L; pt.b = pt.a; pt.na = L; pt.L = L;
+                const double fv = tci_functor_eval(FK ? FK : D.fkind, D.params, D.iparams, &pt);
+                errs[c] = fabs(fv - v[q][0]);
+            }
+        }
+        tm.sync();
+        return true;
+    }
+
     // ------------------------------------------------------ global search
     // DefaultGlobalPivotFinder (globalpivotfinder.jl:143-195), project PRNG.
     static constexpr int kChainReg = 16, kChainCand = 2;
@@ -987,65 +1052,20 @@ struct Engine {
             const int64_t w = t < L ? D.Tdims[3 * t] : 1;
             chi = w > chi ? (int)w : chi;
         }
-        int64_t toff = 0, tmax = 0;
+        int64_t tmax = 0;
         for (int t = 0; t < L; ++t) {
             const int64_t sz = D.Tdims[3 * t] * D.Tdims[3 * t + 1] * D.Tdims[3 * t + 2];
             tmax = sz > tmax ? sz : tmax;
         }
+        bool done = false;
         if (chi <= kChainReg && ncand <= kChainCand * tm.nt()) {
             // bond dimensions up to 16: each thread carries its (at most two)
-            // candidates' running vectors in registers; one site tensor at a
-            // time is staged in LDS (the same products and summation order
-            // as below)
-            toff = loff;
-            double *Ts = lds_take(tmax, toff);
-            double v[kChainCand][kChainReg];
-#pragma unroll
-            for (int q = 0; q < kChainCand; ++q)
-#pragma unroll
-                for (int i = 0; i < kChainReg; ++i) v[q][i] = i == 0 ? 1.0 : 0.0;
-            for (int t = 0; t < L; ++t) {
-                const int64_t d0 = D.Tdims[3 * t], d = D.Tdims[3 * t + 1], d2 = D.Tdims[3 * t + 2];
-                const double *T = D.T + D.tstride * t;
-                if (Ts) {
-                    tm.sync();
-                    for (int64_t q = tm.tid(); q < d0 * d * d2; q += tm.nt()) Ts[q] = T[q];
-                    tm.sync();
-                    T = Ts;
-                }
-#pragma unroll
-                for (int q = 0; q < kChainCand; ++q) {
-                    const int c = tm.tid() + q * tm.nt();
-                    if (c < ncand) {
-                        const double *Tc = T + d0 * (cand[(int64_t)c * L + t] - 1);
-                        double w[kChainReg];
-#pragma unroll
-                        for (int j = 0; j < kChainReg; ++j) {
-                            double acc = 0.0;
-                            if (j < d2) {
-#pragma unroll
-                                for (int i = 0; i < kChainReg; ++i)
-                                    if (i < d0) acc = acc + v[q][i] * Tc[i + d0 * d * j];
-                            }
-                            w[j] = acc;
-                        }
-#pragma unroll
-                        for (int j = 0; j < kChainReg; ++j) v[q][j] = w[j];
-                    }
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < kChainCand; ++q) {
-                const int c = tm.tid() + q * tm.nt();
-                if (c < ncand) {
-                    tci_point pt;
-                    pt.a = cand + (int64_t)c * L; pt.b = pt.a; pt.na = L; pt.L = L;
-                    const double fv = tci_functor_eval(FK ? FK : D.fkind, D.params, D.iparams, &pt);
-                    errs[c] = fabs(fv - v[q][0]);
-                }
-            }
-            tm.sync();
-        } else {
+            // candidates' running vectors in registers
+            if (chi <= 8) done = chain_reg<8>(cand, ncand, errs, loff);
+            else if (chi <= 12) done = chain_reg<12>(cand, ncand, errs, loff);
+            else done = chain_reg<16>(cand, ncand, errs, loff);
+        }
+        if (!done) {
             // TT values of all candidates, site by site (abstracttensortrain.jl:131,
             // each product summed in ascending bond order), chunks of candidates;
             // keep the running vectors in LDS, and each site tensor too when it fits
