@@ -37,7 +37,9 @@ _CASES = [pytest.param(s, 1, id=s[0] + "-1") for s in specs()] + \
          [pytest.param(s, 256, id=s[0] + "-256") for s in specs() if s[0] in _MT]
 
 
-_CASES_LDS = [pytest.param(s, 1, id=s[0] + "-lds") for s in specs()]
+_CASES_LDS = [pytest.param(s, 1, id=s[0] + "-lds") for s in specs()] + \
+             [pytest.param(s, 256, id=s[0] + "-256-lds") for s in specs()
+              if s[0] in {"lorentz5_pivots", "random_tt"}]
 
 
 @pytest.fixture
