@@ -808,6 +808,109 @@ struct Engine {
             for (int i = 0; i < c; ++i) Pi1[q + (int64_t)r * i] = x[sx * i];
     }
 
+    // lu_wave and subst_row for c <= CR <= the wave width, with the rows in
+    // registers: lane i holds row i of A during the elimination (the same
+    // operations per element and the same pivot choice as lu_wave), then the
+    // factors go to the wave's LDS block Ap, padded to CR×CR with the
+    // identity, and lane q substitutes row q of Pi1 (r rows, in place) with
+    // its c values in registers; a padded term subtracts 0·0, which leaves
+    // every value unchanged.  ipiv: 2c ints as in lu_wave.  Returns 1 when a
+    // pivot is zero.
+    template <int CR>
+    ENG_NOINLINE int site_solve_reg(const double *At, int c, int32_t *ipiv, double *Pi1, int r, double *Ap)
+    {
+        const int ln = tm.lane();
+        const bool mine = ln < c;
+        const int lr = mine ? ln : 0;
+        double a[CR];
+#pragma unroll
+        for (int j = 0; j < CR; ++j) {
+            const double v = At[lr + (int64_t)c * (j < c ? j : 0)];
+            a[j] = (mine && j < c) ? v : 0.0;
+        }
+        int singular = 0;
+#pragma unroll
+        for (int k = 0; k < CR; ++k) {
+            if (k >= c) continue;
+            // first maximum |A[i,k]| over i >= k (a NaN never wins; a NaN
+            // diagonal keeps row k)
+            const double akk = tm.wave_bcast_f64(a[k], k);
+            int pr = k;
+            if (akk == akk) {
+                const double v = fabs(a[k]);
+                const double lv = (ln >= k && mine && v == v) ? v : -1.0;
+                const double g = tm.wave_max(lv);
+                pr = (int)tm.wave_min(lv == g ? (uint32_t)ln : 0xffffffffu);
+            }
+            pr = tm.uniform(pr);
+            if (ln == 0) ipiv[k] = pr;
+            if (pr != k) {
+#pragma unroll
+                for (int j = 0; j < CR; ++j) {
+                    const double xk = tm.wave_bcast_f64(a[j], k), xp = tm.wave_bcast_f64(a[j], pr);
+                    a[j] = ln == k ? xp : (ln == pr ? xk : a[j]);
+                }
+            }
+            const double d = tm.wave_bcast_f64(a[k], k);
+            if (d == 0.0) singular = 1;
+            const bool below = ln > k && mine;
+            if (below) a[k] = a[k] / d;
+#pragma unroll
+            for (int j = k + 1; j < CR; ++j) {
+                if (j >= c) continue;
+                const double pj = tm.wave_bcast_f64(a[j], k);
+                if (below) a[j] = a[j] - a[k] * pj;
+            }
+        }
+        // the factors, padded with the identity
+        if (mine) {
+#pragma unroll
+            for (int j = 0; j < CR; ++j) Ap[ln + CR * j] = j < c ? a[j] : 0.0;
+        }
+        for (int i = c + ln; i < CR; i += tm.wsize())
+            for (int j = 0; j < CR; ++j) Ap[i + CR * j] = i == j ? 1.0 : 0.0;
+        if (ln == 0) {
+            int32_t *perm = ipiv + c;
+            for (int i = 0; i < c; ++i) perm[i] = i;
+            for (int k = 0; k < c; ++k) {
+                const int pk = ipiv[k], t = perm[k];
+                perm[k] = perm[pk];
+                perm[pk] = t;
+            }
+        }
+        tm.wave_sync();
+        const int32_t *perm = ipiv + c;
+        for (int q0 = 0; q0 < r; q0 += tm.wsize()) {
+            const int q = q0 + ln;
+            if (q < r) {
+                double x[CR];
+#pragma unroll
+                for (int i = 0; i < CR; ++i) {
+                    const double v = Pi1[q + (int64_t)r * perm[i < c ? i : 0]];
+                    x[i] = i < c ? v : 0.0;
+                }
+#pragma unroll
+                for (int i = 0; i < CR; ++i) {
+                    double s = x[i];
+#pragma unroll
+                    for (int t = 0; t < i; ++t) s = s - Ap[i + CR * t] * x[t];
+                    x[i] = s;
+                }
+#pragma unroll
+                for (int i = CR - 1; i >= 0; --i) {
+                    double s = x[i];
+#pragma unroll
+                    for (int t = i + 1; t < CR; ++t) s = s - Ap[i + CR * t] * x[t];
+                    x[i] = s / Ap[i + CR * i];
+                }
+#pragma unroll
+                for (int i = 0; i < CR; ++i)
+                    if (i < c) Pi1[q + (int64_t)r * i] = x[i];
+            }
+        }
+        return singular;
+    }
+
     // fillsitetensors! (globalsearch.jl:97-104) → setsitetensor!(tci, f, b)
     // (tensorci2.jl:367-394).  When no global search follows (nsearch == 0)
     // the tensors are never read before sweep1site! rebuilds all of them, so
@@ -840,7 +943,11 @@ struct Engine {
         int cmax = 1;
         for (int b = 0; b < nsolve; ++b) cmax = setn(SET_I, b + 1) > cmax ? setn(SET_I, b + 1) : cmax;
         int64_t off0 = 0;
-        double *xsl = lds_take((int64_t)tm.nwaves() * tm.wsize() * cmax, off0);
+        // per-wave padded factor blocks for the register path (bond
+        // dimensions up to 16), else row scratch for the substitutions
+        // (null: in place in HBM)
+        double *rsl = cmax <= 16 && cmax <= tm.wsize() ? lds_take((int64_t)tm.nwaves() * 256, off0) : nullptr;
+        double *xsl = rsl ? nullptr : lds_take((int64_t)tm.nwaves() * tm.wsize() * cmax, off0);
         for (int g0 = 0; g0 < nsolve;) {
             // a group: consecutive sites whose Pᵀ and pivots fit in LDS
             // together; a site that does not fit alone uses the HBM buffers
@@ -871,8 +978,15 @@ struct Engine {
                 double *At = onchip ? lds_take((int64_t)np * np + np, off) : D.pmat;
                 int32_t *ipv = onchip ? (int32_t *)(At + (int64_t)np * np) : D.ipiv;
                 if ((b - g0) % tm.nwaves() != tm.wave()) continue;
-                if (lu_wave(At, np, ipv, onchip) && !singular_site) singular_site = b + 1;
                 double *T = Tslot(b);
+                if (rsl) {
+                    double *Ap = rsl + 256 * tm.wave();
+                    const int sg = np <= 8 ? site_solve_reg<8>(At, np, ipv, T, m, Ap)
+                                           : site_solve_reg<16>(At, np, ipv, T, m, Ap);
+                    if (sg && !singular_site) singular_site = b + 1;
+                    continue;
+                }
+                if (lu_wave(At, np, ipv, onchip) && !singular_site) singular_site = b + 1;
                 double *xw = xsl ? xsl + (int64_t)tm.wave() * tm.wsize() * cmax + tm.lane() : nullptr;
                 for (int q = tm.lane(); q < m; q += tm.wsize())
                     subst_row(T, m, q, At, np, ipv, xw, tm.wsize());
