@@ -320,9 +320,13 @@ def main():
                      "traffic_source": tsrc,
                      "kernel": "tci_solve_kernel", "kernel_ms": round(kms_avg, 3),
                      "bytes_per_solve": bytes_per_solve,
+                     "schur_fp64_tflops": round(B * ws[3] / (kms_avg * 1e-3) / 1e12, 4),
+                     "schur_fp64_frac": round(B * ws[3] / (kms_avg * 1e-3) / 78.6e12, 5),
                      "note": "achieved = algorithmic bytes of the streamed model (8mn per Pi + "
                              "8mn + 16 sum (m-k)(n-k) + 8p(m+n) per rrLU) over the kernel time; "
-                             "traffic = measured HBM bytes per launch: the tiles stay on chip"},
+                             "traffic = measured HBM bytes per launch: the tiles stay on chip, so "
+                             "frac is a streaming-equivalent figure; schur_fp64_* = the Schur "
+                             "update flops 2 sum (m-k)(n-k) over the kernel time vs 78.6 TFLOP/s"},
     }
     if args.served > 0:
         # served: Poisson arrivals of the same request at a fraction of the
