@@ -39,3 +39,11 @@ def test_no_device_is_a_loud_error():
     with pytest.raises(pkg.TCIError) as e:
         pkg.Context(0)
     assert e.value.status == 12        # TCI_ERR_DEVICE: no CPU fallback
+
+
+def test_ctx_stream_rejects_null_arguments():
+    pkg = load_pkg()
+    lib = pkg._abi.lib()
+    sp = C.c_void_p()
+    assert lib.tci_ctx_stream(None, C.byref(sp)) == 3     # TCI_ERR_ARG
+    assert sp.value is None

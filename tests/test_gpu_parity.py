@@ -134,3 +134,28 @@ def test_rrlu_batched_c2_shapes_vs_oracle(tci, ctx):
         assert np.array_equal(g["rowpermutation"], o.rowpermutation), A.shape
         assert np.array_equal(g["colpermutation"], o.colpermutation), A.shape
         assert g["error"] == o.error, A.shape
+
+
+def test_ctx_stream_carries_the_batched_rrlu(tci, ctx):
+    """tci_ctx_stream hands out the stream the batched rrLU runs on: timing
+    events recorded on it around the call bracket the launch, and the
+    results still equal the oracle's."""
+    import ctypes as C
+    import torch
+    lib = tci._abi.lib()
+    sp = C.c_void_p()
+    assert lib.tci_ctx_stream(ctx.handle, C.byref(sp)) == 0 and sp.value
+    s = torch.cuda.ExternalStream(sp.value)
+    rng = np.random.default_rng(11)
+    mats = [rng.standard_normal((60, 8)) @ rng.standard_normal((8, 50)) for _ in range(3)]
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    res = tci.rrlu_batched(mats, reltol=1e-10, ctx=ctx)
+    e1.record(s)
+    e1.synchronize()
+    assert e0.elapsed_time(e1) > 0.0
+    for A, g in zip(mats, res):
+        o = O.rrlu(A, reltol=1e-10)
+        assert g["npivot"] == o.npivot
+        assert np.array_equal(g["rowpermutation"], o.rowpermutation)
+        assert np.array_equal(g["colpermutation"], o.colpermutation)
