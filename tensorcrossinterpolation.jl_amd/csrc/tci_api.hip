@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/tci.h"
@@ -56,6 +57,7 @@ struct PoolBuf {
 struct tci_ctx {
     int device;
     hipStream_t stream;                 // synchronous calls and the first solve stream
+    std::vector<int32_t> lu_order;      // host copy of the last batched rrLU job order
     std::vector<hipStream_t> extra;     // more solve streams (tci_ctx_set_streams)
     size_t next = 0;
     uint64_t budget;
@@ -289,7 +291,23 @@ tci_status tci_rrlu_batched_f64(tci_ctx *ctx, tci_lu_job *jobs, int64_t count)
     double *gstage = nullptr;
     const int64_t lds_cap = 48 * 1024 / 16;
     if (mx > lds_cap) HIPCHK(hipMallocAsync((void **)&gstage, sizeof(double) * 2 * mx * count, ctx->stream));
-    hipError_t e = launch_rrlu(jobs, count, mx, gstage, 2 * mx, ctx->stream);
+    // largest matrices first (longest-processing-time order), so that the
+    // workgroups that start last are the short ones; results do not depend
+    // on the order
+    // (the host copy lives in the context: the next call on this stream
+    // synchronizes before it reuses it)
+    std::vector<int32_t> &order = ctx->lu_order;
+    order.resize((size_t)count);
+    for (int64_t q = 0; q < count; ++q) order[(size_t)q] = (int32_t)q;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        return h[(size_t)a].m * h[(size_t)a].n > h[(size_t)b].m * h[(size_t)b].n;
+    });
+    int32_t *dorder = nullptr;
+    HIPCHK(hipMallocAsync((void **)&dorder, sizeof(int32_t) * (size_t)count, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dorder, order.data(), sizeof(int32_t) * (size_t)count, hipMemcpyHostToDevice,
+                          ctx->stream));
+    hipError_t e = launch_rrlu(jobs, count, mx, gstage, 2 * mx, dorder, ctx->stream);
+    (void)hipFreeAsync(dorder, ctx->stream);
     if (gstage) (void)hipFreeAsync(gstage, ctx->stream);
     if (e != hipSuccess) return set_err(TCI_ERR_DEVICE, "rrlu launch: %s", hipGetErrorString(e));
     return TCI_OK;

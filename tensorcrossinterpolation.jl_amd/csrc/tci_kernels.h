@@ -21,8 +21,9 @@ size_t lds_plan_bytes(const LdsPlan &p);
 int solve_static_lds(int team, int fkind);
 hipError_t launch_solve(const SolveDesc *descs_device, int count, int team, int fkind,
                         const LdsPlan &plan, hipStream_t st);
+// order (device, count entries, or null): the job index each workgroup takes
 hipError_t launch_rrlu(tci_lu_job *jobs_device, int64_t count, int64_t max_side, double *gstage,
-                       int64_t gstage_stride, hipStream_t st);
+                       int64_t gstage_stride, const int32_t *order, hipStream_t st);
 hipError_t launch_lu_unpack(const double *A, int64_t m, int64_t n, int64_t p, int leftorth,
                             double *L, int64_t ldl, double *U, int64_t ldu, hipStream_t st);
 hipError_t launch_kron_center(const uint8_t *left, int64_t nI, int nl, const int32_t *cdims, int M,

@@ -68,13 +68,16 @@ __global__ __launch_bounds__(NT, WPE) void tci_solve_kernel(const SolveDesc *__r
 }
 
 // ------------------------------------------------------------------ rrLU
+// order: the job each workgroup takes (largest first, so the long
+// factorizations start early and the launch ends evenly), or null
 __global__ __launch_bounds__(kTeam) void tci_rrlu_kernel(tci_lu_job *__restrict__ jobs,
                                                          int64_t count, int lds_stage_cap,
                                                          double *__restrict__ gstage,
-                                                         int64_t gstage_stride)
+                                                         int64_t gstage_stride,
+                                                         const int32_t *__restrict__ order)
 {
-    const int64_t s = blockIdx.x;
-    if (s >= count) return;
+    if ((int64_t)blockIdx.x >= count) return;
+    const int64_t s = order ? (int64_t)order[blockIdx.x] : (int64_t)blockIdx.x;
     extern __shared__ double stage[];
     tci_lu_job &J = jobs[s];
     const int m = (int)J.m, n = (int)J.n;
@@ -258,12 +261,12 @@ hipError_t launch_solve(const SolveDesc *descs, int count, int team, int fkind, 
 }
 
 hipError_t launch_rrlu(tci_lu_job *jobs, int64_t count, int64_t max_side, double *gstage,
-                       int64_t gstage_stride, hipStream_t st)
+                       int64_t gstage_stride, const int32_t *order, hipStream_t st)
 {
     const int cap = 48 * 1024 / 16;
     const size_t lds = max_side <= cap ? (size_t)2 * max_side * sizeof(double) : 0;
     hipLaunchKernelGGL(tci_rrlu_kernel, dim3((unsigned)count), dim3(kTeam), lds, st, jobs, count,
-                       max_side <= cap ? (int)max_side : 0, gstage, gstage_stride);
+                       max_side <= cap ? (int)max_side : 0, gstage, gstage_stride, order);
     return hipGetLastError();
 }
 
