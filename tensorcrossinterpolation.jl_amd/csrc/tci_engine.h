@@ -143,11 +143,36 @@ ENG int rrlu_team(Team &tm, double *A, int m, int n, int64_t maxrank, double rel
         best.v = -1.0;
         best.lin = INT64_MAX;
         best.a = 0.0;
-        // columns over waves, rows over lanes: coalesced, no index division
+        // columns over waves, rows over lanes: coalesced, no index division;
+        // four rows per lane at a time, so each wave has four loads in flight
+        // (the candidate order is total, so the visiting order is free)
+        const int ws = tm.wsize();
         for (int j = k + 1 + tm.wave(); j < n; j += tm.nwaves()) {
             double *col = A + lda * j;
             const double yj = tm.ys[j];
-            for (int i = k + 1 + tm.lane(); i < m; i += tm.wsize()) {
+            int i = k + 1 + tm.lane();
+            for (; i + 3 * ws < m; i += 4 * ws) {
+                double c[4], x[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) c[u] = col[i + u * ws];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) x[u] = tm.xs[i + u * ws];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    double t = x[u] * yj;
+                    double a = c[u] - t;
+                    col[i + u * ws] = a;
+                    double v = a * a;
+                    if (v == v) {
+                        ArgBest cnd;
+                        cnd.v = v;
+                        cnd.lin = (i + u * ws) + lda * j;
+                        cnd.a = a;
+                        if (arg_better(cnd, best)) best = cnd;
+                    }
+                }
+            }
+            for (; i < m; i += ws) {
                 double t = tm.xs[i] * yj;
                 double a = col[i] - t;
                 col[i] = a;
