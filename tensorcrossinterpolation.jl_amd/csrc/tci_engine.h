@@ -144,10 +144,61 @@ ENG int rrlu_team(Team &tm, double *A, int m, int n, int64_t maxrank, double rel
         best.lin = INT64_MAX;
         best.a = 0.0;
         // columns over waves, rows over lanes: coalesced, no index division;
-        // four rows per lane at a time, so each wave has four loads in flight
-        // (the candidate order is total, so the visiting order is free)
+        // four rows per lane at a time, so each wave has four loads per column
+        // in flight (the candidate order is total, so the visiting order is
+        // free)
         const int ws = tm.wsize();
-        for (int j = k + 1 + tm.wave(); j < n; j += tm.nwaves()) {
+        const int nw = tm.nwaves();
+        int j = k + 1 + tm.wave();
+        // two columns at a time while both exist (eight loads in flight)
+        for (; j + nw < n; j += 2 * nw) {
+            double *col0 = A + lda * j, *col1 = A + lda * (j + nw);
+            const double y0 = tm.ys[j], y1 = tm.ys[j + nw];
+            int i = k + 1 + tm.lane();
+            for (; i + 3 * ws < m; i += 4 * ws) {
+                double c[8], x[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    c[u] = col0[i + u * ws];
+                    c[4 + u] = col1[i + u * ws];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) x[u] = tm.xs[i + u * ws];
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        double t = x[u] * (h ? y1 : y0);
+                        double a = c[4 * h + u] - t;
+                        (h ? col1 : col0)[i + u * ws] = a;
+                        double v = a * a;
+                        if (v == v) {
+                            ArgBest cnd;
+                            cnd.v = v;
+                            cnd.lin = (i + u * ws) + lda * (j + h * nw);
+                            cnd.a = a;
+                            if (arg_better(cnd, best)) best = cnd;
+                        }
+                    }
+            }
+            for (; i < m; i += ws)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    double *col = h ? col1 : col0;
+                    double t = tm.xs[i] * (h ? y1 : y0);
+                    double a = col[i] - t;
+                    col[i] = a;
+                    double v = a * a;
+                    if (v == v) {
+                        ArgBest cnd;
+                        cnd.v = v;
+                        cnd.lin = i + lda * (j + h * nw);
+                        cnd.a = a;
+                        if (arg_better(cnd, best)) best = cnd;
+                    }
+                }
+        }
+        for (; j < n; j += nw) {
             double *col = A + lda * j;
             const double yj = tm.ys[j];
             int i = k + 1 + tm.lane();
