@@ -144,33 +144,38 @@ ENG int rrlu_team(Team &tm, double *A, int m, int n, int64_t maxrank, double rel
         best.lin = INT64_MAX;
         best.a = 0.0;
         // columns over waves, rows over lanes: coalesced, no index division;
-        // four rows per lane at a time, so each wave has four loads per column
-        // in flight (the candidate order is total, so the visiting order is
-        // free)
+        // four rows per lane and up to four columns per wave at a time, so
+        // each wave has up to sixteen loads in flight (the candidate order is
+        // total, so the visiting order is free)
         const int ws = tm.wsize();
         const int nw = tm.nwaves();
         int j = k + 1 + tm.wave();
-        // two columns at a time while both exist (eight loads in flight)
-        for (; j + nw < n; j += 2 * nw) {
-            double *col0 = A + lda * j, *col1 = A + lda * (j + nw);
-            const double y0 = tm.ys[j], y1 = tm.ys[j + nw];
+        // four columns at a time while they exist (sixteen loads in flight)
+        constexpr int NC = 4;
+        for (; j + (NC - 1) * nw < n; j += NC * nw) {
+            double *cl[NC];
+            double yv[NC];
+#pragma unroll
+            for (int h = 0; h < NC; ++h) {
+                cl[h] = A + lda * (j + h * nw);
+                yv[h] = tm.ys[j + h * nw];
+            }
             int i = k + 1 + tm.lane();
             for (; i + 3 * ws < m; i += 4 * ws) {
-                double c[8], x[4];
+                double c[NC * 4], x[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    c[u] = col0[i + u * ws];
-                    c[4 + u] = col1[i + u * ws];
-                }
+                for (int h = 0; h < NC; ++h)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) c[4 * h + u] = cl[h][i + u * ws];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) x[u] = tm.xs[i + u * ws];
 #pragma unroll
-                for (int h = 0; h < 2; ++h)
+                for (int h = 0; h < NC; ++h)
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
-                        double t = x[u] * (h ? y1 : y0);
+                        double t = x[u] * yv[h];
                         double a = c[4 * h + u] - t;
-                        (h ? col1 : col0)[i + u * ws] = a;
+                        cl[h][i + u * ws] = a;
                         double v = a * a;
                         if (v == v) {
                             ArgBest cnd;
@@ -183,11 +188,10 @@ ENG int rrlu_team(Team &tm, double *A, int m, int n, int64_t maxrank, double rel
             }
             for (; i < m; i += ws)
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    double *col = h ? col1 : col0;
-                    double t = tm.xs[i] * (h ? y1 : y0);
-                    double a = col[i] - t;
-                    col[i] = a;
+                for (int h = 0; h < NC; ++h) {
+                    double t = tm.xs[i] * yv[h];
+                    double a = cl[h][i] - t;
+                    cl[h][i] = a;
                     double v = a * a;
                     if (v == v) {
                         ArgBest cnd;
