@@ -135,7 +135,8 @@ typedef struct tci_options {
     int32_t checkconvglobalpivot;  /* 1                                       */
     uint64_t seed;                 /* global pivot search PRNG seed            */
     int32_t chi_cap;               /* device capacity for any |Iset|/|Jset|;
-                                      0 = min(maxbonddim, 64)                  */
+                                      0 = min(maxbonddim, rank bound, 64); see
+                                      tci_capacity_plan                        */
     int32_t reserved;
 } tci_options;
 
@@ -154,6 +155,17 @@ typedef struct tci_problem {
 } tci_problem;
 
 typedef struct tci_batch tci_batch;
+
+/* Capacity schedule of a problem (host only, no device needed).  *chi_used:
+ * the set capacity tci_solve_submit gives it (opts.chi_cap, or when that is
+ * 0 the default min(maxbonddim, rank bound, 64), the rank bound being
+ * max_b min(Π_{i<=b} d_i, Π_{i>b} d_i)); *chi_limit: min(maxbonddim, rank
+ * bound), the capacity at which TCI_ERR_CAPACITY can no longer occur.  The
+ * reference has no capacities (tensorci2.jl grows its Vectors): a caller
+ * that gets TCI_ERR_CAPACITY for a solve resubmits it with chi_cap doubled,
+ * up to chi_limit, which reproduces the reference's behaviour for every
+ * maxbonddim (the crossinterpolate2 mirrors in Python and Julia do). */
+tci_status tci_capacity_plan(const tci_problem *p, int64_t *chi_used, int64_t *chi_limit);
 
 /* Submit `count` independent solves as one launch: one workgroup per solve,
  * every solve's pivot sets, Pi buffers and site tensors resident in HBM for

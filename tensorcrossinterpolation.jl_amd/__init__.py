@@ -433,29 +433,67 @@ def _holder(f, localdims, initialpivots, kw):
     return ProblemHolder(localdims, f.kind, f.params, f.iparams, initialpivots, tolerance=tol, **kw)
 
 
+def capacity_plan(holder):
+    """(chi_used, chi_limit) of a problem (tci_capacity_plan)."""
+    used, limit = C.c_int64(), C.c_int64()
+    _check(_abi.lib().tci_capacity_plan(C.byref(holder.st), C.byref(used), C.byref(limit)))
+    return used.value, limit.value
+
+
 def crossinterpolate2_batch(problems, ctx=None, return_batch=False):
     """Submit many crossinterpolate2 problems as ONE launch.
 
     problems: list of (f, localdims, initialpivots_or_None, kwargs).
-    Returns a list of (TensorCI2, ranks, errors) in order."""
+    Returns a list of TensorCI2 in order (and the device time of the
+    launches in ms with return_batch).
+
+    The reference's index sets grow without bound; the device's have a
+    capacity fixed at submit.  A solve that ends with TCI_ERR_CAPACITY while
+    its capacity is below min(maxbonddim, rank bound) is resubmitted (with
+    the other such solves, as one launch) at twice the capacity, so the
+    default options succeed wherever the reference does.  A caller that
+    sets chi_cap explicitly gets the CAPACITY status instead."""
     ctx = ctx or default_context()
-    holders = [_holder(f, ld, piv, kw) for f, ld, piv, kw in problems]
-    arr = (_abi.tci_problem * len(holders))(*[h.st for h in holders])
-    b = C.c_void_p()
-    _check(_abi.lib().tci_solve_submit(ctx.handle, len(holders), arr, C.byref(b)))
-    try:
-        _check(_abi.lib().tci_batch_wait(b))
-        out = []
-        for i, (f, ld, piv, kw) in enumerate(problems):
-            t = TensorCI2(b, i, ld)
-            out.append(t)
-        if return_batch:
+    lib = _abi.lib()
+    problems = list(problems)
+    out = [None] * len(problems)
+    chi = [None] * len(problems)     # capacity override per problem (None: as given)
+    pending = list(range(len(problems)))
+    total_ms = 0.0
+    while pending:
+        holders = []
+        for i in pending:
+            f, ld, piv, kw = problems[i]
+            kw = dict(kw)
+            if chi[i] is not None:
+                kw["chi_cap"] = chi[i]
+            holders.append(_holder(f, ld, piv, kw))
+        arr = (_abi.tci_problem * len(holders))(*[h.st for h in holders])
+        b = C.c_void_p()
+        _check(lib.tci_solve_submit(ctx.handle, len(holders), arr, C.byref(b)))
+        retry = []
+        try:
+            _check(lib.tci_batch_wait(b))
+            for k, i in enumerate(pending):
+                t = TensorCI2(b, k, problems[i][1])
+                user_cap = int(problems[i][3].get("chi_cap", 0) or 0)
+                if t.status == 10 and user_cap == 0:
+                    used, limit = capacity_plan(holders[k])
+                    if used < limit:
+                        chi[i] = int(min(2 * used, limit, 8192))
+                        if chi[i] > used:
+                            retry.append(i)
+                            continue
+                out[i] = t
             ms = C.c_float()
-            _abi.lib().tci_batch_elapsed_ms(b, C.byref(ms))
-            return out, ms.value
-        return out
-    finally:
-        _abi.lib().tci_batch_free(b)
+            lib.tci_batch_elapsed_ms(b, C.byref(ms))
+            total_ms += ms.value
+        finally:
+            lib.tci_batch_free(b)
+        pending = retry
+    if return_batch:
+        return out, total_ms
+    return out
 
 
 def crossinterpolate2(ValueType, f, localdims, initialpivots=None, ctx=None, **kwargs):

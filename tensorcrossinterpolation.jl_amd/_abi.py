@@ -163,6 +163,7 @@ def lib():
     L.tci_batch_free.argtypes = [vp]
     L.tci_batch_elapsed_ms.argtypes = [vp, P(C.c_float)]
     L.tci_batch_profile.argtypes = [vp, C.c_int32, P(C.c_double)]
+    L.tci_capacity_plan.argtypes = [P(tci_problem), P(C.c_int64), P(C.c_int64)]
     _LIB = L
     return L
 
@@ -174,5 +175,5 @@ EXPORTED_SYMBOLS = [
     "tci_solve_submit", "tci_batch_query", "tci_batch_wait", "tci_batch_result",
     "tci_batch_history", "tci_batch_linkdims", "tci_batch_indexset", "tci_batch_sitetensor",
     "tci_batch_pivoterrors", "tci_batch_bonderrors", "tci_batch_free", "tci_batch_elapsed_ms",
-    "tci_batch_profile",
+    "tci_batch_profile", "tci_capacity_plan",
 ]

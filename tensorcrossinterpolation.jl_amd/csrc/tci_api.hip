@@ -389,25 +389,26 @@ tci_status tci_pi_eval_f64(tci_ctx *ctx, const tci_functor *f, int32_t L, const 
     if (!ctx || !f || !localdims || L < 1 || nl < 0 || nr < 0 || nl + nr > L || nI < 0 || nJ < 0)
         return set_err(TCI_ERR_ARG, "bad arguments");
     const int M = L - nl - nr;
+    const int64_t lim = (int64_t)1 << 34;   // elements of one evaluation (128 GiB of output)
     int64_t nc = 1;
-    for (int t = 0; t < M; ++t) nc *= localdims[nl + t];
-    const int64_t nrow = nI * nc, tot = nrow * nJ;
+    for (int t = 0; t < M; ++t) {
+        const int64_t d = localdims[nl + t];
+        if (d < 1 || d > 255) return set_err(TCI_ERR_ARG, "local dimensions must be in 1..255");
+        nc *= d;
+        if (nc > lim) return set_err(TCI_ERR_ARG, "Pi tensor too large");
+    }
+    if (nI > lim || nJ > lim || (nI > 0 && nc > lim / nI)) return set_err(TCI_ERR_ARG, "Pi tensor too large");
+    const int64_t nrow = nI * nc;
+    if (nJ > 0 && nrow > lim / nJ) return set_err(TCI_ERR_ARG, "Pi tensor too large");
+    const int64_t tot = nrow * nJ;
     if (tot == 0) return TCI_OK;   // empty Pi (batcheval.jl:40-42)
     if (!out || (nI > 0 && nl > 0 && !left) || (nJ > 0 && nr > 0 && !right))
         return set_err(TCI_ERR_ARG, "NULL buffer");
-    // validate the functor against a one-point problem description
-    tci_problem pr;
-    memset(&pr, 0, sizeof pr);
-    pr.L = L;
-    pr.localdims = localdims;
-    pr.f = *f;
-    tci_options_default(&pr.opts);
-    SolveCaps c;
+    // the functor and local dims are checked for every L, so no parameter
+    // read of the eval kernel goes past its block
     std::string msg;
-    if (L >= 2) {
-        int s = check_problem(pr, c, &msg);
-        if (s != TCI_OK) return set_err((tci_status)s, "%s", msg.c_str());
-    }
+    const int fst = check_functor(*f, L, localdims, &msg);
+    if (fst != TCI_OK) return set_err((tci_status)fst, "%s", msg.c_str());
     for (int64_t i = 0; i < nI * nl; ++i)
         if (left[i] < 1 || left[i] > localdims[i % nl]) return set_err(TCI_ERR_ARG, "left index out of range");
     for (int64_t j = 0; j < nJ * nr; ++j)
@@ -474,6 +475,21 @@ void tci_options_default(tci_options *o)
     o->checkconvglobalpivot = 1;
     o->seed = 0;
     o->chi_cap = 0;
+}
+
+tci_status tci_capacity_plan(const tci_problem *p, int64_t *chi_used, int64_t *chi_limit)
+{
+    if (!p || !chi_used || !chi_limit) return set_err(TCI_ERR_ARG, "bad arguments");
+    SolveCaps c;
+    std::string msg;
+    int s = check_problem(*p, c, &msg);
+    if (s != TCI_OK) return set_err((tci_status)s, "%s", msg.c_str());
+    const int64_t used = p->opts.chi_cap > 0 ? p->opts.chi_cap : default_chi_cap(*p);
+    *chi_used = used < 1 ? 1 : used;
+    int64_t lim = rank_bound(p->L, p->localdims);
+    if (p->opts.maxbonddim < lim) lim = p->opts.maxbonddim;
+    *chi_limit = lim < 1 ? 1 : lim;
+    return TCI_OK;
 }
 
 static void batch_release(tci_batch *b)

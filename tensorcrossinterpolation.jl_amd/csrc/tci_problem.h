@@ -13,32 +13,48 @@
 
 namespace tci {
 
-// Returns TCI_OK or an error with *msg set.
-inline int check_problem(const tci_problem &p, SolveCaps &c, std::string *msg)
+// Largest bond dimension any solve of these local dims can reach: the rank
+// at bond b is at most min(Π_{i<=b} d_i, Π_{i>b} d_i) (saturated at 2^30).
+inline int64_t rank_bound(int L, const int32_t *localdims)
+{
+    const int64_t sat = (int64_t)1 << 30;
+    int64_t best = 1;
+    for (int b = 0; b + 1 < L; ++b) {
+        int64_t lp = 1, rp = 1;
+        for (int i = 0; i <= b && lp < sat; ++i) lp *= localdims[i];
+        for (int i = b + 1; i < L && rp < sat; ++i) rp *= localdims[i];
+        int64_t r = lp < rp ? lp : rp;
+        if (r > sat) r = sat;
+        best = r > best ? r : best;
+    }
+    return best;
+}
+
+// Default capacity of every |Iset|/|Jset| (before the global-pivot extras):
+// min(maxbonddim, the rank bound, kDefaultChi).  A solve whose sets outgrow
+// it ends with TCI_ERR_CAPACITY; the host side then resubmits it at twice
+// the capacity (up to min(maxbonddim, rank bound)), so the default options
+// succeed wherever the reference does.
+constexpr int64_t kDefaultChi = 64;
+inline int64_t default_chi_cap(const tci_problem &p)
+{
+    int64_t chi = kDefaultChi;
+    if (p.opts.maxbonddim < chi) chi = p.opts.maxbonddim;
+    const int64_t rb = rank_bound(p.L, p.localdims);
+    return rb < chi ? rb : chi;
+}
+
+// Checks a functor against the local dims it is evaluated on (any L >= 1):
+// the dims are in 1..255 and the parameter blocks cover every read the
+// functor makes.  Returns TCI_OK or an error with *msg set.
+inline int check_functor(const tci_functor &f, int L, const int32_t *localdims, std::string *msg)
 {
     auto bad = [&](int code, const char *m) { if (msg) *msg = m; return code; };
-    if (p.L < 2) return bad(TCI_ERR_GENERIC, "localdims should have at least 2 elements!");
-    if (p.L > 250) return bad(TCI_ERR_ARG, "at most 250 sites are supported");
-    if (!p.localdims) return bad(TCI_ERR_ARG, "localdims is NULL");
-    const int L = p.L;
-    int dmax = 0, sumd = 0;
-    for (int b = 0; b < L; ++b) {
-        int d = p.localdims[b];
-        if (d < 1 || d > 255) return bad(TCI_ERR_ARG, "local dimensions must be in 1..255");
-        dmax = d > dmax ? d : dmax;
-        sumd += d;
-    }
-    const int npiv = p.initialpivots ? p.npivots : 1;
-    if (npiv < 1) return bad(TCI_ERR_ARG, "at least one initial pivot is required");
-    if (p.initialpivots)
-        for (int q = 0; q < npiv; ++q)
-            for (int b = 0; b < L; ++b) {
-                int v = p.initialpivots[(int64_t)q * L + b];
-                if (v < 1 || v > p.localdims[b]) return bad(TCI_ERR_ARG, "initial pivot out of range");
-            }
-    // functor parameter blocks must cover every read the functor makes
-    const tci_functor &f = p.f;
+    if (L < 1 || !localdims) return bad(TCI_ERR_ARG, "localdims is empty");
+    for (int b = 0; b < L; ++b)
+        if (localdims[b] < 1 || localdims[b] > 255) return bad(TCI_ERR_ARG, "local dimensions must be in 1..255");
     const int np = f.nparams, nip = f.niparams;
+    if (np < 0 || nip < 0) return bad(TCI_ERR_ARG, "negative parameter count");
     if (np > 0 && !f.params) return bad(TCI_ERR_ARG, "functor params is NULL");
     if (nip > 0 && !f.iparams) return bad(TCI_ERR_ARG, "functor iparams is NULL");
     switch (f.kind) {
@@ -68,7 +84,7 @@ inline int check_problem(const tci_problem &p, SolveCaps &c, std::string *msg)
         if (nip < L) return bad(TCI_ERR_ARG, "table functor needs L dims");
         int64_t tot = 1;
         for (int b = 0; b < L; ++b) {
-            if (f.iparams[b] != p.localdims[b]) return bad(TCI_ERR_ARG, "table dims differ from localdims");
+            if (f.iparams[b] != localdims[b]) return bad(TCI_ERR_ARG, "table dims differ from localdims");
             tot *= f.iparams[b];
             if (tot > ((int64_t)1 << 31)) return bad(TCI_ERR_ARG, "table too large");
         }
@@ -82,7 +98,7 @@ inline int check_problem(const tci_problem &p, SolveCaps &c, std::string *msg)
         for (int b = 0; b < L; ++b) {
             int cl = f.iparams[b], cr = f.iparams[b + 1], d = f.iparams[L + 1 + b];
             if (cl < 1 || cl > 64 || cr < 1 || cr > 64) return bad(TCI_ERR_ARG, "TT bond dims must be in 1..64");
-            if (d != p.localdims[b]) return bad(TCI_ERR_ARG, "TT local dims differ from localdims");
+            if (d != localdims[b]) return bad(TCI_ERR_ARG, "TT local dims differ from localdims");
             tot += (int64_t)cl * d * cr;
         }
         if (np < tot) return bad(TCI_ERR_ARG, "TT parameter block too short");
@@ -95,6 +111,34 @@ inline int check_problem(const tci_problem &p, SolveCaps &c, std::string *msg)
     default:
         return bad(TCI_ERR_ARG, "unknown functor kind");
     }
+    return TCI_OK;
+}
+
+// Returns TCI_OK or an error with *msg set.
+inline int check_problem(const tci_problem &p, SolveCaps &c, std::string *msg)
+{
+    auto bad = [&](int code, const char *m) { if (msg) *msg = m; return code; };
+    if (p.L < 2) return bad(TCI_ERR_GENERIC, "localdims should have at least 2 elements!");
+    if (p.L > 250) return bad(TCI_ERR_ARG, "at most 250 sites are supported");
+    if (!p.localdims) return bad(TCI_ERR_ARG, "localdims is NULL");
+    const int L = p.L;
+    const int fst = check_functor(p.f, L, p.localdims, msg);
+    if (fst != TCI_OK) return fst;
+    int dmax = 0, sumd = 0;
+    for (int b = 0; b < L; ++b) {
+        int d = p.localdims[b];
+        dmax = d > dmax ? d : dmax;
+        sumd += d;
+    }
+    const int npiv = p.initialpivots ? p.npivots : 1;
+    if (npiv < 1) return bad(TCI_ERR_ARG, "at least one initial pivot is required");
+    if (p.initialpivots)
+        for (int q = 0; q < npiv; ++q)
+            for (int b = 0; b < L; ++b) {
+                int v = p.initialpivots[(int64_t)q * L + b];
+                if (v < 1 || v > p.localdims[b]) return bad(TCI_ERR_ARG, "initial pivot out of range");
+            }
+    const int np = p.f.nparams, nip = p.f.niparams;
     const tci_options &o = p.opts;
     if (o.maxiter < 0 || o.maxiter > 100000) return bad(TCI_ERR_ARG, "maxiter out of range");
     if (o.ncheckhistory < 1) return bad(TCI_ERR_ARG, "ncheckhistory must be >= 1");
@@ -102,7 +146,8 @@ inline int check_problem(const tci_problem &p, SolveCaps &c, std::string *msg)
         return bad(TCI_ERR_ARG, "global pivot search sizes out of range");
     if (o.sweepstrategy < 0 || o.sweepstrategy > 2) return bad(TCI_ERR_ARG, "unknown sweep strategy");
     if (o.maxbonddim < 0) return bad(TCI_ERR_ARG, "maxbonddim must be >= 0");
-    int64_t chi = o.chi_cap > 0 ? o.chi_cap : (o.maxbonddim < 64 ? o.maxbonddim : 64);
+    if (o.chi_cap < 0) return bad(TCI_ERR_ARG, "chi_cap must be >= 0");
+    int64_t chi = o.chi_cap > 0 ? o.chi_cap : default_chi_cap(p);
     if (chi < 1) chi = 1;
     if (chi > 8192) return bad(TCI_ERR_ARG, "chi_cap too large");
     int extra = o.maxnglobalpivot > npiv ? o.maxnglobalpivot : npiv;

@@ -22,6 +22,18 @@ import numpy as np
 from . import _abi
 
 
+PENDING = 13          # TCI_ERR_PENDING
+TCI_ERR_DEVICE = 12
+
+
+class ServeError(RuntimeError):
+    """A batch failed on the device (tci_status in .status)."""
+
+    def __init__(self, status, msg=""):
+        super().__init__(f"status {status}: {msg}")
+        self.status = status
+
+
 @dataclass
 class ServedResult:
     status: int
@@ -103,27 +115,44 @@ class TCIServer:
                     if st == 11 and cnt > 1:        # nothing running: admit fewer
                         cnt //= 2
                         continue
-                    raise RuntimeError(lib.tci_last_error().decode())
-            # retire finished batches
+                    msg = lib.tci_last_error().decode()
+                    for b2, _, _ in inflight:
+                        lib.tci_batch_free(b2)
+                    raise ServeError(st, f"submit of requests {nxt}..{nxt + cnt - 1} failed: {msg}")
+            # retire finished batches; only TCI_ERR_PENDING means "still
+            # running": any other failure of the query or of the report
+            # fetch frees the batch and raises
             still = []
-            for b, ids, arr in inflight:
-                if lib.tci_batch_query(b) == 0:
-                    lib.tci_batch_wait(b)                    # reports on the host
-                    done = time.perf_counter() - t0
-                    r = _abi.tci_result()
-                    for k, i in enumerate(ids):
-                        lib.tci_batch_result(b, k, C.byref(r))
-                        m = max(r.niter, 1)
-                        ranks = np.zeros(m, np.int64)
-                        errs = np.zeros(m)
-                        lib.tci_batch_history(b, k, _abi.i64ptr(ranks), _abi.dptr(errs), None)
-                        t = TensorCI2(b, k, requests[i][1]) if keep_tci else None
-                        results[i] = ServedResult(r.status, ranks[:r.niter], errs[:r.niter],
-                                                  r.maxsamplevalue, float(arrivals[i]), done, t)
-                    lib.tci_batch_free(b)
-                    progressed = True
-                else:
+            for pos, (b, ids, arr) in enumerate(inflight):
+                q = lib.tci_batch_query(b)
+                if q == PENDING:
                     still.append((b, ids, arr))
+                    continue
+                w = lib.tci_batch_wait(b) if q == 0 else q   # reports on the host
+                if w == 0:
+                    r = _abi.tci_result()
+                    for k in range(len(ids)):
+                        if lib.tci_batch_result(b, k, C.byref(r)) != 0:
+                            w = TCI_ERR_DEVICE
+                            break
+                if w != 0:
+                    msg = lib.tci_last_error().decode()
+                    # this batch, the ones still running and the ones not yet visited
+                    for b2, _, _ in [(b, ids, arr)] + still + inflight[pos + 1:]:
+                        lib.tci_batch_free(b2)
+                    raise ServeError(w, f"batch of requests {ids[0]}..{ids[-1]} failed: {msg}")
+                done = time.perf_counter() - t0
+                for k, i in enumerate(ids):
+                    lib.tci_batch_result(b, k, C.byref(r))
+                    m = max(r.niter, 1)
+                    ranks = np.zeros(m, np.int64)
+                    errs = np.zeros(m)
+                    lib.tci_batch_history(b, k, _abi.i64ptr(ranks), _abi.dptr(errs), None)
+                    t = TensorCI2(b, k, requests[i][1]) if keep_tci else None
+                    results[i] = ServedResult(r.status, ranks[:r.niter], errs[:r.niter],
+                                              r.maxsamplevalue, float(arrivals[i]), done, t)
+                lib.tci_batch_free(b)
+                progressed = True
             inflight = still
             if not progressed:
                 if nxt < n and not inflight:

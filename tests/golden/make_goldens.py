@@ -14,6 +14,9 @@ is not available, nothing of the reference is executed):
    agrees bit for bit.  The reference publishes no pivot sequences.
 3. tci_c1.json — the 8-site Lorentzian (README.md:24-28, tolerance 1e-8)
    solved by the oracle: ranks, errors, pivot sets, maxsamplevalue.
+4. configs.json (`make_goldens.py --configs`, needs no reference checkout) —
+   the full-size solve configurations of tests/problems.py solved by the
+   oracle: ranks, link dims, errors and a digest of the pivot sets.
 """
 import json
 import os
@@ -130,7 +133,37 @@ def tci_c1():
     return res
 
 
+def set_digest(t, L):
+    """sha256 of every Iset and Jset of a solve, in site order."""
+    import hashlib
+    h = hashlib.sha256()
+    for b in range(L):
+        h.update(json.dumps([t.Iset(b), t.Jset(b)]).encode())
+    return h.hexdigest()
+
+
+def configs():
+    """The solve configurations at full size (tests/problems.py config_specs),
+    solved by the oracle: ranks, link dims, errors, global pivots, max sample
+    and a digest of the pivot sets."""
+    import oracle as O
+    sys.path.insert(0, os.path.dirname(HERE))
+    from problems import config_specs, oracle_opts
+    res = {}
+    for name, kind, params, ip, ld, piv, opts in config_specs():
+        t, r, e = O.crossinterpolate2(O.Functor(kind, params, ip), ld, piv, **oracle_opts(opts))
+        res[name] = dict(ranks=r.tolist(), errors=e.tolist(), linkdims=t.linkdims(),
+                         nglobalpivots=t.nglobalpivots.tolist(), maxsamplevalue=t.maxsamplevalue,
+                         sets_sha256=set_digest(t, len(ld)), workstats=t.workstats())
+    return res
+
+
 def main():
+    if sys.argv[1:] == ["--configs"]:
+        with open(os.path.join(HERE, "configs.json"), "w") as fh:
+            json.dump(configs(), fh, indent=1)
+        print("wrote configs.json")
+        return
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     fx = fixtures(ref)
     with open(os.path.join(HERE, "reference_fixtures.json"), "w") as fh:
@@ -139,6 +172,8 @@ def main():
         json.dump(golden_pivots(fx), fh)
     with open(os.path.join(HERE, "tci_c1.json"), "w") as fh:
         json.dump(tci_c1(), fh)
+    with open(os.path.join(HERE, "configs.json"), "w") as fh:
+        json.dump(configs(), fh, indent=1)
     print("wrote", len(fx), "reference matrices")
 
 

@@ -62,6 +62,52 @@ def specs():
     ]
 
 
+def _mode(nsearch):
+    return "_parity" if nsearch == 0 else ""
+
+
+def c1_spec(nsearch=5):
+    """Plain TCI: 1/(1+v.v) on {1..10}^8, tol 1e-8, initial pivot ones (README.md:24-28)."""
+    return ("c1_lorentz8" + _mode(nsearch), LORENTZ, [1.0], [], [10] * 8, None,
+            dict(tolerance=1e-8, nsearchglobalpivot=nsearch))
+
+
+def c3_params(seed=3):
+    """Quantics workload: 6 Gaussians with centers U(0,1), widths 10^U(-6,-1),
+    amplitudes U(0.5,2), plus 0.1 cos(2^20 x) (seeded numpy Generator)."""
+    rng = np.random.default_rng(seed)
+    K = 6
+    a = rng.uniform(0.5, 2, K)
+    c = rng.uniform(0, 1, K)
+    w = 10.0 ** rng.uniform(-6, -1, K)
+    return np.concatenate([[K], np.stack([a, c, w], 1).ravel(), [0.1, 2.0 ** 20]])
+
+
+def c3_spec(nsearch=5):
+    """Quantics TCI: 30 sites of d = 2, tol 1e-10, maxbonddim 200."""
+    return ("c3_quantics30" + _mode(nsearch), QGC, c3_params(), [], [2] * 30, None,
+            dict(tolerance=1e-10, maxbonddim=200, nsearchglobalpivot=nsearch))
+
+
+def c4_params(seed=4, L=40):
+    """High-dimensional workload: 1/(1 + sum c_i (v_i/16)^2), c_i ~ U(0.5, 2)."""
+    rng = np.random.default_rng(seed)
+    return np.concatenate([[16.0], rng.uniform(0.5, 2, L)])
+
+
+def c4_spec(nsearch=5):
+    """High-dimensional TCI: 40 sites of d = 16, tol 1e-10, maxbonddim 256."""
+    return ("c4_highdim40" + _mode(nsearch), WLORENTZ, c4_params(), [], [16] * 40, None,
+            dict(tolerance=1e-10, maxbonddim=256, nsearchglobalpivot=nsearch))
+
+
+def config_specs():
+    """The solve configurations at their full sizes, in the survey's parity
+    mode (no global search, fully deterministic) and with the optimize!
+    defaults (global search with the project PRNG)."""
+    return [c1_spec(0), c1_spec(5), c3_spec(0), c3_spec(5), c4_spec(0), c4_spec(5)]
+
+
 def oracle_opts(opts):
     return {k: v for k, v in opts.items() if k != "chi_cap"}
 

@@ -1,0 +1,133 @@
+"""Host-side logic that needs no GPU: the capacity plan of a problem, the
+serving loop's handling of device failures (with a stubbed library), and the
+functor checks of the Pi-evaluation entry point (which run before any device
+call)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import load_pkg
+
+
+def _holder(pkg, ld, **kw):
+    return pkg._abi.ProblemHolder(ld, pkg._abi.F_LORENTZ_INT, [1.0], **kw)
+
+
+def test_capacity_plan_defaults():
+    pkg = load_pkg()
+    # 8 sites of d = 10: rank bound 10^4, default capacity 64, limit 10^4
+    assert pkg.capacity_plan(_holder(pkg, [10] * 8)) == (64, 10 ** 4)
+    # maxbonddim below 64 caps both
+    assert pkg.capacity_plan(_holder(pkg, [10] * 8, maxbonddim=20)) == (20, 20)
+    # a small domain caps both at its rank bound: 2^2 = 4 at the middle bond
+    assert pkg.capacity_plan(_holder(pkg, [2] * 4)) == (4, 4)
+    assert pkg.capacity_plan(_holder(pkg, [3, 5])) == (3, 3)
+    # an explicit chi_cap is used as given
+    assert pkg.capacity_plan(_holder(pkg, [10] * 8, chi_cap=7)) == (7, 10 ** 4)
+    # saturating products for long chains
+    used, limit = pkg.capacity_plan(_holder(pkg, [16] * 40, maxbonddim=256))
+    assert (used, limit) == (64, 256)
+
+
+def test_capacity_plan_rejects_bad_problems():
+    pkg = load_pkg()
+    h = _holder(pkg, [10] * 8, chi_cap=-1)
+    with pytest.raises(pkg.ArgumentError):
+        pkg.capacity_plan(h)
+
+
+class _StubLib:
+    """Stands in for libtci_hip.so: submits succeed, queries answer from a
+    script of statuses."""
+
+    def __init__(self, query_codes, wait_code=0):
+        self.query_codes = list(query_codes)
+        self.wait_code = wait_code
+        self.freed = []
+        self.nsub = 0
+
+    def tci_ctx_set_streams(self, h, n):
+        return 0
+
+    def tci_solve_submit(self, h, cnt, arr, out):
+        self.nsub += 1
+        out._obj.value = 1000 + self.nsub
+        return 0
+
+    def tci_batch_query(self, b):
+        return self.query_codes.pop(0) if self.query_codes else 0
+
+    def tci_batch_wait(self, b):
+        return self.wait_code
+
+    def tci_batch_result(self, b, k, r):
+        r._obj.status = 0
+        r._obj.niter = 0
+        return 0
+
+    def tci_batch_history(self, *a):
+        return 0
+
+    def tci_batch_free(self, b):
+        self.freed.append(b.value)
+        return 0
+
+    def tci_last_error(self):
+        return b"stub device failure"
+
+
+class _Ctx:
+    handle = C.c_void_p(1)
+
+
+def _serve(monkeypatch, stub, n=3):
+    pkg = load_pkg()
+    serving = __import__("tci_amd.serving", fromlist=["TCIServer"])
+    monkeypatch.setattr(pkg._abi, "lib", lambda: stub)
+    srv = serving.TCIServer(_Ctx(), streams=1)
+    reqs = [(pkg.Lorentz(1.0), [4] * 4, None, {})] * n
+    return serving, srv.run(reqs, np.zeros(n))
+
+
+def test_serving_raises_on_device_error(monkeypatch):
+    # pending, then TCI_ERR_DEVICE: the loop must stop, free the batch and raise
+    stub = _StubLib([13, 12])
+    serving = __import__("tci_amd.serving", fromlist=["ServeError"])
+    with pytest.raises(serving.ServeError) as e:
+        _serve(monkeypatch, stub)
+    assert e.value.status == 12
+    assert stub.freed == [1001]
+
+
+def test_serving_raises_when_report_fetch_fails(monkeypatch):
+    stub = _StubLib([0], wait_code=12)
+    serving = __import__("tci_amd.serving", fromlist=["ServeError"])
+    with pytest.raises(serving.ServeError):
+        _serve(monkeypatch, stub)
+    assert stub.freed == [1001]
+
+
+def test_serving_completes_with_pending_then_ok(monkeypatch):
+    stub = _StubLib([13, 13, 0])
+    _, (results, stats) = _serve(monkeypatch, stub)
+    assert stats.n == 3 and all(r.status == 0 for r in results)
+    assert stub.freed == [1001]
+
+
+def test_pi_eval_checks_functor_for_one_site():
+    # L = 1 used to skip the functor checks; a too-short parameter block must
+    # be refused before any device work (TCI_ERR_ARG), even with no GPU
+    pkg = load_pkg()
+    lib = pkg._abi.lib()
+    ld = np.array([4], np.int32)
+    f = pkg._abi.tci_functor(pkg._abi.F_LORENTZ_WEIGHTED, 1, pkg._abi.dptr(np.array([16.0])), 0, None)
+    out = np.zeros(4)
+    st = lib.tci_pi_eval_f64(C.c_void_p(1), C.byref(f), 1, pkg._abi.i32ptr(ld), None, 1, 0, None, 1, 0,
+                             pkg._abi.dptr(out), None)
+    assert st == 3
+    assert b"weighted" in lib.tci_last_error()
+    # an unknown kind as well
+    f2 = pkg._abi.tci_functor(99, 0, None, 0, None)
+    assert lib.tci_pi_eval_f64(C.c_void_p(1), C.byref(f2), 1, pkg._abi.i32ptr(ld), None, 1, 0, None, 1, 0,
+                               pkg._abi.dptr(out), None) == 3
