@@ -118,6 +118,17 @@ tci_status tci_pi_eval_f64(tci_ctx *ctx, const tci_functor *f, int32_t L,
                            int32_t nl, const uint8_t *right, int64_t nJ, int32_t nr,
                            double *out, double *maxabs_inout);
 
+/* Tensor-train evaluation ---------------------------------------------- */
+/* Replaces evaluate(tt::TensorTrain{Float64}, indexset) / tt(indexset)
+ * (abstracttensortrain.jl:124-132), batched over npts points on the device.
+ * linkdims[0..L] (linkdims[0] = linkdims[L] = 1), localdims[L]; cores: the L
+ * site tensors (linkdims[b] × localdims[b] × linkdims[b+1], column-major)
+ * back to back, host memory; pts: npts × L bytes (row-major, 1-based).
+ * out[npts].  Each product is summed in ascending bond order.  Synchronous. */
+tci_status tci_tt_evaluate_f64(tci_ctx *ctx, int32_t L, const int64_t *linkdims,
+                               const int32_t *localdims, const double *cores, int64_t npts,
+                               const uint8_t *pts, double *out);
+
 /* Whole TCI2 solves ----------------------------------------------------- */
 /* optimize! keyword arguments (tensorci2.jl:700-720) plus two device
  * capacities.  Defaults: tci_options_default(). */
@@ -137,8 +148,14 @@ typedef struct tci_options {
     int32_t chi_cap;               /* device capacity for any |Iset|/|Jset|;
                                       0 = min(maxbonddim, rank bound, 64); see
                                       tci_capacity_plan                        */
-    int32_t reserved;
+    int32_t flags;                 /* TCI_FLAG_* bits, 0 by default           */
 } tci_options;
+
+/* Diagnostic (no counterpart in the reference): end optimize! after its last
+ * iteration without the final sweep1site! (and its _sanitycheck), so the
+ * site tensors are those the last fillsitetensors! built (T_b = Pi1·P⁻¹,
+ * tensorci2.jl:367-394); lets a caller inspect them. */
+#define TCI_FLAG_NO_FINAL_SWEEP 1
 
 void tci_options_default(tci_options *o);
 
@@ -201,6 +218,16 @@ tci_status tci_batch_sitetensor(tci_batch *b, int32_t i, int32_t site, int64_t *
                                 double *out, int64_t cap);
 tci_status tci_batch_pivoterrors(tci_batch *b, int32_t i, double *out, int64_t cap);
 tci_status tci_batch_bonderrors(tci_batch *b, int32_t i, double *out);
+/* Evaluates solve i's tensor train where it lies in device memory (its
+ * TensorTrain(tci), tensortrain.jl:62-64, then evaluate at each point,
+ * abstracttensortrain.jl:124-132), and f through its device functor, at
+ * npts points (npts × L bytes, row-major, 1-based, host memory).  Any output
+ * may be NULL: tt_out[npts], f_out[npts], *maxerr = max|TT - f| and
+ * *maxabsf = max|f| over the points (NaN-propagating).  max|TT - f|/max|f|
+ * is the normalized error of the reference (tensorci2.jl:769-770, 849)
+ * measured on the points instead of the pivots.  Synchronous. */
+tci_status tci_batch_evaluate(tci_batch *b, int32_t i, int64_t npts, const uint8_t *pts,
+                              double *tt_out, double *f_out, double *maxerr, double *maxabsf);
 tci_status tci_batch_free(tci_batch *b);
 /* Device time of the batch's solve kernel (HIP events), milliseconds. */
 tci_status tci_batch_elapsed_ms(tci_batch *b, float *ms);

@@ -104,6 +104,11 @@ def lib():
         L.orc_tci_workstats.argtypes = [C.c_void_p, P(C.c_double)]
         L.orc_convergencecriterion.argtypes = [C.c_int32, P(C.c_int64), P(C.c_double), P(C.c_int64),
                                                C.c_double, C.c_int64, C.c_int32, C.c_int32]
+        L.orc_eval_points.argtypes = [C.c_void_p, C.c_int32, C.c_int64, P(C.c_int32), P(C.c_double)]
+        L.orc_tt_evaluate.argtypes = [C.c_int32, P(C.c_int64), P(C.c_double), C.c_int64, P(C.c_int32),
+                                      P(C.c_double)]
+        L.orc_globalpivotfinder.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_int32, C.c_int32,
+                                            C.c_double, C.c_uint64, P(C.c_int32)]
         L.orc_exp.restype = C.c_double
         L.orc_exp.argtypes = [C.c_double]
         L.orc_cos.restype = C.c_double
@@ -227,6 +232,25 @@ def batcheval(f: Functor, localdims, left, right):
     return out[:int(np.prod(shape))].reshape(shape, order="F")
 
 
+def eval_points(f: Functor, points):
+    """f at each row of points (npts×L, 1-based), host FP64."""
+    x = np.ascontiguousarray(np.asarray(points, np.int32))
+    out = np.zeros(max(x.shape[0], 1))
+    lib().orc_eval_points(C.byref(f.st), x.shape[1], x.shape[0], _i32(x), _dp(out))
+    return out[:x.shape[0]]
+
+
+def tt_evaluate(cores, points):
+    """evaluate(tt, x) (abstracttensortrain.jl:124-132) at each row of points."""
+    L = len(cores)
+    dims = np.ascontiguousarray(np.array([c.shape for c in cores], np.int64).ravel())
+    flat = np.ascontiguousarray(np.concatenate([np.asarray(c, np.float64).ravel(order="F") for c in cores]))
+    x = np.ascontiguousarray(np.asarray(points, np.int32))
+    out = np.zeros(max(x.shape[0], 1))
+    lib().orc_tt_evaluate(L, _ip(dims), _dp(flat), x.shape[0], _i32(x), _dp(out))
+    return out[:x.shape[0]]
+
+
 def make_opts(tolerance=1e-8, maxbonddim=INT64_MAX, maxiter=20, sweepstrategy="backandforth",
               normalizeerror=True, ncheckhistory=3, maxnglobalpivot=5, nsearchglobalpivot=5,
               tolmarginglobalsearch=10.0, strictlynested=False, checkconvglobalpivot=True, seed=0):
@@ -270,6 +294,13 @@ class TCI:
                    strictlynested=False, fillsitetensors=True):
         _check(lib().orc_sweep2site(self.h, C.byref(f.st), niter, abstol, int(maxbonddim), sweepstrategy,
                                     int(strictlynested), int(fillsitetensors)))
+
+    def globalpivotfinder(self, f, abstol, nsearch=5, maxnglobalpivot=5, tolmargin=10.0, seed=0):
+        """DefaultGlobalPivotFinder (globalpivotfinder.jl:143-195) on the current state."""
+        out = np.zeros((max(nsearch, 1), self.L), np.int32)
+        n = lib().orc_globalpivotfinder(self.h, C.byref(f.st), float(abstol), int(nsearch),
+                                        int(maxnglobalpivot), float(tolmargin), int(seed), _i32(out))
+        return [list(map(int, out[i])) for i in range(n)]
 
     def optimize(self, f, **kw):
         o = make_opts(**kw)

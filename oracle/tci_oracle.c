@@ -803,6 +803,44 @@ double orc_tci_evaluate(const orc_tci *t, const int32_t *x)
     return r;
 }
 
+void orc_eval_points(const orc_functor *f, int32_t L, int64_t npts, const int32_t *pts, double *out)
+{
+    for (int64_t q = 0; q < npts; ++q) out[q] = feval(f, pts + q * L, L);
+}
+
+void orc_tt_evaluate(int32_t L, const int64_t *dims3, const double *cores, int64_t npts,
+                     const int32_t *pts, double *out)
+{
+    int64_t chimax = 1;
+    for (int32_t b = 0; b < L; ++b) {
+        if (dims3[3 * b] > chimax) chimax = dims3[3 * b];
+        if (dims3[3 * b + 2] > chimax) chimax = dims3[3 * b + 2];
+    }
+    double *v = (double *)malloc(sizeof(double) * (size_t)chimax);
+    double *w = (double *)malloc(sizeof(double) * (size_t)chimax);
+    for (int64_t q = 0; q < npts; ++q) {
+        const int32_t *x = pts + q * L;
+        int64_t chl = dims3[0];
+        for (int64_t i = 0; i < chl; ++i) v[i] = 1.0;
+        const double *T = cores;
+        for (int32_t b = 0; b < L; ++b) {
+            const int64_t d0 = dims3[3 * b], d = dims3[3 * b + 1], d2 = dims3[3 * b + 2];
+            const int64_t s = x[b] - 1;
+            for (int64_t j = 0; j < d2; ++j) {
+                double acc = 0.0;
+                for (int64_t i = 0; i < d0; ++i) acc = acc + v[i] * T[i + d0 * (s + d * j)];
+                w[j] = acc;
+            }
+            double *tmp = v; v = w; w = tmp;
+            chl = d2;
+            T += d0 * d * d2;
+        }
+        out[q] = chl > 0 ? v[0] : 0.0;
+    }
+    free(v);
+    free(w);
+}
+
 /* the functor math routines, exported for their accuracy tests */
 double orc_exp(double x) { return tci_exp(x); }
 double orc_cos(double x) { return tci_cos(x); }
@@ -865,6 +903,15 @@ static int32_t globalpivotfinder(orc_tci *t, const orc_functor *f, double abstol
     free(pts);
     if (nf > maxng) nf = maxng;
     return nf;
+}
+
+/* the global pivot finder on the current state with its PRNG seeded by
+ * `seed` (for the brute-force checks of the selection rule) */
+int32_t orc_globalpivotfinder(orc_tci *t, const orc_functor *f, double abstol, int32_t nsearch,
+                              int32_t maxng, double margin, uint64_t seed, int32_t *found)
+{
+    orc_rng_seed(t->rng, seed);
+    return globalpivotfinder(t, f, abstol, nsearch, maxng, margin, found);
 }
 
 /* convergencecriterion (tensorci2.jl:609-628) */

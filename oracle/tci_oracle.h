@@ -129,8 +129,20 @@ void orc_tci_pivoterrors(const orc_tci *t, double *out);
 void orc_tci_bonderrors(const orc_tci *t, double *out);
 int32_t orc_tci_niter(const orc_tci *t);
 void orc_tci_history(const orc_tci *t, int64_t *ranks, double *errors, int64_t *nglobal);
+/* DefaultGlobalPivotFinder (globalpivotfinder.jl:143-195) on the current
+ * site tensors, PRNG seeded by `seed`; found: nsearch × L; returns count */
+int32_t orc_globalpivotfinder(orc_tci *t, const orc_functor *f, double abstol, int32_t nsearch,
+                              int32_t maxng, double margin, uint64_t seed, int32_t *found);
 /* TT evaluation of the current site tensors (abstracttensortrain.jl:124-132) */
 double orc_tci_evaluate(const orc_tci *t, const int32_t *x);
+/* f at npts points (npts×L 1-based, row-major), host FP64 */
+void orc_eval_points(const orc_functor *f, int32_t L, int64_t npts, const int32_t *pts, double *out);
+/* evaluate(tt, x) (abstracttensortrain.jl:124-132) of a TT given by its
+ * cores (site b: dims3[3b]×dims3[3b+1]×dims3[3b+2], column-major, stored
+ * back to back) at npts points: the row vector times each slice, each
+ * product summed in ascending bond order */
+void orc_tt_evaluate(int32_t L, const int64_t *dims3, const double *cores, int64_t npts,
+                     const int32_t *pts, double *out);
 /* algorithmic work counters: [0] Pi elements evaluated, [1] rrLU argmax bytes,
  * [2] fused rrLU bytes, [3] Schur flops, [4] number of rrLU, [5] number of evals */
 void orc_tci_workstats(const orc_tci *t, double *out6);

@@ -28,7 +28,7 @@ __all__ = [
     "left", "right", "rowindices", "colindices", "npivots", "pivoterrors", "lastpivoterror",
     "DeviceFunctor", "Lorentz", "WeightedLorentz", "QuanticsGaussCos", "GaussSum", "SumFunctor",
     "TableFunctor", "TTFunctor", "QuanticsExp", "batchevaluate", "TensorCI2", "crossinterpolate2",
-    "crossinterpolate2_batch", "TCI_UNLIMITED",
+    "crossinterpolate2_batch", "TCI_UNLIMITED", "capacity_plan", "tt_evaluate", "sample_points",
 ]
 
 
@@ -340,6 +340,41 @@ def batchevaluate(f: DeviceFunctor, localdims, leftindexset, rightindexset, M, c
 
 
 # --------------------------------------------------------------------------
+# tensor-train evaluation (abstracttensortrain.jl:124-132)
+# --------------------------------------------------------------------------
+def _points(points, L):
+    x = np.asarray(points)
+    if x.ndim == 1:
+        x = x.reshape(1, -1)
+    if x.shape[1] != L:
+        raise ArgumentError(3, f"To evaluate a tt of length {L}, you have to provide {L} indices, "
+                               f"but there were {x.shape[1]}.")
+    return np.ascontiguousarray(x, np.uint8) if x.size else np.zeros((0, L), np.uint8)
+
+
+def tt_evaluate(sitetensors, points, ctx=None):
+    """evaluate(tt, indexset) (abstracttensortrain.jl:124-132) at every row of
+    `points` (1-based, shape npts×L), on the device.  Returns npts values."""
+    ctx = ctx or default_context()
+    L = len(sitetensors)
+    x = _points(points, L)
+    links = np.array([sitetensors[0].shape[0]] + [t.shape[2] for t in sitetensors], np.int64)
+    ld = np.array([t.shape[1] for t in sitetensors], np.int32)
+    cores = np.ascontiguousarray(np.concatenate([np.asarray(t, np.float64).ravel(order="F")
+                                                 for t in sitetensors]))
+    out = np.zeros(max(x.shape[0], 1))
+    _check(_abi.lib().tci_tt_evaluate_f64(ctx.handle, L, _abi.i64ptr(links), _abi.i32ptr(ld),
+                                          _abi.dptr(cores), x.shape[0], _abi.u8ptr(x), _abi.dptr(out)))
+    return out[:x.shape[0]]
+
+
+def sample_points(localdims, n, seed=0):
+    """n seeded random points of the domain (1-based, n×L bytes)."""
+    rng = np.random.default_rng(seed)
+    return np.stack([rng.integers(1, int(d) + 1, n) for d in localdims], 1).astype(np.uint8)
+
+
+# --------------------------------------------------------------------------
 # TensorCI2 / crossinterpolate2 (tensorci2.jl)
 # --------------------------------------------------------------------------
 class TensorCI2:
@@ -405,6 +440,14 @@ class TensorCI2:
         """max(bonderrors) (tensorci2.jl:157-159)."""
         return float(np.max(self.bonderrors))
 
+    def evaluate(self, points, ctx=None):
+        """TensorTrain(tci) evaluated at each row of points (tensortrain.jl:62-64,
+        abstracttensortrain.jl:124-132), on the device."""
+        return tt_evaluate(self.sitetensors, points, ctx)
+
+    def __call__(self, indexset):
+        return float(self.evaluate(np.asarray(indexset).reshape(1, -1))[0])
+
 
 def _resolve_tolerance(tolerance, pivottolerance):
     """optimize! tolerance/pivottolerance resolution (tensorci2.jl:734-746)."""
@@ -440,7 +483,7 @@ def capacity_plan(holder):
     return used.value, limit.value
 
 
-def crossinterpolate2_batch(problems, ctx=None, return_batch=False):
+def crossinterpolate2_batch(problems, ctx=None, return_batch=False, error_sample=None):
     """Submit many crossinterpolate2 problems as ONE launch.
 
     problems: list of (f, localdims, initialpivots_or_None, kwargs).
@@ -452,7 +495,12 @@ def crossinterpolate2_batch(problems, ctx=None, return_batch=False):
     its capacity is below min(maxbonddim, rank bound) is resubmitted (with
     the other such solves, as one launch) at twice the capacity, so the
     default options succeed wherever the reference does.  A caller that
-    sets chi_cap explicitly gets the CAPACITY status instead."""
+    sets chi_cap explicitly gets the CAPACITY status instead.
+
+    error_sample=(npts, seed): every successful solve's TT and f are
+    evaluated on the device at npts seeded random points (sample_points)
+    while the solve is still resident; t.sample_error holds
+    (max|TT - f|, max|f|) over them."""
     ctx = ctx or default_context()
     lib = _abi.lib()
     problems = list(problems)
@@ -484,6 +532,13 @@ def crossinterpolate2_batch(problems, ctx=None, return_batch=False):
                         if chi[i] > used:
                             retry.append(i)
                             continue
+                if error_sample and t.status == 0:
+                    npts, seed = error_sample
+                    x = sample_points(problems[i][1], int(npts), seed)
+                    me, mf = C.c_double(), C.c_double()
+                    _check(lib.tci_batch_evaluate(b, k, x.shape[0], _abi.u8ptr(x), None, None,
+                                                  C.byref(me), C.byref(mf)))
+                    t.sample_error = (me.value, mf.value)
                 out[i] = t
             ms = C.c_float()
             lib.tci_batch_elapsed_ms(b, C.byref(ms))

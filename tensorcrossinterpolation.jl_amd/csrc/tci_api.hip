@@ -753,6 +753,147 @@ tci_status tci_batch_profile(tci_batch *b, int32_t i, double *us8)
     return TCI_OK;
 }
 
+// every point coordinate in 1..d_b (the kernel indexes the cores with it)
+static bool points_in_range(int L, const int32_t *localdims, int64_t npts, const uint8_t *pts)
+{
+    for (int64_t q = 0; q < npts; ++q)
+        for (int b = 0; b < L; ++b) {
+            const int v = pts[q * L + b];
+            if (v < 1 || v > localdims[b]) return false;
+        }
+    return true;
+}
+
+// device part shared by the two TT evaluation entry points: points upload,
+// launch, results back; dims3/coff/cores/params are device pointers
+static tci_status tt_eval_device(tci_ctx *ctx, hipStream_t st, int L, const int64_t *d_dims3,
+                                 const int64_t *d_coff, const double *d_cores, int chimax, int64_t npts,
+                                 const uint8_t *pts, int fkind, const double *d_params,
+                                 const int32_t *d_iparams, double *tt_out, double *f_out, double *maxerr,
+                                 double *maxabsf)
+{
+    const int np = tt_eval_points_per_block(chimax);
+    const int64_t blocks = (npts + np - 1) / np;
+    size_t need = 0;
+    auto a = [&](size_t bytes) { size_t at = need; need = lay_align(need + (bytes ? bytes : 1)); return at; };
+    const size_t o_pts = a((size_t)(npts * L));
+    const size_t o_tt = a(sizeof(double) * (size_t)npts);
+    const size_t o_f = a(sizeof(double) * (size_t)npts);
+    const size_t o_bm = a(sizeof(double) * (size_t)(2 * blocks));
+    uint8_t *d = nullptr;
+    HIPCHK(hipMallocAsync((void **)&d, need, st));
+    hipError_t e = hipMemcpyAsync(d + o_pts, pts, (size_t)(npts * L), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = launch_tt_eval(L, d_dims3, d_coff, d_cores, d + o_pts, npts, chimax, fkind, d_params, d_iparams,
+                           (double *)(d + o_tt), fkind ? (double *)(d + o_f) : nullptr,
+                           (double *)(d + o_bm), st);
+    std::vector<double> bm;
+    if (e == hipSuccess && tt_out)
+        e = hipMemcpyAsync(tt_out, d + o_tt, sizeof(double) * (size_t)npts, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && fkind && f_out)
+        e = hipMemcpyAsync(f_out, d + o_f, sizeof(double) * (size_t)npts, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && fkind) {
+        bm.resize((size_t)(2 * blocks));
+        e = hipMemcpyAsync(bm.data(), d + o_bm, sizeof(double) * bm.size(), hipMemcpyDeviceToHost, st);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFreeAsync(d, st);
+    if (e != hipSuccess) return set_err(TCI_ERR_DEVICE, "TT evaluation: %s", hipGetErrorString(e));
+    if (fkind) {
+        double me = 0.0, mf = 0.0;
+        for (int64_t q = 0; q < blocks; ++q) {
+            me = nan_max(me, bm[(size_t)(2 * q)]);
+            mf = nan_max(mf, bm[(size_t)(2 * q + 1)]);
+        }
+        if (maxerr) *maxerr = me;
+        if (maxabsf) *maxabsf = mf;
+    }
+    return TCI_OK;
+}
+
+tci_status tci_tt_evaluate_f64(tci_ctx *ctx, int32_t L, const int64_t *linkdims, const int32_t *localdims,
+                               const double *cores, int64_t npts, const uint8_t *pts, double *out)
+{
+    if (!ctx || L < 1 || !linkdims || !localdims || npts < 0 || (npts > 0 && (!pts || !out)))
+        return set_err(TCI_ERR_ARG, "bad arguments");
+    if (linkdims[0] != 1 || linkdims[L] != 1)
+        return set_err(TCI_ERR_ARG, "a tensor train has bond dimension 1 at both ends");
+    int64_t tot = 0, chimax = 1;
+    std::vector<int64_t> dims3((size_t)(3 * L)), coff((size_t)L);
+    for (int b = 0; b < L; ++b) {
+        if (localdims[b] < 1 || localdims[b] > 255) return set_err(TCI_ERR_ARG, "local dimensions must be in 1..255");
+        if (linkdims[b + 1] < 1 || linkdims[b + 1] > 8192) return set_err(TCI_ERR_ARG, "bond dimensions must be in 1..8192");
+        dims3[(size_t)(3 * b)] = linkdims[b];
+        dims3[(size_t)(3 * b + 1)] = localdims[b];
+        dims3[(size_t)(3 * b + 2)] = linkdims[b + 1];
+        coff[(size_t)b] = tot;
+        tot += linkdims[b] * localdims[b] * linkdims[b + 1];
+        chimax = linkdims[b + 1] > chimax ? linkdims[b + 1] : chimax;
+    }
+    if (npts == 0) return TCI_OK;
+    if (!cores) return set_err(TCI_ERR_ARG, "cores is NULL");
+    if (!points_in_range(L, localdims, npts, pts)) return set_err(TCI_ERR_ARG, "point index out of range");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    uint8_t *d = nullptr;
+    size_t need = 0;
+    auto a = [&](size_t bytes) { size_t at = need; need = lay_align(need + (bytes ? bytes : 1)); return at; };
+    const size_t o_c = a(sizeof(double) * (size_t)tot);
+    const size_t o_d = a(sizeof(int64_t) * dims3.size());
+    const size_t o_o = a(sizeof(int64_t) * coff.size());
+    HIPCHK(hipMallocAsync((void **)&d, need, st));
+    hipError_t e = hipMemcpyAsync(d + o_c, cores, sizeof(double) * (size_t)tot, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d + o_d, dims3.data(), sizeof(int64_t) * dims3.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d + o_o, coff.data(), sizeof(int64_t) * coff.size(), hipMemcpyHostToDevice, st);
+    tci_status s = TCI_OK;
+    if (e != hipSuccess) s = set_err(TCI_ERR_DEVICE, "TT upload: %s", hipGetErrorString(e));
+    else s = tt_eval_device(ctx, st, L, (const int64_t *)(d + o_d), (const int64_t *)(d + o_o),
+                            (const double *)(d + o_c), (int)chimax, npts, pts, 0, nullptr, nullptr, out,
+                            nullptr, nullptr, nullptr);
+    (void)hipFreeAsync(d, st);
+    (void)hipStreamSynchronize(st);
+    return s;
+}
+
+tci_status tci_batch_evaluate(tci_batch *b, int32_t i, int64_t npts, const uint8_t *pts, double *tt_out,
+                              double *f_out, double *maxerr, double *maxabsf)
+{
+    SolveDesc R;
+    if (!report_view(b, i, R) || npts < 0 || (npts > 0 && !pts)) return set_err(TCI_ERR_ARG, "bad arguments");
+    if (R.ints[INT_STATUS] != TCI_OK || !R.ints[INT_DONE])
+        return set_err(TCI_ERR_ARG, "solve %d did not finish successfully", i);
+    if (npts == 0) {
+        if (maxerr) *maxerr = 0.0;
+        if (maxabsf) *maxabsf = 0.0;
+        return TCI_OK;
+    }
+    tci_ctx *ctx = b->ctx;
+    HIPCHK(hipSetDevice(ctx->device));
+    const int L = R.L;
+    std::vector<int32_t> ld((size_t)L);
+    std::vector<int64_t> dims3((size_t)(3 * L)), coff((size_t)L);
+    HIPCHK(hipMemcpy(ld.data(), R.localdims, sizeof(int32_t) * (size_t)L, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(dims3.data(), R.Tdims, sizeof(int64_t) * dims3.size(), hipMemcpyDeviceToHost));
+    if (!points_in_range(L, ld.data(), npts, pts)) return set_err(TCI_ERR_ARG, "point index out of range");
+    int64_t chimax = 1;
+    for (int s = 0; s < L; ++s) {
+        coff[(size_t)s] = R.tstride * s;
+        chimax = dims3[(size_t)(3 * s + 2)] > chimax ? dims3[(size_t)(3 * s + 2)] : chimax;
+        if (dims3[(size_t)(3 * s + 1)] != ld[(size_t)s]) return set_err(TCI_ERR_GENERIC, "site tensor %d is missing", s + 1);
+    }
+    hipStream_t st = ctx->stream;
+    int64_t *d_off = nullptr;
+    HIPCHK(hipMallocAsync((void **)&d_off, sizeof(int64_t) * coff.size(), st));
+    hipError_t e = hipMemcpyAsync(d_off, coff.data(), sizeof(int64_t) * coff.size(), hipMemcpyHostToDevice, st);
+    tci_status s = TCI_OK;
+    if (e != hipSuccess) s = set_err(TCI_ERR_DEVICE, "offsets upload: %s", hipGetErrorString(e));
+    else s = tt_eval_device(ctx, st, L, R.Tdims, d_off, R.T, (int)chimax, npts, pts, R.fkind, R.params,
+                            R.iparams, tt_out, f_out, maxerr, maxabsf);
+    (void)hipFreeAsync(d_off, st);
+    (void)hipStreamSynchronize(st);
+    return s;
+}
+
 tci_status tci_batch_free(tci_batch *b)
 {
     if (!b) return TCI_OK;

@@ -38,6 +38,9 @@ enum {
     NSCAL = 16
 };
 
+// bits of SolveDesc::optflags (the values of tci.h's TCI_FLAG_*)
+enum { OPT_NO_FINAL_SWEEP = 1 };
+
 struct SolveDesc {
     // ---- problem ----
     int32_t L;
@@ -56,6 +59,7 @@ struct SolveDesc {
     int32_t dmax;
     int32_t sumd;            // Σ_b d_b
     int32_t maxiter_cap;     // capacity of ranks/errors/nglob
+    int32_t optflags;        // TCI_FLAG_* of tci_options
     double tolerance;
     double tolmargin;
     int64_t maxbonddim;

@@ -1134,7 +1134,11 @@ struct Engine {
             }
         }
         const uint64_t t0 = tm.clock();
-        int st = fillsitetensors(D.nsearchglobalpivot > 0);
+        // optimize! always calls sweep2site! with fillsitetensors=true
+        // (tensorci2.jl:777-787), so its errors (a singular or non-square
+        // pivot matrix) are raised even when no global search reads the
+        // tensors
+        int st = fillsitetensors(true);
         if (tm.tid() == 0) S.tph[PH_FILL] += tm.clock() - t0;
         return st;
     }
@@ -1521,16 +1525,18 @@ struct Engine {
             if (conv) break;
         }
         const double norm = D.normalizeerror ? S.maxsample : 1.0;
-        const uint64_t t1 = tm.clock();
-        st = sweep1site_forward(tol * norm);
-        if (tm.tid() == 0) S.tph[PH_SWEEP1] += tm.clock() - t1;
-        if (st != ST_OK) return finish(st);
-        // _sanitycheck (globalsearch.jl:106-111)
-        for (int b = 0; b < L - 1; ++b)
-            if (setn(SET_I, b + 1) != setn(SET_J, b)) {
-                if (tm.tid() == 0) D.ints[INT_ERRSITE] = b + 1;
-                return finish(ST_NONSQUARE);
-            }
+        if (!(D.optflags & OPT_NO_FINAL_SWEEP)) {
+            const uint64_t t1 = tm.clock();
+            st = sweep1site_forward(tol * norm);
+            if (tm.tid() == 0) S.tph[PH_SWEEP1] += tm.clock() - t1;
+            if (st != ST_OK) return finish(st);
+            // _sanitycheck (globalsearch.jl:106-111)
+            for (int b = 0; b < L - 1; ++b)
+                if (setn(SET_I, b + 1) != setn(SET_J, b)) {
+                    if (tm.tid() == 0) D.ints[INT_ERRSITE] = b + 1;
+                    return finish(ST_NONSQUARE);
+                }
+        }
         if (tm.tid() == 0)
             for (int i = 0; i < niter; ++i) D.errors[i] = D.errors[i] / norm;
         return finish(ST_OK);

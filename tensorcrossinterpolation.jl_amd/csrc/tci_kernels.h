@@ -29,6 +29,13 @@ hipError_t launch_lu_unpack(const double *A, int64_t m, int64_t n, int64_t p, in
 hipError_t launch_kron_center(const uint8_t *left, int64_t nI, int nl, const int32_t *cdims, int M,
                               int64_t nc, uint8_t *rowsx, int stride, hipStream_t st);
 int eval_blocks(int64_t tot);
+// TT evaluation at npts points (and f, |TT - f| and |f| maxima when fkind != 0);
+// blockmax holds 2 doubles per block of tt_eval_points_per_block(chimax) points
+int tt_eval_points_per_block(int chimax);
+hipError_t launch_tt_eval(int L, const int64_t *dims3, const int64_t *coff, const double *cores,
+                          const uint8_t *pts, int64_t npts, int chimax, int fkind, const double *params,
+                          const int32_t *iparams, double *out_tt, double *out_f, double *blockmax,
+                          hipStream_t st);
 hipError_t launch_eval(int kind, const double *params, const int32_t *iparams, int L,
                        const uint8_t *rows, int64_t nrow, int rstride, int nl, const uint8_t *cols,
                        int64_t ncol, int cstride, double *out, double *blockmax, hipStream_t st);

@@ -173,7 +173,107 @@ __global__ void tci_eval_kernel(int kind, const double *__restrict__ params,
     }
 }
 
+// --------------------------------------------------------- TT evaluation
+// evaluate(tt, x) (abstracttensortrain.jl:124-132) at npts points: for each
+// point the row vector v (χ_0 = 1 entries of 1) times each site's slice
+// T_b[:, x_b, :], every product summed in ascending bond order from 0.0.
+// A workgroup takes np points; their running vectors live in LDS (two
+// buffers of np·chimax doubles), and thread t computes entries (p, j) of
+// the next vector.  Site b's core starts at cores + coff[b]; its shape is
+// dims3[3b..3b+2].  With fkind != 0, f is evaluated at the same points and
+// the workgroup writes the NaN-propagating maxima of |TT - f| and |f| to
+// blockmax[2·block], [2·block + 1].
+__global__ __launch_bounds__(256) void tci_tt_eval_kernel(
+    int L, const int64_t *__restrict__ dims3, const int64_t *__restrict__ coff,
+    const double *__restrict__ cores, const uint8_t *__restrict__ pts, int64_t npts, int np,
+    int chimax, int fkind, const double *__restrict__ params, const int32_t *__restrict__ iparams,
+    double *__restrict__ out_tt, double *__restrict__ out_f, double *__restrict__ blockmax)
+{
+    extern __shared__ double vbuf[];
+    __shared__ double red[8];
+    const int64_t p0 = (int64_t)blockIdx.x * np;
+    const int cn = (int)(npts - p0 < np ? npts - p0 : np);
+    double *V = vbuf, *W = vbuf + (int64_t)np * chimax;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int q = tid; q < cn; q += nt) V[(int64_t)q * chimax] = 1.0;
+    __syncthreads();
+    for (int b = 0; b < L; ++b) {
+        const int64_t d0 = dims3[3 * b], d = dims3[3 * b + 1], d2 = dims3[3 * b + 2];
+        const double *T = cores + coff[b];
+        for (int64_t o = tid; o < (int64_t)cn * d2; o += nt) {
+            const int64_t q = o / d2, j = o - q * d2;
+            const int64_t s = pts[(p0 + q) * L + b] - 1;
+            const double *Tc = T + d0 * (s + d * j);
+            const double *v = V + q * chimax;
+            double acc = 0.0;
+            for (int64_t i = 0; i < d0; ++i) acc = acc + v[i] * Tc[i];
+            W[q * chimax + j] = acc;
+        }
+        __syncthreads();
+        double *t = V; V = W; W = t;
+    }
+    double le = 0.0, lf = 0.0;
+    for (int q = tid; q < cn; q += nt) {
+        const double tt = V[(int64_t)q * chimax];
+        out_tt[p0 + q] = tt;
+        if (fkind) {
+            tci_point x;
+            x.a = pts + (p0 + q) * L;
+            x.b = x.a;
+            x.na = L;
+            x.L = L;
+            const double fv = tci_functor_eval(fkind, params, iparams, &x);
+            if (out_f) out_f[p0 + q] = fv;
+            le = jl_max(le, fabs(tt - fv));
+            lf = jl_max(lf, fabs(fv));
+        }
+    }
+    if (!fkind) return;
+    for (int off = 32; off >= 1; off >>= 1) {
+        le = jl_max(le, __shfl_xor(le, off));
+        lf = jl_max(lf, __shfl_xor(lf, off));
+    }
+    if ((tid & 63) == 0) {
+        red[2 * (tid >> 6)] = le;
+        red[2 * (tid >> 6) + 1] = lf;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double e = red[0], f = red[1];
+        for (int w = 1; w < nt / 64; ++w) {
+            e = jl_max(e, red[2 * w]);
+            f = jl_max(f, red[2 * w + 1]);
+        }
+        blockmax[2 * blockIdx.x] = e;
+        blockmax[2 * blockIdx.x + 1] = f;
+    }
+}
+
 // ------------------------------------------------------------- launchers
+int tt_eval_points_per_block(int chimax)
+{
+    // two running-vector buffers of np·chimax doubles in at most 64 KiB
+    int np = (int)((64 * 1024) / (16 * (int64_t)(chimax > 0 ? chimax : 1)));
+    if (np > 256) np = 256;
+    return np < 1 ? 1 : np;
+}
+
+hipError_t launch_tt_eval(int L, const int64_t *dims3, const int64_t *coff, const double *cores,
+                          const uint8_t *pts, int64_t npts, int chimax, int fkind, const double *params,
+                          const int32_t *iparams, double *out_tt, double *out_f, double *blockmax,
+                          hipStream_t st)
+{
+    if (npts <= 0) return hipSuccess;
+    const int np = tt_eval_points_per_block(chimax);
+    const int64_t blocks = (npts + np - 1) / np;
+    if (blocks > INT32_MAX) return hipErrorInvalidValue;
+    const size_t lds = (size_t)2 * np * chimax * sizeof(double);
+    if (lds > 64 * 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(tci_tt_eval_kernel, dim3((unsigned)blocks), dim3(256), lds, st, L, dims3, coff, cores,
+                       pts, npts, np, chimax, fkind, params, iparams, out_tt, out_f, blockmax);
+    return hipGetLastError();
+}
+
 size_t lds_plan_bytes(const LdsPlan &p)
 {
     return (size_t)2 * p.stage_cap * sizeof(double) + 2 * (size_t)p.tab_cap + (size_t)p.sets_cap +

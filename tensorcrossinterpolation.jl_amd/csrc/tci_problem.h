@@ -164,6 +164,8 @@ inline int check_problem(const tci_problem &p, SolveCaps &c, std::string *msg)
     return TCI_OK;
 }
 
+static_assert(OPT_NO_FINAL_SWEEP == TCI_FLAG_NO_FINAL_SWEEP, "option bits differ");
+
 // Copy the problem-owned fields into a descriptor (pointers excluded).
 inline void fill_desc_scalars(const tci_problem &p, SolveDesc &D)
 {
@@ -181,6 +183,7 @@ inline void fill_desc_scalars(const tci_problem &p, SolveDesc &D)
     D.tolmargin = o.tolmarginglobalsearch;
     D.maxbonddim = o.maxbonddim;
     D.seed = o.seed;
+    D.optflags = o.flags;
 }
 
 // Pack the inputs (localdims, params, iparams, initial pivots) into a host

@@ -174,7 +174,7 @@ struct COptions
     checkconvglobalpivot::Int32
     seed::UInt64
     chi_cap::Int32
-    reserved::Int32
+    flags::Int32
 end
 
 struct CProblem

@@ -16,7 +16,8 @@ is not available, nothing of the reference is executed):
    solved by the oracle: ranks, errors, pivot sets, maxsamplevalue.
 4. configs.json (`make_goldens.py --configs`, needs no reference checkout) —
    the full-size solve configurations of tests/problems.py solved by the
-   oracle: ranks, link dims, errors and a digest of the pivot sets.
+   oracle: ranks, link dims, errors and a digest of the pivot sets; and
+   config_errors.json, their max|TT - f| on 10^5 seeded random points.
 """
 import json
 import os
@@ -158,10 +159,35 @@ def configs():
     return res
 
 
+def config_errors(npts=100_000, seed=12345):
+    """max|TT - f| over npts seeded random points (numpy Generator(seed), one
+    integers(1, d+1, npts) draw per site), normalized by max|f| over the
+    points and by the solve's maxsamplevalue (the reference's normalization,
+    tensorci2.jl:769-770); f in host FP64."""
+    import oracle as O
+    sys.path.insert(0, os.path.dirname(HERE))
+    from problems import config_specs, oracle_opts
+    res = {}
+    for name, kind, params, ip, ld, piv, opts in config_specs():
+        f = O.Functor(kind, params, ip)
+        t, r, e = O.crossinterpolate2(f, ld, piv, **oracle_opts(opts))
+        rng = np.random.default_rng(seed)
+        x = np.stack([rng.integers(1, int(d) + 1, npts) for d in ld], 1).astype(np.int32)
+        tt = O.tt_evaluate(t.sitetensors(), x)
+        fv = O.eval_points(f, x)
+        me, mf = float(np.max(np.abs(tt - fv))), float(np.max(np.abs(fv)))
+        res[name] = dict(npts=npts, seed=seed, max_abs_err=me, max_abs_f=mf, maxsamplevalue=t.maxsamplevalue,
+                         err_maxf=me / mf, err_maxsample=me / t.maxsamplevalue, tolerance=opts["tolerance"],
+                         meets_tol=bool(me / t.maxsamplevalue <= opts["tolerance"]))
+    return res
+
+
 def main():
     if sys.argv[1:] == ["--configs"]:
         with open(os.path.join(HERE, "configs.json"), "w") as fh:
             json.dump(configs(), fh, indent=1)
+        with open(os.path.join(HERE, "config_errors.json"), "w") as fh:
+            json.dump(config_errors(), fh, indent=1)
         print("wrote configs.json")
         return
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
@@ -174,6 +200,8 @@ def main():
         json.dump(tci_c1(), fh)
     with open(os.path.join(HERE, "configs.json"), "w") as fh:
         json.dump(configs(), fh, indent=1)
+    with open(os.path.join(HERE, "config_errors.json"), "w") as fh:
+        json.dump(config_errors(), fh, indent=1)
     print("wrote", len(fx), "reference matrices")
 
 
