@@ -97,6 +97,15 @@ typedef struct tci_lu_job {
  * array; the call is asynchronous on the context stream. */
 tci_status tci_rrlu_batched_f64(tci_ctx *ctx, tci_lu_job *jobs_device, int64_t count);
 
+/* Which kernel the context's standalone rrLU calls (tci_rrlu_f64,
+ * tci_rrlu_batched_f64) use; no counterpart in the reference, and the results
+ * are the same bits either way.  AUTO (the default): the register-tile
+ * factorization for matrices up to 128×128, the streaming one otherwise;
+ * STREAM: always streaming; TILE: always the register tile (jobs larger than
+ * 128×128 are refused with TCI_ERR_ARG). */
+enum { TCI_RRLU_AUTO = 0, TCI_RRLU_STREAM = 1, TCI_RRLU_TILE = 2 };
+tci_status tci_ctx_set_rrlu_algo(tci_ctx *ctx, int32_t algo);
+
 /* Pi-tensor evaluation ------------------------------------------------- */
 /* Replaces _batchevaluate_dispatch(Float64, f, localdims, Iset, Jset, Val(M))
  * (batcheval.jl:32-61) for a device functor f (include/tci_functors.h).

@@ -66,6 +66,7 @@ struct tci_ctx {
     int lds_wg;         // LDS bytes (static + dynamic) a solve workgroup may use
     int team;           // threads per solve workgroup
     int pi_lds_kb;      // LDS budget for Pi matrices (KiB)
+    int rrlu_algo = TCI_RRLU_AUTO;   // factorization kernel of the standalone rrLU
 };
 
 // Launch shape of the solve kernel: a team of `team` threads per solve; LDS
@@ -229,6 +230,13 @@ tci_status tci_ctx_stream(tci_ctx *ctx, void **stream)
     return TCI_OK;
 }
 
+tci_status tci_ctx_set_rrlu_algo(tci_ctx *ctx, int32_t algo)
+{
+    if (!ctx || algo < TCI_RRLU_AUTO || algo > TCI_RRLU_TILE) return set_err(TCI_ERR_ARG, "bad arguments");
+    ctx->rrlu_algo = algo;
+    return TCI_OK;
+}
+
 tci_status tci_ctx_set_streams(tci_ctx *ctx, int32_t n)
 {
     if (!ctx || n < 1 || n > 64) return set_err(TCI_ERR_ARG, "need 1 <= n <= 64 streams");
@@ -288,6 +296,49 @@ tci_status tci_rrlu_batched_f64(tci_ctx *ctx, tci_lu_job *jobs, int64_t count)
         int64_t s = j.m > j.n ? j.m : j.n;
         mx = s > mx ? s : mx;
     }
+    // jobs for the register tile (they fit 128×128 and the algorithm allows
+    // it) and for the streaming factorization; results do not depend on it
+    std::vector<int32_t> tile, stream;
+    std::vector<int64_t> soff;
+    int64_t sc_tot = 0;
+    for (int64_t q = 0; q < count; ++q) {
+        const tci_lu_job &j = h[(size_t)q];
+        const bool fits = j.m >= 1 && j.n >= 1 && j.m <= kTileSide && j.n <= kTileSide;
+        if (ctx->rrlu_algo == TCI_RRLU_TILE && !fits && j.m > 0 && j.n > 0)
+            return set_err(TCI_ERR_ARG, "job %lld (%lld x %lld) does not fit the 128 x 128 register tile",
+                           (long long)q, (long long)j.m, (long long)j.n);
+        if (fits && ctx->rrlu_algo != TCI_RRLU_STREAM) {
+            int64_t mr = j.maxrank < j.m ? j.maxrank : j.m;
+            if (j.n < mr) mr = j.n;
+            tile.push_back((int32_t)q);
+            soff.push_back(sc_tot);
+            sc_tot += mr * (j.m + j.n);
+        } else {
+            stream.push_back((int32_t)q);
+        }
+    }
+    if (!tile.empty()) {
+        uint8_t *d = nullptr;
+        const size_t b_idx = lay_align(sizeof(int32_t) * tile.size());
+        const size_t b_off = lay_align(sizeof(int64_t) * soff.size());
+        HIPCHK(hipMallocAsync((void **)&d, b_idx + b_off + sizeof(double) * (size_t)(sc_tot + 1), ctx->stream));
+        hipError_t e = hipMemcpyAsync(d, tile.data(), sizeof(int32_t) * tile.size(), hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d + b_idx, soff.data(), sizeof(int64_t) * soff.size(), hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess)
+            e = launch_rrlu_tile(jobs, (const int32_t *)d, (int64_t)tile.size(), (double *)(d + b_idx + b_off),
+                                 (const int64_t *)(d + b_idx), ctx->stream);
+        // the host vectors must outlive the copies
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        (void)hipFreeAsync(d, ctx->stream);
+        if (e != hipSuccess) return set_err(TCI_ERR_DEVICE, "rrlu tile launch: %s", hipGetErrorString(e));
+    }
+    if (stream.empty()) return TCI_OK;
+    mx = 0;
+    for (int32_t q : stream) {
+        const int64_t s = h[(size_t)q].m > h[(size_t)q].n ? h[(size_t)q].m : h[(size_t)q].n;
+        mx = s > mx ? s : mx;
+    }
     double *gstage = nullptr;
     const int64_t lds_cap = 48 * 1024 / 16;
     if (mx > lds_cap) HIPCHK(hipMallocAsync((void **)&gstage, sizeof(double) * 2 * mx * count, ctx->stream));
@@ -297,16 +348,17 @@ tci_status tci_rrlu_batched_f64(tci_ctx *ctx, tci_lu_job *jobs, int64_t count)
     // (the host copy lives in the context: the next call on this stream
     // synchronizes before it reuses it)
     std::vector<int32_t> &order = ctx->lu_order;
-    order.resize((size_t)count);
-    for (int64_t q = 0; q < count; ++q) order[(size_t)q] = (int32_t)q;
+    order = stream;
+    const int64_t ns = (int64_t)order.size();
     std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
         return h[(size_t)a].m * h[(size_t)a].n > h[(size_t)b].m * h[(size_t)b].n;
     });
     int32_t *dorder = nullptr;
-    HIPCHK(hipMallocAsync((void **)&dorder, sizeof(int32_t) * (size_t)count, ctx->stream));
-    HIPCHK(hipMemcpyAsync(dorder, order.data(), sizeof(int32_t) * (size_t)count, hipMemcpyHostToDevice,
+    HIPCHK(hipMallocAsync((void **)&dorder, sizeof(int32_t) * (size_t)ns, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dorder, order.data(), sizeof(int32_t) * (size_t)ns, hipMemcpyHostToDevice,
                           ctx->stream));
-    hipError_t e = launch_rrlu(jobs, count, mx, gstage, 2 * mx, dorder, ctx->stream);
+    // (staging is indexed by job number, so it covers all `count` jobs)
+    hipError_t e = launch_rrlu(jobs, ns, mx, gstage, 2 * mx, dorder, ctx->stream);
     (void)hipFreeAsync(dorder, ctx->stream);
     if (gstage) (void)hipFreeAsync(gstage, ctx->stream);
     if (e != hipSuccess) return set_err(TCI_ERR_DEVICE, "rrlu launch: %s", hipGetErrorString(e));

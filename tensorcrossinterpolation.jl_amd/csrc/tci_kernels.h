@@ -24,6 +24,11 @@ hipError_t launch_solve(const SolveDesc *descs_device, int count, int team, int 
 // order (device, count entries, or null): the job index each workgroup takes
 hipError_t launch_rrlu(tci_lu_job *jobs_device, int64_t count, int64_t max_side, double *gstage,
                        int64_t gstage_stride, const int32_t *order, hipStream_t st);
+// the register-tile rrLU of the jobs idx[0..count) (each m, n <= 128);
+// scratch + soff[q]: m·mr + mr·n doubles for job idx[q] (mr = min(m, n, maxrank))
+constexpr int kTileSide = 128;
+hipError_t launch_rrlu_tile(tci_lu_job *jobs_device, const int32_t *idx, int64_t count, double *scratch,
+                            const int64_t *soff, hipStream_t st);
 hipError_t launch_lu_unpack(const double *A, int64_t m, int64_t n, int64_t p, int leftorth,
                             double *L, int64_t ldl, double *U, int64_t ldu, hipStream_t st);
 hipError_t launch_kron_center(const uint8_t *left, int64_t nI, int nl, const int32_t *cdims, int M,

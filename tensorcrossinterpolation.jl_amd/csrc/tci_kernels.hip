@@ -99,6 +99,73 @@ __global__ __launch_bounds__(kTeam) void tci_rrlu_kernel(tci_lu_job *__restrict_
     }
 }
 
+// The register-tile factorization of one job (tci_regtile.h) with tile shape
+// T, then rrlu!'s packed output written to J.A: the L columns (recorded by
+// original row in Lsc, m × mr) and U rows (by original column in Usc,
+// mr × n) placed at their final positions, the pivots on the diagonal and
+// the trailing block from the registers.
+template <class T, class Team>
+__device__ void tile_job(Team &tm, tci_lu_job &J, int m, int n, double *Lsc, double *Usc, double *pv)
+{
+    int64_t mr = J.maxrank;
+    if (m < mr) mr = m;
+    if (n < mr) mr = n;
+    T R;
+    R.setup(tm, m, n);
+    R.load(tm, J.A, m);
+    __syncthreads();   // every load of A done before A is rewritten
+    double err, st[2];
+    const int p = R.factor(tm, J.maxrank, J.reltol, J.abstol, J.leftorth != 0, pv, &err, st, Lsc,
+                           J.rowperm, J.colperm, Usc, mr);
+    R.store_trailing(tm, J.A, m, p > 0 ? p : 0);
+    __syncthreads();   // permutations, pivots and recorded factors complete
+    const int pp = p > 0 ? p : 0;
+    for (int64_t q = tm.tid(); q < (int64_t)m * pp; q += tm.nt()) {
+        const int k = (int)(q / m), pos = (int)(q - (int64_t)k * m);
+        if (pos > k) J.A[pos + (int64_t)m * k] = Lsc[J.rowperm[pos] + (int64_t)m * k];
+        else if (pos == k) J.A[k + (int64_t)m * k] = pv[k];
+    }
+    for (int64_t q = tm.tid(); q < (int64_t)pp * n; q += tm.nt()) {
+        const int pj = (int)(q / pp), k = (int)(q - (int64_t)pj * pp);
+        if (pj > k) J.A[k + (int64_t)m * pj] = Usc[k + mr * J.colperm[pj]];
+    }
+    __syncthreads();
+    for (int i = tm.tid(); i < m; i += tm.nt()) J.rowperm[i] += 1;
+    for (int j = tm.tid(); j < n; j += tm.nt()) J.colperm[j] += 1;
+    if (tm.tid() == 0) {
+        J.status = p < 0 ? -p : 0;
+        J.npivot = pp;
+        J.error = err;
+    }
+}
+
+// rrlu!(A) of the jobs that fit a 128×128 register tile, one workgroup per
+// job: the matrix stays in registers during the factorization (the solve
+// engine's tile, here with its packed factors written out).  idx: the job
+// indices; scratch + soff[q]: the job's Lsc (m × mr) then Usc (mr × n).
+__global__ __launch_bounds__(kTeam, 2) void tci_rrlu_tile_kernel(tci_lu_job *__restrict__ jobs,
+                                                                const int32_t *__restrict__ idx,
+                                                                int64_t count, double *__restrict__ scratch,
+                                                                const int64_t *__restrict__ soff)
+{
+    if ((int64_t)blockIdx.x >= count) return;
+    using Team = DevTeam<kTeam, 2>;
+    __shared__ double pv[128];
+    tci_lu_job &J = jobs[idx[blockIdx.x]];
+    const int m = (int)J.m, n = (int)J.n;
+    Team tm;
+    tm.init(nullptr, nullptr);
+    int64_t mr = J.maxrank;
+    if (m < mr) mr = m;
+    if (n < mr) mr = n;
+    double *Lsc = scratch + soff[blockIdx.x];
+    double *Usc = Lsc + (int64_t)m * mr;
+    if (TileLU<Team, 1, 8>::fits(m, n)) tile_job<TileLU<Team, 1, 8>>(tm, J, m, n, Lsc, Usc, pv);
+    else if (TileLU<Team, 1, 32>::fits(m, n)) tile_job<TileLU<Team, 1, 32>>(tm, J, m, n, Lsc, Usc, pv);
+    else if (TileLU<Team, 2, 8>::fits(m, n)) tile_job<TileLU<Team, 2, 8>>(tm, J, m, n, Lsc, Usc, pv);
+    else tile_job<TileLU<Team, 2, 32>>(tm, J, m, n, Lsc, Usc, pv);
+}
+
 __global__ void tci_lu_unpack_kernel(const double *__restrict__ A, int64_t m, int64_t n, int64_t p,
                                      int leftorth, double *__restrict__ L, int64_t ldl,
                                      double *__restrict__ U, int64_t ldu)
@@ -367,6 +434,15 @@ hipError_t launch_rrlu(tci_lu_job *jobs, int64_t count, int64_t max_side, double
     const size_t lds = max_side <= cap ? (size_t)2 * max_side * sizeof(double) : 0;
     hipLaunchKernelGGL(tci_rrlu_kernel, dim3((unsigned)count), dim3(kTeam), lds, st, jobs, count,
                        max_side <= cap ? (int)max_side : 0, gstage, gstage_stride, order);
+    return hipGetLastError();
+}
+
+hipError_t launch_rrlu_tile(tci_lu_job *jobs, const int32_t *idx, int64_t count, double *scratch,
+                            const int64_t *soff, hipStream_t st)
+{
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tci_rrlu_tile_kernel, dim3((unsigned)count), dim3(kTeam), 0, st, jobs, idx, count,
+                       scratch, soff);
     return hipGetLastError();
 }
 

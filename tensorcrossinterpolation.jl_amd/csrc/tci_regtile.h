@@ -68,6 +68,20 @@ struct TileLU {
         jw = 0;
     }
 
+    // the trailing block (positions >= p in both directions) as rrlu! leaves
+    // it in A after factor returned p: A[rowpos + lda·colpos] = entry
+    ENG_INLINE void store_trailing(Team &tm, double *A, int64_t lda, int p) const
+    {
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            const int c = colpos(tm, j);
+            if (c < p || c >= n) continue;
+#pragma unroll
+            for (int i = 0; i < RA; ++i)
+                if (rp[i] >= p && rp[i] < m) A[rp[i] + lda * c] = a[i][j];
+        }
+    }
+
     // entries outside the matrix hold NaN
     ENG_INLINE void load(const Team &tm, const double *A, int64_t lda)
     {
@@ -242,11 +256,15 @@ struct TileLU {
     // and NaN verdicts as rrlu_team.  Requires entries outside the matrix to
     // hold NaN.  Outputs: rowperm/colperm (original index at each position,
     // written when non-null), pivval[k], *perr, stats[0..1] (argmax and Schur
-    // element counts), and with Lorig non-null the L column k by original row
-    // (Lorig[orig + m·k]).  Returns p, or -status for the NaN errors.
+    // element counts), with Lorig non-null the L column k by original row
+    // (Lorig[orig + m·k]) and with Uorig non-null the U row k by original
+    // column (Uorig[k + ldu·orig], the pivot row as addpivot! leaves it:
+    // divided by the pivot when !leftorth).  Returns p, or -status for the
+    // NaN errors.
     ENG_INLINE int factor(Team &tm, int64_t maxrank, double reltol, double abstol, bool leftorth,
                           double *pivval, double *perr, double *stats, double *Lorig,
-                          int32_t *rowperm, int32_t *colperm)
+                          int32_t *rowperm, int32_t *colperm, double *Uorig = nullptr,
+                          int64_t ldu = 0)
     {
         const double NaN = 0.0 / 0.0;
         double lmax = -1.0;
@@ -331,6 +349,7 @@ struct TileLU {
                 if (j < CB && cpl[c] > k && cpl[c] < n) {
                     const double v = tm.tY(tm.wave(), j);
                     if (v != v) nany = 1;
+                    if (Uorig) Uorig[k + ldu * (tm.wave() + NW * j)] = v;
                 }
             }
             // trailing update and the next search.  With a full register
