@@ -1,12 +1,22 @@
-"""Benchmark: TCI solves/s and p50/p99 latency at tolerance 1e-8, batched vs plain.
+"""Benchmark: TCI solves/s and p50/p99 latency at tolerance 1e-8, batched and
+served vs plain, on one MI355X.
 
-Workload: crossinterpolate2(Float64, f, fill(10, 8); tolerance=1e-8) with
-f(v) = 1/(1 + v·v) and the initial pivot ones(8) (the README example,
+Headline workload: crossinterpolate2(Float64, f, fill(10, 8); tolerance=1e-8)
+with f(v) = 1/(1 + v·v) and the initial pivot ones(8) (the README example,
 README.md:24-28), every optimize! default kept (maxiter 20, back-and-forth
 sweeps, full pivot search, global pivot search with 5 points).  A "step" is
 one batched launch of B such solves, each a complete solve including the
-final sweep1site!, with results returned to the host.  The plain path is the
-same solve submitted alone (one solve per launch).
+final sweep1site!, results returned to the host.  The plain path is the same
+solve submitted alone (one solve per launch).
+
+The same JSON line carries, as sub-objects, the other workloads at their full
+sizes: the batched standalone rrLU (4096 uneven matrices in one launch, with
+a seeded sample of pivot sequences checked against the oracle outside the
+timed region), the quantics (L=30, d=2) and high-dimensional (L=40, d=16)
+solves, each with its error max|TT - f| on 10^5 seeded random points, and the
+served Gaussian-sum stream (10 000 requests, Poisson arrivals, rate swept).
+The CPU baseline beside each is the oracle (the C restatement of the
+reference's CPU path) on this host's cores.
 
 Prints ONE JSON line.  --gpus > 1 is refused: this build runs on one GPU.
 """
@@ -25,6 +35,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "tensorcrossinterpolation.jl_amd")
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 
 
 def load_pkg():
@@ -36,7 +47,81 @@ def load_pkg():
     return mod
 
 
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    return O
+
+
+def _problems():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import problems
+    return problems
+
+
+# ------------------------------------------------------------------ host CPU
+def host_cpu():
+    """CPU model, nproc, the CPUs this process may run on and the cgroup CPU
+    quota; `threads` = the CPUs usable at once (affinity, capped by the quota)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = nproc
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    threads = aff if quota is None else max(1, min(aff, int(quota)))
+    return {"model": model, "nproc": nproc, "affinity": aff, "cgroup_quota_cpus": quota, "threads": threads}
+
+
+def cpu_throughput(fn, threads, seconds):
+    """fn() run repeatedly on `threads` threads (the oracle releases the GIL
+    in its C calls) for about `seconds`; returns (calls/s, calls, wall s)."""
+    counts = [0] * threads
+    stop = time.perf_counter() + seconds
+
+    def work(k):
+        while time.perf_counter() < stop:
+            fn()
+            counts[k] += 1
+
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=work, args=(k,)) for k in range(threads)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    dt = time.perf_counter() - t0
+    return sum(counts) / dt, sum(counts), dt
+
+
+def cpu_latency(fn, reps):
+    fn()   # warm-up
+    lat = []
+    for _ in range(reps):
+        s = time.perf_counter()
+        fn()
+        lat.append(time.perf_counter() - s)
+    return statistics.median(lat)
+
+
+# ------------------------------------------------------------------ solves
 def run_batch(tci, ctx, arr, count):
+    """One launch of `count` solves; returns (wall s, kernel ms, ranks of
+    solve 0, [workstats of solve 0 + phase profile])."""
     lib = tci._abi.lib()
     b = C.c_void_p()
     t0 = time.perf_counter()
@@ -67,50 +152,74 @@ def run_batch(tci, ctx, arr, count):
     return t1 - t0, ms.value, ranks[:r.niter].tolist(), ws
 
 
-def cpu_baseline(threads, seconds):
-    """The oracle (C restatement of the reference CPU path), one solve per thread."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-    f = O.Functor(1, [1.0])
-    O.crossinterpolate2(f, [10] * 8, tolerance=1e-8)   # warm-up
-    counts = [0] * threads
-    stop = time.perf_counter() + seconds
-
-    def work(k):
-        while time.perf_counter() < stop:
-            O.crossinterpolate2(f, [10] * 8, tolerance=1e-8)
-            counts[k] += 1
-
-    t0 = time.perf_counter()
-    th = [threading.Thread(target=work, args=(k,)) for k in range(threads)]
-    for t in th:
-        t.start()
-    for t in th:
-        t.join()
-    dt = time.perf_counter() - t0
-    # single-thread latency
-    lat = []
-    for _ in range(5):
-        s = time.perf_counter()
-        O.crossinterpolate2(f, [10] * 8, tolerance=1e-8)
-        lat.append(time.perf_counter() - s)
-    return sum(counts) / dt, sum(counts), dt, statistics.median(lat)
-
-
-def profiled_traffic(name, batch, ref_batch):
-    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE + WRITE_SIZE
-    (copy-calibrated) and recorded in profiles/<name>.md, for the same
-    workload; None when the batch differs or no profile is committed."""
+def profiled(name, key):
+    """A figure recorded in profiles/<name>.md as a `key: value` line, or None."""
     path = os.path.join(ROOT, "profiles", name + ".md")
-    if batch != ref_batch or not os.path.exists(path):
-        return None, None
+    if not os.path.exists(path):
+        return None
     for line in open(path):
         line = line.strip()
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1]), "profiles/%s.md" % name
-    return None, None
+        if line.startswith(key + ":"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
+def sample_error(tci, spec, npts=100_000, seed=12345):
+    """max|TT - f| of one device solve on npts seeded random points (f by
+    the device functor, whose bits equal the host FP64 evaluation), divided
+    by max|f| over the points and by the solve's maxsamplevalue."""
+    name, kind, params, ip, ld, piv, opts = spec
+    f = tci.DeviceFunctor(kind, np.asarray(params, float), np.asarray(ip, np.int32))
+    (t,) = tci.crossinterpolate2_batch([(f, ld, piv, dict(opts))], error_sample=(npts, seed))
+    me, mf = t.sample_error
+    return {"points": npts, "seed": seed, "max_abs_err": me, "err_over_max_f": me / mf,
+            "err_over_maxsample": me / t.maxsamplevalue, "tolerance": opts["tolerance"],
+            "meets_tolerance": bool(me / t.maxsamplevalue <= opts["tolerance"]),
+            "ranks": list(map(int, t.ranks))}
+
+
+def solve_config(tci, ctx, spec, batch, steps, plain_reps, cpu, cpu_seconds, cpu_reps):
+    """Batched and plain GPU rates of one solve configuration, its sample
+    error, and the oracle's single-thread latency and multi-thread rate."""
+    name, kind, params, ip, ld, piv, opts = spec
+    holder = tci._abi.ProblemHolder(ld, kind, params, ip, piv, **opts)
+    arr = (tci._abi.tci_problem * batch)(*([holder.st] * batch))
+    one = (tci._abi.tci_problem * 1)(holder.st)
+    run_batch(tci, ctx, one, 1)
+    plain = [run_batch(tci, ctx, one, 1)[0] for _ in range(plain_reps)]
+    run_batch(tci, ctx, arr, batch)
+    walls, kms = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        w, k, ranks, ws = run_batch(tci, ctx, arr, batch)
+        walls.append(w)
+        kms.append(k)
+    elapsed = time.perf_counter() - t0
+    value = batch * steps / elapsed
+    out = {"workload": name, "batch_solves": batch, "solves_per_s": round(value, 2),
+           "ms_per_launch": round(elapsed / steps * 1e3, 3), "kernel_ms": round(float(np.mean(kms)), 3),
+           "plain": {"solves_per_s": round(1.0 / statistics.median(plain), 2),
+                     "latency_ms_p50": round(statistics.median(plain) * 1e3, 3)},
+           "batched_vs_plain": round(value * statistics.median(plain), 1), "ranks": ranks,
+           "error": sample_error(tci, spec)}
+    if cpu_seconds > 0:
+        O = _oracle()
+        of = O.Functor(kind, params, ip)
+        kw = {k: v for k, v in opts.items() if k != "chi_cap"}
+
+        def one_solve():
+            O.crossinterpolate2(of, ld, piv, **kw)
+
+        lat = cpu_latency(one_solve, cpu_reps)
+        cps, n, dt = cpu_throughput(one_solve, cpu["threads"], cpu_seconds)
+        out["cpu_baseline"] = {"value": round(cps, 3), "unit": "solves/s", "cores": cpu["threads"], "kind": "port",
+                               "sample": f"{n} solves of the same problem in {dt:.1f} s on {cpu['threads']} threads",
+                               "latency_ms_1thread": round(lat * 1e3, 3)}
+        out["vs_cpu"] = round(value / cps, 1)
+    return out
+
+
+# ------------------------------------------------------------------ rrLU
 def c2_shapes(count, seed=2024):
     """Shapes of the batched-rrLU workload: rows, cols ~ U{32..1024}, rank ~ U{8..128}."""
     rng = np.random.default_rng(seed)
@@ -139,10 +248,11 @@ def c2_matrices_device(count, seed=2024):
     return A, offs, [(int(m), int(n)) for m, n in zip(ms, ns)]
 
 
-def bench_rrlu(args, tci, ctx):
-    """Batched standalone rrLU: every matrix of the workload in ONE launch,
-    full pivoting to relative tolerance 1e-10.  Matrices resident in HBM; a
-    pristine copy restores them (untimed) before each step."""
+def rrlu_section(args, tci, ctx, cpu):
+    """Batched standalone rrLU: every matrix in ONE launch, full pivoting to
+    relative tolerance 1e-10, matrices resident in HBM (a pristine copy
+    restores them, untimed, before each step).  Afterwards a seeded sample of
+    the matrices is factored by the oracle and its pivot sequences compared."""
     import torch
     cnt = args.rrlu_count
     A, offs, shapes = c2_matrices_device(cnt)
@@ -151,6 +261,7 @@ def bench_rrlu(args, tci, ctx):
     perm = torch.zeros(sum(m + n for m, n in shapes), dtype=torch.int32, device="cuda")
     jobs = (tci._abi.tci_lu_job * cnt)()
     po = 0
+    poff = []
     for i, (m, n) in enumerate(shapes):
         j = jobs[i]
         j.A = A.data_ptr() + 8 * offs[i]
@@ -158,6 +269,7 @@ def bench_rrlu(args, tci, ctx):
         j.colperm = perm.data_ptr() + 4 * (po + m)
         j.m, j.n, j.maxrank = m, n, tci.TCI_UNLIMITED
         j.reltol, j.abstol, j.leftorth = 1e-10, 0.0, 1
+        poff.append(po)
         po += m + n
     jb = torch.frombuffer(bytearray(jobs), dtype=torch.uint8).to("cuda")
     # HIP events on the library's own stream time the launch
@@ -166,7 +278,7 @@ def bench_rrlu(args, tci, ctx):
         raise RuntimeError(lib.tci_last_error().decode())
     lstream = torch.cuda.ExternalStream(sp.value)
     times, kms = [], []
-    for it in range(args.warmup + args.steps):
+    for it in range(args.rrlu_warmup + args.rrlu_steps):
         A.copy_(pristine)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -177,7 +289,7 @@ def bench_rrlu(args, tci, ctx):
             raise RuntimeError(lib.tci_last_error().decode())
         e1.record(lstream)
         ctx.synchronize()
-        if it >= args.warmup:
+        if it >= args.rrlu_warmup:
             times.append(time.perf_counter() - t0)
             kms.append(e0.elapsed_time(e1))
     C.memmove(jobs, bytes(jb.cpu().numpy().tobytes()), C.sizeof(jobs))
@@ -191,52 +303,125 @@ def bench_rrlu(args, tci, ctx):
         tot_bytes += 8.0 * m * n + 16.0 * float(np.sum((m - k) * (n - k))) + 8.0 * p * (m + n)
     kms_avg = float(np.mean(kms))
     achieved = tot_bytes / (kms_avg * 1e-3) / 1e9
-    traffic, tsrc = profiled_traffic("tci_rrlu_kernel_c2_4096", cnt, 4096)
-    out = {"metric": "batched standalone rrLU matrices/s (FP64, full pivoting, reltol 1e-10)",
-           "value": round(value, 2), "unit": "matrices/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": round(elapsed / len(times) * 1e3, 3),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-           "data": "synthetic",
-           "config": {"workload": "rrlu of %d matrices, rows/cols U{32..1024}, A = X.Y + 1e-12 E, "
-                                  "rank U{8..128}, one launch" % cnt,
-                      "mean_pivots": round(float(ps.mean()), 2), "failed": int(bad)},
-           "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                        "traffic_source": tsrc, "kernel": "tci_rrlu_kernel",
-                        "kernel_ms": round(kms_avg, 3), "bytes_per_launch": tot_bytes,
-                        "note": "achieved = algorithmic bytes (8mn + 16 sum (m-k)(n-k) + 8p(m+n) "
-                                "per matrix) over the launch time from HIP events on the "
-                                "library's stream"}}
+    traffic = profiled("tci_rrlu_kernel_c2_4096", "traffic_bytes_per_launch") if cnt == 4096 else None
+    out = {"workload": "rrlu of %d matrices, rows/cols U{32..1024}, A = X.Y + 1e-12 E, rank U{8..128}, "
+                       "reltol 1e-10, one launch" % cnt,
+           "matrices_per_s": round(value, 2), "ms_per_launch": round(elapsed / len(times) * 1e3, 3),
+           "mean_pivots": round(float(ps.mean()), 2), "failed": int(bad),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                        "kernel": "tci_rrlu_kernel", "kernel_ms": round(kms_avg, 3), "bytes_per_launch": tot_bytes,
+                        "note": "achieved = algorithmic bytes 8mn + 16 sum (m-k)(n-k) + 8p(m+n) per matrix over "
+                                "the launch time (HIP events on the library's stream); the trailing block is "
+                                "streamed through L2/HBM once per pivot"}}
+    O = _oracle()
+    host = pristine.cpu().numpy()
+    permh = perm.cpu().numpy()
+    rng = np.random.default_rng(77)
+    sample = sorted(rng.choice(cnt, size=min(args.rrlu_check, cnt), replace=False).tolist())
+    mism = 0
+    for i in sample:
+        m, n = shapes[i]
+        o = O.rrlu(host[offs[i]:offs[i] + m * n].reshape((m, n), order="F"), reltol=1e-10)
+        same = (o.npivot == jobs[i].npivot and
+                np.array_equal(permh[poff[i]:poff[i] + m], o.rowpermutation) and
+                np.array_equal(permh[poff[i] + m:poff[i] + m + n], o.colpermutation) and
+                o.error == jobs[i].error)
+        mism += not same
+    out["pivot_check"] = {"matrices": len(sample), "mismatches": int(mism),
+                          "note": "seeded sample of the same launch's matrices; rowpermutation, colpermutation, "
+                                  "npivot and error compared with the oracle bit for bit, outside the timed region"}
     if args.cpu_seconds > 0:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
-        host = pristine.cpu().numpy()
-        t0 = time.perf_counter()
-        k = 0
-        while time.perf_counter() - t0 < args.cpu_seconds and k < cnt:
-            m, n = shapes[k]
-            O.rrlu(host[offs[k]:offs[k] + m * n].reshape((m, n), order="F"), reltol=1e-10)
-            k += 1
-        dt = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": round(k / dt, 3), "unit": "matrices/s", "cores": 1, "kind": "port",
-                               "sample": f"the first {k} matrices of the same batch, one thread, {dt:.1f} s"}
-    print(json.dumps(out), flush=True)
+        k = [0]
+        lock = threading.Lock()
+
+        def one():
+            with lock:
+                q = k[0] % cnt
+                k[0] += 1
+            m, n = shapes[q]
+            O.rrlu(host[offs[q]:offs[q] + m * n].reshape((m, n), order="F"), reltol=1e-10)
+
+        cps, nmat, dt = cpu_throughput(one, cpu["threads"], args.cpu_seconds)
+        out["cpu_baseline"] = {"value": round(cps, 3), "unit": "matrices/s", "cores": cpu["threads"], "kind": "port",
+                               "sample": f"{nmat} matrices of the same batch (in order) in {dt:.1f} s "
+                                         f"on {cpu['threads']} threads"}
+        out["vs_cpu"] = round(value / cps, 1)
+    return out
 
 
+# ------------------------------------------------------------------ served
+def served_section(args, tci, ctx, cpu):
+    """The served workload: 10 000 requests, each a sum of K ~ U{1..8}
+    separable Gaussians in n ~ U{4..12} dims, one local dimension
+    d ∈ {2,4,8,16,32}, tolerance 10^U(-10,-6), maxbonddim 128, arriving as a
+    Poisson stream at λ × the measured capacity, admitted by continuous
+    batching; λ swept."""
+    from importlib import import_module
+    serving = import_module("tci_amd.serving")
+    P = _problems()
+    mix = P.served_mix(args.c5_requests, seed=5)
+    reqs = [(tci.DeviceFunctor(k, np.asarray(pa, float), np.asarray(ip, np.int32)), ld, piv, dict(o))
+            for _, k, pa, ip, ld, piv, o in mix]
+    srv = serving.TCIServer(ctx, streams=args.streams, max_batch=args.c5_max_batch)
+    # capacity: every request present at t = 0
+    res, st = srv.run(reqs, np.zeros(len(reqs)))
+    cap = st.solves_per_s
+    out = {"workload": f"{len(reqs)} Gaussian-sum requests (K~U{{1..8}}, n~U{{4..12}}, d in {{2,4,8,16,32}}, "
+                       "tol 10^U(-10,-6), maxbonddim 128), Poisson arrivals at lambda x capacity",
+           "capacity_solves_per_s": round(cap, 2), "streams": args.streams, "max_batch": args.c5_max_batch,
+           "failed": int(sum(r.status != 0 for r in res)), "capacity_retries": st.retries, "sweep": []}
+    for lam in args.c5_lambdas:
+        arr = serving.poisson_arrivals(len(reqs), lam * cap, seed=5)
+        res, st = srv.run(reqs, arr)
+        out["sweep"].append({"lambda": lam, "arrival_rate_per_s": round(lam * cap, 1),
+                             "solves_per_s": round(st.solves_per_s, 2), "p50_ms": round(st.p50_ms, 3),
+                             "p99_ms": round(st.p99_ms, 3), "batches": st.batches,
+                             "mean_batch": round(st.mean_batch, 1),
+                             "failed": int(sum(r.status != 0 for r in res))})
+    if args.cpu_seconds > 0:
+        O = _oracle()
+        k = [0]
+        lock = threading.Lock()
+
+        def one():
+            with lock:
+                q = k[0] % len(mix)
+                k[0] += 1
+            _, kind, pa, ip, ld, piv, o = mix[q]
+            O.crossinterpolate2(O.Functor(kind, pa, ip), ld, piv, **{a: b for a, b in o.items() if a != "chi_cap"})
+
+        cps, n, dt = cpu_throughput(one, cpu["threads"], args.cpu_seconds)
+        out["cpu_baseline"] = {"value": round(cps, 3), "unit": "solves/s", "cores": cpu["threads"], "kind": "port",
+                               "sample": f"the first {n} requests of the stream in {dt:.1f} s on {cpu['threads']} threads"}
+        out["vs_cpu"] = round(cap / cps, 1)
+    return out
+
+
+# ------------------------------------------------------------------ main
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["solve", "rrlu"], default="solve")
-    ap.add_argument("--rrlu-count", type=int, default=4096)
+    ap.add_argument("--workload", choices=["all", "solve", "rrlu"], default="all",
+                    help="all: the headline line with every sub-object; solve: the headline only; "
+                         "rrlu: the batched rrLU only")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--plain", type=int, default=20)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--served", type=int, default=8192, help="requests in the served (Poisson) run; 0: skip")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
+    ap.add_argument("--served", type=int, default=8192, help="requests in the headline's served run; 0: skip")
     ap.add_argument("--served-load", type=float, default=0.5, help="arrival rate as a fraction of batched capacity")
     ap.add_argument("--streams", type=int, default=4)
+    ap.add_argument("--rrlu-count", type=int, default=4096)
+    ap.add_argument("--rrlu-steps", type=int, default=3)
+    ap.add_argument("--rrlu-warmup", type=int, default=1)
+    ap.add_argument("--rrlu-check", type=int, default=48, help="matrices whose pivots are checked against the oracle")
+    ap.add_argument("--c3-batch", type=int, default=2048)
+    ap.add_argument("--c4-batch", type=int, default=512)
+    ap.add_argument("--c5-requests", type=int, default=10000)
+    ap.add_argument("--c5-max-batch", type=int, default=4096)
+    ap.add_argument("--c5-lambdas", type=float, nargs="*", default=[0.25, 0.5, 1.0, 2.0, 4.0])
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 plain device copies of 1 GiB (the reference kernel for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -257,11 +442,21 @@ def main():
     import torch
     tci = load_pkg()
     ctx = tci.default_context()
+    cpu = host_cpu()
     if args.workload == "rrlu":
-        bench_rrlu(args, tci, ctx)
+        r = rrlu_section(args, tci, ctx, cpu)
+        out = {"metric": "batched standalone rrLU matrices/s (FP64, full pivoting, reltol 1e-10)",
+               "value": r["matrices_per_s"], "unit": "matrices/s", "n_gpus": 1, "steps": args.rrlu_steps,
+               "warmup": args.rrlu_warmup, "ms_per_step": r["ms_per_launch"], "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+               "config": {"workload": r["workload"]}, "roofline": r["roofline"], "rrlu": r,
+               "cpu_baseline": r.get("cpu_baseline"), "host_cpu": cpu}
+        print(json.dumps(out), flush=True)
         return
+    P = _problems()
+    spec = P.c1_spec(5)
     B = args.batch
-    holder = tci._abi.ProblemHolder([10] * 8, tci._abi.F_LORENTZ_INT, [1.0], tolerance=1e-8, chi_cap=32)
+    holder = tci._abi.ProblemHolder([10] * 8, tci._abi.F_LORENTZ_INT, [1.0], tolerance=1e-8)
     arr = (tci._abi.tci_problem * B)(*([holder.st] * B))
     one = (tci._abi.tci_problem * 1)(holder.st)
     for _ in range(args.warmup):
@@ -273,7 +468,7 @@ def main():
         wall, kms, ranks1, _ = run_batch(tci, ctx, one, 1)
         plain.append(wall)
     plain_lat = statistics.median(plain)
-    # batched path: timed steps
+    # batched path: timed steps, bracketed by device synchronization
     torch.cuda.synchronize()
     walls, kms_all = [], []
     t0 = time.perf_counter()
@@ -282,19 +477,18 @@ def main():
         walls.append(wall)
         kms_all.append(kms)
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
+    elapsed = time.perf_counter() - t0
     value = B * args.steps / elapsed
-    lat_sorted = sorted(walls)
-    p50 = float(np.percentile(lat_sorted, 50)) * 1e3
-    p99 = float(np.percentile(lat_sorted, 99)) * 1e3
+    p50 = float(np.percentile(walls, 50)) * 1e3
+    p99 = float(np.percentile(walls, 99)) * 1e3
     kms_avg = float(np.mean(kms_all))
-    # algorithmic bytes per launch: Pi evaluation writes 8 B per element, the
-    # fused rrLU moves 8mn + 16Σ(m-k)(n-k) + 8p(m+n) per factorization
-    bytes_per_solve = 8.0 * ws[0] + ws[2]
     phases = dict(zip(["eval", "rrlu", "sets", "fill", "gsearch", "sweep1", "total", "pinv"], ws[6][:8]))
-    achieved = B * bytes_per_solve / (kms_avg * 1e-3) / 1e9
-    traffic, tsrc = profiled_traffic("tci_solve_kernel_lorentz8_b4096", B, 4096)
+    # the Schur updates run on the FP64 units with the trailing block in
+    # registers: the roof is FP64 (MI355X FP64 vector and matrix peaks are
+    # both 78.6 TFLOP/s); F_schur = 2 sum (m-k)(n-k) over every rrLU of a solve
+    schur_tflops = B * ws[3] / (kms_avg * 1e-3) / 1e12
+    traffic = profiled("tci_solve_kernel_lorentz8_b4096", "traffic_bytes_per_launch") if B == 4096 else None
+    streamed = 8.0 * ws[0] + ws[2]
     out = {
         "metric": "TCI solves/s + p50/p99 latency at tol 1e-8, batched/served vs plain on MI355X",
         "value": round(value, 2),
@@ -315,18 +509,23 @@ def main():
         "plain": {"solves_per_s": round(1.0 / plain_lat, 2), "latency_ms_p50": round(plain_lat * 1e3, 3)},
         "batched_vs_plain": round(value * plain_lat, 1),
         "solve_phases_us": {k: round(v, 1) for k, v in phases.items()},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "traffic_source": tsrc,
+        "roofline": {"bound": "mfma", "achieved": round(schur_tflops, 4), "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(schur_tflops / FP64_PEAK_TFLOPS, 5), "traffic": traffic,
                      "kernel": "tci_solve_kernel", "kernel_ms": round(kms_avg, 3),
-                     "bytes_per_solve": bytes_per_solve,
-                     "schur_fp64_tflops": round(B * ws[3] / (kms_avg * 1e-3) / 1e12, 4),
-                     "schur_fp64_frac": round(B * ws[3] / (kms_avg * 1e-3) / 78.6e12, 5),
-                     "note": "achieved = algorithmic bytes of the streamed model (8mn per Pi + "
-                             "8mn + 16 sum (m-k)(n-k) + 8p(m+n) per rrLU) over the kernel time; "
-                             "traffic = measured HBM bytes per launch: the tiles stay on chip, so "
-                             "frac is a streaming-equivalent figure; schur_fp64_* = the Schur "
-                             "update flops 2 sum (m-k)(n-k) over the kernel time vs 78.6 TFLOP/s"},
+                     "flops_per_solve": ws[3],
+                     "hbm_measured": None if traffic is None else {
+                         "GB_per_s": round(traffic / (kms_avg * 1e-3) / 1e9, 2),
+                         "frac": round(traffic / (kms_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)},
+                     "streamed_model_bytes_per_solve": streamed,
+                     "note": "FP64 roof: the Schur updates (2 sum (m-k)(n-k) flops per rrLU) run on the FP64 "
+                             "vector units with every Pi matrix and trailing block in registers/LDS "
+                             "(MI355X FP64 vector = FP64 matrix peak = 78.6 TFLOP/s, no MFMA used); the "
+                             "step is bound by the latency of the dependent pivot chain. traffic = measured "
+                             "HBM bytes per launch (copy-calibrated FETCH_SIZE + WRITE_SIZE); "
+                             "streamed_model_bytes_per_solve = what a design streaming each Pi through HBM "
+                             "would move (8mn per evaluation + 8mn + 16 sum (m-k)(n-k) + 8p(m+n) per rrLU)"},
+        "error": sample_error(tci, spec),
+        "host_cpu": cpu,
     }
     if args.served > 0:
         # served: Poisson arrivals of the same request at a fraction of the
@@ -335,7 +534,7 @@ def main():
         serving = import_module("tci_amd.serving")
         rate = args.served_load * value
         f = tci.Lorentz(1.0)
-        reqs = [(f, [10] * 8, None, dict(tolerance=1e-8, chi_cap=32))] * args.served
+        reqs = [(f, [10] * 8, None, dict(tolerance=1e-8))] * args.served
         arrivals = serving.poisson_arrivals(args.served, rate, seed=5)
         srv = serving.TCIServer(ctx, streams=args.streams, max_batch=B)
         res, st = srv.run(reqs, arrivals)
@@ -346,13 +545,25 @@ def main():
                          "mean_batch": round(st.mean_batch, 1), "streams": args.streams, "failed": bad,
                          "note": "latency from arrival to results on the host"}
     if args.cpu_seconds > 0:
-        thr = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cps, n, dt, lat1 = cpu_baseline(thr, args.cpu_seconds)
-        out["cpu_baseline"] = {"value": round(cps, 2), "unit": "solves/s", "cores": thr, "kind": "port",
-                               "sample": f"{n} solves of the same workload in {dt:.1f} s on {thr} threads "
-                                         f"(oracle C restatement, -O2, one solve per thread)",
+        O = _oracle()
+        of = O.Functor(1, [1.0])
+
+        def one_solve():
+            O.crossinterpolate2(of, [10] * 8, tolerance=1e-8)
+
+        lat1 = cpu_latency(one_solve, 5)
+        cps, n, dt = cpu_throughput(one_solve, cpu["threads"], 1.5 * args.cpu_seconds)
+        out["cpu_baseline"] = {"value": round(cps, 2), "unit": "solves/s", "cores": cpu["threads"], "kind": "port",
+                               "sample": f"{n} solves of the same workload in {dt:.1f} s on {cpu['threads']} threads "
+                                         f"(oracle C restatement, -O2, one solve per thread; host: {cpu['model']}, "
+                                         f"nproc {cpu['nproc']})",
                                "latency_ms_1thread": round(lat1 * 1e3, 3)}
         out["vs_cpu"] = round(value / cps, 1)
+    if args.workload == "all":
+        out["rrlu_batched"] = rrlu_section(args, tci, ctx, cpu)
+        out["c3_quantics"] = solve_config(tci, ctx, P.c3_spec(5), args.c3_batch, 3, 5, cpu, args.cpu_seconds / 2, 3)
+        out["c4_highdim"] = solve_config(tci, ctx, P.c4_spec(5), args.c4_batch, 2, 3, cpu, args.cpu_seconds, 3)
+        out["c5_served"] = served_section(args, tci, ctx, cpu)
     print(json.dumps(out), flush=True)
 
 

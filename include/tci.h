@@ -155,7 +155,7 @@ typedef struct tci_options {
     int32_t checkconvglobalpivot;  /* 1                                       */
     uint64_t seed;                 /* global pivot search PRNG seed            */
     int32_t chi_cap;               /* device capacity for any |Iset|/|Jset|;
-                                      0 = min(maxbonddim, rank bound, 64); see
+                                      0 = min(maxbonddim, rank bound, 32); see
                                       tci_capacity_plan                        */
     int32_t flags;                 /* TCI_FLAG_* bits, 0 by default           */
 } tci_options;
@@ -184,8 +184,9 @@ typedef struct tci_batch tci_batch;
 
 /* Capacity schedule of a problem (host only, no device needed).  *chi_used:
  * the set capacity tci_solve_submit gives it (opts.chi_cap, or when that is
- * 0 the default min(maxbonddim, rank bound, 64), the rank bound being
- * max_b min(Π_{i<=b} d_i, Π_{i>b} d_i)); *chi_limit: min(maxbonddim, rank
+ * 0 the default min(maxbonddim, rank bound, 32), the rank bound being
+ * max_b min(Π_{i<=b} d_i, Π_{i>b} d_i), lowered to no less than 8 while the
+ * solve's Pi buffer would exceed 8 MiB); *chi_limit: min(maxbonddim, rank
  * bound), the capacity at which TCI_ERR_CAPACITY can no longer occur.  The
  * reference has no capacities (tensorci2.jl grows its Vectors): a caller
  * that gets TCI_ERR_CAPACITY for a solve resubmits it with chi_cap doubled,

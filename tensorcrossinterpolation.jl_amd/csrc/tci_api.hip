@@ -275,48 +275,93 @@ static hipStream_t pick_stream(tci_ctx *ctx)
 }
 
 // ------------------------------------------------------------------- rrLU
-tci_status tci_rrlu_batched_f64(tci_ctx *ctx, tci_lu_job *jobs, int64_t count)
+// The job shapes of a batched rrLU, read once, and their partition between
+// the register-tile kernel (jobs up to 128×128, when the context's
+// algorithm allows it) and the streaming one.  tile_budget: the most tile
+// scratch (doubles) the tile jobs may take, -1 for no limit; a job past it
+// goes to the streaming kernel (the results are the same bits).
+struct LuPlan {
+    std::vector<tci_lu_job> h;
+    std::vector<int32_t> tile, stream;
+    std::vector<int64_t> soff;
+    int64_t sc_tot = 0;   // tile scratch, doubles
+    int64_t mx = 0;       // largest side of any job
+};
+
+static tci_status lu_plan(tci_ctx *ctx, tci_lu_job *jobs, int64_t count, int64_t tile_budget, LuPlan &P)
 {
-    if (!ctx || (!jobs && count > 0) || count < 0) return set_err(TCI_ERR_ARG, "bad arguments");
-    if (count == 0) return TCI_OK;
-    if (count > INT32_MAX) return set_err(TCI_ERR_ARG, "too many jobs");
-    HIPCHK(hipSetDevice(ctx->device));
-    // job shapes decide the staging placement; read them once
-    std::vector<tci_lu_job> h((size_t)count);
-    HIPCHK(hipMemcpyAsync(h.data(), jobs, sizeof(tci_lu_job) * (size_t)count, hipMemcpyDeviceToHost,
+    P.h.resize((size_t)count);
+    HIPCHK(hipMemcpyAsync(P.h.data(), jobs, sizeof(tci_lu_job) * (size_t)count, hipMemcpyDeviceToHost,
                           ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    int64_t mx = 0;
-    for (const tci_lu_job &j : h) {
+    for (const tci_lu_job &j : P.h) {
         if (j.m < 0 || j.n < 0 || j.m > INT32_MAX / 2 || j.n > INT32_MAX / 2 || j.m * j.n > ((int64_t)1 << 31))
             return set_err(TCI_ERR_ARG, "job shape out of range");
         if ((j.m > 0 && (!j.A || !j.rowperm)) || (j.n > 0 && (!j.A || !j.colperm)))
             return set_err(TCI_ERR_ARG, "job has NULL buffers");
         if (j.maxrank < 0) return set_err(TCI_ERR_ARG, "maxrank must be >= 0");
         int64_t s = j.m > j.n ? j.m : j.n;
-        mx = s > mx ? s : mx;
+        P.mx = s > P.mx ? s : P.mx;
     }
-    // jobs for the register tile (they fit 128×128 and the algorithm allows
-    // it) and for the streaming factorization; results do not depend on it
-    std::vector<int32_t> tile, stream;
-    std::vector<int64_t> soff;
-    int64_t sc_tot = 0;
     for (int64_t q = 0; q < count; ++q) {
-        const tci_lu_job &j = h[(size_t)q];
+        const tci_lu_job &j = P.h[(size_t)q];
         const bool fits = j.m >= 1 && j.n >= 1 && j.m <= kTileSide && j.n <= kTileSide;
         if (ctx->rrlu_algo == TCI_RRLU_TILE && !fits && j.m > 0 && j.n > 0)
             return set_err(TCI_ERR_ARG, "job %lld (%lld x %lld) does not fit the 128 x 128 register tile",
                            (long long)q, (long long)j.m, (long long)j.n);
-        if (fits && ctx->rrlu_algo != TCI_RRLU_STREAM) {
-            int64_t mr = j.maxrank < j.m ? j.maxrank : j.m;
-            if (j.n < mr) mr = j.n;
-            tile.push_back((int32_t)q);
-            soff.push_back(sc_tot);
-            sc_tot += mr * (j.m + j.n);
+        int64_t mr = j.maxrank < j.m ? j.maxrank : j.m;
+        if (j.n < mr) mr = j.n;
+        const int64_t need = mr * (j.m + j.n);
+        if (fits && ctx->rrlu_algo != TCI_RRLU_STREAM && (tile_budget < 0 || P.sc_tot + need <= tile_budget)) {
+            P.tile.push_back((int32_t)q);
+            P.soff.push_back(P.sc_tot);
+            P.sc_tot += need;
         } else {
-            stream.push_back((int32_t)q);
+            if (ctx->rrlu_algo == TCI_RRLU_TILE && fits)
+                return set_err(TCI_ERR_ARG, "the register-tile scratch cannot hold job %lld", (long long)q);
+            P.stream.push_back((int32_t)q);
         }
     }
+    return TCI_OK;
+}
+
+// largest side of the streaming jobs
+static int64_t stream_side(const LuPlan &P)
+{
+    int64_t mx = 0;
+    for (int32_t q : P.stream) {
+        const int64_t s = P.h[(size_t)q].m > P.h[(size_t)q].n ? P.h[(size_t)q].m : P.h[(size_t)q].n;
+        mx = s > mx ? s : mx;
+    }
+    return mx;
+}
+
+// longest-processing-time order of the streaming jobs (largest first), so
+// that the workgroups that start last are the short ones; results do not
+// depend on the order
+static void stream_order(const LuPlan &P, std::vector<int32_t> &order)
+{
+    order = P.stream;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        return P.h[(size_t)a].m * P.h[(size_t)a].n > P.h[(size_t)b].m * P.h[(size_t)b].n;
+    });
+}
+
+constexpr int64_t kLuLdsStageCap = 48 * 1024 / 16;   // doubles per staging vector in LDS
+
+tci_status tci_rrlu_batched_f64(tci_ctx *ctx, tci_lu_job *jobs, int64_t count)
+{
+    if (!ctx || (!jobs && count > 0) || count < 0) return set_err(TCI_ERR_ARG, "bad arguments");
+    if (count == 0) return TCI_OK;
+    if (count > INT32_MAX) return set_err(TCI_ERR_ARG, "too many jobs");
+    HIPCHK(hipSetDevice(ctx->device));
+    LuPlan P;
+    tci_status ps = lu_plan(ctx, jobs, count, -1, P);
+    if (ps != TCI_OK) return ps;
+    std::vector<int32_t> &tile = P.tile, &stream = P.stream;
+    std::vector<int64_t> &soff = P.soff;
+    const int64_t sc_tot = P.sc_tot;
+    int64_t mx;
     if (!tile.empty()) {
         uint8_t *d = nullptr;
         const size_t b_idx = lay_align(sizeof(int32_t) * tile.size());
@@ -334,25 +379,14 @@ tci_status tci_rrlu_batched_f64(tci_ctx *ctx, tci_lu_job *jobs, int64_t count)
         if (e != hipSuccess) return set_err(TCI_ERR_DEVICE, "rrlu tile launch: %s", hipGetErrorString(e));
     }
     if (stream.empty()) return TCI_OK;
-    mx = 0;
-    for (int32_t q : stream) {
-        const int64_t s = h[(size_t)q].m > h[(size_t)q].n ? h[(size_t)q].m : h[(size_t)q].n;
-        mx = s > mx ? s : mx;
-    }
+    mx = stream_side(P);
     double *gstage = nullptr;
-    const int64_t lds_cap = 48 * 1024 / 16;
-    if (mx > lds_cap) HIPCHK(hipMallocAsync((void **)&gstage, sizeof(double) * 2 * mx * count, ctx->stream));
-    // largest matrices first (longest-processing-time order), so that the
-    // workgroups that start last are the short ones; results do not depend
-    // on the order
-    // (the host copy lives in the context: the next call on this stream
-    // synchronizes before it reuses it)
+    if (mx > kLuLdsStageCap) HIPCHK(hipMallocAsync((void **)&gstage, sizeof(double) * 2 * mx * count, ctx->stream));
+    // (the host copy of the order lives in the context: the next call on
+    // this stream synchronizes before it reuses it)
     std::vector<int32_t> &order = ctx->lu_order;
-    order = stream;
+    stream_order(P, order);
     const int64_t ns = (int64_t)order.size();
-    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
-        return h[(size_t)a].m * h[(size_t)a].n > h[(size_t)b].m * h[(size_t)b].n;
-    });
     int32_t *dorder = nullptr;
     HIPCHK(hipMallocAsync((void **)&dorder, sizeof(int32_t) * (size_t)ns, ctx->stream));
     HIPCHK(hipMemcpyAsync(dorder, order.data(), sizeof(int32_t) * (size_t)ns, hipMemcpyHostToDevice,

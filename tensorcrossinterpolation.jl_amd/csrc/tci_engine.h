@@ -1373,7 +1373,9 @@ struct Engine {
 
     // ------------------------------------------------ final sweep1site!
     // tensorci2.jl:402-461, forward, updatetensors = true
-    ENG int sweep1site_forward(double abstol)
+    // (inlined, and finish kept out of line: the kernel body measured
+    // fastest with these two call decisions)
+    ENG_INLINE int sweep1site_forward(double abstol)
     {
         if (tm.tid() == 0) S.npe = 0;
         for (int b = 0; b < L - 1; ++b) {
@@ -1542,7 +1544,7 @@ struct Engine {
         return finish(ST_OK);
     }
 
-    ENG int finish(int st)
+    ENG_NOINLINE int finish(int st)
     {
         tm.sync();
         if (sets != D.sets) {   // sets held in LDS: write them to the solve's arena

@@ -31,17 +31,31 @@ inline int64_t rank_bound(int L, const int32_t *localdims)
 }
 
 // Default capacity of every |Iset|/|Jset| (before the global-pivot extras):
-// min(maxbonddim, the rank bound, kDefaultChi).  A solve whose sets outgrow
-// it ends with TCI_ERR_CAPACITY; the host side then resubmits it at twice
-// the capacity (up to min(maxbonddim, rank bound)), so the default options
-// succeed wherever the reference does.
-constexpr int64_t kDefaultChi = 64;
+// min(maxbonddim, the rank bound, kDefaultChi), lowered (not below
+// kMinDefaultChi) until the solve's Pi buffer, mrcap² doubles with
+// mrcap = (dmax + 1)·(chi + extras), fits kDefaultPiBytes, so that a batch
+// of many solves over large local dimensions stays small in HBM.  A solve
+// whose sets outgrow it ends with TCI_ERR_CAPACITY; the host side then
+// resubmits it at twice the capacity (up to min(maxbonddim, rank bound)), so
+// the default options succeed wherever the reference does.
+constexpr int64_t kDefaultChi = 32, kMinDefaultChi = 8;
+constexpr int64_t kDefaultPiBytes = (int64_t)8 << 20;
 inline int64_t default_chi_cap(const tci_problem &p)
 {
     int64_t chi = kDefaultChi;
     if (p.opts.maxbonddim < chi) chi = p.opts.maxbonddim;
     const int64_t rb = rank_bound(p.L, p.localdims);
-    return rb < chi ? rb : chi;
+    if (rb < chi) chi = rb;
+    int64_t dmax = 1;
+    for (int b = 0; b < p.L; ++b) dmax = p.localdims[b] > dmax ? p.localdims[b] : dmax;
+    const int64_t npiv = p.initialpivots ? p.npivots : 1;
+    const int64_t extra = p.opts.maxnglobalpivot > npiv ? p.opts.maxnglobalpivot : npiv;
+    while (chi > kMinDefaultChi) {
+        const int64_t mr = (dmax + 1) * (chi + extra);
+        if (mr * mr * 8 <= kDefaultPiBytes) break;
+        --chi;
+    }
+    return chi;
 }
 
 // Checks a functor against the local dims it is evaluated on (any L >= 1):

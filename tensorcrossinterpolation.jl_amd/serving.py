@@ -24,6 +24,7 @@ from . import _abi
 
 PENDING = 13          # TCI_ERR_PENDING
 TCI_ERR_DEVICE = 12
+TCI_ERR_CAPACITY = 10
 
 
 class ServeError(RuntimeError):
@@ -59,6 +60,7 @@ class ServeStats:
     batches: int
     mean_batch: float
     oom_waits: int = 0
+    retries: int = 0
     latencies_ms: list = field(default_factory=list)
 
 
@@ -79,33 +81,46 @@ class TCIServer:
 
     def run(self, requests, arrivals, keep_tci=False):
         """requests[i] = (f, localdims, initialpivots_or_None, kwargs); arrivals
-        sorted, seconds from the start.  Returns (results, ServeStats)."""
-        from . import TensorCI2, _holder
+        sorted, seconds from the start.  Returns (results, ServeStats).
+
+        A request whose solve outgrew its capacity (TCI_ERR_CAPACITY) while
+        the caller left chi_cap unset is admitted again, ahead of new
+        arrivals, at twice the capacity (as crossinterpolate2_batch does);
+        its latency counts both runs."""
+        from . import TensorCI2, _holder, capacity_plan
         lib = _abi.lib()
         n = len(requests)
         holders = [_holder(f, ld, piv, kw) for f, ld, piv, kw in requests]
         results = [None] * n
         inflight = []                       # (batch handle, [request ids], arrays kept alive)
+        retry = []                          # request ids to admit again (larger capacity)
         nxt = 0
         nbatches = 0
+        nretries = 0
         oom_waits = 0
         t0 = time.perf_counter()
-        while nxt < n or inflight:
+        while nxt < n or inflight or retry:
             now = time.perf_counter() - t0
             progressed = False
-            # admit everything that has arrived, when a stream is free
-            if nxt < n and arrivals[nxt] <= now and len(inflight) < self.streams:
+            # admit the retries and everything that has arrived, when a stream is free
+            ready = nxt < n and arrivals[nxt] <= now
+            if (retry or ready) and len(inflight) < self.streams:
+                ids = retry[:self.max_batch]
                 hi = nxt
-                while hi < n and arrivals[hi] <= now and hi - nxt < self.max_batch:
+                while hi < n and arrivals[hi] <= now and len(ids) + (hi - nxt) < self.max_batch:
                     hi += 1
-                cnt = hi - nxt
+                ids = ids + list(range(nxt, hi))
+                cnt = len(ids)
                 while cnt > 0:
-                    arr = (_abi.tci_problem * cnt)(*[h.st for h in holders[nxt:nxt + cnt]])
+                    arr = (_abi.tci_problem * cnt)(*[holders[i].st for i in ids[:cnt]])
                     b = C.c_void_p()
                     st = lib.tci_solve_submit(self.ctx.handle, cnt, arr, C.byref(b))
                     if st == 0:
-                        inflight.append((b, list(range(nxt, nxt + cnt)), arr))
-                        nxt += cnt
+                        taken = ids[:cnt]
+                        inflight.append((b, taken, arr))
+                        nr = min(len(retry), cnt)
+                        retry = retry[nr:]
+                        nxt += cnt - nr
                         nbatches += 1
                         progressed = True
                         break
@@ -118,7 +133,7 @@ class TCIServer:
                     msg = lib.tci_last_error().decode()
                     for b2, _, _ in inflight:
                         lib.tci_batch_free(b2)
-                    raise ServeError(st, f"submit of requests {nxt}..{nxt + cnt - 1} failed: {msg}")
+                    raise ServeError(st, f"submit of {cnt} requests failed: {msg}")
             # retire finished batches; only TCI_ERR_PENDING means "still
             # running": any other failure of the query or of the report
             # fetch frees the batch and raises
@@ -144,6 +159,15 @@ class TCIServer:
                 done = time.perf_counter() - t0
                 for k, i in enumerate(ids):
                     lib.tci_batch_result(b, k, C.byref(r))
+                    if r.status == TCI_ERR_CAPACITY and not requests[i][3].get("chi_cap"):
+                        used, limit = capacity_plan(holders[i])
+                        chi = int(min(2 * used, limit, 8192))
+                        if chi > used:
+                            f, ld, piv, kw = requests[i]
+                            holders[i] = _holder(f, ld, piv, dict(kw, chi_cap=chi))
+                            retry.append(i)
+                            nretries += 1
+                            continue
                     m = max(r.niter, 1)
                     ranks = np.zeros(m, np.int64)
                     errs = np.zeros(m)
@@ -155,7 +179,7 @@ class TCIServer:
                 progressed = True
             inflight = still
             if not progressed:
-                if nxt < n and not inflight:
+                if nxt < n and not inflight and not retry:
                     wait = arrivals[nxt] - (time.perf_counter() - t0)
                     if wait > 0:
                         time.sleep(min(wait, 1e-3))
@@ -166,6 +190,7 @@ class TCIServer:
         stats = ServeStats(n=n, wall_s=wall, solves_per_s=n / wall if wall > 0 else 0.0,
                            p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
                            batches=nbatches, mean_batch=n / max(nbatches, 1), oom_waits=oom_waits,
+                           retries=nretries,
                            latencies_ms=lat.tolist())
         return results, stats
 

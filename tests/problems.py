@@ -112,7 +112,7 @@ def oracle_opts(opts):
     return {k: v for k, v in opts.items() if k != "chi_cap"}
 
 
-def served_mix(count, seed=5, chi_cap=16):
+def served_mix(count, seed=5, chi_cap=0):
     """Requests of the served workload: each a sum of K ~ U{1..8} separable
     Gaussians in n ~ U{4..12} dims, one local dimension d ∈ {2,4,8,16,32} for
     all sites, tolerance 10^U(-10,-6), maxbonddim 128.  Returns specs in the

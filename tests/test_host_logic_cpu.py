@@ -16,8 +16,8 @@ def _holder(pkg, ld, **kw):
 
 def test_capacity_plan_defaults():
     pkg = load_pkg()
-    # 8 sites of d = 10: rank bound 10^4, default capacity 64, limit 10^4
-    assert pkg.capacity_plan(_holder(pkg, [10] * 8)) == (64, 10 ** 4)
+    # 8 sites of d = 10: rank bound 10^4, default capacity 32, limit 10^4
+    assert pkg.capacity_plan(_holder(pkg, [10] * 8)) == (32, 10 ** 4)
     # maxbonddim below 64 caps both
     assert pkg.capacity_plan(_holder(pkg, [10] * 8, maxbonddim=20)) == (20, 20)
     # a small domain caps both at its rank bound: 2^2 = 4 at the middle bond
@@ -27,7 +27,12 @@ def test_capacity_plan_defaults():
     assert pkg.capacity_plan(_holder(pkg, [10] * 8, chi_cap=7)) == (7, 10 ** 4)
     # saturating products for long chains
     used, limit = pkg.capacity_plan(_holder(pkg, [16] * 40, maxbonddim=256))
-    assert (used, limit) == (64, 256)
+    assert (used, limit) == (32, 256)
+    # d = 64: the Pi buffer of (65·(chi+5))² doubles is kept within 8 MiB
+    used, limit = pkg.capacity_plan(_holder(pkg, [64] * 6))
+    assert used == 10 and (65 * 15) ** 2 * 8 <= 8 << 20 < (65 * 16) ** 2 * 8
+    # d = 128: never below 8
+    assert pkg.capacity_plan(_holder(pkg, [128] * 6))[0] == 8
 
 
 def test_capacity_plan_rejects_bad_problems():
