@@ -106,6 +106,21 @@ tci_status tci_rrlu_batched_f64(tci_ctx *ctx, tci_lu_job *jobs_device, int64_t c
 enum { TCI_RRLU_AUTO = 0, TCI_RRLU_STREAM = 1, TCI_RRLU_TILE = 2 };
 tci_status tci_ctx_set_rrlu_algo(tci_ctx *ctx, int32_t algo);
 
+/* tci_rrlu_batched_f64 that allocates no device memory.  No counterpart in
+ * the reference: it is for callers that cannot call hipMalloc (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid it).  The
+ * job table, permutations and matrices are the caller's, as in
+ * tci_rrlu_batched_f64, and the results are the same bits.  Its working data
+ * (job order, staging vectors, register-tile scratch) lives in a fixed
+ * 32 MiB device array that each GPU holds until the process ends; a batch
+ * needing more than that for its order and staging vectors is refused with
+ * TCI_ERR_ARG (tile jobs that do not fit the rest go to the streaming
+ * kernel).  Every call on one GPU shares that array: calls that overlap on
+ * one GPU (different streams or contexts) race; calls on one context are
+ * ordered.  Returns after the job shapes are read and the kernels are
+ * queued on the context stream. */
+tci_status tci_rrlu_batched_noalloc_f64(tci_ctx *ctx, tci_lu_job *jobs_device, int64_t count);
+
 /* Pi-tensor evaluation ------------------------------------------------- */
 /* Replaces _batchevaluate_dispatch(Float64, f, localdims, Iset, Jset, Val(M))
  * (batcheval.jl:32-61) for a device functor f (include/tci_functors.h).

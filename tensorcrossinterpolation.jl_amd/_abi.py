@@ -165,6 +165,7 @@ def lib():
     L.tci_batch_profile.argtypes = [vp, C.c_int32, P(C.c_double)]
     L.tci_capacity_plan.argtypes = [P(tci_problem), P(C.c_int64), P(C.c_int64)]
     L.tci_ctx_set_rrlu_algo.argtypes = [vp, C.c_int32]
+    L.tci_rrlu_batched_noalloc_f64.argtypes = [vp, vp, C.c_int64]
     L.tci_tt_evaluate_f64.argtypes = [vp, C.c_int32, P(C.c_int64), P(C.c_int32), P(C.c_double), C.c_int64,
                                       P(C.c_uint8), P(C.c_double)]
     L.tci_batch_evaluate.argtypes = [vp, C.c_int32, C.c_int64, P(C.c_uint8), P(C.c_double), P(C.c_double),
@@ -181,5 +182,5 @@ EXPORTED_SYMBOLS = [
     "tci_batch_history", "tci_batch_linkdims", "tci_batch_indexset", "tci_batch_sitetensor",
     "tci_batch_pivoterrors", "tci_batch_bonderrors", "tci_batch_free", "tci_batch_elapsed_ms",
     "tci_batch_profile", "tci_capacity_plan", "tci_tt_evaluate_f64", "tci_batch_evaluate",
-    "tci_ctx_set_rrlu_algo",
+    "tci_ctx_set_rrlu_algo", "tci_rrlu_batched_noalloc_f64",
 ]

@@ -11,6 +11,7 @@
 
 #include <string>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "../../include/tci.h"
@@ -145,6 +146,41 @@ struct tci_batch {
     hipStream_t stream = nullptr;
 };
 
+// ---- scratch of tci_rrlu_batched_noalloc_f64, the entry point that
+// allocates nothing.  A file-scope device array: every GPU the library runs
+// on holds its own copy, at its own address, until the process ends.  Host
+// code never passes the array's name to a kernel or a copy (it is a host-side
+// stand-in there); it uses the per-device address that hipGetSymbolAddress
+// returns, looked up once per device at context creation.
+static constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static uint8_t *g_scratch_ptr[kMaxDevices];
+
+// with `device` current: this device's address of `scratch` (looked up once)
+static tci_status scratch_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return set_err(TCI_ERR_DEVICE, "device %d is beyond the %d devices the scratch table holds", device,
+                       kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return TCI_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return set_err(TCI_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[device] = (uint8_t *)p;
+    return TCI_OK;
+}
+
+static uint8_t *scratch_of(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
 extern "C" {
 
 const char *tci_last_error(void) { return g_err.c_str(); }
@@ -164,6 +200,8 @@ tci_status tci_ctx_create(tci_ctx **out, int32_t device, uint64_t hbm_budget_byt
         return set_err(TCI_ERR_DEVICE, "device %d is %s; this library is built for gfx950 only",
                        device, prop.gcnArchName);
     HIPCHK(hipSetDevice(device));
+    const tci_status ss = scratch_setup(device);
+    if (ss != TCI_OK) return ss;
     tci_ctx *c = new tci_ctx();
     c->device = device;
     c->used = 0;
@@ -288,7 +326,10 @@ struct LuPlan {
     int64_t mx = 0;       // largest side of any job
 };
 
-static tci_status lu_plan(tci_ctx *ctx, tci_lu_job *jobs, int64_t count, int64_t tile_budget, LuPlan &P)
+static tci_status lu_partition(tci_ctx *ctx, int64_t tile_budget, LuPlan &P);
+
+// reads and checks the job table (and the largest side), partitions it
+static tci_status lu_read(tci_ctx *ctx, tci_lu_job *jobs, int64_t count, LuPlan &P)
 {
     P.h.resize((size_t)count);
     HIPCHK(hipMemcpyAsync(P.h.data(), jobs, sizeof(tci_lu_job) * (size_t)count, hipMemcpyDeviceToHost,
@@ -303,6 +344,18 @@ static tci_status lu_plan(tci_ctx *ctx, tci_lu_job *jobs, int64_t count, int64_t
         int64_t s = j.m > j.n ? j.m : j.n;
         P.mx = s > P.mx ? s : P.mx;
     }
+    return TCI_OK;
+}
+
+static tci_status lu_plan(tci_ctx *ctx, tci_lu_job *jobs, int64_t count, int64_t tile_budget, LuPlan &P)
+{
+    const tci_status s = lu_read(ctx, jobs, count, P);
+    return s != TCI_OK ? s : lu_partition(ctx, tile_budget, P);
+}
+
+static tci_status lu_partition(tci_ctx *ctx, int64_t tile_budget, LuPlan &P)
+{
+    const int64_t count = (int64_t)P.h.size();
     for (int64_t q = 0; q < count; ++q) {
         const tci_lu_job &j = P.h[(size_t)q];
         const bool fits = j.m >= 1 && j.n >= 1 && j.m <= kTileSide && j.n <= kTileSide;
@@ -396,6 +449,63 @@ tci_status tci_rrlu_batched_f64(tci_ctx *ctx, tci_lu_job *jobs, int64_t count)
     (void)hipFreeAsync(dorder, ctx->stream);
     if (gstage) (void)hipFreeAsync(gstage, ctx->stream);
     if (e != hipSuccess) return set_err(TCI_ERR_DEVICE, "rrlu launch: %s", hipGetErrorString(e));
+    return TCI_OK;
+}
+
+// tci_rrlu_batched_f64 without any device allocation: the job indices, the
+// streaming order, the staging vectors and the register-tile scratch live in
+// this device's `scratch` (kNoAllocScratchBytes).  Layout (offsets from the
+// scratch base, each part 256-byte aligned): tile job indices (int32 ×
+// count), their tile-scratch offsets (int64 × count), the streaming order
+// (int32 × count), the staging vectors (2·side doubles per job, only when a
+// side exceeds the LDS staging), then the tile scratch.  Every part is
+// written (copies or kernels) before it is read, so nothing is cleared.
+tci_status tci_rrlu_batched_noalloc_f64(tci_ctx *ctx, tci_lu_job *jobs, int64_t count)
+{
+    if (!ctx || (!jobs && count > 0) || count < 0) return set_err(TCI_ERR_ARG, "bad arguments");
+    if (count == 0) return TCI_OK;
+    if (count > INT32_MAX) return set_err(TCI_ERR_ARG, "too many jobs");
+    HIPCHK(hipSetDevice(ctx->device));
+    uint8_t *base = scratch_of(ctx->device);
+    if (!base) return set_err(TCI_ERR_DEVICE, "device %d has no scratch (create its context first)", ctx->device);
+    LuPlan P;
+    tci_status ps = lu_read(ctx, jobs, count, P);
+    if (ps != TCI_OK) return ps;
+    const size_t o_idx = 0;
+    const size_t o_soff = o_idx + lay_align(sizeof(int32_t) * (size_t)count);
+    const size_t o_order = o_soff + lay_align(sizeof(int64_t) * (size_t)count);
+    const size_t o_stage = o_order + lay_align(sizeof(int32_t) * (size_t)count);
+    const bool staged = P.mx > kLuLdsStageCap;
+    const size_t stage_bytes = staged ? sizeof(double) * 2 * (size_t)P.mx * (size_t)count : 0;
+    const size_t o_tile = o_stage + lay_align(stage_bytes);
+    if (o_tile > kNoAllocScratchBytes)
+        return set_err(TCI_ERR_ARG, "this batch needs %zu bytes of scratch; the limit is %zu bytes per device",
+                       o_tile, kNoAllocScratchBytes);
+    // tile jobs while their scratch fits, the others stream (same bits)
+    ps = lu_partition(ctx, (int64_t)((kNoAllocScratchBytes - o_tile) / sizeof(double)), P);
+    if (ps != TCI_OK) return ps;
+    std::vector<int32_t> order;
+    stream_order(P, order);
+    hipStream_t st = ctx->stream;
+    hipError_t e = hipSuccess;
+    if (!P.tile.empty()) {
+        e = hipMemcpyAsync(base + o_idx, P.tile.data(), sizeof(int32_t) * P.tile.size(), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(base + o_soff, P.soff.data(), sizeof(int64_t) * P.soff.size(), hipMemcpyHostToDevice, st);
+    }
+    if (e == hipSuccess && !order.empty())
+        e = hipMemcpyAsync(base + o_order, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice, st);
+    // the host vectors must outlive the copies
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess && !P.tile.empty())
+        e = launch_rrlu_tile(jobs, (const int32_t *)(base + o_idx), (int64_t)P.tile.size(),
+                             (double *)(base + o_tile), (const int64_t *)(base + o_soff), st);
+    if (e == hipSuccess && !order.empty()) {
+        const int64_t mx = stream_side(P);
+        e = launch_rrlu(jobs, (int64_t)order.size(), mx, staged ? (double *)(base + o_stage) : nullptr, 2 * mx,
+                        (const int32_t *)(base + o_order), st);
+    }
+    if (e != hipSuccess) return set_err(TCI_ERR_DEVICE, "rrlu (no allocation): %s", hipGetErrorString(e));
     return TCI_OK;
 }
 
