@@ -74,6 +74,19 @@ tci_status tci_rrlu_f64(tci_ctx *ctx, const double *A, int64_t lda, int64_t m, i
                         int64_t *rowperm, int64_t *colperm, double *L, int64_t ldl,
                         double *U, int64_t ldu, int64_t *npivot, double *error);
 
+/* Replaces MatrixLUCI(A; maxrank, reltol, abstol, leftorthogonal) with its
+ * left(luci) and right(luci) (matrixluci.jl:5-7, 40-84): with leftorth,
+ * left = colstimespivotinv (m×p, rows of [I; L21·L11⁻¹] in original order)
+ * and right = rowmatrix (L11·U, p×n); otherwise left = colmatrix (L·U11,
+ * m×p) and right = pivotinvtimesrows ([I U11⁻¹·U12], columns in original
+ * order).  The factorization is tci_rrlu_f64's; rowindices/colindices are
+ * rowperm[1..p]/colperm[1..p].  Synchronous; host buffers; left has leading
+ * dimension ldleft >= m, right ldright >= min(m, n, maxrank). */
+tci_status tci_luci_f64(tci_ctx *ctx, const double *A, int64_t lda, int64_t m, int64_t n,
+                        int64_t maxrank, double reltol, double abstol, int32_t leftorth,
+                        int64_t *rowperm, int64_t *colperm, double *left, int64_t ldleft,
+                        double *right, int64_t ldright, int64_t *npivot, double *error);
+
 /* One factorization job of a batch (device memory).  A is overwritten with
  * the packed factors exactly as rrlu!(A) leaves it (matrixlu.jl:194-204):
  * tril(A[:,1:p]) holds L (before its unit diagonal is imposed when
@@ -186,6 +199,27 @@ void tci_options_default(tci_options *o);
 /* One TCI problem: crossinterpolate2(Float64, f, localdims, initialpivots;
  * opts...) (tensorci2.jl:943-953).  Pointers are host memory, copied at
  * submit time. */
+/* A solve's starting state instead of initial pivots: the pivot sets of
+ * TensorCI2{T}(f, localdims, Iset, Jset) (tensorci2.jl:58-72), or the state
+ * of an existing TensorCI2 that optimize! continues from (tensorci2.jl:700;
+ * only the last history entry is ever read, sweep2site! 874-877).  Host
+ * memory, copied at submit.  Set b (0-based site) holds nI[b] multi-indices
+ * of length b (Iset) and nJ[b] of length L-1-b (Jset), 1-based bytes,
+ * entries back to back and sets back to back in site order. */
+typedef struct tci_state {
+    const int32_t *nI;             /* L counts                                 */
+    const uint8_t *Iset;
+    const int32_t *nJ;             /* L counts                                 */
+    const uint8_t *Jset;
+    const int32_t *nI_hist;        /* Iset_history[end], Jset_history[end] in  */
+    const uint8_t *Iset_hist;      /* the same layout; NULL counts: no history */
+    const int32_t *nJ_hist;
+    const uint8_t *Jset_hist;
+    double maxsamplevalue;         /* > 0: continue with it (optimize!); 0: the
+                                      constructor's max |f| over every
+                                      Iset[b] ⊕ σ ⊕ Jset[b] (tensorci2.jl:67-69) */
+} tci_state;
+
 typedef struct tci_problem {
     int32_t L;
     const int32_t *localdims;
@@ -193,6 +227,9 @@ typedef struct tci_problem {
     int32_t npivots;
     const int32_t *initialpivots;  /* npivots × L, 1-based; NULL = [ones(L)]   */
     tci_options opts;
+    const tci_state *state;        /* NULL: start from the initial pivots
+                                      (tensorci2.jl:42-53); else from this state
+                                      and the initial pivots are ignored      */
 } tci_problem;
 
 typedef struct tci_batch tci_batch;
@@ -235,7 +272,9 @@ tci_status tci_batch_history(tci_batch *b, int32_t i, int64_t *ranks, double *er
 /* linkdims[L-1] = |Iset[b+1]| (tensorci2.jl:83-85) */
 tci_status tci_batch_linkdims(tci_batch *b, int32_t i, int64_t *linkdims);
 /* which = 0: Iset[site] (entries of length site-1), 1: Jset[site] (length
- * L-site); site is 1-based.  out receives count×len bytes, 1-based values. */
+ * L-site), 2 and 3: Iset_history[end][site] and Jset_history[end][site]
+ * (tensorci2.jl:12-13; count 0 when there is no history); site is 1-based.
+ * out receives count×len bytes, 1-based values. */
 tci_status tci_batch_indexset(tci_batch *b, int32_t i, int32_t which, int32_t site,
                               int32_t *count, uint8_t *out, int64_t cap_bytes);
 /* site tensor T[site] (χ_{site-1} × d × χ_site column-major). */

@@ -24,11 +24,12 @@ from ._abi import (F_GAUSS_SUM, F_LORENTZ_INT, F_LORENTZ_WEIGHTED, F_QUANTICS_EX
                    F_QUANTICS_GAUSS_COS, F_SUM, F_TABLE, F_TT, TCI_UNLIMITED, ProblemHolder)
 
 __all__ = [
-    "TCIError", "ArgumentError", "Context", "default_context", "rrLU", "rrlu", "rrlu_batched",
+    "TCIError", "ArgumentError", "Context", "default_context", "rrLU", "rrlu", "rrlu_batched", "MatrixLUCI",
     "left", "right", "rowindices", "colindices", "npivots", "pivoterrors", "lastpivoterror",
     "DeviceFunctor", "Lorentz", "WeightedLorentz", "QuanticsGaussCos", "GaussSum", "SumFunctor",
     "TableFunctor", "TTFunctor", "QuanticsExp", "batchevaluate", "TensorCI2", "crossinterpolate2",
     "crossinterpolate2_batch", "TCI_UNLIMITED", "capacity_plan", "tt_evaluate", "sample_points",
+    "tensorci2_from_sets", "optimize",
 ]
 
 
@@ -147,8 +148,58 @@ def rrlu(A, maxrank=TCI_UNLIMITED, reltol=1e-14, abstol=0.0, leftorthogonal=True
                 bool(leftorthogonal), pv, err.value)
 
 
-def left(lu: rrLU, permute=True):
-    """left(lu; permute) (matrixlu.jl:374-382)."""
+class MatrixLUCI:
+    """MatrixLUCI(A; maxrank, reltol, abstol, leftorthogonal) (matrixluci.jl:1-7)
+    with left(luci)/right(luci) (matrixluci.jl:40-84) computed on the device
+    (tci_luci_f64): with leftorthogonal, left = colstimespivotinv and
+    right = rowmatrix; otherwise left = colmatrix and right =
+    pivotinvtimesrows."""
+
+    def __init__(self, A, maxrank=TCI_UNLIMITED, reltol=1e-14, abstol=0.0, leftorthogonal=True, ctx=None):
+        ctx = ctx or default_context()
+        A = np.asfortranarray(A, dtype=np.float64)
+        m, n = A.shape
+        cap = max(min(m, n, int(maxrank)), 0)
+        rp = np.zeros(max(m, 1), np.int64)
+        cp = np.zeros(max(n, 1), np.int64)
+        lb = np.zeros((max(m, 1), max(cap, 1)), order="F")
+        rb = np.zeros((max(cap, 1), max(n, 1)), order="F")
+        p = C.c_int64()
+        err = C.c_double()
+        _check(_abi.lib().tci_luci_f64(ctx.handle, _abi.dptr(A) if A.size else None, max(m, 1), m, n,
+                                       int(maxrank), float(reltol), float(abstol), int(bool(leftorthogonal)),
+                                       _abi.i64ptr(rp), _abi.i64ptr(cp), _abi.dptr(lb), max(m, 1),
+                                       _abi.dptr(rb), max(cap, 1), C.byref(p), C.byref(err)))
+        pv = p.value
+        self.shape = (m, n)
+        self.rowpermutation, self.colpermutation = rp[:m].copy(), cp[:n].copy()
+        self.leftorthogonal = bool(leftorthogonal)
+        self.npivot, self.error = pv, err.value
+        self._left, self._right = np.array(lb[:m, :pv]), np.array(rb[:pv, :n])
+
+    def left(self):
+        return self._left
+
+    def right(self):
+        return self._right
+
+    def npivots(self):
+        return self.npivot
+
+    def rowindices(self):
+        return self.rowpermutation[:self.npivot]
+
+    def colindices(self):
+        return self.colpermutation[:self.npivot]
+
+    def lastpivoterror(self):
+        return self.error
+
+
+def left(lu, permute=True):
+    """left(lu; permute) (matrixlu.jl:374-382), or left(luci) (matrixluci.jl:70-76)."""
+    if isinstance(lu, MatrixLUCI):
+        return lu.left()
     if not permute:
         return lu.L
     out = np.empty_like(lu.L)
@@ -156,8 +207,10 @@ def left(lu: rrLU, permute=True):
     return out
 
 
-def right(lu: rrLU, permute=True):
-    """right(lu; permute) (matrixlu.jl:384-392)."""
+def right(lu, permute=True):
+    """right(lu; permute) (matrixlu.jl:384-392), or right(luci) (matrixluci.jl:78-84)."""
+    if isinstance(lu, MatrixLUCI):
+        return lu.right()
     if not permute:
         return lu.U
     out = np.empty_like(lu.U)
@@ -407,9 +460,10 @@ class TensorCI2:
         lib.tci_batch_bonderrors(batch, i, _abi.dptr(be))
         self.bonderrors = be[:L - 1]
         self.Iset, self.Jset = [], []
-        for which, dst in ((0, self.Iset), (1, self.Jset)):
+        hI, hJ = [], []
+        for which, dst in ((0, self.Iset), (1, self.Jset), (2, hI), (3, hJ)):
             for site in range(1, L + 1):
-                ln = site - 1 if which == 0 else L - site
+                ln = site - 1 if which % 2 == 0 else L - site
                 cnt = C.c_int32()
                 buf = np.zeros(1 << 16, np.uint8)
                 lib.tci_batch_indexset(batch, i, which, site, C.byref(cnt), _abi.u8ptr(buf), len(buf))
@@ -417,6 +471,11 @@ class TensorCI2:
                     buf = np.zeros(cnt.value * ln, np.uint8)
                     lib.tci_batch_indexset(batch, i, which, site, C.byref(cnt), _abi.u8ptr(buf), len(buf))
                 dst.append([list(map(int, buf[e * ln:(e + 1) * ln])) for e in range(cnt.value)])
+        # Iset_history / Jset_history (tensorci2.jl:12-13): only the last
+        # entry is ever read (sweep2site!, 874-877), so only it is kept
+        has_hist = any(len(s) for s in hI)
+        self.Iset_history = [hI] if has_hist else []
+        self.Jset_history = [hJ] if has_hist else []
         self.sitetensors = []
         if self.status == 0:
             for site in range(1, L + 1):
@@ -471,9 +530,48 @@ def _holder(f, localdims, initialpivots, kw):
     kw.pop("checkbatchevaluatable", None)   # device functors are batch evaluators
     kw.pop("verbosity", None)
     kw.pop("loginterval", None)
+    state = kw.pop("_state", None)
     if not isinstance(f, DeviceFunctor):
         raise TypeError("f must be a DeviceFunctor (a Julia/Python closure cannot run on the GPU)")
-    return ProblemHolder(localdims, f.kind, f.params, f.iparams, initialpivots, tolerance=tol, **kw)
+    return ProblemHolder(localdims, f.kind, f.params, f.iparams, initialpivots, state=state, tolerance=tol, **kw)
+
+
+def _state_of(tci, maxsamplevalue):
+    L = len(tci.localdims)
+    hist = (tci.Iset_history[-1], tci.Jset_history[-1]) if tci.Iset_history else (None, None)
+    return _abi.StateHolder(L, tci.Iset, tci.Jset, hist[0], hist[1], maxsamplevalue)
+
+
+def tensorci2_from_sets(f, localdims, Iset, Jset, ctx=None):
+    """TensorCI2{Float64}(f, localdims, Iset, Jset) (tensorci2.jl:58-72): a
+    TCI with the given pivot sets, no history, and maxsamplevalue the
+    maximum of |f| over every Iset[b] ⊕ σ ⊕ Jset[b], evaluated on the device.
+    Iset[b]/Jset[b] are lists of 1-based multi-indices (b = 0..L-1)."""
+    L = len(localdims)
+    if len(Iset) != L or len(Jset) != L:
+        raise ArgumentError(3, "Iset and Jset need one set per site")
+    st = _abi.StateHolder(L, Iset, Jset, None, None, 0.0)
+    (t,) = crossinterpolate2_batch([(f, localdims, None, dict(maxiter=0, flags=_abi.FLAG_NO_FINAL_SWEEP,
+                                                             _state=st))], ctx=ctx)
+    if t.status != 0:
+        _raise(t.status, _MESSAGES.get(t.status, ""))
+    return t
+
+
+def optimize(tci, f, ctx=None, **kwargs):
+    """optimize!(tci, f; kwargs...) (tensorci2.jl:700-850) continuing from an
+    existing TensorCI2 (its Iset, Jset, last history entry and
+    maxsamplevalue), on the device.  Updates tci in place, as the reference
+    mutates it, and returns (ranks, errors) of this call."""
+    kw = dict(kwargs)
+    kw["_state"] = _state_of(tci, tci.maxsamplevalue)
+    (t,) = crossinterpolate2_batch([(f, tci.localdims, None, kw)], ctx=ctx)
+    if t.status != 0:
+        _raise(t.status, _MESSAGES.get(t.status, ""))
+    for k in ("Iset", "Jset", "Iset_history", "Jset_history", "sitetensors", "pivoterrors", "bonderrors",
+              "maxsamplevalue", "status", "niter", "ranks", "errors", "nglobalpivots", "workstats"):
+        setattr(tci, k, getattr(t, k))
+    return t.ranks, t.errors
 
 
 def capacity_plan(holder):

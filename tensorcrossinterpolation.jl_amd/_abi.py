@@ -35,6 +35,7 @@ F_SUM = 5
 F_TABLE = 6
 F_TT = 7
 F_QUANTICS_EXP = 8
+FLAG_NO_FINAL_SWEEP = 1        # TCI_FLAG_NO_FINAL_SWEEP
 
 
 class tci_functor(C.Structure):
@@ -51,10 +52,46 @@ class tci_options(C.Structure):
                 ("seed", C.c_uint64), ("chi_cap", C.c_int32), ("flags", C.c_int32)]
 
 
+class tci_state(C.Structure):
+    _fields_ = [("nI", C.POINTER(C.c_int32)), ("Iset", C.POINTER(C.c_uint8)),
+                ("nJ", C.POINTER(C.c_int32)), ("Jset", C.POINTER(C.c_uint8)),
+                ("nI_hist", C.POINTER(C.c_int32)), ("Iset_hist", C.POINTER(C.c_uint8)),
+                ("nJ_hist", C.POINTER(C.c_int32)), ("Jset_hist", C.POINTER(C.c_uint8)),
+                ("maxsamplevalue", C.c_double)]
+
+
 class tci_problem(C.Structure):
     _fields_ = [("L", C.c_int32), ("localdims", C.POINTER(C.c_int32)), ("f", tci_functor),
                 ("npivots", C.c_int32), ("initialpivots", C.POINTER(C.c_int32)),
-                ("opts", tci_options)]
+                ("opts", tci_options), ("state", C.POINTER(tci_state))]
+
+
+def _pack_sets(sets, L, iside):
+    """L sets of multi-indices → (counts int32[L], bytes back to back)."""
+    counts = np.array([len(s) for s in sets], np.int32)
+    chunks = []
+    for b, s in enumerate(sets):
+        ln = b if iside else L - 1 - b
+        if len(s) and ln:
+            chunks.append(np.asarray(s, np.uint8).reshape(len(s), ln).ravel())
+    flat = np.concatenate(chunks) if chunks else np.zeros(1, np.uint8)
+    return np.ascontiguousarray(counts), np.ascontiguousarray(flat)
+
+
+class StateHolder:
+    """Owns the buffers of a tci_state: the Iset/Jset of every site, the last
+    history entry (or none) and maxsamplevalue (0: computed at the start)."""
+
+    def __init__(self, L, Iset, Jset, Iset_hist=None, Jset_hist=None, maxsamplevalue=0.0):
+        self.nI, self.I = _pack_sets(Iset, L, True)
+        self.nJ, self.J = _pack_sets(Jset, L, False)
+        self.st = tci_state(i32ptr(self.nI), u8ptr(self.I), i32ptr(self.nJ), u8ptr(self.J),
+                            None, None, None, None, float(maxsamplevalue))
+        if Iset_hist is not None:
+            self.nIh, self.Ih = _pack_sets(Iset_hist, L, True)
+            self.nJh, self.Jh = _pack_sets(Jset_hist, L, False)
+            self.st.nI_hist, self.st.Iset_hist = i32ptr(self.nIh), u8ptr(self.Ih)
+            self.st.nJ_hist, self.st.Jset_hist = i32ptr(self.nJh), u8ptr(self.Jh)
 
 
 class tci_result(C.Structure):
@@ -104,7 +141,9 @@ def make_options(tolerance=1e-8, maxbonddim=TCI_UNLIMITED, maxiter=20, sweepstra
 class ProblemHolder:
     """Owns the numpy buffers a tci_problem points into."""
 
-    def __init__(self, localdims, kind, params=(), iparams=(), initialpivots=None, **opts):
+    def __init__(self, localdims, kind, params=(), iparams=(), initialpivots=None, state=None, **opts):
+        """state: a StateHolder to start from instead of initialpivots."""
+        self.state = state
         self.localdims = np.ascontiguousarray(localdims, np.int32)
         self.L = len(self.localdims)
         self.params = np.ascontiguousarray(np.asarray(params, np.float64).ravel())
@@ -115,7 +154,8 @@ class ProblemHolder:
         f = tci_functor(int(kind), len(self.params), dptr(self.params) if len(self.params) else None,
                         len(self.iparams), i32ptr(self.iparams) if len(self.iparams) else None)
         self.st = tci_problem(self.L, i32ptr(self.localdims), f, self.pivots.shape[0],
-                              i32ptr(self.pivots), make_options(**opts))
+                              i32ptr(self.pivots), make_options(**opts),
+                              C.pointer(state.st) if state is not None else None)
 
 
 _LIB = None
@@ -142,6 +182,7 @@ def lib():
                                C.c_double, C.c_double, C.c_int32, P(C.c_int64), P(C.c_int64),
                                P(C.c_double), C.c_int64, P(C.c_double), C.c_int64, P(C.c_int64),
                                P(C.c_double)]
+    L.tci_luci_f64.argtypes = L.tci_rrlu_f64.argtypes
     L.tci_rrlu_batched_f64.argtypes = [vp, vp, C.c_int64]
     L.tci_pi_eval_f64.argtypes = [vp, P(tci_functor), C.c_int32, P(C.c_int32), P(C.c_uint8),
                                   C.c_int64, C.c_int32, P(C.c_uint8), C.c_int64, C.c_int32,
@@ -177,7 +218,7 @@ def lib():
 EXPORTED_SYMBOLS = [
     "tci_last_error", "tci_version", "tci_ctx_create", "tci_ctx_destroy", "tci_ctx_synchronize",
     "tci_ctx_set_streams", "tci_ctx_stream",
-    "tci_rrlu_f64", "tci_rrlu_batched_f64", "tci_pi_eval_f64", "tci_options_default",
+    "tci_rrlu_f64", "tci_luci_f64", "tci_rrlu_batched_f64", "tci_pi_eval_f64", "tci_options_default",
     "tci_solve_submit", "tci_batch_query", "tci_batch_wait", "tci_batch_result",
     "tci_batch_history", "tci_batch_linkdims", "tci_batch_indexset", "tci_batch_sitetensor",
     "tci_batch_pivoterrors", "tci_batch_bonderrors", "tci_batch_free", "tci_batch_elapsed_ms",

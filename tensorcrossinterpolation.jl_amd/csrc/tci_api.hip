@@ -577,6 +577,76 @@ tci_status tci_rrlu_f64(tci_ctx *ctx, const double *A, int64_t lda, int64_t m, i
     return TCI_OK;
 }
 
+tci_status tci_luci_f64(tci_ctx *ctx, const double *A, int64_t lda, int64_t m, int64_t n, int64_t maxrank,
+                        double reltol, double abstol, int32_t leftorth, int64_t *rowperm, int64_t *colperm,
+                        double *left, int64_t ldleft, double *right, int64_t ldright, int64_t *npivot,
+                        double *error)
+{
+    if (!ctx || m < 0 || n < 0 || lda < (m > 0 ? m : 1) || maxrank < 0 || !npivot || !error)
+        return set_err(TCI_ERR_ARG, "bad arguments");
+    if ((m > 0 && n > 0 && !A) || (m > 0 && !rowperm) || (n > 0 && !colperm))
+        return set_err(TCI_ERR_ARG, "NULL buffer");
+    int64_t cap = m < n ? m : n;
+    if (maxrank < cap) cap = maxrank;
+    if ((cap > 0 && m > 0 && (!left || ldleft < m)) || (cap > 0 && n > 0 && (!right || ldright < cap)))
+        return set_err(TCI_ERR_ARG, "left/right buffers too small");
+    if (m * n > ((int64_t)1 << 31)) return set_err(TCI_ERR_ARG, "matrix too large");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const size_t mn = (size_t)(m * n);
+    double *dA = nullptr, *dLU = nullptr;
+    int32_t *dperm = nullptr;
+    tci_lu_job *dj = nullptr;
+    HIPCHK(hipMallocAsync((void **)&dA, sizeof(double) * (mn ? mn : 1), st));
+    HIPCHK(hipMallocAsync((void **)&dperm, sizeof(int32_t) * (size_t)(m + n + 1), st));
+    HIPCHK(hipMallocAsync((void **)&dj, sizeof(tci_lu_job), st));
+    if (mn) HIPCHK(hipMemcpy2DAsync(dA, sizeof(double) * m, A, sizeof(double) * lda, sizeof(double) * m, n,
+                                    hipMemcpyHostToDevice, st));
+    tci_lu_job j;
+    memset(&j, 0, sizeof j);
+    j.A = dA;
+    j.rowperm = dperm;
+    j.colperm = dperm + m;
+    j.m = m;
+    j.n = n;
+    j.maxrank = maxrank;
+    j.reltol = reltol;
+    j.abstol = abstol;
+    j.leftorth = leftorth ? 1 : 0;
+    HIPCHK(hipMemcpyAsync(dj, &j, sizeof j, hipMemcpyHostToDevice, st));
+    tci_status s = tci_rrlu_batched_f64(ctx, dj, 1);
+    if (s != TCI_OK) return s;
+    HIPCHK(hipMemcpyAsync(&j, dj, sizeof j, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const int64_t p = j.npivot;
+    if (j.status == TCI_OK && p > 0) {
+        // unpacked L, U, then left and right (device), then host copies
+        const size_t nl = (size_t)(m * p), nu = (size_t)(p * n);
+        HIPCHK(hipMallocAsync((void **)&dLU, sizeof(double) * (2 * (nl + nu) + 1), st));
+        double *dL = dLU, *dU = dLU + nl, *dleft = dU + nu, *dright = dleft + nl;
+        HIPCHK(launch_lu_unpack(dA, m, n, p, j.leftorth, dL, m, dU, p, st));
+        HIPCHK(launch_luci(m, n, p, j.leftorth, dperm, dperm + m, dL, dU, dleft, m, dright, p, st));
+        HIPCHK(hipMemcpy2DAsync(left, sizeof(double) * ldleft, dleft, sizeof(double) * m, sizeof(double) * m, p,
+                                hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpy2DAsync(right, sizeof(double) * ldright, dright, sizeof(double) * p, sizeof(double) * p, n,
+                                hipMemcpyDeviceToHost, st));
+    }
+    std::vector<int32_t> perm((size_t)(m + n + 1));
+    HIPCHK(hipMemcpyAsync(perm.data(), dperm, sizeof(int32_t) * (size_t)(m + n), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int64_t i = 0; i < m; ++i) rowperm[i] = perm[(size_t)i];
+    for (int64_t q = 0; q < n; ++q) colperm[q] = perm[(size_t)(m + q)];
+    (void)hipFreeAsync(dA, st);
+    (void)hipFreeAsync(dperm, st);
+    (void)hipFreeAsync(dj, st);
+    if (dLU) (void)hipFreeAsync(dLU, st);
+    *npivot = p;
+    *error = j.error;
+    if (j.status == TCI_ERR_NAN_L) return set_err(TCI_ERR_NAN_L, "lu.L contains NaNs");
+    if (j.status == TCI_ERR_NAN_U) return set_err(TCI_ERR_NAN_U, "lu.U contains NaNs");
+    return TCI_OK;
+}
+
 // ------------------------------------------------------------ Pi tensors
 tci_status tci_pi_eval_f64(tci_ctx *ctx, const tci_functor *f, int32_t L, const int32_t *localdims,
                            const uint8_t *left, int64_t nI, int32_t nl, const uint8_t *right,
@@ -680,7 +750,8 @@ tci_status tci_capacity_plan(const tci_problem *p, int64_t *chi_used, int64_t *c
     std::string msg;
     int s = check_problem(*p, c, &msg);
     if (s != TCI_OK) return set_err((tci_status)s, "%s", msg.c_str());
-    const int64_t used = p->opts.chi_cap > 0 ? p->opts.chi_cap : default_chi_cap(*p);
+    int64_t used = p->opts.chi_cap > 0 ? p->opts.chi_cap : default_chi_cap(*p);
+    if (p->state && largest_state_set(*p) > used) used = largest_state_set(*p);
     *chi_used = used < 1 ? 1 : used;
     int64_t lim = rank_bound(p->L, p->localdims);
     if (p->opts.maxbonddim < lim) lim = p->opts.maxbonddim;
@@ -886,11 +957,15 @@ tci_status tci_batch_indexset(tci_batch *b, int32_t i, int32_t which, int32_t si
                               uint8_t *out, int64_t cap_bytes)
 {
     SolveDesc R;
-    if (!count || !report_view(b, i, R) || site < 1 || site > R.L || which < 0 || which > 1)
+    if (!count || !report_view(b, i, R) || site < 1 || site > R.L || which < 0 || which > 3)
         return set_err(TCI_ERR_ARG, "bad arguments");
-    const int L = R.L, b0 = site - 1, kind = which ? SET_J : SET_I;
-    const int len = which ? L - 1 - b0 : b0;
-    const int n = R.setn[kind * L + b0];
+    const int L = R.L, b0 = site - 1, jside = which & 1;
+    // which 2/3: the last history entry, in the slot the solve wrote last
+    const int hist = R.ints[INT_HIST];
+    const int kind = which < 2 ? (jside ? SET_J : SET_I)
+                               : (hist == 2 ? (jside ? SET_HB_J : SET_HB_I) : (jside ? SET_HA_J : SET_HA_I));
+    const int len = jside ? L - 1 - b0 : b0;
+    const int n = (which >= 2 && hist == 0) ? 0 : R.setn[kind * L + b0];
     *count = n;
     if ((int64_t)n * len > cap_bytes || (!out && n * len > 0)) return set_err(TCI_ERR_ARG, "output buffer too small");
     if (n == 0 || len == 0) return TCI_OK;

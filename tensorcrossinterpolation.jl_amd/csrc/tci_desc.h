@@ -69,6 +69,10 @@ struct SolveDesc {
     const double *params;
     const int32_t *iparams;
     const uint8_t *initpivots;   // npivinit × L, 1-based
+    const uint8_t *initsets;     // starting pivot sets in the set arena's layout,
+                                 // then their NSETKIND·L int32 counts; null: none
+    double init_maxsample;       // > 0: the starting maxsamplevalue
+    int32_t init_hist;           // the starting sets include a history entry (slot A)
     // ---- state (device) ----
     uint8_t *sets;           // 6L sets × setcap entries × L bytes
     int32_t *setn;           // 6L counts

@@ -1472,8 +1472,22 @@ struct Engine {
         }
         tm.sync();
         build_lut();
-        int st = addglobalpivots(D.initpivots, D.npivinit);
+        int st = D.initsets ? load_state() : start_from_pivots();
         if (st != ST_OK) return finish(st);
+        // optimize! argument checks (730-754)
+        if (D.nsearchglobalpivot > 0 && D.nsearchglobalpivot < D.maxnglobalpivot) return finish(ST_GENERIC);
+        const double tol = D.tolerance;
+        if (D.maxbonddim == INT64_MAX && tol <= 0) return finish(ST_ARG);
+        if (tm.tid() == 0) eng_rng_seed(S.rng, D.seed);
+        tm.sync();
+        return finish(optimize(tol));
+    }
+
+    // TensorCI2{T}(f, localdims, initialpivots) (tensorci2.jl:42-53)
+    ENG_INLINE int start_from_pivots()
+    {
+        int st = addglobalpivots(D.initpivots, D.npivinit);
+        if (st != ST_OK) return st;
         // maxsamplevalue = maximum(abs, f(x) for x in initialpivots) (49-50)
         if (tm.tid() == 0) {
             double mx = 0.0;
@@ -1487,19 +1501,50 @@ struct Engine {
             if (!(fabs(mx) > 0.0)) S.status = ST_ZERO_MAXSAMPLE;
         }
         tm.sync();
-        if (S.status) return finish(S.status);
-        // optimize! argument checks (730-754)
-        if (D.nsearchglobalpivot > 0 && D.nsearchglobalpivot < D.maxnglobalpivot) return finish(ST_GENERIC);
-        const double tol = D.tolerance;
-        if (D.maxbonddim == INT64_MAX && tol <= 0) return finish(ST_ARG);
-        if (tm.tid() == 0) eng_rng_seed(S.rng, D.seed);
+        return S.status;
+    }
+
+    // TensorCI2{T}(f, localdims, Iset, Jset) (tensorci2.jl:58-72), or the
+    // state of an existing TensorCI2 that optimize! continues from: the sets
+    // (and the last history entry) from the input image; maxsamplevalue as
+    // given, else the maximum of |f| over every Iset[b] ⊕ σ ⊕ Jset[b], the
+    // points reconstractglobalpivotsfromijset enumerates (171-190)
+    ENG_INLINE int load_state()
+    {
+        const int64_t per = (int64_t)D.setcap * L;
+        const int32_t *cn = (const int32_t *)(D.initsets + (int64_t)NSETKIND * L * per);
+        for (int q = tm.tid(); q < NSETKIND * L; q += tm.nt()) setcnt[q] = cn[q];
+        for (int64_t x = tm.tid(); x < (int64_t)NSETKIND * L * per; x += tm.nt()) sets[x] = D.initsets[x];
+        if (tm.tid() == 0) {
+            D.ints[INT_HIST] = D.init_hist ? 1 : 0;
+            S.maxsample = D.init_maxsample;
+        }
         tm.sync();
+        if (D.init_maxsample > 0.0) return ST_OK;
+        for (int b = 0; b < L; ++b) {
+            const int m = kron_rows(b);
+            if (m > D.mrcap) return fail(ST_CAPACITY);
+            load_set(cols, SET_J, b, jlen(b));
+            tm.sync();
+            eval(rows, b + 1, m, cols, setn(SET_J, b), nullptr, 0, false, true);
+        }
+        if (tm.tid() == 0 && !(fabs(S.maxsample) > 0.0)) S.status = ST_ZERO_MAXSAMPLE;
+        tm.sync();
+        return S.status;
+    }
+
+    // optimize! (tensorci2.jl:700-850) from the current state: the 2-site
+    // sweeps, the global search and the convergence test, then the final
+    // sweep1site! and _sanitycheck
+    ENG_INLINE int optimize(double tol)
+    {
+        int st = ST_OK;
         int niter = 0;
         for (int iter = 1; iter <= D.maxiter; ++iter) {
             const double norm = D.normalizeerror ? S.maxsample : 1.0;
             const double abstol = tol * norm;
             st = sweep2site(abstol);
-            if (st != ST_OK) return finish(st);
+            if (st != ST_OK) return st;
             double perr = D.bonderrors[0];
             for (int b = 1; b < L - 1; ++b) perr = jl_max(perr, D.bonderrors[b]);
             int ng = 0;
@@ -1507,7 +1552,7 @@ struct Engine {
                 const uint64_t t0 = tm.clock();
                 ng = globalsearch(abstol);
                 if (tm.tid() == 0) S.tph[PH_GSEARCH] += tm.clock() - t0;
-                if (ng < 0) return finish(S.status);
+                if (ng < 0) return S.status;
             }
             if (tm.tid() == 0) {
                 if (niter < D.maxiter_cap) {
@@ -1531,17 +1576,17 @@ struct Engine {
             const uint64_t t1 = tm.clock();
             st = sweep1site_forward(tol * norm);
             if (tm.tid() == 0) S.tph[PH_SWEEP1] += tm.clock() - t1;
-            if (st != ST_OK) return finish(st);
+            if (st != ST_OK) return st;
             // _sanitycheck (globalsearch.jl:106-111)
             for (int b = 0; b < L - 1; ++b)
                 if (setn(SET_I, b + 1) != setn(SET_J, b)) {
                     if (tm.tid() == 0) D.ints[INT_ERRSITE] = b + 1;
-                    return finish(ST_NONSQUARE);
+                    return ST_NONSQUARE;
                 }
         }
         if (tm.tid() == 0)
             for (int i = 0; i < niter; ++i) D.errors[i] = D.errors[i] / norm;
-        return finish(ST_OK);
+        return ST_OK;
     }
 
     ENG_NOINLINE int finish(int st)

@@ -188,6 +188,84 @@ __global__ void tci_lu_unpack_kernel(const double *__restrict__ A, int64_t m, in
     }
 }
 
+// ---------------------------------------------------------- LUCI factors
+// left(luci) and right(luci) of MatrixLUCI (matrixluci.jl:40-84) from the
+// unpacked factors L (m×p, lda m) and U (p×n, lda p) of one rrLU with
+// 1-based permutations rowperm/colperm (int32).  One thread per row of left
+// (position i, written to row rowperm[i]-1) and one per column of right
+// (position j, written to column colperm[j]-1); each substitution or dot
+// product runs in a fixed order (ascending t for products from 0.0, back to
+// front for the triangular solves).
+__global__ void tci_luci_kernel(int64_t m, int64_t n, int64_t p, int leftorth, const int32_t *__restrict__ rowperm,
+                                const int32_t *__restrict__ colperm, const double *__restrict__ L,
+                                const double *__restrict__ U, double *__restrict__ left, int64_t ldl,
+                                double *__restrict__ right, int64_t ldr)
+{
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < m + n;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        if (q < m) {
+            const int64_t i = q, r = rowperm[i] - 1;
+            double *out = left + r;   // row r of left, stride ldl
+            if (leftorth) {
+                // colstimespivotinv (48-57): [I_p; L21·L11⁻¹]; x·L11 = L[i,1:p]
+                if (i < p) {
+                    for (int64_t j = 0; j < p; ++j) out[ldl * j] = i == j ? 1.0 : 0.0;
+                } else {
+                    for (int64_t j = p - 1; j >= 0; --j) {
+                        double s = L[i + m * j];
+                        for (int64_t t = j + 1; t < p; ++t) s = s - out[ldl * t] * L[t + m * j];
+                        out[ldl * j] = s / L[j + m * j];
+                    }
+                }
+            } else {
+                // colmatrix (40-42): L · U[:,1:p]
+                for (int64_t j = 0; j < p; ++j) {
+                    double s = 0.0;
+                    const int64_t tmax = i < j ? i : j;
+                    for (int64_t t = 0; t <= tmax && t < p; ++t) s = s + L[i + m * t] * U[t + p * j];
+                    out[ldl * j] = s;
+                }
+            }
+        } else {
+            const int64_t j = q - m, c = colperm[j] - 1;
+            double *out = right + ldr * c;   // column c of right
+            if (leftorth) {
+                // rowmatrix (44-46): L[1:p,:] · U
+                for (int64_t i = 0; i < p; ++i) {
+                    double s = 0.0;
+                    const int64_t tmax = i < j ? i : j;
+                    for (int64_t t = 0; t <= tmax; ++t) s = s + L[i + m * t] * U[t + p * j];
+                    out[i] = s;
+                }
+            } else {
+                // pivotinvtimesrows (59-68): [I_p  U11⁻¹·U12]
+                if (j < p) {
+                    for (int64_t i = 0; i < p; ++i) out[i] = i == j ? 1.0 : 0.0;
+                } else {
+                    for (int64_t i = p - 1; i >= 0; --i) {
+                        double s = U[i + p * j];
+                        for (int64_t t = i + 1; t < p; ++t) s = s - U[i + p * t] * out[t];
+                        out[i] = s / U[i + p * i];
+                    }
+                }
+            }
+        }
+    }
+}
+
+hipError_t launch_luci(int64_t m, int64_t n, int64_t p, int leftorth, const int32_t *rowperm, const int32_t *colperm,
+                       const double *L, const double *U, double *left, int64_t ldl, double *right, int64_t ldr,
+                       hipStream_t st)
+{
+    const int64_t tot = m + n;
+    if (tot == 0 || p == 0) return hipSuccess;
+    int64_t blocks = (tot + 255) / 256;
+    if (blocks > 65535) blocks = 65535;
+    hipLaunchKernelGGL(tci_luci_kernel, dim3((unsigned)blocks), dim3(256), 0, st, m, n, p, leftorth, rowperm, colperm,
+                       L, U, left, ldl, right, ldr);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------ evaluation
 // rowsx[r = i + nI·c] = left[i] ⊕ central digits of c (first fastest)
 __global__ void tci_kron_center_kernel(const uint8_t *__restrict__ left, int64_t nI, int nl,

@@ -27,7 +27,15 @@ struct SolveCaps {
     int32_t niparams;
     int32_t nsearch;
     int32_t maxiter;
+    int32_t has_init;   // the solve starts from given pivot sets (tci_state)
 };
+
+// bytes of a starting-state image: the pivot sets in the set arena's own
+// layout (NSETKIND·L sets of setcap entries of L bytes) and their counts
+inline size_t init_image_bytes(const SolveCaps &c)
+{
+    return (size_t)NSETKIND * c.L * c.setcap * c.L + 4 * (size_t)NSETKIND * c.L;
+}
 
 inline size_t lay_align(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -46,6 +54,7 @@ inline void layout_solve(const SolveCaps &c, SolveDesc *D, uint8_t *in, uint8_t 
     size_t o_par = itake((size_t)c.nparams * 8);
     size_t o_ipar = itake((size_t)c.niparams * 4);
     size_t o_piv = itake((size_t)c.npivinit * L);
+    size_t o_init = itake(c.has_init ? init_image_bytes(c) : 0);
     *input_bytes = i;
 
     size_t o = 0;
@@ -94,6 +103,7 @@ inline void layout_solve(const SolveCaps &c, SolveDesc *D, uint8_t *in, uint8_t 
         D->params = (const double *)(in + o_par);
         D->iparams = (const int32_t *)(in + o_ipar);
         D->initpivots = in + o_piv;
+        D->initsets = c.has_init ? in + o_init : nullptr;
     }
     if (base) {
         D->sets = base + o_sets;

@@ -128,6 +128,64 @@ inline int check_functor(const tci_functor &f, int L, const int32_t *localdims, 
     return TCI_OK;
 }
 
+// The four set families of a starting state, in SET_I, SET_J, SET_HA_I,
+// SET_HA_J order (history entries may be absent).
+inline void state_sets(const tci_state &s, const int32_t *cnt[4], const uint8_t *bytes[4])
+{
+    cnt[0] = s.nI; bytes[0] = s.Iset;
+    cnt[1] = s.nJ; bytes[1] = s.Jset;
+    cnt[2] = s.nI_hist; bytes[2] = s.Iset_hist;
+    cnt[3] = s.nJ_hist; bytes[3] = s.Jset_hist;
+}
+
+// The largest set of a starting state (the capacity must hold it).
+inline int64_t largest_state_set(const tci_problem &p)
+{
+    const int32_t *cnt[4];
+    const uint8_t *bytes[4];
+    state_sets(*p.state, cnt, bytes);
+    int64_t big = 0;
+    for (int f = 0; f < 4; ++f)
+        if (cnt[f])
+            for (int b = 0; b < p.L; ++b) big = cnt[f][b] > big ? cnt[f][b] : big;
+    return big;
+}
+
+// A starting state is well formed: counts >= 0 (at least one entry in Iset
+// and Jset), the history given as a pair or not at all, every entry within
+// 1..d of its site, maxsamplevalue >= 0.
+inline int check_state(const tci_problem &p, std::string *msg)
+{
+    auto bad = [&](int code, const char *m) { if (msg) *msg = m; return code; };
+    const tci_state &s = *p.state;
+    const int L = p.L;
+    if (!s.nI || !s.nJ) return bad(TCI_ERR_ARG, "state has no Iset/Jset counts");
+    if ((s.nI_hist == nullptr) != (s.nJ_hist == nullptr)) return bad(TCI_ERR_ARG, "state history must hold both Iset and Jset");
+    if (!(s.maxsamplevalue >= 0.0)) return bad(TCI_ERR_ARG, "state maxsamplevalue must be >= 0");
+    const int32_t *cnt[4];
+    const uint8_t *bytes[4];
+    state_sets(s, cnt, bytes);
+    for (int f = 0; f < 4; ++f) {
+        if (!cnt[f]) continue;
+        const bool iside = (f & 1) == 0;
+        int64_t off = 0;
+        for (int b = 0; b < L; ++b) {
+            const int n = cnt[f][b];
+            if (n < 0 || n > 8192) return bad(TCI_ERR_ARG, "state set size out of range");
+            if (f < 2 && n < 1) return bad(TCI_ERR_ARG, "state Iset/Jset entries must be non-empty");
+            const int len = iside ? b : L - 1 - b;
+            if ((int64_t)n * len > 0 && !bytes[f]) return bad(TCI_ERR_ARG, "state set bytes are NULL");
+            for (int64_t q = 0; q < (int64_t)n * len; ++q) {
+                const int t = (int)(q % len), site = iside ? t : b + 1 + t;
+                const int v = bytes[f][off + q];
+                if (v < 1 || v > p.localdims[site]) return bad(TCI_ERR_ARG, "state index out of range");
+            }
+            off += (int64_t)n * len;
+        }
+    }
+    return TCI_OK;
+}
+
 // Returns TCI_OK or an error with *msg set.
 inline int check_problem(const tci_problem &p, SolveCaps &c, std::string *msg)
 {
@@ -163,6 +221,12 @@ inline int check_problem(const tci_problem &p, SolveCaps &c, std::string *msg)
     if (o.chi_cap < 0) return bad(TCI_ERR_ARG, "chi_cap must be >= 0");
     int64_t chi = o.chi_cap > 0 ? o.chi_cap : default_chi_cap(p);
     if (chi < 1) chi = 1;
+    if (p.state) {
+        const int sst = check_state(p, msg);
+        if (sst != TCI_OK) return sst;
+        const int64_t big = largest_state_set(p);
+        if (big > chi) chi = big;
+    }
     if (chi > 8192) return bad(TCI_ERR_ARG, "chi_cap too large");
     int extra = o.maxnglobalpivot > npiv ? o.maxnglobalpivot : npiv;
     c.L = L;
@@ -175,6 +239,7 @@ inline int check_problem(const tci_problem &p, SolveCaps &c, std::string *msg)
     c.niparams = nip;
     c.nsearch = o.nsearchglobalpivot;
     c.maxiter = o.maxiter > 0 ? o.maxiter : 1;
+    c.has_init = p.state ? 1 : 0;
     return TCI_OK;
 }
 
@@ -198,6 +263,8 @@ inline void fill_desc_scalars(const tci_problem &p, SolveDesc &D)
     D.maxbonddim = o.maxbonddim;
     D.seed = o.seed;
     D.optflags = o.flags;
+    D.init_maxsample = p.state ? p.state->maxsamplevalue : 0.0;
+    D.init_hist = (p.state && p.state->nI_hist) ? 1 : 0;
 }
 
 // Pack the inputs (localdims, params, iparams, initial pivots) into a host
@@ -213,6 +280,31 @@ inline void pack_inputs(const tci_problem &p, const SolveCaps &c, const SolveDes
     for (int q = 0; q < c.npivinit; ++q)
         for (int b = 0; b < c.L; ++b)
             piv[(int64_t)q * c.L + b] = p.initialpivots ? (uint8_t)p.initialpivots[(int64_t)q * c.L + b] : 1;
+    if (c.has_init && p.state) {
+        // the starting sets in the set arena's layout, then the counts
+        uint8_t *img = at(Ddev.initsets);
+        const int L = c.L;
+        const size_t setbytes = (size_t)NSETKIND * L * c.setcap * L;
+        memset(img, 0, init_image_bytes(c));
+        int32_t *cnts = (int32_t *)(img + setbytes);
+        const int32_t *cnt[4];
+        const uint8_t *bytes[4];
+        state_sets(*p.state, cnt, bytes);
+        const int kinds[4] = {SET_I, SET_J, SET_HA_I, SET_HA_J};
+        for (int f = 0; f < 4; ++f) {
+            if (!cnt[f]) continue;
+            const bool iside = (f & 1) == 0;
+            int64_t off = 0;
+            for (int b = 0; b < L; ++b) {
+                const int n = cnt[f][b], len = iside ? b : L - 1 - b;
+                uint8_t *dst = img + ((size_t)(kinds[f] * L + b) * c.setcap) * L;
+                for (int e = 0; e < n; ++e)
+                    for (int t = 0; t < len; ++t) dst[(size_t)e * L + t] = bytes[f][off + (int64_t)e * len + t];
+                cnts[kinds[f] * L + b] = n;
+                off += (int64_t)n * len;
+            }
+        }
+    }
 }
 
 }  // namespace tci

@@ -68,7 +68,7 @@ class EngineSolve:
 
     def set(self, which, b):
         n = lib().eh_setn(self.h, which, b)
-        ln = b if which == 0 else self.L - 1 - b
+        ln = b if which % 2 == 0 else self.L - 1 - b
         out = np.zeros(max(n * ln, 1), np.int32)
         lib().eh_set(self.h, which, b, out.ctypes.data_as(C.POINTER(C.c_int32)))
         return [list(map(int, out[i * ln:(i + 1) * ln])) for i in range(n)]

@@ -411,16 +411,24 @@ extern "C" void eh_history(void *hp, int64_t *ranks, double *errors, int64_t *ng
         nglob[i] = D.nglob[i];
     }
 }
+// which 0/1: Iset/Jset, 2/3: the last history entry (as tci_batch_indexset)
+static int eh_kind(const SolveDesc &D, int which)
+{
+    const int jside = which & 1, hist = D.ints[INT_HIST];
+    if (which < 2) return jside ? SET_J : SET_I;
+    return hist == 2 ? (jside ? SET_HB_J : SET_HB_I) : (jside ? SET_HA_J : SET_HA_I);
+}
 extern "C" int eh_setn(void *hp, int which, int b)
 {
     const SolveDesc &D = ((EhSolve *)hp)->D;
-    return D.setn[(which ? SET_J : SET_I) * D.L + b];
+    if (which >= 2 && D.ints[INT_HIST] == 0) return 0;
+    return D.setn[eh_kind(D, which) * D.L + b];
 }
 extern "C" void eh_set(void *hp, int which, int b, int32_t *out)
 {
     const SolveDesc &D = ((EhSolve *)hp)->D;
-    const int L = D.L, kind = which ? SET_J : SET_I, len = which ? L - 1 - b : b;
-    const int n = D.setn[kind * L + b];
+    const int L = D.L, kind = eh_kind(D, which), len = (which & 1) ? L - 1 - b : b;
+    const int n = eh_setn(hp, which, b);
     const uint8_t *s = D.sets + ((int64_t)(kind * L + b) * D.setcap) * L;
     for (int i = 0; i < n; ++i)
         for (int t = 0; t < len; ++t) out[i * len + t] = s[(int64_t)i * L + t];
