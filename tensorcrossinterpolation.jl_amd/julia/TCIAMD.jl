@@ -159,6 +159,33 @@ end
 (f::DeviceFunctor{Float64})(x::Vector{Int}) =
     f([x[1:1]], [x[2:end]], Val(0))[1, 1]
 
+# ---------------------------------------------------------------- LUCI
+"""
+    leftright(A::Matrix{Float64}; maxrank, reltol, abstol, leftorthogonal)
+
+`(left(luci), right(luci), luci_rowindices, luci_colindices)` of
+`MatrixLUCI(A; kwargs...)` (matrixluci.jl:5-7, 40-84) with the factorization
+and both factors computed on the device (tci_luci_f64).
+"""
+function leftright(A::Matrix{Float64}; maxrank::Int=UNLIMITED, reltol::Number=1e-14,
+                   abstol::Number=0.0, leftorthogonal::Bool=true)
+    m, n = size(A)
+    cap = max(min(m, n, maxrank), 0)
+    rp = zeros(Int64, max(m, 1)); cp = zeros(Int64, max(n, 1))
+    lb = zeros(Float64, max(m, 1), max(cap, 1)); rb = zeros(Float64, max(cap, 1), max(n, 1))
+    p = Ref{Int64}(0); err = Ref{Float64}(0.0)
+    GC.@preserve A rp cp lb rb begin
+        throwstatus(ccall((:tci_luci_f64, libtci), Cint,
+                          (Ptr{Cvoid}, Ptr{Float64}, Int64, Int64, Int64, Int64, Float64, Float64, Int32,
+                           Ptr{Int64}, Ptr{Int64}, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ref{Int64},
+                           Ref{Float64}),
+                          defaultcontext().handle, A, max(m, 1), m, n, maxrank, reltol, abstol, leftorthogonal,
+                          rp, cp, lb, max(m, 1), rb, max(cap, 1), p, err), "tci_luci_f64")
+    end
+    k = p[]
+    return lb[1:m, 1:k], rb[1:k, 1:n], rp[1:k], cp[1:k]
+end
+
 # ---------------------------------------------------------------- solves
 struct COptions
     tolerance::Float64
@@ -177,6 +204,21 @@ struct COptions
     flags::Int32
 end
 
+const FLAG_NO_FINAL_SWEEP = Int32(1)
+
+# tci_state: a solve's starting pivot sets (and the last history entry)
+struct CState
+    nI::Ptr{Int32}
+    Iset::Ptr{UInt8}
+    nJ::Ptr{Int32}
+    Jset::Ptr{UInt8}
+    nI_hist::Ptr{Int32}
+    Iset_hist::Ptr{UInt8}
+    nJ_hist::Ptr{Int32}
+    Jset_hist::Ptr{UInt8}
+    maxsamplevalue::Float64
+end
+
 struct CProblem
     L::Int32
     localdims::Ptr{Int32}
@@ -184,6 +226,7 @@ struct CProblem
     npivots::Int32
     initialpivots::Ptr{Int32}
     opts::COptions
+    state::Ptr{CState}
 end
 
 struct CResult
@@ -197,56 +240,165 @@ end
 
 const _sweep = Dict(:backandforth => 0, :forward => 1, :backward => 2)
 
+# counts and bytes of one family of sets (site b: entries of length len(b))
+function _packsets(sets::Vector{Vector{Vector{Int}}}, len)
+    cnt = Int32[length(s) for s in sets]
+    bytes = UInt8[]
+    for (b, s) in enumerate(sets), x in s
+        length(x) == len(b) || throw(ArgumentError("index set $b has an entry of length $(length(x))"))
+        append!(bytes, UInt8.(x))
+    end
+    return cnt, isempty(bytes) ? UInt8[0] : bytes
+end
+
+# optimize! keywords (tensorci2.jl:700-720) as the device options
+function _options(; tolerance::Union{Float64,Nothing}=nothing, pivottolerance::Union{Float64,Nothing}=nothing,
+                  maxbonddim::Int=typemax(Int), maxiter::Int=20, sweepstrategy::Symbol=:backandforth,
+                  pivotsearch::Symbol=:full, verbosity::Int=0, loginterval::Int=10,
+                  normalizeerror::Bool=true, ncheckhistory::Int=3, maxnglobalpivot::Int=5,
+                  nsearchglobalpivot::Int=5, tolmarginglobalsearch::Float64=10.0,
+                  strictlynested::Bool=false, checkbatchevaluatable::Bool=false,
+                  checkconvglobalpivot::Bool=true, seed::Integer=0, chi_cap::Integer=0, flags::Integer=0)
+    # tolerance / pivottolerance resolution (tensorci2.jl:734-746)
+    if pivottolerance !== nothing
+        if tolerance !== nothing && tolerance != pivottolerance
+            throw(ArgumentError("Got different values for pivottolerance and tolerance in optimize!(TCI2). " *
+                                "For TCI2, both of these options have the same meaning. Please assign only `tolerance`."))
+        end
+        @warn "The option `pivottolerance` of `optimize!(tci::TensorCI2, f)` is deprecated. Please update your " *
+              "code to use `tolerance`, as `pivottolerance` will be removed in the future."
+        tolerance = pivottolerance
+    end
+    tol = something(tolerance, 1e-8)
+    pivotsearch == :full || throw(ArgumentError("only pivotsearch=:full runs on the device"))
+    # verbosity / loginterval only print in the reference; checkbatchevaluatable
+    # holds trivially for a DeviceFunctor (a BatchEvaluator)
+    return COptions(tol, maxbonddim, maxiter, _sweep[sweepstrategy], normalizeerror, ncheckhistory,
+                    maxnglobalpivot, nsearchglobalpivot, tolmarginglobalsearch, strictlynested,
+                    checkconvglobalpivot, UInt64(seed), chi_cap, flags)
+end
+
+# one device solve; a solve that outgrows the capacity chosen at submit is
+# resubmitted at twice the capacity (tci_capacity_plan) unless chi_cap was set
+function _solve(f::DeviceFunctor{Float64}, localdims::Vector{Int}, piv::Vector{Vector{Int}},
+                opts::COptions, state::Union{Nothing,NTuple{9,Any}})
+    L = length(localdims)
+    ld = Int32.(localdims)
+    pv = Int32.(reduce(vcat, piv))
+    while true
+        batch = Ref{Ptr{Cvoid}}(C_NULL)
+        cst = state === nothing ? nothing :
+              Ref(CState(pointer(state[1]), pointer(state[2]), pointer(state[3]), pointer(state[4]),
+                         state[5] === nothing ? C_NULL : pointer(state[5]),
+                         state[6] === nothing ? C_NULL : pointer(state[6]),
+                         state[7] === nothing ? C_NULL : pointer(state[7]),
+                         state[8] === nothing ? C_NULL : pointer(state[8]), state[9]))
+        GC.@preserve f ld pv state cst begin
+            sp = cst === nothing ? Ptr{CState}(C_NULL) : Base.unsafe_convert(Ptr{CState}, cst)
+            prob = Ref(CProblem(L, pointer(ld), _cf(f), length(piv), pointer(pv), opts, sp))
+            throwstatus(ccall((:tci_solve_submit, libtci), Cint,
+                              (Ptr{Cvoid}, Int32, Ref{CProblem}, Ref{Ptr{Cvoid}}),
+                              defaultcontext().handle, 1, prob, batch), "tci_solve_submit")
+            b = batch[]
+            try
+                throwstatus(ccall((:tci_batch_wait, libtci), Cint, (Ptr{Cvoid},), b), "tci_batch_wait")
+                r = Ref{CResult}()
+                ccall((:tci_batch_result, libtci), Cint, (Ptr{Cvoid}, Int32, Ref{CResult}), b, 0, r)
+                if r[].status == 10 && opts.chi_cap == 0
+                    used = Ref{Int64}(0); limit = Ref{Int64}(0)
+                    throwstatus(ccall((:tci_capacity_plan, libtci), Cint, (Ref{CProblem}, Ref{Int64}, Ref{Int64}),
+                                      prob, used, limit), "tci_capacity_plan")
+                    if used[] < limit[]
+                        o = opts
+                        opts = COptions(o.tolerance, o.maxbonddim, o.maxiter, o.sweepstrategy, o.normalizeerror,
+                                        o.ncheckhistory, o.maxnglobalpivot, o.nsearchglobalpivot,
+                                        o.tolmarginglobalsearch, o.strictlynested, o.checkconvglobalpivot, o.seed,
+                                        Int32(min(2 * used[], limit[], 8192)), o.flags)
+                        continue
+                    end
+                end
+                throwstatus(r[].status, "solve")
+                return _totci(b, r[], localdims)
+            finally
+                ccall((:tci_batch_free, libtci), Cint, (Ptr{Cvoid},), b)
+            end
+        end
+    end
+end
+
 """
     TCI.crossinterpolate2(Float64, f::DeviceFunctor, localdims, initialpivots; kwargs...)
 
 The whole TCI2 solve (optimize! and the final sweep1site!) on the GPU, one
 workgroup per solve.  Returns `(tci::TensorCI2{Float64}, ranks, errors)` like
-the reference (tensorci2.jl:943-953).  The global pivot search draws from the
-library's PRNG (seed keyword) instead of Julia's default RNG.
+the reference (tensorci2.jl:943-953) and takes its keyword arguments.  The
+global pivot search draws from the library's PRNG (seed keyword) instead of
+Julia's default RNG.
 """
 function TCI.crossinterpolate2(::Type{Float64}, f::DeviceFunctor{Float64},
                                localdims::Union{Vector{Int},NTuple{N,Int}},
                                initialpivots::Vector{Vector{Int}}=[ones(Int, length(localdims))];
-                               tolerance::Float64=1e-8, maxbonddim::Int=typemax(Int),
-                               maxiter::Int=20, sweepstrategy::Symbol=:backandforth,
-                               normalizeerror::Bool=true, ncheckhistory::Int=3,
-                               maxnglobalpivot::Int=5, nsearchglobalpivot::Int=5,
-                               tolmarginglobalsearch::Float64=10.0, strictlynested::Bool=false,
-                               checkconvglobalpivot::Bool=true, seed::Integer=0,
-                               chi_cap::Integer=0) where {N}
-    L = length(localdims)
-    ld = Int32.(collect(localdims))
-    piv = Int32.(reduce(vcat, initialpivots))
-    opts = COptions(tolerance, maxbonddim, maxiter, _sweep[sweepstrategy], normalizeerror,
-                    ncheckhistory, maxnglobalpivot, nsearchglobalpivot, tolmarginglobalsearch,
-                    strictlynested, checkconvglobalpivot, UInt64(seed), chi_cap, 0)
-    batch = Ref{Ptr{Cvoid}}(C_NULL)
-    GC.@preserve f ld piv begin
-        prob = Ref(CProblem(L, pointer(ld), _cf(f), length(initialpivots), pointer(piv), opts))
-        throwstatus(ccall((:tci_solve_submit, libtci), Cint,
-                          (Ptr{Cvoid}, Int32, Ref{CProblem}, Ref{Ptr{Cvoid}}),
-                          defaultcontext().handle, 1, prob, batch), "tci_solve_submit")
-    end
-    b = batch[]
-    try
-        throwstatus(ccall((:tci_batch_wait, libtci), Cint, (Ptr{Cvoid},), b), "tci_batch_wait")
-        r = Ref{CResult}()
-        ccall((:tci_batch_result, libtci), Cint, (Ptr{Cvoid}, Int32, Ref{CResult}), b, 0, r)
-        throwstatus(r[].status, "solve")
-        return _totci(b, r[], collect(localdims))
-    finally
-        ccall((:tci_batch_free, libtci), Cint, (Ptr{Cvoid},), b)
+                               kwargs...) where {N}
+    return _solve(f, collect(localdims), initialpivots, _options(; kwargs...), nothing)
+end
+
+_statebuffers(tci, L, maxsample) = begin
+    nI, I = _packsets(tci.Iset, b -> b - 1)
+    nJ, J = _packsets(tci.Jset, b -> L - b)
+    if isempty(tci.Iset_history)
+        (nI, I, nJ, J, nothing, nothing, nothing, nothing, Float64(maxsample))
+    else
+        nIh, Ih = _packsets(tci.Iset_history[end], b -> b - 1)
+        nJh, Jh = _packsets(tci.Jset_history[end], b -> L - b)
+        (nI, I, nJ, J, nIh, Ih, nJh, Jh, Float64(maxsample))
     end
 end
 
-function _totci(b::Ptr{Cvoid}, r::CResult, localdims::Vector{Int})
+"""
+    TCI.optimize!(tci::TensorCI2{Float64}, f::DeviceFunctor; kwargs...)
+
+optimize! (tensorci2.jl:700-850) continuing from tci's pivot sets, last
+history entry and maxsamplevalue, on the device.  Updates tci in place and
+returns `(ranks, errors)`.
+"""
+function TCI.optimize!(tci::TCI.TensorCI2{Float64}, f::DeviceFunctor{Float64}; kwargs...)
+    L = length(tci.localdims)
+    st = _statebuffers(tci, L, tci.maxsamplevalue)
+    t, ranks, errors = _solve(f, collect(tci.localdims), [ones(Int, L)], _options(; kwargs...), st)
+    for fld in (:Iset, :Jset, :sitetensors, :pivoterrors, :bonderrors, :maxsamplevalue)
+        setfield!(tci, fld, getfield(t, fld))
+    end
+    append!(tci.Iset_history, t.Iset_history)
+    append!(tci.Jset_history, t.Jset_history)
+    return ranks, errors
+end
+
+"""
+    TCI.TensorCI2{Float64}(f::DeviceFunctor, localdims, Iset, Jset)
+
+The constructor from local pivot lists (tensorci2.jl:58-72): maxsamplevalue
+is the maximum of |f| over every Iset[b] ⊕ σ ⊕ Jset[b], evaluated on the device.
+"""
+function TCI.TensorCI2{Float64}(f::DeviceFunctor{Float64}, localdims::Union{Vector{Int},NTuple{N,Int}},
+                                Iset::Vector{Vector{Vector{Int}}}, Jset::Vector{Vector{Vector{Int}}}) where {N}
     L = length(localdims)
-    tci = TCI.TensorCI2{Float64}(localdims)
-    for site in 1:L, which in (0, 1)
-        len = which == 0 ? site - 1 : L - site
+    nI, I = _packsets(Iset, b -> b - 1)
+    nJ, J = _packsets(Jset, b -> L - b)
+    st = (nI, I, nJ, J, nothing, nothing, nothing, nothing, 0.0)
+    t, _, _ = _solve(f, collect(localdims), [ones(Int, L)],
+                     _options(; maxiter=0, flags=FLAG_NO_FINAL_SWEEP), st)
+    empty!(t.Iset_history); empty!(t.Jset_history)
+    return t
+end
+
+# one set family of a finished solve (which: 0 Iset, 1 Jset, 2/3 the last
+# history entry of each)
+function _indexsets(b::Ptr{Cvoid}, L::Int, which::Integer)
+    sets = Vector{Vector{Vector{Int}}}(undef, L)
+    for site in 1:L
+        len = iseven(which) ? site - 1 : L - site
         cnt = Ref{Int32}(0)
-        # first call reports the count (and a too-small-buffer status)
+        # the first call reports the count (and a too-small-buffer status)
         ccall((:tci_batch_indexset, libtci), Cint,
               (Ptr{Cvoid}, Int32, Int32, Int32, Ref{Int32}, Ptr{UInt8}, Int64),
               b, 0, which, site, cnt, C_NULL, 0)
@@ -254,8 +406,22 @@ function _totci(b::Ptr{Cvoid}, r::CResult, localdims::Vector{Int})
         throwstatus(ccall((:tci_batch_indexset, libtci), Cint,
                           (Ptr{Cvoid}, Int32, Int32, Int32, Ref{Int32}, Ptr{UInt8}, Int64),
                           b, 0, which, site, cnt, buf, length(buf)), "tci_batch_indexset")
-        set = [Int.(buf[(i-1)*len+1:i*len]) for i in 1:cnt[]]
-        which == 0 ? (tci.Iset[site] = set) : (tci.Jset[site] = set)
+        sets[site] = [Int.(buf[(i-1)*len+1:i*len]) for i in 1:cnt[]]
+    end
+    return sets
+end
+
+function _totci(b::Ptr{Cvoid}, r::CResult, localdims::Vector{Int})
+    L = length(localdims)
+    tci = TCI.TensorCI2{Float64}(localdims)
+    tci.Iset = _indexsets(b, L, 0)
+    tci.Jset = _indexsets(b, L, 1)
+    # Iset_history / Jset_history: the device keeps the last entry, the only
+    # one sweep2site! reads (tensorci2.jl:874-877)
+    hI, hJ = _indexsets(b, L, 2), _indexsets(b, L, 3)
+    if any(!isempty, hI)
+        push!(tci.Iset_history, hI)
+        push!(tci.Jset_history, hJ)
     end
     for site in 1:L
         d3 = zeros(Int64, 3)
