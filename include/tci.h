@@ -186,6 +186,15 @@ typedef struct tci_options {
                                       0 = min(maxbonddim, rank bound, 32); see
                                       tci_capacity_plan                        */
     int32_t flags;                 /* TCI_FLAG_* bits, 0 by default           */
+    /* f memoized on the device as CachedFunction{Float64,K}(f, localdims)
+     * does (cachedfunction.jl:8-203): 0 = off, else a table of 2^cache_log2
+     * entries (4..26) per solve, keyed by Σ_i coeff_i·(σ_i - 1) with
+     * coeff_1 = 1, coeff_i = d_{i-1}·coeff_{i-1} (177-184) held in 256 bits
+     * (domains of more than 2^256 points are refused, as the reference
+     * refuses a key type that overflows, 19-21).  Values are f's own bits,
+     * so results are the same with or without the table. */
+    int32_t cache_log2;
+    int32_t reserved;
 } tci_options;
 
 /* Diagnostic (no counterpart in the reference): end optimize! after its last
@@ -292,6 +301,17 @@ tci_status tci_batch_bonderrors(tci_batch *b, int32_t i, double *out);
  * measured on the points instead of the pivots.  Synchronous. */
 tci_status tci_batch_evaluate(tci_batch *b, int32_t i, int64_t npts, const uint8_t *pts,
                               double *tt_out, double *f_out, double *maxerr, double *maxabsf);
+/* Solve i's memo table of f (opts.cache_log2): its size, the evaluations
+ * asked of it and how many it answered (the reference keeps no counts; this
+ * shows what the CachedFunction wrapper saved). */
+tci_status tci_batch_cachestats(tci_batch *b, int32_t i, int64_t *entries, int64_t *lookups,
+                                int64_t *hits);
+/* cachedata(cf) (cachedfunction.jl:198-203) of solve i's memo table: the
+ * cached points (cap × L bytes, 1-based, decoded from their keys as
+ * decodecachekey does, 188-196) and values.  *count = the entries held;
+ * TCI_ERR_ARG when they exceed cap (pass NULL buffers to ask the count). */
+tci_status tci_batch_cachedata(tci_batch *b, int32_t i, int64_t cap, uint8_t *points, double *values,
+                               int64_t *count);
 tci_status tci_batch_free(tci_batch *b);
 /* Device time of the batch's solve kernel (HIP events), milliseconds. */
 tci_status tci_batch_elapsed_ms(tci_batch *b, float *ms);

@@ -29,7 +29,7 @@ __all__ = [
     "DeviceFunctor", "Lorentz", "WeightedLorentz", "QuanticsGaussCos", "GaussSum", "SumFunctor",
     "TableFunctor", "TTFunctor", "QuanticsExp", "batchevaluate", "TensorCI2", "crossinterpolate2",
     "crossinterpolate2_batch", "TCI_UNLIMITED", "capacity_plan", "tt_evaluate", "sample_points",
-    "tensorci2_from_sets", "optimize",
+    "tensorci2_from_sets", "optimize", "CachedFunction",
 ]
 
 
@@ -366,6 +366,54 @@ def QuanticsExp(alpha1, beta1, alpha2=0.0, beta2=0.0):
     return DeviceFunctor(F_QUANTICS_EXP, np.array([alpha1, beta1, alpha2, beta2], np.float64))
 
 
+class CachedFunction:
+    """CachedFunction{Float64,K}(f, localdims) (cachedfunction.jl:8-60) for a
+    device functor: passed to crossinterpolate2 in place of f, every
+    evaluation of the solve goes through a memo table on the device
+    (2^log2size entries), keyed by Σ coeff_i·(σ_i - 1) (177-184) in 256 bits.
+    keybits is K's width; a domain whose largest key does not fit below
+    typemax(K) is refused as the reference refuses it (19-21).  The reference's
+    table lives as long as the wrapper; here each solve starts an empty table,
+    and after a solve `cachedata()` holds that solve's table."""
+
+    def __init__(self, f: DeviceFunctor, localdims, keybits=128, log2size=20):
+        self.f = f
+        self.localdims = [int(d) for d in localdims]
+        self.keybits = int(keybits)
+        self.log2size = int(log2size)
+        if self.keybits > 256:
+            raise ArgumentError(3, "keys are at most 256 bits on the device")
+        self.coeffs = [1]
+        for d in self.localdims[:-1]:
+            self.coeffs.append(self.coeffs[-1] * d)
+        if sum(c * (d - 1) for c, d in zip(self.coeffs, self.localdims)) >= (1 << self.keybits) - 1:
+            raise TCIError(8, "Overflow in CachedFunction. Use ValueType = a bigger type with fixed size, "
+                              "e.g., BitIntegers.UInt256")
+        self._data = {}
+        self.stats = None
+
+    def encodecachekey(self, index):
+        """_key / encodecachekey (cachedfunction.jl:177-186)."""
+        if len(index) != len(self.coeffs):
+            raise TCIError(8, "Invalid length of indexset")
+        return sum(c * (int(x) - 1) for c, x in zip(self.coeffs, index))
+
+    def decodecachekey(self, key):
+        """decodecachekey (cachedfunction.jl:188-196)."""
+        out = []
+        for d in self.localdims:
+            key, r = divmod(key, d)
+            out.append(r + 1)
+        return out
+
+    def cachedata(self):
+        """cachedata(cf) (cachedfunction.jl:198-203) of the last solve's table."""
+        return dict(self._data)
+
+    def iscached(self, index):
+        return tuple(int(x) for x in index) in self._data
+
+
 def batchevaluate(f: DeviceFunctor, localdims, leftindexset, rightindexset, M, ctx=None):
     """_batchevaluate_dispatch(Float64, f, localdims, Iset, Jset, Val(M)) (batcheval.jl:32-61).
 
@@ -531,6 +579,9 @@ def _holder(f, localdims, initialpivots, kw):
     kw.pop("verbosity", None)
     kw.pop("loginterval", None)
     state = kw.pop("_state", None)
+    if isinstance(f, CachedFunction):
+        kw["cache_log2"] = f.log2size
+        f = f.f
     if not isinstance(f, DeviceFunctor):
         raise TypeError("f must be a DeviceFunctor (a Julia/Python closure cannot run on the GPU)")
     return ProblemHolder(localdims, f.kind, f.params, f.iparams, initialpivots, state=state, tolerance=tol, **kw)
@@ -572,6 +623,19 @@ def optimize(tci, f, ctx=None, **kwargs):
               "maxsamplevalue", "status", "niter", "ranks", "errors", "nglobalpivots", "workstats"):
         setattr(tci, k, getattr(t, k))
     return t.ranks, t.errors
+
+
+def _fetch_cache(lib, b, k, cf):
+    """The memo table of solve k of batch b into cf (stats and cachedata)."""
+    ne, nl, nh, cnt = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+    _check(lib.tci_batch_cachestats(b, k, C.byref(ne), C.byref(nl), C.byref(nh)))
+    _check(lib.tci_batch_cachedata(b, k, 0, None, None, C.byref(cnt)))
+    L = len(cf.localdims)
+    pts = np.zeros((max(cnt.value, 1), L), np.uint8)
+    vals = np.zeros(max(cnt.value, 1))
+    _check(lib.tci_batch_cachedata(b, k, cnt.value, _abi.u8ptr(pts), _abi.dptr(vals), C.byref(cnt)))
+    cf._data = {tuple(int(v) for v in pts[q]): float(vals[q]) for q in range(cnt.value)}
+    cf.stats = dict(entries=ne.value, lookups=nl.value, hits=nh.value)
 
 
 def capacity_plan(holder):
@@ -637,6 +701,8 @@ def crossinterpolate2_batch(problems, ctx=None, return_batch=False, error_sample
                     _check(lib.tci_batch_evaluate(b, k, x.shape[0], _abi.u8ptr(x), None, None,
                                                   C.byref(me), C.byref(mf)))
                     t.sample_error = (me.value, mf.value)
+                if isinstance(problems[i][0], CachedFunction) and t.status == 0:
+                    _fetch_cache(lib, b, k, problems[i][0])
                 out[i] = t
             ms = C.c_float()
             lib.tci_batch_elapsed_ms(b, C.byref(ms))

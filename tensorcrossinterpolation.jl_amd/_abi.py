@@ -49,7 +49,8 @@ class tci_options(C.Structure):
                 ("ncheckhistory", C.c_int32), ("maxnglobalpivot", C.c_int32),
                 ("nsearchglobalpivot", C.c_int32), ("tolmarginglobalsearch", C.c_double),
                 ("strictlynested", C.c_int32), ("checkconvglobalpivot", C.c_int32),
-                ("seed", C.c_uint64), ("chi_cap", C.c_int32), ("flags", C.c_int32)]
+                ("seed", C.c_uint64), ("chi_cap", C.c_int32), ("flags", C.c_int32),
+                ("cache_log2", C.c_int32), ("reserved", C.c_int32)]
 
 
 class tci_state(C.Structure):
@@ -129,13 +130,14 @@ SWEEP = {"backandforth": 0, "forward": 1, "backward": 2}
 def make_options(tolerance=1e-8, maxbonddim=TCI_UNLIMITED, maxiter=20, sweepstrategy="backandforth",
                  normalizeerror=True, ncheckhistory=3, maxnglobalpivot=5, nsearchglobalpivot=5,
                  tolmarginglobalsearch=10.0, strictlynested=False, checkconvglobalpivot=True,
-                 seed=0, chi_cap=0, flags=0):
+                 seed=0, chi_cap=0, flags=0, cache_log2=0):
     if isinstance(sweepstrategy, str):
         sweepstrategy = SWEEP[sweepstrategy]
     return tci_options(float(tolerance), int(maxbonddim), int(maxiter), int(sweepstrategy),
                        int(bool(normalizeerror)), int(ncheckhistory), int(maxnglobalpivot),
                        int(nsearchglobalpivot), float(tolmarginglobalsearch), int(bool(strictlynested)),
-                       int(bool(checkconvglobalpivot)), int(seed) & ((1 << 64) - 1), int(chi_cap), int(flags))
+                       int(bool(checkconvglobalpivot)), int(seed) & ((1 << 64) - 1), int(chi_cap), int(flags),
+                       int(cache_log2), 0)
 
 
 class ProblemHolder:
@@ -206,6 +208,8 @@ def lib():
     L.tci_batch_profile.argtypes = [vp, C.c_int32, P(C.c_double)]
     L.tci_capacity_plan.argtypes = [P(tci_problem), P(C.c_int64), P(C.c_int64)]
     L.tci_ctx_set_rrlu_algo.argtypes = [vp, C.c_int32]
+    L.tci_batch_cachestats.argtypes = [vp, C.c_int32, P(C.c_int64), P(C.c_int64), P(C.c_int64)]
+    L.tci_batch_cachedata.argtypes = [vp, C.c_int32, C.c_int64, P(C.c_uint8), P(C.c_double), P(C.c_int64)]
     L.tci_rrlu_batched_noalloc_f64.argtypes = [vp, vp, C.c_int64]
     L.tci_tt_evaluate_f64.argtypes = [vp, C.c_int32, P(C.c_int64), P(C.c_int32), P(C.c_double), C.c_int64,
                                       P(C.c_uint8), P(C.c_double)]
@@ -223,5 +227,5 @@ EXPORTED_SYMBOLS = [
     "tci_batch_history", "tci_batch_linkdims", "tci_batch_indexset", "tci_batch_sitetensor",
     "tci_batch_pivoterrors", "tci_batch_bonderrors", "tci_batch_free", "tci_batch_elapsed_ms",
     "tci_batch_profile", "tci_capacity_plan", "tci_tt_evaluate_f64", "tci_batch_evaluate",
-    "tci_ctx_set_rrlu_algo", "tci_rrlu_batched_noalloc_f64",
+    "tci_ctx_set_rrlu_algo", "tci_rrlu_batched_noalloc_f64", "tci_batch_cachestats", "tci_batch_cachedata",
 ]

@@ -1024,6 +1024,65 @@ tci_status tci_batch_profile(tci_batch *b, int32_t i, double *us8)
     return TCI_OK;
 }
 
+tci_status tci_batch_cachestats(tci_batch *b, int32_t i, int64_t *entries, int64_t *lookups, int64_t *hits)
+{
+    SolveDesc R;
+    if (!report_view(b, i, R)) return set_err(TCI_ERR_ARG, "bad batch/index");
+    if (entries) *entries = R.cache_log2 ? (int64_t)1 << R.cache_log2 : 0;
+    if (lookups) *lookups = (int64_t)R.scal[SCAL_CACHE_LOOKUPS];
+    if (hits) *hits = (int64_t)R.scal[SCAL_CACHE_HITS];
+    return TCI_OK;
+}
+
+tci_status tci_batch_cachedata(tci_batch *b, int32_t i, int64_t cap, uint8_t *points, double *values,
+                               int64_t *count)
+{
+    SolveDesc R;
+    if (!count || !report_view(b, i, R) || cap < 0) return set_err(TCI_ERR_ARG, "bad arguments");
+    *count = 0;
+    if (!R.cache_log2) return TCI_OK;
+    const int64_t ne = (int64_t)1 << R.cache_log2;
+    const int L = R.L;
+    std::vector<uint32_t> st((size_t)ne);
+    std::vector<uint64_t> keys((size_t)ne * CACHE_LIMBS);
+    std::vector<double> vals((size_t)ne);
+    std::vector<int32_t> ld((size_t)L);
+    HIPCHK(hipSetDevice(b->ctx->device));
+    HIPCHK(hipMemcpy(st.data(), R.cache_state, sizeof(uint32_t) * st.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(keys.data(), R.cache_keys, sizeof(uint64_t) * keys.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(vals.data(), R.cache_vals, sizeof(double) * vals.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ld.data(), R.localdims, sizeof(int32_t) * ld.size(), hipMemcpyDeviceToHost));
+    int64_t n = 0;
+    for (int64_t e = 0; e < ne; ++e) {
+        if (st[(size_t)e] != CACHE_READY) continue;
+        if (n < cap && points && values) {
+            // decodecachekey (cachedfunction.jl:188-196): divrem by d_1, d_2, …
+            uint64_t k[CACHE_LIMBS];
+            for (int q = 0; q < CACHE_LIMBS; ++q) k[q] = keys[(size_t)(e * CACHE_LIMBS + q)];
+            for (int s = 0; s < L; ++s) {
+                const uint64_t d = (uint64_t)ld[(size_t)s];
+                uint64_t rem = 0;
+                for (int q = CACHE_LIMBS - 1; q >= 0; --q) {
+                    // (rem·2^64 + k[q]) / d in 32-bit halves (rem < d < 2^8)
+                    const uint64_t hi = (rem << 32) | (k[q] >> 32);
+                    const uint64_t qh = hi / d, rh = hi % d;
+                    const uint64_t lo = (rh << 32) | (k[q] & 0xffffffffULL);
+                    const uint64_t ql = lo / d;
+                    rem = lo % d;
+                    k[q] = (qh << 32) | ql;
+                }
+                points[(size_t)(n * L + s)] = (uint8_t)(rem + 1);
+            }
+            values[(size_t)n] = vals[(size_t)e];
+        }
+        ++n;
+    }
+    *count = n;
+    if (n > cap && (points || values)) return set_err(TCI_ERR_ARG, "output buffers hold %lld of %lld entries",
+                                                      (long long)cap, (long long)n);
+    return TCI_OK;
+}
+
 // every point coordinate in 1..d_b (the kernel indexes the cores with it)
 static bool points_in_range(int L, const int32_t *localdims, int64_t npts, const uint8_t *pts)
 {

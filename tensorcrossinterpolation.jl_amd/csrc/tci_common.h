@@ -73,7 +73,38 @@ struct EngShared {
     int32_t nf;        // global pivots found
     int32_t npe;       // length of pivoterrors
     double pe[kPivErrCap];   // pivoterrors while the solve runs, when they fit
+    uint64_t cache_lookups, cache_hits;   // memo table of f (CachedFunction)
 };
+
+// ---- 256-bit keys of the memo table (four 64-bit limbs, least significant
+// first): k += c·s for a key coefficient c and a small factor s
+ENG inline void key_muladd(uint64_t *k, const uint64_t *c, uint64_t s)
+{
+    uint64_t carry = 0;
+    for (int q = 0; q < 4; ++q) {
+        // the 128-bit product c[q]·s + carry, split in 32-bit halves
+        const uint64_t lo = (c[q] & 0xffffffffULL) * s, hi = (c[q] >> 32) * s;
+        const uint64_t mid = (lo >> 32) + (hi & 0xffffffffULL);
+        const uint64_t plo = (lo & 0xffffffffULL) | (mid << 32);
+        const uint64_t phi = (hi >> 32) + (mid >> 32);
+        const uint64_t t = plo + carry;
+        const uint64_t c1 = t < plo;
+        const uint64_t u = k[q] + t;
+        const uint64_t c2 = u < t;
+        k[q] = u;
+        carry = phi + c1 + c2;
+    }
+}
+
+// a hash of a 256-bit key (SplitMix64's finalizer over the folded limbs)
+ENG inline uint64_t key_hash(const uint64_t *k)
+{
+    uint64_t x = k[0] ^ (k[1] * 0x9e3779b97f4a7c15ULL) ^ (k[2] * 0xbf58476d1ce4e5b9ULL) ^
+                 (k[3] * 0x94d049bb133111ebULL);
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+    return x ^ (x >> 31);
+}
 
 // ---- project PRNG: SplitMix64-seeded xoshiro256** --------------------------
 ENG inline uint64_t eng_splitmix(uint64_t *x)

@@ -35,8 +35,14 @@ enum {
     SCAL_WS_NEVAL = 6,
     SCAL_TPH = 8,         // 8 phase timers (device clock ticks): eval, rrlu, sets,
                           // fill, global search, sweep1site, total
-    NSCAL = 16
+    SCAL_CACHE_LOOKUPS = 16,   // f evaluations asked of the memo table
+    SCAL_CACHE_HITS = 17,      // of which answered from it
+    NSCAL = 18
 };
+
+// the memo table of f (CachedFunction): one state word per entry (empty,
+// being written, ready), the 256-bit key as four 64-bit limbs, the value
+enum { CACHE_EMPTY = 0, CACHE_BUSY = 1, CACHE_READY = 2, CACHE_LIMBS = 4 };
 
 // bits of SolveDesc::optflags (the values of tci.h's TCI_FLAG_*)
 enum { OPT_NO_FINAL_SWEEP = 1 };
@@ -73,6 +79,11 @@ struct SolveDesc {
                                  // then their NSETKIND·L int32 counts; null: none
     double init_maxsample;       // > 0: the starting maxsamplevalue
     int32_t init_hist;           // the starting sets include a history entry (slot A)
+    int32_t cache_log2;          // memo table of f: 2^cache_log2 entries, 0 = none
+    const uint64_t *cache_coef;  // L × CACHE_LIMBS: the key coefficient of each site
+    uint32_t *cache_state;       // 2^cache_log2
+    uint64_t *cache_keys;        // 2^cache_log2 × CACHE_LIMBS
+    double *cache_vals;          // 2^cache_log2
     // ---- state (device) ----
     uint8_t *sets;           // 6L sets × setcap entries × L bytes
     int32_t *setn;           // 6L counts

@@ -310,7 +310,9 @@ struct Engine {
     // the team has room for it (the same division the functor performs)
     ENG void build_lut()
     {
-        if (!(FK == TCI_F_LORENTZ_INT || (FK == 0 && D.fkind == TCI_F_LORENTZ_INT)) || !tm.llut) return;
+        // (with a memo table every evaluation goes through it instead)
+        if (!(FK == TCI_F_LORENTZ_INT || (FK == 0 && D.fkind == TCI_F_LORENTZ_INT)) || !tm.llut || D.cache_log2)
+            return;
         int64_t smax = 0;
         for (int t = 0; t < L; ++t) smax += (int64_t)D.localdims[t] * D.localdims[t];
         if (smax + 2 > tm.llut_cap) return;
@@ -528,6 +530,47 @@ struct Engine {
         if (tm.tid() == 0) setn(kind, b) = p;
     }
 
+    // ------------------------------------------------------- memo table of f
+    // f(x) through the solve's memo table, as CachedFunction (cachedfunction.jl:
+    // 62-110) memoizes it: the key Σ_t coef[t]·(σ_t - 1) (177-184, 256 bits),
+    // open addressing with linear probing.  A thread that finds a free entry
+    // claims it (empty → busy), computes f and publishes key and value
+    // (busy → ready); a thread that meets a busy entry or a full probe run
+    // computes f itself without waiting.  f is deterministic, so every path
+    // returns f's own bits.  *hit counts the answers from the table.
+    ENG_INLINE double cached_f(const tci_point &x, uint64_t &hit)
+    {
+        uint64_t k[CACHE_LIMBS] = {0, 0, 0, 0};
+        for (int t = 0; t < L; ++t) {
+            const uint64_t s = (uint64_t)(tci_pt(&x, t) - 1);
+            if (s) key_muladd(k, D.cache_coef + CACHE_LIMBS * t, s);
+        }
+        const uint64_t mask = ((uint64_t)1 << D.cache_log2) - 1;
+        uint64_t h = key_hash(k) & mask;
+        for (int probe = 0; probe < 64; ++probe, h = (h + 1) & mask) {
+            uint32_t st = __atomic_load_n(D.cache_state + h, __ATOMIC_ACQUIRE);
+            if (st == CACHE_EMPTY) {
+                uint32_t expect = CACHE_EMPTY;
+                if (__atomic_compare_exchange_n(D.cache_state + h, &expect, (uint32_t)CACHE_BUSY, false,
+                                                __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE)) {
+                    const double v = tci_functor_eval(FK ? FK : D.fkind, D.params, D.iparams, &x);
+                    for (int q = 0; q < CACHE_LIMBS; ++q) D.cache_keys[CACHE_LIMBS * h + q] = k[q];
+                    D.cache_vals[h] = v;
+                    __atomic_store_n(D.cache_state + h, (uint32_t)CACHE_READY, __ATOMIC_RELEASE);
+                    return v;
+                }
+                st = expect;
+            }
+            if (st != CACHE_READY) break;   // being written: do not wait
+            const uint64_t *kk = D.cache_keys + CACHE_LIMBS * h;
+            if (kk[0] == k[0] && kk[1] == k[1] && kk[2] == k[2] && kk[3] == k[3]) {
+                ++hit;
+                return D.cache_vals[h];
+            }
+        }
+        return tci_functor_eval(FK ? FK : D.fkind, D.params, D.iparams, &x);
+    }
+
     // ---------------------------------------------------------- evaluation
     // out[i + ld·j] = f(rows[i] ⊕ cols[j]) (batcheval.jl:50-57); returns the
     // NaN-propagating max of |out| folded into S.maxsample when fold != 0.
@@ -555,6 +598,21 @@ struct Engine {
                     lmax = jl_max(lmax, fabs(v));
                 }
             }
+        } else if (D.cache_log2) {
+            uint64_t hits = 0;
+            for (int j = tm.wave(); j < n; j += tm.nwaves())
+                for (int i = tm.lane(); i < m; i += tm.wsize()) {
+                    tci_point x;
+                    x.a = rows + (int64_t)i * L;
+                    x.b = cols + (int64_t)j * L;
+                    x.na = nl;
+                    x.L = L;
+                    const double v = cached_f(x, hits);
+                    if (out) out[transpose ? (j + ld * i) : (i + ld * j)] = v;
+                    lmax = jl_max(lmax, fabs(v));
+                }
+            if (hits) __atomic_fetch_add(&S.cache_hits, hits, __ATOMIC_RELAXED);
+            if (tm.tid() == 0) S.cache_lookups += (uint64_t)cnt;
         } else
         for (int j = tm.wave(); j < n; j += tm.nwaves())
             for (int i = tm.lane(); i < m; i += tm.wsize()) {
@@ -669,7 +727,7 @@ struct Engine {
     // sums of v², never stored.  Returns p or -status like rrlu().
     ENG_INLINE bool lorentz_tile(int m, int n) const
     {
-        return (FK == TCI_F_LORENTZ_INT || (FK == 0 && D.fkind == TCI_F_LORENTZ_INT)) &&
+        return (FK == TCI_F_LORENTZ_INT || (FK == 0 && D.fkind == TCI_F_LORENTZ_INT)) && !D.cache_log2 &&
                Team::RA > 0 && Tile::fits(m, n) &&
                m <= D.mrcap && n <= D.mrcap;
     }
@@ -1461,6 +1519,8 @@ struct Engine {
             S.tph[PH_TOTAL] = tm.clock();
             S.status = ST_OK;
             for (int i = 0; i < 6; ++i) S.ws[i] = 0.0;
+            S.cache_lookups = 0;
+            S.cache_hits = 0;
             for (int q = 0; q < NSETKIND * L; ++q) setcnt[q] = 0;
             D.ints[INT_NITER] = 0;
             D.ints[INT_NPE] = 0;
@@ -1471,6 +1531,12 @@ struct Engine {
             for (int b = 0; b < L; ++b) { D.Tdims[3 * b] = 0; D.Tdims[3 * b + 1] = 0; D.Tdims[3 * b + 2] = 0; }
         }
         tm.sync();
+        if (D.cache_log2) {
+            // an empty memo table (a new CachedFunction per solve)
+            const int64_t ne = (int64_t)1 << D.cache_log2;
+            for (int64_t q = tm.tid(); q < ne; q += tm.nt()) D.cache_state[q] = CACHE_EMPTY;
+            tm.sync();
+        }
         build_lut();
         int st = D.initsets ? load_state() : start_from_pivots();
         if (st != ST_OK) return finish(st);
@@ -1610,6 +1676,8 @@ struct Engine {
             for (int i = 0; i < 8; ++i) D.scal[SCAL_TPH + i] = (double)S.tph[i];
             D.scal[SCAL_MAXSAMPLE] = S.maxsample;
             for (int i = 0; i < 6; ++i) D.scal[SCAL_WS_PI + i] = S.ws[i];
+            D.scal[SCAL_CACHE_LOOKUPS] = (double)S.cache_lookups;
+            D.scal[SCAL_CACHE_HITS] = (double)S.cache_hits;
             D.ints[INT_DONE] = 1;
         }
         tm.sync();

@@ -28,6 +28,7 @@ struct SolveCaps {
     int32_t nsearch;
     int32_t maxiter;
     int32_t has_init;   // the solve starts from given pivot sets (tci_state)
+    int32_t cache_log2; // memo table of f: 2^cache_log2 entries, 0 = none
 };
 
 // bytes of a starting-state image: the pivot sets in the set arena's own
@@ -55,6 +56,7 @@ inline void layout_solve(const SolveCaps &c, SolveDesc *D, uint8_t *in, uint8_t 
     size_t o_ipar = itake((size_t)c.niparams * 4);
     size_t o_piv = itake((size_t)c.npivinit * L);
     size_t o_init = itake(c.has_init ? init_image_bytes(c) : 0);
+    size_t o_coef = itake(c.cache_log2 ? L * CACHE_LIMBS * 8 : 0);
     *input_bytes = i;
 
     size_t o = 0;
@@ -74,6 +76,10 @@ inline void layout_solve(const SolveCaps &c, SolveDesc *D, uint8_t *in, uint8_t 
     size_t o_work = take(ncand * (1 + 2 * sc) * 8);
     size_t o_gpts = take((2 * (size_t)c.nsearch + ncand) * L);
     size_t o_stage = take(2 * mr * 8);
+    const size_t ncache = c.cache_log2 ? (size_t)1 << c.cache_log2 : 0;
+    size_t o_cst = take(ncache * 4);
+    size_t o_ckey = take(ncache * CACHE_LIMBS * 8);
+    size_t o_cval = take(ncache * 8);
     *state_bytes = o;
 
     size_t r = 0;
@@ -104,6 +110,13 @@ inline void layout_solve(const SolveCaps &c, SolveDesc *D, uint8_t *in, uint8_t 
         D->iparams = (const int32_t *)(in + o_ipar);
         D->initpivots = in + o_piv;
         D->initsets = c.has_init ? in + o_init : nullptr;
+        D->cache_coef = c.cache_log2 ? (const uint64_t *)(in + o_coef) : nullptr;
+    }
+    D->cache_log2 = c.cache_log2;
+    if (base && c.cache_log2) {
+        D->cache_state = (uint32_t *)(base + o_cst);
+        D->cache_keys = (uint64_t *)(base + o_ckey);
+        D->cache_vals = (double *)(base + o_cval);
     }
     if (base) {
         D->sets = base + o_sets;

@@ -128,6 +128,34 @@ inline int check_functor(const tci_functor &f, int L, const int32_t *localdims, 
     return TCI_OK;
 }
 
+// The key coefficients of the memo table (cachedfunction.jl:13-25):
+// coef_1 = 1, coef_i = d_{i-1}·coef_{i-1}, as 256-bit numbers (four 64-bit
+// limbs each, written to out when non-null).  False when the largest key,
+// Π d_i - 1, would not stay below 2^256 - 1 (the reference's overflow check
+// with a 256-bit key type, 19-21).
+inline bool cache_key_coefs(int L, const int32_t *localdims, uint64_t *out)
+{
+    uint64_t c[5] = {1, 0, 0, 0, 0};
+    for (int i = 0; i < L; ++i) {
+        if (out)
+            for (int q = 0; q < 4; ++q) out[4 * i + q] = c[q];
+        // c *= d_i (five limbs, to see the overflow)
+        uint64_t carry = 0;
+        const uint64_t d = (uint64_t)localdims[i];
+        for (int q = 0; q < 5; ++q) {
+            const uint64_t lo = (c[q] & 0xffffffffULL) * d, hi = (c[q] >> 32) * d;
+            const uint64_t mid = (lo >> 32) + (hi & 0xffffffffULL);
+            const uint64_t plo = (lo & 0xffffffffULL) | (mid << 32);
+            const uint64_t phi = (hi >> 32) + (mid >> 32);
+            const uint64_t t = plo + carry;
+            carry = phi + (t < plo);
+            c[q] = t;
+        }
+        if (carry || c[4]) return false;   // Π d >= 2^256
+    }
+    return true;   // Π d < 2^256, so every key Π d - 1 < 2^256 - 1 fits
+}
+
 // The four set families of a starting state, in SET_I, SET_J, SET_HA_I,
 // SET_HA_J order (history entries may be absent).
 inline void state_sets(const tci_state &s, const int32_t *cnt[4], const uint8_t *bytes[4])
@@ -240,6 +268,11 @@ inline int check_problem(const tci_problem &p, SolveCaps &c, std::string *msg)
     c.nsearch = o.nsearchglobalpivot;
     c.maxiter = o.maxiter > 0 ? o.maxiter : 1;
     c.has_init = p.state ? 1 : 0;
+    if (o.cache_log2 != 0 && (o.cache_log2 < 4 || o.cache_log2 > 26))
+        return bad(TCI_ERR_ARG, "cache_log2 must be 0 or in 4..26");
+    if (o.cache_log2 && !cache_key_coefs(L, p.localdims, nullptr))
+        return bad(TCI_ERR_ARG, "Overflow in CachedFunction: the domain has more than 2^256 points");
+    c.cache_log2 = o.cache_log2;
     return TCI_OK;
 }
 
@@ -280,6 +313,7 @@ inline void pack_inputs(const tci_problem &p, const SolveCaps &c, const SolveDes
     for (int q = 0; q < c.npivinit; ++q)
         for (int b = 0; b < c.L; ++b)
             piv[(int64_t)q * c.L + b] = p.initialpivots ? (uint8_t)p.initialpivots[(int64_t)q * c.L + b] : 1;
+    if (c.cache_log2) cache_key_coefs(c.L, p.localdims, (uint64_t *)at(Ddev.cache_coef));
     if (c.has_init && p.state) {
         // the starting sets in the set arena's layout, then the counts
         uint8_t *img = at(Ddev.initsets);

@@ -202,6 +202,8 @@ struct COptions
     seed::UInt64
     chi_cap::Int32
     flags::Int32
+    cache_log2::Int32
+    reserved::Int32
 end
 
 const FLAG_NO_FINAL_SWEEP = Int32(1)
@@ -258,7 +260,8 @@ function _options(; tolerance::Union{Float64,Nothing}=nothing, pivottolerance::U
                   normalizeerror::Bool=true, ncheckhistory::Int=3, maxnglobalpivot::Int=5,
                   nsearchglobalpivot::Int=5, tolmarginglobalsearch::Float64=10.0,
                   strictlynested::Bool=false, checkbatchevaluatable::Bool=false,
-                  checkconvglobalpivot::Bool=true, seed::Integer=0, chi_cap::Integer=0, flags::Integer=0)
+                  checkconvglobalpivot::Bool=true, seed::Integer=0, chi_cap::Integer=0, flags::Integer=0,
+                  cache_log2::Integer=0)
     # tolerance / pivottolerance resolution (tensorci2.jl:734-746)
     if pivottolerance !== nothing
         if tolerance !== nothing && tolerance != pivottolerance
@@ -275,7 +278,7 @@ function _options(; tolerance::Union{Float64,Nothing}=nothing, pivottolerance::U
     # holds trivially for a DeviceFunctor (a BatchEvaluator)
     return COptions(tol, maxbonddim, maxiter, _sweep[sweepstrategy], normalizeerror, ncheckhistory,
                     maxnglobalpivot, nsearchglobalpivot, tolmarginglobalsearch, strictlynested,
-                    checkconvglobalpivot, UInt64(seed), chi_cap, flags)
+                    checkconvglobalpivot, UInt64(seed), chi_cap, flags, cache_log2, 0)
 end
 
 # one device solve; a solve that outgrows the capacity chosen at submit is
@@ -313,7 +316,7 @@ function _solve(f::DeviceFunctor{Float64}, localdims::Vector{Int}, piv::Vector{V
                         opts = COptions(o.tolerance, o.maxbonddim, o.maxiter, o.sweepstrategy, o.normalizeerror,
                                         o.ncheckhistory, o.maxnglobalpivot, o.nsearchglobalpivot,
                                         o.tolmarginglobalsearch, o.strictlynested, o.checkconvglobalpivot, o.seed,
-                                        Int32(min(2 * used[], limit[], 8192)), o.flags)
+                                        Int32(min(2 * used[], limit[], 8192)), o.flags, o.cache_log2, 0)
                         continue
                     end
                 end
