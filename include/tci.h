@@ -87,6 +87,46 @@ tci_status tci_luci_f64(tci_ctx *ctx, const double *A, int64_t lda, int64_t m, i
                         int64_t *rowperm, int64_t *colperm, double *left, int64_t ldleft,
                         double *right, int64_t ldright, int64_t *npivot, double *error);
 
+/* C = A·B (M×K times K×N, column-major, host buffers) on the device, each
+ * sum in ascending k.  The matrix products of compress! and of the TT →
+ * TensorCI2 conversion (tensortrain.jl:161-180, conversion.jl:122-126),
+ * which the reference does with BLAS.  Synchronous. */
+tci_status tci_gemm_f64(tci_ctx *ctx, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
+                        const double *B, int64_t ldb, double *C, int64_t ldc);
+
+/* ---- tensor trains (tensortrain.jl, conversion.jl) ----
+ * A tensor train is given by L+1 bond dimensions (linkdims[0] and
+ * linkdims[L] are the outer ones, left unchanged) and L local dimensions,
+ * with the cores back to back: site b (0-based) is linkdims[b] ×
+ * localdims[b] × linkdims[b+1], column-major.  Both entries upload the
+ * train once, sweep on the device, and write linkdims and cores back in
+ * place (the cores repacked with the new bonds; no site grows, so the
+ * buffer the caller passed always holds the result). */
+#define TCI_FACTOR_LU 0    /* :LU — left/right of rrlu (matrixlu.jl:374-392)      */
+#define TCI_FACTOR_CI 1    /* :CI — left/right of MatrixLUCI (matrixluci.jl:70-84) */
+#define TCI_FACTOR_SVD 2   /* :SVD — refused with TCI_ERR_ARG                      */
+
+/* compress!(tt, method; tolerance, maxbonddim, normalizeerror)
+ * (tensortrain.jl:147-183 with _factorize at 95-145): left to right with
+ * no truncation, then right to left with reltol = tolerance
+ * (normalizeerror) or abstol = tolerance, maxrank = maxbonddim. */
+tci_status tci_tt_compress_f64(tci_ctx *ctx, int32_t L, int64_t *linkdims, const int32_t *localdims,
+                               double *cores, int32_t method, double tolerance, int64_t maxbonddim,
+                               int32_t normalizeerror);
+
+/* One sweep of sweep1sitegetindices! (conversion.jl:73-156) — and of each
+ * direction of compress! — with the given rrLU options and
+ * leftorthogonal = forward.  pivots (optional, 2·Σ_{b=1}^{L-1} linkdims[b]
+ * entries on entry): per bond in bond order, rowindices(luci) then
+ * colindices(luci) of the factored site (the new bond dimension's worth of
+ * 1-based indices each; the site is b when forward, b+1 when backward).
+ * pivoterrors (optional, rank(tt)+1
+ * entries with rank(tt) the largest inner bond on entry): the running
+ * maximum of pivoterrors(luci) over the bonds (conversion.jl:141). */
+tci_status tci_tt_sweep_f64(tci_ctx *ctx, int32_t L, int64_t *linkdims, const int32_t *localdims, double *cores,
+                            int32_t method, int32_t forward, int64_t maxrank, double reltol, double abstol,
+                            int32_t *pivots, double *pivoterrors);
+
 /* One factorization job of a batch (device memory).  A is overwritten with
  * the packed factors exactly as rrlu!(A) leaves it (matrixlu.jl:194-204):
  * tril(A[:,1:p]) holds L (before its unit diagonal is imposed when

@@ -107,6 +107,10 @@ def lib():
         L.orc_eval_points.argtypes = [C.c_void_p, C.c_int32, C.c_int64, P(C.c_int32), P(C.c_double)]
         L.orc_tt_evaluate.argtypes = [C.c_int32, P(C.c_int64), P(C.c_double), C.c_int64, P(C.c_int32),
                                       P(C.c_double)]
+        L.orc_tt_sweep.argtypes = [C.c_int32, P(C.c_int64), P(C.c_int32), P(C.c_double), C.c_int32, C.c_int32,
+                                   C.c_int64, C.c_double, C.c_double, P(C.c_int32), P(C.c_double)]
+        L.orc_tt_compress.argtypes = [C.c_int32, P(C.c_int64), P(C.c_int32), P(C.c_double), C.c_int32,
+                                      C.c_double, C.c_int64, C.c_int32]
         L.orc_globalpivotfinder.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_int32, C.c_int32,
                                             C.c_double, C.c_uint64, P(C.c_int32)]
         L.orc_exp.restype = C.c_double
@@ -249,6 +253,52 @@ def tt_evaluate(cores, points):
     out = np.zeros(max(x.shape[0], 1))
     lib().orc_tt_evaluate(L, _ip(dims), _dp(flat), x.shape[0], _i32(x), _dp(out))
     return out[:x.shape[0]]
+
+
+def _tt_pack(cores):
+    links = np.array([cores[0].shape[0]] + [c.shape[2] for c in cores], np.int64)
+    ld = np.array([c.shape[1] for c in cores], np.int32)
+    flat = np.ascontiguousarray(np.concatenate([np.asarray(c, np.float64).ravel(order="F") for c in cores]))
+    return links, ld, flat
+
+
+def _tt_unpack(links, ld, flat):
+    out, off = [], 0
+    for b in range(len(ld)):
+        sz = int(links[b] * ld[b] * links[b + 1])
+        out.append(flat[off:off + sz].reshape((int(links[b]), int(ld[b]), int(links[b + 1])), order="F").copy())
+        off += sz
+    return out
+
+
+_METHODS = {"LU": 0, "CI": 1}
+
+
+def tt_sweep(cores, method="CI", forward=True, maxrank=INT64_MAX, reltol=1e-14, abstol=0.0):
+    """One factorize-and-carry sweep over a TT (tensortrain.jl:147-183,
+    conversion.jl:73-156).  Returns (cores, (rowindices, colindices) per bond,
+    pivoterrors)."""
+    links, ld, flat = _tt_pack(cores)
+    L = len(cores)
+    R = int(max(links[1:L])) if L > 1 else 0
+    piv = np.zeros(max(2 * int(links[1:L].sum()), 1), np.int32)
+    pe = np.zeros(R + 1)
+    _check(lib().orc_tt_sweep(L, _ip(links), _i32(ld), _dp(flat), _METHODS[method], int(bool(forward)),
+                              int(maxrank), float(reltol), float(abstol), _i32(piv), _dp(pe)))
+    pivots, q = [], 0
+    for b in range(L - 1):
+        p = int(links[b + 1])
+        pivots.append((piv[q:q + p].astype(np.int64).copy(), piv[q + p:q + 2 * p].astype(np.int64).copy()))
+        q += 2 * p
+    return _tt_unpack(links, ld, flat), pivots, pe
+
+
+def tt_compress(cores, method="LU", tolerance=1e-12, maxbonddim=INT64_MAX, normalizeerror=True):
+    """compress!(tt, method; tolerance, maxbonddim, normalizeerror) (tensortrain.jl:147-183)."""
+    links, ld, flat = _tt_pack(cores)
+    _check(lib().orc_tt_compress(len(cores), _ip(links), _i32(ld), _dp(flat), _METHODS[method],
+                                 float(tolerance), int(maxbonddim), int(bool(normalizeerror))))
+    return _tt_unpack(links, ld, flat)
 
 
 def make_opts(tolerance=1e-8, maxbonddim=INT64_MAX, maxiter=20, sweepstrategy="backandforth",

@@ -841,6 +841,162 @@ void orc_tt_evaluate(int32_t L, const int64_t *dims3, const double *cores, int64
     free(w);
 }
 
+/* ===================================================================== */
+/* Tensor-train sweeps — compress! (tensortrain.jl:95-183) and            */
+/* sweep1sitegetindices! (conversion.jl:73-156)                           */
+/* ===================================================================== */
+
+/* C = A·B (M×K · K×N, column-major, lda = M, ldb = K), each sum in
+ * ascending k from 0.0; the reference calls BLAS for these products */
+static void gemm_fixed(int64_t M, int64_t N, int64_t K, const double *A, const double *B, double *Cm)
+{
+    for (int64_t j = 0; j < N; ++j)
+        for (int64_t i = 0; i < M; ++i) {
+            double acc = 0.0;
+            for (int64_t k = 0; k < K; ++k) acc = acc + A[i + M * k] * B[k + K * j];
+            Cm[i + M * j] = acc;
+        }
+}
+
+/* The factors of one grouped site matrix (m×n): for method 0 left(lu),
+ * right(lu) with the permutations applied (matrixlu.jl:374-392), for
+ * method 1 left(luci), right(luci) (matrixluci.jl:70-84).  The diagonal of
+ * the factored matrix goes to diagp (|U_kk| or |L_kk|, pivoterrors at
+ * matrixlu.jl:394-416). */
+static int tt_factor(const double *A, int64_t m, int64_t n, int32_t method, int32_t leftorth,
+                     int64_t maxrank, double reltol, double abstol, int64_t **rp, int64_t **cp,
+                     double **left, double **right, int64_t *p_out, double **pe)
+{
+    int64_t cap = m < n ? m : n;
+    if (maxrank < cap) cap = maxrank;
+    int64_t *rowperm = (int64_t *)malloc(sizeof(int64_t) * (size_t)(m + 1));
+    int64_t *colperm = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    double *Lf = (double *)calloc((size_t)(m * cap + 1), sizeof(double));
+    double *Uf = (double *)calloc((size_t)(cap * n + 1), sizeof(double));
+    int64_t p = 0;
+    double err = 0.0;
+    int st = orc_rrlu(A, m, n, maxrank, reltol, abstol, leftorth, rowperm, colperm, Lf, Uf,
+                      cap > 0 ? cap : 1, &p, &err);
+    if (st != ORC_OK) {
+        free(rowperm); free(colperm); free(Lf); free(Uf);
+        return st;
+    }
+    double *l = (double *)calloc((size_t)(m * p + 1), sizeof(double));
+    double *r = (double *)calloc((size_t)(p * n + 1), sizeof(double));
+    const int64_t ldu = cap > 0 ? cap : 1;
+    if (method == 0) {
+        for (int64_t j = 0; j < p; ++j)
+            for (int64_t i = 0; i < m; ++i) l[(rowperm[i] - 1) + m * j] = Lf[i + m * j];
+        for (int64_t j = 0; j < n; ++j)
+            for (int64_t i = 0; i < p; ++i) r[i + p * (colperm[j] - 1)] = Uf[i + ldu * j];
+    } else {
+        orc_luci_left(m, n, p, leftorth, rowperm, colperm, Lf, Uf, ldu, l);
+        orc_luci_right(m, n, p, leftorth, rowperm, colperm, Lf, Uf, ldu, r);
+    }
+    /* pivoterrors(lu) = [|diag(lu)|..., lu.error]: diag of U with
+     * leftorthogonal, of L without (matrixlu.jl:394-400) */
+    double *e = (double *)malloc(sizeof(double) * (size_t)(p + 1));
+    for (int64_t k = 0; k < p; ++k) e[k] = fabs(leftorth ? Uf[k + ldu * k] : Lf[k + m * k]);
+    e[p] = err;
+    free(Lf); free(Uf);
+    *rp = rowperm; *cp = colperm; *left = l; *right = r; *p_out = p; *pe = e;
+    return ORC_OK;
+}
+
+int orc_tt_sweep(int32_t L, int64_t *linkdims, const int32_t *localdims, double *cores, int32_t method,
+                 int32_t forward, int64_t maxrank, double reltol, double abstol, int32_t *pivots,
+                 double *pivoterrors)
+{
+    if (L < 1) return fail(ORC_ERR_ARG, "a tensor train needs at least one site");
+    if (method != 0 && method != 1) return fail(ORC_ERR_ARG, "method must be :LU or :CI");
+    int64_t R = 0;
+    for (int32_t b = 1; b < L; ++b) R = linkdims[b] > R ? linkdims[b] : R;
+    if (pivoterrors)
+        for (int64_t k = 0; k <= R; ++k) pivoterrors[k] = 0.0;   /* zeros(rank(tt) + 1) */
+    double **T = (double **)malloc(sizeof(double *) * (size_t)L);
+    int64_t off = 0;
+    for (int32_t b = 0; b < L; ++b) {
+        const int64_t sz = linkdims[b] * localdims[b] * linkdims[b + 1];
+        T[b] = (double *)malloc(sizeof(double) * (size_t)(sz + 1));
+        memcpy(T[b], cores + off, sizeof(double) * (size_t)sz);
+        off += sz;
+    }
+    int64_t *perms = (int64_t *)malloc(sizeof(int64_t) * (size_t)(2 * R + 2) * (size_t)(L > 1 ? L - 1 : 1));
+    int64_t *npiv = (int64_t *)calloc((size_t)(L > 1 ? L - 1 : 1), sizeof(int64_t));
+    int st = ORC_OK;
+    for (int32_t i = 0; i < L - 1 && st == ORC_OK; ++i) {
+        /* the bond b sits between sites b and b+1; forward factors site b
+         * as (χ_b d_b)×χ_{b+1}, backward site b+1 as χ_{b+1}×(d_{b+1} χ_{b+2}) */
+        const int32_t b = forward ? i : L - 2 - i;
+        const int32_t s = forward ? b : b + 1;
+        const int64_t m = forward ? linkdims[b] * localdims[b] : linkdims[b + 1];
+        const int64_t n = forward ? linkdims[b + 1] : localdims[b + 1] * linkdims[b + 2];
+        int64_t *rp, *cp, p;
+        double *l, *r, *e;
+        st = tt_factor(T[s], m, n, method, forward, maxrank, reltol, abstol, &rp, &cp, &l, &r, &p, &e);
+        if (st != ORC_OK) break;
+        if (forward) {
+            /* site b ← left; site b+1 ← right · (χ_{b+1} × d_{b+1} χ_{b+2}) */
+            const int64_t nn = localdims[b + 1] * linkdims[b + 2];
+            double *nx = (double *)malloc(sizeof(double) * (size_t)(p * nn + 1));
+            gemm_fixed(p, nn, n, r, T[b + 1], nx);
+            free(T[b]); free(T[b + 1]); free(r);
+            T[b] = l;
+            T[b + 1] = nx;
+        } else {
+            /* site b+1 ← right; site b ← (χ_b d_b × χ_{b+1}) · left */
+            const int64_t mm = linkdims[b] * localdims[b];
+            double *nx = (double *)malloc(sizeof(double) * (size_t)(mm * p + 1));
+            gemm_fixed(mm, p, m, T[b], l, nx);
+            free(T[b]); free(T[b + 1]); free(l);
+            T[b + 1] = r;
+            T[b] = nx;
+        }
+        for (int64_t k = 0; k < p; ++k) {
+            perms[(size_t)b * (size_t)(2 * R + 2) + (size_t)k] = rp[k];
+            perms[(size_t)b * (size_t)(2 * R + 2) + (size_t)(p + k)] = cp[k];
+        }
+        linkdims[b + 1] = p;
+        npiv[b] = p;
+        if (pivoterrors)
+            for (int64_t k = 0; k <= p; ++k) pivoterrors[k] = jl_max(pivoterrors[k], e[k]);
+        free(rp); free(cp); free(e);
+    }
+    if (st == ORC_OK) {
+        off = 0;
+        for (int32_t b = 0; b < L; ++b) {
+            const int64_t sz = linkdims[b] * localdims[b] * linkdims[b + 1];
+            memcpy(cores + off, T[b], sizeof(double) * (size_t)sz);
+            off += sz;
+        }
+        if (pivots) {
+            int64_t q = 0;
+            for (int32_t b = 0; b < L - 1; ++b)
+                for (int64_t k = 0; k < 2 * npiv[b]; ++k)
+                    pivots[q++] = (int32_t)perms[(size_t)b * (size_t)(2 * R + 2) + (size_t)k];
+        }
+    }
+    for (int32_t b = 0; b < L; ++b) free(T[b]);
+    free(T);
+    free(perms);
+    free(npiv);
+    return st;
+}
+
+int orc_tt_compress(int32_t L, int64_t *linkdims, const int32_t *localdims, double *cores, int32_t method,
+                    double tolerance, int64_t maxbonddim, int32_t normalizeerror)
+{
+    /* left to right without truncation: _factorize with tolerance 0.0 and
+     * the default normalizeerror, so reltol = abstol = 0 */
+    int st = orc_tt_sweep(L, linkdims, localdims, cores, method, 1, INT64_MAX, 0.0, 0.0, NULL, NULL);
+    if (st != ORC_OK) return st;
+    /* right to left: reltol = tolerance (normalizeerror) or reltol = 1e-14
+     * and abstol = tolerance (tensortrain.jl:98-104) */
+    const double reltol = normalizeerror ? tolerance : 1e-14;
+    const double abstol = normalizeerror ? 0.0 : tolerance;
+    return orc_tt_sweep(L, linkdims, localdims, cores, method, 0, maxbonddim, reltol, abstol, NULL, NULL);
+}
+
 /* the functor math routines, exported for their accuracy tests */
 double orc_exp(double x) { return tci_exp(x); }
 double orc_cos(double x) { return tci_cos(x); }

@@ -143,6 +143,23 @@ void orc_eval_points(const orc_functor *f, int32_t L, int64_t npts, const int32_
  * product summed in ascending bond order */
 void orc_tt_evaluate(int32_t L, const int64_t *dims3, const double *cores, int64_t npts,
                      const int32_t *pts, double *out);
+/* One sweep over a tensor train whose cores are stored back to back
+ * (site b: linkdims[b]×localdims[b]×linkdims[b+1], column-major), as
+ * compress! does in each direction (tensortrain.jl:147-183) and
+ * sweep1sitegetindices! does (conversion.jl:73-156).  method 0 :LU
+ * (left/right of rrlu), 1 :CI (left/right of MatrixLUCI); leftorthogonal =
+ * forward.  linkdims and cores are updated in place (cores repacked with the
+ * new bonds).  pivots (optional): per bond in bond order, rowindices then
+ * colindices of the factored site (site b forward, b+1 backward), each the
+ * new bond dimension's worth of 1-based indices.  pivoterrors (optional, rank(tt)+1
+ * entries): the running max of pivoterrors(lu) over the bonds. */
+int orc_tt_sweep(int32_t L, int64_t *linkdims, const int32_t *localdims, double *cores, int32_t method,
+                 int32_t forward, int64_t maxrank, double reltol, double abstol, int32_t *pivots,
+                 double *pivoterrors);
+/* compress!(tt, method; tolerance, maxbonddim, normalizeerror)
+ * (tensortrain.jl:95-183) for method :LU (0) or :CI (1) */
+int orc_tt_compress(int32_t L, int64_t *linkdims, const int32_t *localdims, double *cores, int32_t method,
+                    double tolerance, int64_t maxbonddim, int32_t normalizeerror);
 /* algorithmic work counters: [0] Pi elements evaluated, [1] rrLU argmax bytes,
  * [2] fused rrLU bytes, [3] Schur flops, [4] number of rrLU, [5] number of evals */
 void orc_tci_workstats(const orc_tci *t, double *out6);

@@ -29,7 +29,8 @@ __all__ = [
     "DeviceFunctor", "Lorentz", "WeightedLorentz", "QuanticsGaussCos", "GaussSum", "SumFunctor",
     "TableFunctor", "TTFunctor", "QuanticsExp", "batchevaluate", "TensorCI2", "crossinterpolate2",
     "crossinterpolate2_batch", "TCI_UNLIMITED", "capacity_plan", "tt_evaluate", "sample_points",
-    "tensorci2_from_sets", "optimize", "CachedFunction",
+    "tensorci2_from_sets", "optimize", "CachedFunction", "TensorTrain", "compress", "tt_sweep",
+    "tensorci2_from_tt",
 ]
 
 
@@ -467,6 +468,172 @@ def tt_evaluate(sitetensors, points, ctx=None):
     _check(_abi.lib().tci_tt_evaluate_f64(ctx.handle, L, _abi.i64ptr(links), _abi.i32ptr(ld),
                                           _abi.dptr(cores), x.shape[0], _abi.u8ptr(x), _abi.dptr(out)))
     return out[:x.shape[0]]
+
+
+class TensorTrain:
+    """TensorTrain{Float64,3} (tensortrain.jl:1-64): site tensors of shape
+    (χ_b, d_b, χ_{b+1}); evaluate/compress run on the device."""
+
+    def __init__(self, sitetensors):
+        ts = [np.asarray(t, np.float64) for t in (sitetensors.sitetensors if hasattr(sitetensors, "sitetensors")
+                                                  else sitetensors)]
+        for b, t in enumerate(ts):
+            if t.ndim != 3:
+                raise ArgumentError(3, f"site tensor {b + 1} must have 3 dimensions")
+            if b + 1 < len(ts) and t.shape[2] != ts[b + 1].shape[0]:
+                raise ArgumentError(3, f"The tensors at {b + 1} and {b + 2} must share a common link index")
+        self.sitetensors = [np.array(t, order="F") for t in ts]
+
+    def __len__(self):
+        return len(self.sitetensors)
+
+    def linkdims(self):
+        """[size(T, 3) for T in sitetensors[1:end-1]] (abstracttensortrain.jl:29-31)."""
+        return [t.shape[2] for t in self.sitetensors[:-1]]
+
+    def rank(self):
+        return max(self.linkdims()) if len(self) > 1 else 0
+
+    def sitedims(self):
+        return [[t.shape[1]] for t in self.sitetensors]
+
+    def evaluate(self, points, ctx=None):
+        return tt_evaluate(self.sitetensors, points, ctx)
+
+    def __call__(self, indexset):
+        return float(self.evaluate(np.asarray(indexset).reshape(1, -1))[0])
+
+    def _packed(self):
+        links = np.array([self.sitetensors[0].shape[0]] + [t.shape[2] for t in self.sitetensors], np.int64)
+        ld = np.array([t.shape[1] for t in self.sitetensors], np.int32)
+        flat = np.ascontiguousarray(np.concatenate([t.ravel(order="F") for t in self.sitetensors]))
+        return links, ld, flat
+
+    def _unpack(self, links, ld, flat):
+        out, off = [], 0
+        for b in range(len(ld)):
+            sz = int(links[b] * ld[b] * links[b + 1])
+            out.append(np.array(flat[off:off + sz].reshape((int(links[b]), int(ld[b]), int(links[b + 1])),
+                                                           order="F"), order="F"))
+            off += sz
+        self.sitetensors = out
+
+
+_FACTOR = {"LU": _abi.FACTOR_LU, "CI": _abi.FACTOR_CI, "SVD": _abi.FACTOR_SVD}
+
+
+def _method(method):
+    key = method.lstrip(":").upper() if isinstance(method, str) else method
+    if key not in _FACTOR:
+        raise TCIError(8, "Not implemented yet.")
+    return _FACTOR[key]
+
+
+def compress(tt, method="LU", tolerance=1e-12, maxbonddim=TCI_UNLIMITED, normalizeerror=True, ctx=None):
+    """compress!(tt, method; tolerance, maxbonddim, normalizeerror)
+    (tensortrain.jl:147-183) on the device, in place: a left-to-right sweep
+    without truncation, then a right-to-left one truncating each bond with
+    rrLU (:LU) or MatrixLUCI (:CI).  :SVD is refused (ArgumentError)."""
+    code = _method(method)
+    ctx = ctx or default_context()
+    links, ld, flat = tt._packed()
+    _check(_abi.lib().tci_tt_compress_f64(ctx.handle, len(tt), _abi.i64ptr(links), _abi.i32ptr(ld),
+                                          _abi.dptr(flat), code, float(tolerance), int(maxbonddim),
+                                          int(bool(normalizeerror))))
+    tt._unpack(links, ld, flat)
+
+
+def _kron_left(Iset, d):
+    """kronecker(Iset, localdim) (tensorci2.jl:315-320): Iset index fastest."""
+    return [list(i) + [s] for s in range(1, d + 1) for i in Iset]
+
+
+def _kron_right(d, Jset):
+    """kronecker(localdim, Jset) (tensorci2.jl:322-327): local index fastest."""
+    return [[s] + list(j) for j in Jset for s in range(1, d + 1)]
+
+
+def tt_sweep(tt, method="CI", forward=True, maxrank=TCI_UNLIMITED, reltol=1e-14, abstol=0.0, ctx=None):
+    """One sweep of factorizations over tt on the device (tci_tt_sweep_f64),
+    in place: per bond, the rrLU of the grouped site (leftorthogonal =
+    forward), the site replaced by left (forward) or right (backward) of
+    rrLU (:LU) or MatrixLUCI (:CI) and the other factor multiplied into the
+    neighbour (tensortrain.jl:155-182, conversion.jl:105-141).  Returns the
+    per-bond rowindices, colindices and the running max of pivoterrors."""
+    code = _method(method)
+    ctx = ctx or default_context()
+    L = len(tt)
+    links, ld, flat = tt._packed()
+    R = int(max(links[1:L])) if L > 1 else 0
+    piv = np.zeros(max(2 * int(links[1:L].sum()), 1), np.int32)
+    pe = np.zeros(R + 1)
+    _check(_abi.lib().tci_tt_sweep_f64(ctx.handle, L, _abi.i64ptr(links), _abi.i32ptr(ld), _abi.dptr(flat),
+                                       code, int(bool(forward)), int(maxrank), float(reltol),
+                                       float(abstol), _abi.i32ptr(piv), _abi.dptr(pe)))
+    tt._unpack(links, ld, flat)
+    rows, cols, q = [], [], 0
+    for b in range(L - 1):
+        p = int(links[b + 1])
+        rows.append([int(v) for v in piv[q:q + p]])
+        cols.append([int(v) for v in piv[q + p:q + 2 * p]])
+        q += 2 * p
+    return rows, cols, pe
+
+
+def _sweep1sitegetindices(tt, forward, spectator=None, maxbonddim=TCI_UNLIMITED, tolerance=0.0, ctx=None):
+    """sweep1sitegetindices!(tt, forwardsweep, spectatorindices; maxbonddim,
+    tolerance) (conversion.jl:73-156): the factorizations and products on the
+    device (MatrixLUCI with abstol = tolerance, leftorthogonal = forward),
+    the index bookkeeping here.  Mutates tt and spectator."""
+    L = len(tt)
+    ld = [s[0] for s in tt.sitedims()]
+    rows, cols, pe = tt_sweep(tt, "CI", forward, maxbonddim, 1e-14, tolerance, ctx)
+    indexset = [[[]]]
+    for i in range(L - 1):
+        if forward:
+            b = i   # site b, bond b
+            indexset.append([_kron_left(indexset[-1], ld[b])[r - 1] for r in rows[b]])
+            if spectator:
+                spectator[b] = [spectator[b][c - 1] for c in cols[b]]
+        else:
+            b = L - 2 - i   # site b+1, bond b
+            indexset.append([_kron_right(ld[b + 1], indexset[-1])[c - 1] for c in cols[b]])
+            if spectator:
+                spectator[b + 1] = [spectator[b + 1][r - 1] for r in rows[b]]
+    return (indexset if forward else indexset[::-1]), pe
+
+
+def tensorci2_from_tt(tt, tolerance=1e-12, maxbonddim=TCI_UNLIMITED, maxiter=3, ctx=None):
+    """TensorCI2{Float64}(tt::TensorTrain; tolerance, maxbonddim, maxiter)
+    (conversion.jl:158-176): pivot sets from alternating MatrixLUCI sweeps
+    over tt (which is modified in place, as in the reference), the site
+    tensors of the swept train, maxsamplevalue the largest |entry|."""
+    if len(tt) < 2:
+        raise TCIError(8, "localdims should have at least 2 elements!")
+    Iset, _ = _sweep1sitegetindices(tt, True, maxbonddim=maxbonddim, tolerance=tolerance, ctx=ctx)
+    Jset, pivoterrors = _sweep1sitegetindices(tt, False, maxbonddim=maxbonddim, tolerance=tolerance, ctx=ctx)
+    for it in range(3, maxiter + 1):
+        if it % 2 == 1:
+            Isetnew, pivoterrors = _sweep1sitegetindices(tt, True, Jset, ctx=ctx)
+            if Isetnew == Iset:
+                break
+        else:
+            Jsetnew, pivoterrors = _sweep1sitegetindices(tt, False, Iset, ctx=ctx)
+            if Jsetnew == Jset:
+                break
+    t = TensorCI2.__new__(TensorCI2)
+    t.localdims = [s[0] for s in tt.sitedims()]
+    L = len(t.localdims)
+    t.Iset, t.Jset = Iset, Jset
+    t.sitetensors = tt.sitetensors
+    t.pivoterrors = np.asarray(pivoterrors)
+    t.bonderrors = np.zeros(L - 1)
+    t.maxsamplevalue = float(max(np.max(np.abs(s)) for s in t.sitetensors))
+    t.Iset_history, t.Jset_history = [], []
+    t.status, t.niter = 0, 0
+    t.ranks, t.errors, t.nglobalpivots = np.zeros(0, np.int64), np.zeros(0), np.zeros(0, np.int64)
+    t.workstats = {}
+    return t
 
 
 def sample_points(localdims, n, seed=0):

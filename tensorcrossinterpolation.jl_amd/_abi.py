@@ -35,6 +35,7 @@ F_SUM = 5
 F_TABLE = 6
 F_TT = 7
 F_QUANTICS_EXP = 8
+FACTOR_LU, FACTOR_CI, FACTOR_SVD = 0, 1, 2
 FLAG_NO_FINAL_SWEEP = 1        # TCI_FLAG_NO_FINAL_SWEEP
 
 
@@ -215,6 +216,12 @@ def lib():
                                       P(C.c_uint8), P(C.c_double)]
     L.tci_batch_evaluate.argtypes = [vp, C.c_int32, C.c_int64, P(C.c_uint8), P(C.c_double), P(C.c_double),
                                      P(C.c_double), P(C.c_double)]
+    L.tci_gemm_f64.argtypes = [vp, C.c_int64, C.c_int64, C.c_int64, P(C.c_double), C.c_int64, P(C.c_double),
+                               C.c_int64, P(C.c_double), C.c_int64]
+    L.tci_tt_compress_f64.argtypes = [vp, C.c_int32, P(C.c_int64), P(C.c_int32), P(C.c_double), C.c_int32,
+                                      C.c_double, C.c_int64, C.c_int32]
+    L.tci_tt_sweep_f64.argtypes = [vp, C.c_int32, P(C.c_int64), P(C.c_int32), P(C.c_double), C.c_int32, C.c_int32,
+                                   C.c_int64, C.c_double, C.c_double, P(C.c_int32), P(C.c_double)]
     _LIB = L
     return L
 
@@ -228,4 +235,5 @@ EXPORTED_SYMBOLS = [
     "tci_batch_pivoterrors", "tci_batch_bonderrors", "tci_batch_free", "tci_batch_elapsed_ms",
     "tci_batch_profile", "tci_capacity_plan", "tci_tt_evaluate_f64", "tci_batch_evaluate",
     "tci_ctx_set_rrlu_algo", "tci_rrlu_batched_noalloc_f64", "tci_batch_cachestats", "tci_batch_cachedata",
+    "tci_gemm_f64", "tci_tt_compress_f64", "tci_tt_sweep_f64",
 ]

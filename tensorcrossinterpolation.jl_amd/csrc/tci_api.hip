@@ -647,6 +647,262 @@ tci_status tci_luci_f64(tci_ctx *ctx, const double *A, int64_t lda, int64_t m, i
     return TCI_OK;
 }
 
+tci_status tci_gemm_f64(tci_ctx *ctx, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
+                        const double *B, int64_t ldb, double *Cm, int64_t ldc)
+{
+    if (!ctx || M < 0 || N < 0 || K < 0 || lda < (M > 0 ? M : 1) || ldb < (K > 0 ? K : 1) || ldc < (M > 0 ? M : 1))
+        return set_err(TCI_ERR_ARG, "bad arguments");
+    if (M * N == 0) return TCI_OK;
+    if ((K > 0 && (!A || !B)) || !Cm) return set_err(TCI_ERR_ARG, "NULL buffer");
+    if (M * K > ((int64_t)1 << 31) || K * N > ((int64_t)1 << 31) || M * N > ((int64_t)1 << 31))
+        return set_err(TCI_ERR_ARG, "matrix too large");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    double *d = nullptr;
+    const size_t na = (size_t)(M * K), nb = (size_t)(K * N), nc = (size_t)(M * N);
+    HIPCHK(hipMallocAsync((void **)&d, sizeof(double) * (na + nb + nc + 1), st));
+    hipError_t e = hipSuccess;
+    if (na) e = hipMemcpy2DAsync(d, sizeof(double) * M, A, sizeof(double) * lda, sizeof(double) * M, K,
+                                 hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && nb)
+        e = hipMemcpy2DAsync(d + na, sizeof(double) * K, B, sizeof(double) * ldb, sizeof(double) * K, N,
+                             hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_gemm(M, N, K, d, M, d + na, K > 0 ? K : 1, d + na + nb, M, st);
+    if (e == hipSuccess)
+        e = hipMemcpy2DAsync(Cm, sizeof(double) * ldc, d + na + nb, sizeof(double) * M, sizeof(double) * M, N,
+                             hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFreeAsync(d, st);
+    if (e != hipSuccess) return set_err(TCI_ERR_DEVICE, "gemm: %s", hipGetErrorString(e));
+    return TCI_OK;
+}
+
+// ---------------------------------------------------------- tensor trains
+// A tensor train resident on the device for compress! and the TT →
+// TensorCI2 sweeps: each site keeps the slot (and capacity) it had on
+// upload, since no sweep step makes a site larger (a new bond is at most the
+// old one).  Work buffers hold the unpacked L and U, the factor that is
+// carried to the neighbour and the product, each sized for the largest site.
+struct TtDevice {
+    tci_ctx *ctx = nullptr;
+    int L = 0;
+    std::vector<int64_t> links;
+    std::vector<int32_t> d;
+    std::vector<int64_t> off;   // slot of each site in `cores` (doubles)
+    int64_t maxsite = 0, maxside = 0;
+    uint8_t *arena = nullptr;
+    double *cores = nullptr, *wl = nullptr, *wu = nullptr, *wc = nullptr, *wp = nullptr, *diag = nullptr;
+    int32_t *perm = nullptr;
+    tci_lu_job *job = nullptr;
+    ~TtDevice()
+    {
+        if (arena) (void)hipFreeAsync(arena, ctx->stream);
+    }
+};
+
+static int64_t tt_site_size(const TtDevice &T, int b) { return T.links[(size_t)b] * T.d[(size_t)b] * T.links[(size_t)b + 1]; }
+
+static tci_status tt_check(int32_t L, const int64_t *linkdims, const int32_t *localdims, const double *cores)
+{
+    if (L < 1 || !linkdims || !localdims || !cores) return set_err(TCI_ERR_ARG, "bad arguments");
+    for (int32_t b = 0; b <= L; ++b)
+        if (linkdims[b] < 1 || linkdims[b] > INT32_MAX / 2) return set_err(TCI_ERR_ARG, "bond %d out of range", b);
+    for (int32_t b = 0; b < L; ++b) {
+        if (localdims[b] < 1) return set_err(TCI_ERR_ARG, "local dimension %d out of range", b + 1);
+        if (linkdims[b] * localdims[b] > ((int64_t)1 << 31) / linkdims[b + 1] ||
+            localdims[b] * linkdims[b + 1] > ((int64_t)1 << 31) / linkdims[b])
+            return set_err(TCI_ERR_ARG, "site %d too large", b + 1);
+    }
+    return TCI_OK;
+}
+
+static tci_status tt_upload(tci_ctx *ctx, TtDevice &T, int32_t L, const int64_t *linkdims, const int32_t *localdims,
+                            const double *cores)
+{
+    T.ctx = ctx;
+    T.L = L;
+    T.links.assign(linkdims, linkdims + L + 1);
+    T.d.assign(localdims, localdims + L);
+    T.off.resize((size_t)L);
+    int64_t tot = 0;
+    for (int b = 0; b < L; ++b) {
+        T.off[(size_t)b] = tot;
+        const int64_t sz = tt_site_size(T, b);
+        tot += sz;
+        T.maxsite = sz > T.maxsite ? sz : T.maxsite;
+        const int64_t s1 = T.links[(size_t)b] * T.d[(size_t)b], s2 = T.d[(size_t)b] * T.links[(size_t)b + 1];
+        const int64_t side = (s1 > s2 ? s1 : s2) + T.links[(size_t)b] + T.links[(size_t)b + 1];
+        T.maxside = side > T.maxside ? side : T.maxside;
+    }
+    // cores | L | U | carried factor | product | diagonal | perm | job
+    const size_t b_cores = lay_align(sizeof(double) * (size_t)tot);
+    const size_t b_w = lay_align(sizeof(double) * (size_t)T.maxsite);
+    const size_t b_diag = lay_align(sizeof(double) * (size_t)T.maxside);
+    const size_t b_perm = lay_align(sizeof(int32_t) * (size_t)T.maxside);
+    const size_t bytes = b_cores + 4 * b_w + b_diag + b_perm + lay_align(sizeof(tci_lu_job));
+    HIPCHK(hipMallocAsync((void **)&T.arena, bytes, ctx->stream));
+    uint8_t *p = T.arena;
+    T.cores = (double *)p;
+    p += b_cores;
+    T.wl = (double *)p;
+    p += b_w;
+    T.wu = (double *)p;
+    p += b_w;
+    T.wc = (double *)p;
+    p += b_w;
+    T.wp = (double *)p;
+    p += b_w;
+    T.diag = (double *)p;
+    p += b_diag;
+    T.perm = (int32_t *)p;
+    p += b_perm;
+    T.job = (tci_lu_job *)p;
+    HIPCHK(hipMemcpyAsync(T.cores, cores, sizeof(double) * (size_t)tot, hipMemcpyHostToDevice, ctx->stream));
+    return TCI_OK;
+}
+
+// the cores back to back with the current bonds
+static tci_status tt_download(TtDevice &T, int64_t *linkdims, double *cores)
+{
+    int64_t o = 0;
+    for (int b = 0; b < T.L; ++b) {
+        const int64_t sz = tt_site_size(T, b);
+        HIPCHK(hipMemcpyAsync(cores + o, T.cores + T.off[(size_t)b], sizeof(double) * (size_t)sz,
+                              hipMemcpyDeviceToHost, T.ctx->stream));
+        o += sz;
+    }
+    HIPCHK(hipStreamSynchronize(T.ctx->stream));
+    for (int b = 0; b <= T.L; ++b) linkdims[b] = T.links[(size_t)b];
+    return TCI_OK;
+}
+
+// One sweep (tensortrain.jl:147-183 in either direction, conversion.jl:95-147):
+// per bond, the rrLU of the grouped site matrix, its left/right factors
+// (method 0 rrLU with permutations, 1 MatrixLUCI), the factored site
+// replaced by one of them and the other multiplied into the neighbour.
+static tci_status tt_sweep(TtDevice &T, int32_t method, int32_t forward, int64_t maxrank, double reltol,
+                           double abstol, int32_t *pivots, double *pivoterrors)
+{
+    tci_ctx *ctx = T.ctx;
+    hipStream_t st = ctx->stream;
+    const int L = T.L;
+    int64_t R = 0;
+    for (int b = 1; b < L; ++b) R = T.links[(size_t)b] > R ? T.links[(size_t)b] : R;
+    if (pivoterrors)
+        for (int64_t k = 0; k <= R; ++k) pivoterrors[k] = 0.0;
+    std::vector<std::vector<int32_t>> piv((size_t)(L > 1 ? L - 1 : 0));
+    std::vector<int32_t> hperm((size_t)T.maxside);
+    std::vector<double> hdiag((size_t)T.maxside);
+    for (int i = 0; i < L - 1; ++i) {
+        const int b = forward ? i : L - 2 - i;
+        const int s = forward ? b : b + 1;
+        const int64_t m = forward ? T.links[(size_t)b] * T.d[(size_t)b] : T.links[(size_t)b + 1];
+        const int64_t n = forward ? T.links[(size_t)b + 1] : T.d[(size_t)b + 1] * T.links[(size_t)b + 2];
+        double *A = T.cores + T.off[(size_t)s];
+        tci_lu_job j;
+        memset(&j, 0, sizeof j);
+        j.A = A;
+        j.rowperm = T.perm;
+        j.colperm = T.perm + m;
+        j.m = m;
+        j.n = n;
+        j.maxrank = maxrank;
+        j.reltol = reltol;
+        j.abstol = abstol;
+        j.leftorth = forward ? 1 : 0;
+        HIPCHK(hipMemcpyAsync(T.job, &j, sizeof j, hipMemcpyHostToDevice, st));
+        tci_status ls = tci_rrlu_batched_f64(ctx, T.job, 1);
+        if (ls != TCI_OK) return ls;
+        // the pivots are the diagonal of the factored matrix
+        const int64_t mn = m < n ? m : n;
+        HIPCHK(hipMemcpyAsync(&j, T.job, sizeof j, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpy2DAsync(hdiag.data(), sizeof(double), A, sizeof(double) * (size_t)(m + 1), sizeof(double),
+                                (size_t)mn, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(hperm.data(), T.perm, sizeof(int32_t) * (size_t)(m + n), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (j.status == TCI_ERR_NAN_L) return set_err(TCI_ERR_NAN_L, "lu.L contains NaNs");
+        if (j.status == TCI_ERR_NAN_U) return set_err(TCI_ERR_NAN_U, "lu.U contains NaNs");
+        if (j.status != TCI_OK) return set_err((tci_status)j.status, "rrlu failed at bond %d", b + 1);
+        const int64_t p = j.npivot;
+        HIPCHK(launch_lu_unpack(A, m, n, p, j.leftorth, T.wl, m, T.wu, p, st));
+        // forward: the site takes left (m×p), right goes to wc; backward:
+        // the site takes right (p×n), left goes to wc
+        double *lft = forward ? A : T.wc, *rgt = forward ? T.wc : A;
+        if (method == 0)
+            HIPCHK(launch_lu_permute(m, n, p, T.perm, T.perm + m, T.wl, T.wu, lft, m, rgt, p, st));
+        else
+            HIPCHK(launch_luci(m, n, p, j.leftorth, T.perm, T.perm + m, T.wl, T.wu, lft, m, rgt, p, st));
+        if (forward) {
+            // site b+1 ← right · (χ_{b+1} × d_{b+1} χ_{b+2})
+            const int64_t nn = T.d[(size_t)b + 1] * T.links[(size_t)b + 2];
+            double *nx = T.cores + T.off[(size_t)b + 1];
+            HIPCHK(launch_gemm(p, nn, n, T.wc, p, nx, n, T.wp, p, st));
+            HIPCHK(hipMemcpyAsync(nx, T.wp, sizeof(double) * (size_t)(p * nn), hipMemcpyDeviceToDevice, st));
+        } else {
+            // site b ← (χ_b d_b × χ_{b+1}) · left
+            const int64_t mm = T.links[(size_t)b] * T.d[(size_t)b];
+            double *pv = T.cores + T.off[(size_t)b];
+            HIPCHK(launch_gemm(mm, p, m, pv, mm, T.wc, m, T.wp, mm, st));
+            HIPCHK(hipMemcpyAsync(pv, T.wp, sizeof(double) * (size_t)(mm * p), hipMemcpyDeviceToDevice, st));
+        }
+        T.links[(size_t)b + 1] = p;
+        piv[(size_t)b].assign(hperm.data(), hperm.data() + p);
+        piv[(size_t)b].insert(piv[(size_t)b].end(), hperm.data() + m, hperm.data() + m + p);
+        if (pivoterrors) {
+            for (int64_t k = 0; k < p; ++k) pivoterrors[k] = nan_max(pivoterrors[k], fabs(hdiag[(size_t)k]));
+            pivoterrors[p] = nan_max(pivoterrors[p], j.error);
+        }
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    if (pivots) {
+        int64_t q = 0;
+        for (const auto &v : piv)
+            for (int32_t x : v) pivots[q++] = x;
+    }
+    return TCI_OK;
+}
+
+tci_status tci_tt_sweep_f64(tci_ctx *ctx, int32_t L, int64_t *linkdims, const int32_t *localdims, double *cores,
+                            int32_t method, int32_t forward, int64_t maxrank, double reltol, double abstol,
+                            int32_t *pivots, double *pivoterrors)
+{
+    if (!ctx) return set_err(TCI_ERR_ARG, "bad arguments");
+    tci_status s = tt_check(L, linkdims, localdims, cores);
+    if (s != TCI_OK) return s;
+    if (method != TCI_FACTOR_LU && method != TCI_FACTOR_CI)
+        return set_err(TCI_ERR_ARG, "method must be TCI_FACTOR_LU or TCI_FACTOR_CI");
+    if (maxrank < 1) return set_err(TCI_ERR_ARG, "maxrank must be >= 1");
+    HIPCHK(hipSetDevice(ctx->device));
+    TtDevice T;
+    s = tt_upload(ctx, T, L, linkdims, localdims, cores);
+    if (s == TCI_OK) s = tt_sweep(T, method, forward ? 1 : 0, maxrank, reltol, abstol, pivots, pivoterrors);
+    if (s == TCI_OK) s = tt_download(T, linkdims, cores);
+    return s;
+}
+
+tci_status tci_tt_compress_f64(tci_ctx *ctx, int32_t L, int64_t *linkdims, const int32_t *localdims, double *cores,
+                               int32_t method, double tolerance, int64_t maxbonddim, int32_t normalizeerror)
+{
+    if (!ctx) return set_err(TCI_ERR_ARG, "bad arguments");
+    tci_status s = tt_check(L, linkdims, localdims, cores);
+    if (s != TCI_OK) return s;
+    if (method == TCI_FACTOR_SVD)
+        return set_err(TCI_ERR_ARG, "compress! with :SVD is not provided by this library (use :LU or :CI)");
+    if (method != TCI_FACTOR_LU && method != TCI_FACTOR_CI) return set_err(TCI_ERR_GENERIC, "Not implemented yet.");
+    if (maxbonddim < 1) return set_err(TCI_ERR_ARG, "maxbonddim must be >= 1");
+    HIPCHK(hipSetDevice(ctx->device));
+    TtDevice T;
+    s = tt_upload(ctx, T, L, linkdims, localdims, cores);
+    // left to right with no truncation (tolerance 0.0: reltol = abstol = 0),
+    // then right to left with the caller's truncation (tensortrain.jl:98-104)
+    if (s == TCI_OK) s = tt_sweep(T, method, 1, TCI_UNLIMITED, 0.0, 0.0, nullptr, nullptr);
+    if (s == TCI_OK)
+        s = tt_sweep(T, method, 0, maxbonddim, normalizeerror ? tolerance : 1e-14, normalizeerror ? 0.0 : tolerance,
+                     nullptr, nullptr);
+    if (s == TCI_OK) s = tt_download(T, linkdims, cores);
+    return s;
+}
+
 // ------------------------------------------------------------ Pi tensors
 tci_status tci_pi_eval_f64(tci_ctx *ctx, const tci_functor *f, int32_t L, const int32_t *localdims,
                            const uint8_t *left, int64_t nI, int32_t nl, const uint8_t *right,

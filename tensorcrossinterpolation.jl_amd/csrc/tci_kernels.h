@@ -29,11 +29,18 @@ hipError_t launch_rrlu(tci_lu_job *jobs_device, int64_t count, int64_t max_side,
 constexpr int kTileSide = 128;
 hipError_t launch_rrlu_tile(tci_lu_job *jobs_device, const int32_t *idx, int64_t count, double *scratch,
                             const int64_t *soff, hipStream_t st);
+// C = A·B with each sum in ascending k (M×K · K×N, column-major)
+hipError_t launch_gemm(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
+                       double *Cm, int64_t ldc, hipStream_t st);
 // left(luci) (m×p, ld ldl) and right(luci) (p×n, ld ldr) of MatrixLUCI from
 // the unpacked L (m×p) and U (p×n) and the 1-based permutations
 hipError_t launch_luci(int64_t m, int64_t n, int64_t p, int leftorth, const int32_t *rowperm, const int32_t *colperm,
                        const double *L, const double *U, double *left, int64_t ldl, double *right, int64_t ldr,
                        hipStream_t st);
+// left(lu)/right(lu) with the permutations applied, from the unpacked L, U
+hipError_t launch_lu_permute(int64_t m, int64_t n, int64_t p, const int32_t *rowperm, const int32_t *colperm,
+                             const double *L, const double *U, double *left, int64_t ldl, double *right, int64_t ldr,
+                             hipStream_t st);
 hipError_t launch_lu_unpack(const double *A, int64_t m, int64_t n, int64_t p, int leftorth,
                             double *L, int64_t ldl, double *U, int64_t ldu, hipStream_t st);
 hipError_t launch_kron_center(const uint8_t *left, int64_t nI, int nl, const int32_t *cdims, int M,

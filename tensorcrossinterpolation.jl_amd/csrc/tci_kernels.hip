@@ -188,6 +188,33 @@ __global__ void tci_lu_unpack_kernel(const double *__restrict__ A, int64_t m, in
     }
 }
 
+// ------------------------------------------------------------ products
+// C = A·B (M×K times K×N, column-major), one thread per entry of C, each
+// sum in ascending k from 0.0 with a rounding per product and per sum (the
+// order of the oracle's loops; the reference calls BLAS here).
+__global__ void tci_gemm_kernel(int64_t M, int64_t N, int64_t K, const double *__restrict__ A, int64_t lda,
+                                const double *__restrict__ B, int64_t ldb, double *__restrict__ Cm, int64_t ldc)
+{
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < M * N;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = q / M, i = q - j * M;
+        double acc = 0.0;
+        for (int64_t k = 0; k < K; ++k) acc = acc + A[i + lda * k] * B[k + ldb * j];
+        Cm[i + ldc * j] = acc;
+    }
+}
+
+hipError_t launch_gemm(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
+                       double *Cm, int64_t ldc, hipStream_t st)
+{
+    const int64_t tot = M * N;
+    if (tot == 0) return hipSuccess;
+    int64_t blocks = (tot + 255) / 256;
+    if (blocks > 65535) blocks = 65535;
+    hipLaunchKernelGGL(tci_gemm_kernel, dim3((unsigned)blocks), dim3(256), 0, st, M, N, K, A, lda, B, ldb, Cm, ldc);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------- LUCI factors
 // left(luci) and right(luci) of MatrixLUCI (matrixluci.jl:40-84) from the
 // unpacked factors L (m×p, lda m) and U (p×n, lda p) of one rrLU with
@@ -262,6 +289,40 @@ hipError_t launch_luci(int64_t m, int64_t n, int64_t p, int leftorth, const int3
     int64_t blocks = (tot + 255) / 256;
     if (blocks > 65535) blocks = 65535;
     hipLaunchKernelGGL(tci_luci_kernel, dim3((unsigned)blocks), dim3(256), 0, st, m, n, p, leftorth, rowperm, colperm,
+                       L, U, left, ldl, right, ldr);
+    return hipGetLastError();
+}
+
+// left(lu) and right(lu) with permute=true (matrixlu.jl:374-392): row i of
+// L goes to row rowperm[i]-1 of left, column j of U to column colperm[j]-1
+// of right (L m×p with lda m, U p×n with lda p; permutations 1-based)
+__global__ void tci_lu_permute_kernel(int64_t m, int64_t n, int64_t p, const int32_t *__restrict__ rowperm,
+                                      const int32_t *__restrict__ colperm, const double *__restrict__ L,
+                                      const double *__restrict__ U, double *__restrict__ left, int64_t ldl,
+                                      double *__restrict__ right, int64_t ldr)
+{
+    const int64_t nl = m * p, nu = p * n;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nl + nu;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        if (q < nl) {
+            const int64_t j = q / m, i = q - j * m;
+            left[(rowperm[i] - 1) + ldl * j] = L[i + m * j];
+        } else {
+            const int64_t r = q - nl, j = r / p, i = r - j * p;
+            right[i + ldr * (colperm[j] - 1)] = U[i + p * j];
+        }
+    }
+}
+
+hipError_t launch_lu_permute(int64_t m, int64_t n, int64_t p, const int32_t *rowperm, const int32_t *colperm,
+                             const double *L, const double *U, double *left, int64_t ldl, double *right, int64_t ldr,
+                             hipStream_t st)
+{
+    const int64_t tot = m * p + p * n;
+    if (tot == 0) return hipSuccess;
+    int64_t blocks = (tot + 255) / 256;
+    if (blocks > 65535) blocks = 65535;
+    hipLaunchKernelGGL(tci_lu_permute_kernel, dim3((unsigned)blocks), dim3(256), 0, st, m, n, p, rowperm, colperm,
                        L, U, left, ldl, right, ldr);
     return hipGetLastError();
 }
