@@ -206,6 +206,22 @@ tci_status tci_tt_evaluate_f64(tci_ctx *ctx, int32_t L, const int64_t *linkdims,
                                const int32_t *localdims, const double *cores, int64_t npts,
                                const uint8_t *pts, double *out);
 
+/* The per-point search of estimatetrueerror(tt, f; nsearch, initialpoints)
+ * (globalsearch.jl:18-41): _floatingzone(ttcache, f; earlystoptol,
+ * nsweeps, initp) (globalsearch.jl:43-95) from each of npts start points,
+ * one workgroup per point.  points: npts × L bytes (1-based), the start
+ * points on entry and the found pivots on return; errors[npts]: the
+ * largest |f − TT| seen from each start.  The caller orders the pairs by
+ * descending error and drops repeats (globalsearch.jl:37-40).  TT values
+ * are l·T_p[:,σ,:]·R with the left row vector and the right environment of
+ * the current point, each sum in ascending bond order.  A NaN error ends
+ * that point's search (the reference would sweep without end).  nsweeps:
+ * at most that many sweeps (>= 1; TCI_UNLIMITED for the reference's
+ * default).  Synchronous; host buffers. */
+tci_status tci_tt_floatingzone_f64(tci_ctx *ctx, const tci_functor *f, int32_t L, const int64_t *linkdims,
+                                   const int32_t *localdims, const double *cores, int64_t npts, uint8_t *points,
+                                   double *errors, int64_t nsweeps, double earlystoptol);
+
 /* Whole TCI2 solves ----------------------------------------------------- */
 /* optimize! keyword arguments (tensorci2.jl:700-720) plus two device
  * capacities.  Defaults: tci_options_default(). */

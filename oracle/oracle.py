@@ -111,6 +111,8 @@ def lib():
                                    C.c_int64, C.c_double, C.c_double, P(C.c_int32), P(C.c_double)]
         L.orc_tt_compress.argtypes = [C.c_int32, P(C.c_int64), P(C.c_int32), P(C.c_double), C.c_int32,
                                       C.c_double, C.c_int64, C.c_int32]
+        L.orc_floatingzone.argtypes = [C.c_void_p, C.c_int32, P(C.c_int64), P(C.c_double), C.c_int64, P(C.c_int32),
+                                       P(C.c_double), C.c_int64, C.c_double]
         L.orc_globalpivotfinder.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_int32, C.c_int32,
                                             C.c_double, C.c_uint64, P(C.c_int32)]
         L.orc_exp.restype = C.c_double
@@ -253,6 +255,19 @@ def tt_evaluate(cores, points):
     out = np.zeros(max(x.shape[0], 1))
     lib().orc_tt_evaluate(L, _ip(dims), _dp(flat), x.shape[0], _i32(x), _dp(out))
     return out[:x.shape[0]]
+
+
+def floatingzone(f: Functor, cores, points, nsweeps=INT64_MAX, earlystoptol=float("inf")):
+    """_floatingzone (globalsearch.jl:43-95) from each row of points.
+    Returns (pivots npts×L, errors)."""
+    L = len(cores)
+    dims = np.ascontiguousarray(np.array([c.shape for c in cores], np.int64).ravel())
+    flat = np.ascontiguousarray(np.concatenate([np.asarray(c, np.float64).ravel(order="F") for c in cores]))
+    x = np.array(np.asarray(points, np.int32).reshape(-1, L), order="C")
+    errs = np.zeros(max(x.shape[0], 1))
+    lib().orc_floatingzone(C.byref(f.st), L, _ip(dims), _dp(flat), x.shape[0], _i32(x), _dp(errs), int(nsweeps),
+                           float(earlystoptol))
+    return x, errs[:x.shape[0]]
 
 
 def _tt_pack(cores):

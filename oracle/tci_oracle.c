@@ -841,6 +841,96 @@ void orc_tt_evaluate(int32_t L, const int64_t *dims3, const double *cores, int64
     free(w);
 }
 
+/* Julia's isless on Float64 for the argmax of _floatingzone: NaN sorts
+ * above every number (and equal to itself) */
+static int jl_isless(double a, double b)
+{
+    if (a != a) return 0;
+    if (b != b) return 1;
+    return a < b;
+}
+
+/* _floatingzone (globalsearch.jl:43-95) from each start point; TT values
+ * through the left row vector l and the right environment R of the current
+ * point: TT_σ = Σ_k (Σ_j l[j] T_p[j,σ,k]) R_{p+1}[k], sums ascending.  A NaN
+ * error ends the search (the reference would not stop). */
+void orc_floatingzone(const orc_functor *f, int32_t L, const int64_t *dims3, const double *cores, int64_t npts,
+                      int32_t *pts, double *errs, int64_t nsweeps, double earlystoptol)
+{
+    int64_t *coff = (int64_t *)malloc(sizeof(int64_t) * (size_t)(L + 1));
+    int64_t chimax = 1, dmax = 1;
+    coff[0] = 0;
+    for (int32_t b = 0; b < L; ++b) {
+        coff[b + 1] = coff[b] + dims3[3 * b] * dims3[3 * b + 1] * dims3[3 * b + 2];
+        if (dims3[3 * b + 2] > chimax) chimax = dims3[3 * b + 2];
+        if (dims3[3 * b] > chimax) chimax = dims3[3 * b];
+        if (dims3[3 * b + 1] > dmax) dmax = dims3[3 * b + 1];
+    }
+    /* E[b] (b = 1..L): the product of sites b..L-1 at the point, χ_b long */
+    double **E = (double **)malloc(sizeof(double *) * (size_t)(L + 1));
+    for (int32_t b = 0; b <= L; ++b) E[b] = (double *)malloc(sizeof(double) * (size_t)chimax);
+    double *l = (double *)malloc(sizeof(double) * (size_t)chimax);
+    double *w = (double *)malloc(sizeof(double) * (size_t)(dmax * chimax));
+    double *err = (double *)malloc(sizeof(double) * (size_t)dmax);
+    int32_t *x = (int32_t *)malloc(sizeof(int32_t) * (size_t)L);
+    for (int64_t q = 0; q < npts; ++q) {
+        int32_t *piv = pts + q * L;
+        double maxerror = 0.0;
+        for (int64_t sweep = 0;; ++sweep) {
+            E[L][0] = 1.0;
+            for (int32_t b = L - 1; b >= 1; --b) {
+                const int64_t d0 = dims3[3 * b], d = dims3[3 * b + 1], d2 = dims3[3 * b + 2];
+                const double *T = cores + coff[b];
+                for (int64_t j = 0; j < d0; ++j) {
+                    double acc = 0.0;
+                    for (int64_t k = 0; k < d2; ++k) acc = acc + T[j + d0 * ((piv[b] - 1) + d * k)] * E[b + 1][k];
+                    E[b][j] = acc;
+                }
+            }
+            if (sweep == 0) {
+                /* maxerror = abs(f(pivot) - ttcache(pivot)) (globalsearch.jl:62) */
+                const int64_t d = dims3[1], d2 = dims3[2];
+                double acc = 0.0;
+                for (int64_t k = 0; k < d2; ++k) acc = acc + (0.0 + 1.0 * cores[(piv[0] - 1) + d * k]) * E[1][k];
+                maxerror = fabs(feval(f, piv, L) - acc);
+            }
+            const double prev = maxerror;
+            l[0] = 1.0;
+            for (int32_t p = 0; p < L; ++p) {
+                const int64_t d0 = dims3[3 * p], d = dims3[3 * p + 1], d2 = dims3[3 * p + 2];
+                const double *T = cores + coff[p];
+                for (int64_t s = 0; s < d; ++s)
+                    for (int64_t k = 0; k < d2; ++k) {
+                        double acc = 0.0;
+                        for (int64_t j = 0; j < d0; ++j) acc = acc + l[j] * T[j + d0 * (s + d * k)];
+                        w[s * chimax + k] = acc;
+                    }
+                for (int64_t s = 0; s < d; ++s) {
+                    double acc = 0.0;
+                    for (int64_t k = 0; k < d2; ++k) acc = acc + w[s * chimax + k] * E[p + 1][k];
+                    memcpy(x, piv, sizeof(int32_t) * (size_t)L);
+                    x[p] = (int32_t)(s + 1);
+                    err[s] = fabs(feval(f, x, L) - acc);
+                }
+                /* pivot[ipos] = argmax(err); maxerror = max(maximum(err), maxerror) */
+                int64_t best = 0;
+                double mx = err[0];
+                for (int64_t s = 1; s < d; ++s) {
+                    if (jl_isless(err[best], err[s])) best = s;
+                    mx = jl_max(mx, err[s]);
+                }
+                piv[p] = (int32_t)(best + 1);
+                maxerror = jl_max(mx, maxerror);
+                for (int64_t k = 0; k < d2; ++k) l[k] = w[best * chimax + k];
+            }
+            if (maxerror == prev || maxerror > earlystoptol || maxerror != maxerror || sweep + 1 >= nsweeps) break;
+        }
+        errs[q] = maxerror;
+    }
+    for (int32_t b = 0; b <= L; ++b) free(E[b]);
+    free(E); free(l); free(w); free(err); free(x); free(coff);
+}
+
 /* ===================================================================== */
 /* Tensor-train sweeps — compress! (tensortrain.jl:95-183) and            */
 /* sweep1sitegetindices! (conversion.jl:73-156)                           */

@@ -143,6 +143,12 @@ void orc_eval_points(const orc_functor *f, int32_t L, int64_t npts, const int32_
  * product summed in ascending bond order */
 void orc_tt_evaluate(int32_t L, const int64_t *dims3, const double *cores, int64_t npts,
                      const int32_t *pts, double *out);
+/* _floatingzone (globalsearch.jl:43-95) from each of npts start points
+ * (pts: npts×L, 1-based, row-major; the found pivots on return), the
+ * largest error seen in errs; the per-point part of estimatetrueerror
+ * (globalsearch.jl:18-41).  TT cores as orc_tt_evaluate. */
+void orc_floatingzone(const orc_functor *f, int32_t L, const int64_t *dims3, const double *cores, int64_t npts,
+                      int32_t *pts, double *errs, int64_t nsweeps, double earlystoptol);
 /* One sweep over a tensor train whose cores are stored back to back
  * (site b: linkdims[b]×localdims[b]×linkdims[b+1], column-major), as
  * compress! does in each direction (tensortrain.jl:147-183) and

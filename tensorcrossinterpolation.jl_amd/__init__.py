@@ -30,7 +30,7 @@ __all__ = [
     "TableFunctor", "TTFunctor", "QuanticsExp", "batchevaluate", "TensorCI2", "crossinterpolate2",
     "crossinterpolate2_batch", "TCI_UNLIMITED", "capacity_plan", "tt_evaluate", "sample_points",
     "tensorci2_from_sets", "optimize", "CachedFunction", "TensorTrain", "compress", "tt_sweep",
-    "tensorci2_from_tt",
+    "tensorci2_from_tt", "floatingzone", "estimatetrueerror",
 ]
 
 
@@ -634,6 +634,51 @@ def tensorci2_from_tt(tt, tolerance=1e-12, maxbonddim=TCI_UNLIMITED, maxiter=3, 
     t.ranks, t.errors, t.nglobalpivots = np.zeros(0, np.int64), np.zeros(0), np.zeros(0, np.int64)
     t.workstats = {}
     return t
+
+
+def floatingzone(tt, f, initialpoints, nsweeps=TCI_UNLIMITED, earlystoptol=float("inf"), ctx=None):
+    """_floatingzone(ttcache, f; earlystoptol, nsweeps, initp)
+    (globalsearch.jl:43-95) from every row of initialpoints, one workgroup
+    per point (tci_tt_floatingzone_f64).  Returns (pivots npts×L, errors)."""
+    if not isinstance(f, DeviceFunctor):
+        raise TypeError("f must be a DeviceFunctor")
+    if nsweeps <= 0:
+        raise TCIError(8, "nsweeps should be positive!")
+    ctx = ctx or default_context()
+    tt = tt if isinstance(tt, TensorTrain) else TensorTrain(tt)
+    L = len(tt)
+    x = np.array(_points(initialpoints, L), order="C")
+    links, ld, flat = tt._packed()
+    errs = np.zeros(max(x.shape[0], 1))
+    fs = f.struct()
+    _check(_abi.lib().tci_tt_floatingzone_f64(ctx.handle, C.byref(fs), L, _abi.i64ptr(links),
+                                              _abi.i32ptr(ld), _abi.dptr(flat), x.shape[0], _abi.u8ptr(x),
+                                              _abi.dptr(errs), int(nsweeps), float(earlystoptol)))
+    return x.astype(np.int64), errs[:x.shape[0]]
+
+
+def estimatetrueerror(tt, f, nsearch=100, initialpoints=None, seed=0, ctx=None):
+    """estimatetrueerror(tt, f; nsearch, initialpoints) (globalsearch.jl:18-41):
+    the floating-zone search from each start point on the device, then the
+    (pivot, error) pairs in descending error order (ties keep their order,
+    NaN first), repeats dropped.  Random start points come from a seeded
+    numpy generator (the reference draws from Julia's default RNG)."""
+    if nsearch <= 0 and initialpoints is None:
+        raise TCIError(8, "No search is performed")
+    if nsearch < 0:
+        raise TCIError(8, "nsearch must be non-negative")
+    tt = tt if isinstance(tt, TensorTrain) else TensorTrain(tt)
+    if initialpoints is None:
+        initialpoints = sample_points([s[0] for s in tt.sitedims()], nsearch, seed)
+    piv, errs = floatingzone(tt, f, initialpoints, ctx=ctx)
+    order = sorted(range(len(errs)), key=lambda q: (0 if errs[q] != errs[q] else 1, -errs[q] if errs[q] == errs[q] else 0))
+    out, seen = [], set()
+    for q in order:
+        key = (tuple(int(v) for v in piv[q]), "nan" if errs[q] != errs[q] else float(errs[q]))
+        if key not in seen:
+            seen.add(key)
+            out.append(([int(v) for v in piv[q]], float(errs[q])))
+    return out
 
 
 def sample_points(localdims, n, seed=0):

@@ -222,6 +222,8 @@ def lib():
                                       C.c_double, C.c_int64, C.c_int32]
     L.tci_tt_sweep_f64.argtypes = [vp, C.c_int32, P(C.c_int64), P(C.c_int32), P(C.c_double), C.c_int32, C.c_int32,
                                    C.c_int64, C.c_double, C.c_double, P(C.c_int32), P(C.c_double)]
+    L.tci_tt_floatingzone_f64.argtypes = [vp, P(tci_functor), C.c_int32, P(C.c_int64), P(C.c_int32), P(C.c_double),
+                                          C.c_int64, P(C.c_uint8), P(C.c_double), C.c_int64, C.c_double]
     _LIB = L
     return L
 
@@ -235,5 +237,5 @@ EXPORTED_SYMBOLS = [
     "tci_batch_pivoterrors", "tci_batch_bonderrors", "tci_batch_free", "tci_batch_elapsed_ms",
     "tci_batch_profile", "tci_capacity_plan", "tci_tt_evaluate_f64", "tci_batch_evaluate",
     "tci_ctx_set_rrlu_algo", "tci_rrlu_batched_noalloc_f64", "tci_batch_cachestats", "tci_batch_cachedata",
-    "tci_gemm_f64", "tci_tt_compress_f64", "tci_tt_sweep_f64",
+    "tci_gemm_f64", "tci_tt_compress_f64", "tci_tt_sweep_f64", "tci_tt_floatingzone_f64",
 ]

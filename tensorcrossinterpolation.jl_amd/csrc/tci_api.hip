@@ -1441,6 +1441,78 @@ tci_status tci_tt_evaluate_f64(tci_ctx *ctx, int32_t L, const int64_t *linkdims,
     return s;
 }
 
+tci_status tci_tt_floatingzone_f64(tci_ctx *ctx, const tci_functor *f, int32_t L, const int64_t *linkdims,
+                                   const int32_t *localdims, const double *cores, int64_t npts, uint8_t *points,
+                                   double *errors, int64_t nsweeps, double earlystoptol)
+{
+    if (!ctx || !f || L < 1 || L > 255 || !linkdims || !localdims || npts < 0 || (npts > 0 && (!points || !errors)))
+        return set_err(TCI_ERR_ARG, "bad arguments");
+    if (nsweeps < 1) return set_err(TCI_ERR_GENERIC, "nsweeps should be positive!");
+    if (linkdims[0] != 1 || linkdims[L] != 1)
+        return set_err(TCI_ERR_ARG, "a tensor train has bond dimension 1 at both ends");
+    std::string msg;
+    const int fst = check_functor(*f, L, localdims, &msg);
+    if (fst != TCI_OK) return set_err((tci_status)fst, "%s", msg.c_str());
+    int64_t tot = 0, chimax = 1, dmax = 1;
+    std::vector<int64_t> dims3((size_t)(3 * L)), coff((size_t)L), roff((size_t)L + 1);
+    for (int b = 0; b < L; ++b) {
+        if (localdims[b] < 1 || localdims[b] > 255) return set_err(TCI_ERR_ARG, "local dimensions must be in 1..255");
+        if (linkdims[b + 1] < 1 || linkdims[b + 1] > 8192) return set_err(TCI_ERR_ARG, "bond dimensions must be in 1..8192");
+        dims3[(size_t)(3 * b)] = linkdims[b];
+        dims3[(size_t)(3 * b + 1)] = localdims[b];
+        dims3[(size_t)(3 * b + 2)] = linkdims[b + 1];
+        coff[(size_t)b] = tot;
+        tot += linkdims[b] * localdims[b] * linkdims[b + 1];
+        chimax = linkdims[b + 1] > chimax ? linkdims[b + 1] : chimax;
+        chimax = linkdims[b] > chimax ? linkdims[b] : chimax;
+        dmax = localdims[b] > dmax ? localdims[b] : dmax;
+        roff[(size_t)b + 1] = roff[(size_t)b] + linkdims[b + 1];   // E_{b+1} at roff[b]
+    }
+    if (npts == 0) return TCI_OK;
+    if (!cores) return set_err(TCI_ERR_ARG, "cores is NULL");
+    if (!points_in_range(L, localdims, npts, points)) return set_err(TCI_ERR_ARG, "point index out of range");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int blocks = (int)(npts < 1024 ? npts : 1024);
+    const int64_t wstride = roff[(size_t)L] + chimax + dmax * chimax + dmax;
+    size_t need = 0;
+    auto a = [&](size_t bytes) { size_t at = need; need = lay_align(need + (bytes ? bytes : 1)); return at; };
+    const size_t o_c = a(sizeof(double) * (size_t)tot);
+    const size_t o_d = a(sizeof(int64_t) * dims3.size());
+    const size_t o_o = a(sizeof(int64_t) * coff.size());
+    const size_t o_r = a(sizeof(int64_t) * roff.size());
+    const size_t o_par = a(sizeof(double) * (size_t)f->nparams);
+    const size_t o_ipar = a(sizeof(int32_t) * (size_t)f->niparams);
+    const size_t o_pts = a((size_t)(npts * L));
+    const size_t o_err = a(sizeof(double) * (size_t)npts);
+    const size_t o_w = a(sizeof(double) * (size_t)(wstride * blocks));
+    const size_t o_dig = a((size_t)(dmax * L * blocks));
+    uint8_t *d = nullptr;
+    HIPCHK(hipMallocAsync((void **)&d, need, st));
+    hipError_t e = hipMemcpyAsync(d + o_c, cores, sizeof(double) * (size_t)tot, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d + o_d, dims3.data(), sizeof(int64_t) * dims3.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d + o_o, coff.data(), sizeof(int64_t) * coff.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d + o_r, roff.data(), sizeof(int64_t) * roff.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && f->nparams)
+        e = hipMemcpyAsync(d + o_par, f->params, sizeof(double) * f->nparams, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && f->niparams)
+        e = hipMemcpyAsync(d + o_ipar, f->iparams, sizeof(int32_t) * f->niparams, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d + o_pts, points, (size_t)(npts * L), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = launch_floatingzone(L, (const int64_t *)(d + o_d), (const int64_t *)(d + o_o), (const double *)(d + o_c),
+                                (const int64_t *)(d + o_r), (int)chimax, (int)dmax, f->kind,
+                                (const double *)(d + o_par), (const int32_t *)(d + o_ipar), d + o_pts,
+                                (double *)(d + o_err), npts, nsweeps, earlystoptol, (double *)(d + o_w), wstride,
+                                d + o_dig, blocks, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(points, d + o_pts, (size_t)(npts * L), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(errors, d + o_err, sizeof(double) * (size_t)npts, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFreeAsync(d, st);
+    (void)hipStreamSynchronize(st);
+    if (e != hipSuccess) return set_err(TCI_ERR_DEVICE, "floating zone search: %s", hipGetErrorString(e));
+    return TCI_OK;
+}
+
 tci_status tci_batch_evaluate(tci_batch *b, int32_t i, int64_t npts, const uint8_t *pts, double *tt_out,
                               double *f_out, double *maxerr, double *maxabsf)
 {

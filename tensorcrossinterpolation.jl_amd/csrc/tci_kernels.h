@@ -53,6 +53,14 @@ hipError_t launch_tt_eval(int L, const int64_t *dims3, const int64_t *coff, cons
                           const uint8_t *pts, int64_t npts, int chimax, int fkind, const double *params,
                           const int32_t *iparams, double *out_tt, double *out_f, double *blockmax,
                           hipStream_t st);
+// _floatingzone from each of npts points (in/out, npts×L digits), one
+// workgroup per point; work: wstride doubles per workgroup, digits: dmax·L
+// bytes per workgroup; roff: L+1 offsets of the right environments
+hipError_t launch_floatingzone(int L, const int64_t *dims3, const int64_t *coff, const double *cores,
+                               const int64_t *roff, int chimax, int dmax, int fkind, const double *params,
+                               const int32_t *iparams, uint8_t *pts, double *errs, int64_t npts, int64_t nsweeps,
+                               double earlystoptol, double *work, int64_t wstride, uint8_t *digits, int blocks,
+                               hipStream_t st);
 hipError_t launch_eval(int kind, const double *params, const int32_t *iparams, int L,
                        const uint8_t *rows, int64_t nrow, int rstride, int nl, const uint8_t *cols,
                        int64_t ncol, int cstride, double *out, double *blockmax, hipStream_t st);

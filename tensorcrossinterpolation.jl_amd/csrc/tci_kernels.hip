@@ -455,6 +455,147 @@ __global__ __launch_bounds__(256) void tci_tt_eval_kernel(
     }
 }
 
+// _floatingzone (globalsearch.jl:43-95) from each start point: one
+// workgroup per point (grid-stride).  Per sweep, the right environments
+// R_b = T_b[:, x_b, :]·R_{b+1} (R_L = [1], each entry summed in ascending
+// bond order) are built from the current point; then for each site p, with
+// l the row vector of the sites before p at the current point,
+//   w_σ[k] = Σ_j l[j] T_p[j,σ,k]  (ascending j),  TT_σ = Σ_k w_σ[k] R_{p+1}[k]
+// (ascending k), err_σ = |f(x with x_p = σ) − TT_σ|; x_p moves to Julia's
+// argmax of err (first maximum, NaN above every number) and the running
+// error takes the NaN-propagating max.  A sweep that leaves the error
+// unchanged, or lifts it above earlystoptol, ends the search; so does a
+// NaN error (with which the reference would sweep without end).
+// Work area per workgroup (wstride doubles): renv (Σ_{b=1..L} χ_b, at
+// roff[b-1]) | l (chimax) | w (dmax·chimax) | err (dmax); digit rows
+// (dmax·L bytes per workgroup) hold the candidate points.
+__global__ __launch_bounds__(256) void tci_floatingzone_kernel(
+    int L, const int64_t *__restrict__ dims3, const int64_t *__restrict__ coff, const double *__restrict__ cores,
+    const int64_t *__restrict__ roff, int chimax, int dmax, int fkind, const double *__restrict__ params,
+    const int32_t *__restrict__ iparams, uint8_t *__restrict__ pts, double *__restrict__ errs, int64_t npts,
+    int64_t nsweeps, double earlystoptol, double *__restrict__ work, int64_t wstride, uint8_t *__restrict__ digits)
+{
+    __shared__ uint8_t piv[256];
+    __shared__ double s_maxerr;
+    __shared__ int s_best, s_stop;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double *renv = work + (int64_t)blockIdx.x * wstride;
+    double *lv = renv + roff[L];
+    double *w = lv + chimax;
+    double *err = w + (int64_t)dmax * chimax;
+    uint8_t *rows = digits + (int64_t)blockIdx.x * dmax * L;
+    for (int64_t q = blockIdx.x; q < npts; q += gridDim.x) {
+        for (int t = tid; t < L; t += nt) piv[t] = pts[q * L + t];
+        __syncthreads();
+        for (int64_t sweep = 0;; ++sweep) {
+            // right environments of the current point
+            if (tid == 0) renv[roff[L - 1]] = 1.0;   // R_L
+            __syncthreads();
+            for (int b = L - 1; b >= 1; --b) {
+                const int64_t d0 = dims3[3 * b], d = dims3[3 * b + 1], d2 = dims3[3 * b + 2];
+                const double *Tc = cores + coff[b] + d0 * (int64_t)(piv[b] - 1);
+                const double *Rn = renv + roff[b];
+                double *Rb = renv + roff[b - 1];
+                for (int64_t j = tid; j < d0; j += nt) {
+                    double acc = 0.0;
+                    for (int64_t k = 0; k < d2; ++k) acc = acc + Tc[j + d0 * d * k] * Rn[k];
+                    Rb[j] = acc;
+                }
+                __syncthreads();
+            }
+            if (sweep == 0) {
+                // the start point's error: TT(x) = T_1[1, x_1, :]·R_2
+                if (tid == 0) {
+                    const int64_t d = dims3[1], d2 = dims3[2];
+                    const double *Tc = cores + coff[0] + (int64_t)(piv[0] - 1);
+                    const double *Rn = renv + roff[0];
+                    double acc = 0.0;
+                    for (int64_t k = 0; k < d2; ++k) acc = acc + (0.0 + 1.0 * Tc[d * k]) * Rn[k];
+                    tci_point x;
+                    x.a = piv;
+                    x.b = piv;
+                    x.na = L;
+                    x.L = L;
+                    s_maxerr = fabs(tci_functor_eval(fkind, params, iparams, &x) - acc);
+                }
+                __syncthreads();
+            }
+            const double prev = s_maxerr;
+            if (tid == 0) lv[0] = 1.0;
+            __syncthreads();
+            for (int p = 0; p < L; ++p) {
+                const int64_t d0 = dims3[3 * p], d = dims3[3 * p + 1], d2 = dims3[3 * p + 2];
+                const double *T = cores + coff[p];
+                const double *Rn = renv + roff[p];   // R_{p+1}
+                for (int64_t o = tid; o < d * d2; o += nt) {
+                    const int64_t s = o / d2, k = o - s * d2;
+                    const double *Tc = T + d0 * (s + d * k);
+                    double acc = 0.0;
+                    for (int64_t j = 0; j < d0; ++j) acc = acc + lv[j] * Tc[j];
+                    w[s * chimax + k] = acc;
+                }
+                for (int64_t o = tid; o < d * L; o += nt) {
+                    const int64_t s = o / L, t = o - s * L;
+                    rows[s * L + t] = t == p ? (uint8_t)(s + 1) : piv[t];
+                }
+                __syncthreads();
+                for (int64_t s = tid; s < d; s += nt) {
+                    double acc = 0.0;
+                    for (int64_t k = 0; k < d2; ++k) acc = acc + w[s * chimax + k] * Rn[k];
+                    tci_point x;
+                    x.a = rows + s * L;
+                    x.b = x.a;
+                    x.na = L;
+                    x.L = L;
+                    err[s] = fabs(tci_functor_eval(fkind, params, iparams, &x) - acc);
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    int best = 0;
+                    double bv = err[0], mx = err[0];
+                    for (int s = 1; s < (int)d; ++s) {
+                        const double e = err[s];
+                        // isless(bv, e): NaN is above every number, the first NaN stays
+                        if (bv == bv && (e != e || bv < e)) {
+                            best = s;
+                            bv = e;
+                        }
+                        mx = jl_max(mx, e);
+                    }
+                    piv[p] = (uint8_t)(best + 1);
+                    s_best = best;
+                    s_maxerr = jl_max(mx, s_maxerr);
+                }
+                __syncthreads();
+                for (int64_t k = tid; k < d2; k += nt) lv[k] = w[s_best * chimax + k];
+                __syncthreads();
+            }
+            if (tid == 0) {
+                const double me = s_maxerr;
+                s_stop = me == prev || me > earlystoptol || me != me || sweep + 1 >= nsweeps;
+            }
+            __syncthreads();
+            if (s_stop) break;
+        }
+        for (int t = tid; t < L; t += nt) pts[q * L + t] = piv[t];
+        if (tid == 0) errs[q] = s_maxerr;
+        __syncthreads();
+    }
+}
+
+hipError_t launch_floatingzone(int L, const int64_t *dims3, const int64_t *coff, const double *cores,
+                               const int64_t *roff, int chimax, int dmax, int fkind, const double *params,
+                               const int32_t *iparams, uint8_t *pts, double *errs, int64_t npts, int64_t nsweeps,
+                               double earlystoptol, double *work, int64_t wstride, uint8_t *digits, int blocks,
+                               hipStream_t st)
+{
+    if (npts <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tci_floatingzone_kernel, dim3((unsigned)blocks), dim3(256), 0, st, L, dims3, coff, cores, roff,
+                       chimax, dmax, fkind, params, iparams, pts, errs, npts, nsweeps, earlystoptol, work, wstride,
+                       digits);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------- launchers
 int tt_eval_points_per_block(int chimax)
 {

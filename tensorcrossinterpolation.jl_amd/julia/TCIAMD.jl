@@ -9,6 +9,17 @@
 #       (f)(Iset, Jset, Val(M)) → tci_pi_eval_f64      (batcheval.jl:67-83)
 #   TCI.crossinterpolate2(Float64, f::DeviceFunctor, localdims, initialpivots; kw...)
 #       → tci_solve_submit / tci_batch_wait            (tensorci2.jl:943-953)
+#   TCI.optimize!(tci::TensorCI2{Float64}, f::DeviceFunctor; kw...)
+#       → tci_solve_submit with a tci_state            (tensorci2.jl:700-850)
+#   TCI.TensorCI2{Float64}(f::DeviceFunctor, localdims, Iset, Jset)
+#       → a zero-iteration solve from the sets         (tensorci2.jl:58-72)
+#   leftright(A; kw...) → left/right of MatrixLUCI     (matrixluci.jl:40-84)
+#   TCI.compress!(tt::TensorTrain{Float64,3}, :LU | :CI; kw...)
+#       → tci_tt_compress_f64                          (tensortrain.jl:147-183)
+#   TCI.TensorCI2{Float64}(tt::TensorTrain{Float64,3}; kw...)
+#       → tci_tt_sweep_f64 per sweep                   (conversion.jl:73-176)
+#   TCI.estimatetrueerror(tt::TensorTrain{Float64,3}, f::DeviceFunctor; kw...)
+#       → tci_tt_floatingzone_f64                      (globalsearch.jl:18-95)
 #
 # Every entry point is a plain ccall on the C ABI of include/tci.h; arrays
 # are passed as pointers under GC.@preserve.  Nonzero status codes become
@@ -392,6 +403,150 @@ function TCI.TensorCI2{Float64}(f::DeviceFunctor{Float64}, localdims::Union{Vect
                      _options(; maxiter=0, flags=FLAG_NO_FINAL_SWEEP), st)
     empty!(t.Iset_history); empty!(t.Jset_history)
     return t
+end
+
+# ---------------------------------------------------------------- tensor trains
+const _factor = Dict(:LU => Int32(0), :CI => Int32(1), :SVD => Int32(2))
+
+function _packtt(tt::TCI.TensorTrain{Float64,3})
+    T = tt.sitetensors
+    links = Int64[size(T[1], 1); [size(t, 3) for t in T]]
+    ld = Int32[size(t, 2) for t in T]
+    return links, ld, reduce(vcat, vec.(T))
+end
+
+function _unpacktt!(tt::TCI.TensorTrain{Float64,3}, links, ld, flat)
+    o = 0
+    for b in eachindex(ld)
+        sz = links[b] * ld[b] * links[b+1]
+        tt.sitetensors[b] = reshape(flat[o+1:o+sz], links[b], ld[b], links[b+1])
+        o += sz
+    end
+    return tt
+end
+
+"""
+    TCI.compress!(tt::TensorTrain{Float64,3}, method=:LU; tolerance, maxbonddim, normalizeerror)
+
+compress! (tensortrain.jl:147-183) on the device (tci_tt_compress_f64): the
+train is uploaded once, swept left to right without truncation and right to
+left with the truncation, and written back.  :LU and :CI; :SVD throws.
+"""
+function TCI.compress!(tt::TCI.TensorTrain{Float64,3}, method::Symbol=:LU; tolerance::Float64=1e-12,
+                       maxbonddim::Int=typemax(Int), normalizeerror::Bool=true)
+    haskey(_factor, method) || error("Not implemented yet.")
+    links, ld, flat = _packtt(tt)
+    GC.@preserve links ld flat begin
+        throwstatus(ccall((:tci_tt_compress_f64, libtci), Cint,
+                          (Ptr{Cvoid}, Int32, Ptr{Int64}, Ptr{Int32}, Ptr{Float64}, Int32, Float64, Int64, Int32),
+                          defaultcontext().handle, length(ld), links, ld, flat, _factor[method], tolerance,
+                          maxbonddim, normalizeerror), "tci_tt_compress_f64")
+    end
+    _unpacktt!(tt, links, ld, flat)
+    nothing
+end
+
+# sweep1sitegetindices! (conversion.jl:73-156): the MatrixLUCI sweep on the
+# device (tci_tt_sweep_f64), the index bookkeeping here
+function _sweep1sitegetindices!(tt::TCI.TensorTrain{Float64,3}, forwardsweep::Bool,
+                                spectatorindices::Vector{Vector{Vector{Int}}}=Vector{Vector{Int}}[];
+                                maxbonddim=typemax(Int), tolerance=0.0)
+    L = length(tt)
+    links, ld, flat = _packtt(tt)
+    piv = zeros(Int32, max(2 * sum(links[2:L]), 1))
+    pe = zeros(Float64, maximum(links[2:L]) + 1)
+    GC.@preserve links ld flat piv pe begin
+        throwstatus(ccall((:tci_tt_sweep_f64, libtci), Cint,
+                          (Ptr{Cvoid}, Int32, Ptr{Int64}, Ptr{Int32}, Ptr{Float64}, Int32, Int32, Int64, Float64,
+                           Float64, Ptr{Int32}, Ptr{Float64}),
+                          defaultcontext().handle, L, links, ld, flat, Int32(1), forwardsweep, maxbonddim, 1e-14,
+                          tolerance, piv, pe), "tci_tt_sweep_f64")
+    end
+    _unpacktt!(tt, links, ld, flat)
+    rows = Vector{Vector{Int}}(undef, L - 1); cols = Vector{Vector{Int}}(undef, L - 1)
+    q = 0
+    for b in 1:L-1
+        p = links[b+1]
+        rows[b] = Int.(piv[q+1:q+p]); cols[b] = Int.(piv[q+p+1:q+2p])
+        q += 2p
+    end
+    indexset = Vector{Vector{Int}}[Vector{Int}[Int[]]]
+    for i in 1:L-1
+        if forwardsweep
+            push!(indexset, TCI.kronecker(last(indexset), Int(ld[i]))[rows[i]])
+            isempty(spectatorindices) || (spectatorindices[i] = spectatorindices[i][cols[i]])
+        else
+            b = L - i                       # bond b, site b+1
+            push!(indexset, TCI.kronecker(Int(ld[b+1]), last(indexset))[cols[b]])
+            isempty(spectatorindices) || (spectatorindices[b+1] = spectatorindices[b+1][rows[b]])
+        end
+    end
+    return (forwardsweep ? indexset : reverse(indexset)), pe
+end
+
+"""
+    TCI.TensorCI2{Float64}(tt::TensorTrain{Float64,3}; tolerance, maxbonddim, maxiter)
+
+The TT → TensorCI2 conversion (conversion.jl:158-176) with every sweep on
+the device; tt is swept in place as in the reference.
+"""
+function TCI.TensorCI2{Float64}(tt::TCI.TensorTrain{Float64,3}; tolerance=1e-12, maxbonddim=typemax(Int),
+                                maxiter=3)
+    Iset, = _sweep1sitegetindices!(tt, true; maxbonddim, tolerance)
+    Jset, pivoterrors = _sweep1sitegetindices!(tt, false; maxbonddim, tolerance)
+    for iter in 3:maxiter
+        if isodd(iter)
+            Isetnew, pivoterrors = _sweep1sitegetindices!(tt, true, Jset)
+            Isetnew == Iset && break
+        else
+            Jsetnew, pivoterrors = _sweep1sitegetindices!(tt, false, Iset)
+            Jsetnew == Jset && break
+        end
+    end
+    tci2 = TCI.TensorCI2{Float64}([size(t, 2) for t in tt.sitetensors])
+    tci2.Iset = Iset
+    tci2.Jset = Jset
+    tci2.sitetensors = tt.sitetensors
+    tci2.pivoterrors = pivoterrors
+    tci2.maxsamplevalue = maximum(maximum.(abs, tci2.sitetensors))
+    return tci2
+end
+
+"""
+    TCI.estimatetrueerror(tt::TensorTrain{Float64,3}, f::DeviceFunctor; nsearch, initialpoints)
+
+estimatetrueerror (globalsearch.jl:18-41) with every _floatingzone search
+on the device, one workgroup per start point (tci_tt_floatingzone_f64).
+"""
+function TCI.estimatetrueerror(tt::TCI.TensorTrain{Float64,3}, f::DeviceFunctor{Float64}; nsearch::Int=100,
+                               initialpoints::Union{Nothing,AbstractVector{Vector{Int}}}=nothing)
+    if nsearch <= 0 && initialpoints === nothing
+        error("No search is performed")
+    end
+    nsearch >= 0 || error("nsearch must be non-negative")
+    localdims = [size(t, 2) for t in tt.sitetensors]
+    if initialpoints === nothing
+        initialpoints = [[rand(1:d) for d in localdims] for _ in 1:nsearch]
+    end
+    L = length(tt)
+    n = length(initialpoints)
+    pts = Vector{UInt8}(undef, n * L)
+    for (q, p) in enumerate(initialpoints), t in 1:L
+        pts[(q-1)*L+t] = UInt8(p[t])
+    end
+    errs = zeros(Float64, n)
+    links, ld, flat = _packtt(tt)
+    GC.@preserve f links ld flat pts errs begin
+        cf = Ref(_cf(f))
+        throwstatus(ccall((:tci_tt_floatingzone_f64, libtci), Cint,
+                          (Ptr{Cvoid}, Ref{CFunctor}, Int32, Ptr{Int64}, Ptr{Int32}, Ptr{Float64}, Int64, Ptr{UInt8},
+                           Ptr{Float64}, Int64, Float64),
+                          defaultcontext().handle, cf, L, links, ld, flat, n, pts, errs, typemax(Int64), Inf),
+                    "tci_tt_floatingzone_f64")
+    end
+    pivoterror = [(Int.(pts[(q-1)*L+1:q*L]), errs[q]) for q in 1:n]
+    p = sortperm([e for (_, e) in pivoterror], rev=true)
+    return unique(pivoterror[p])
 end
 
 # one set family of a finished solve (which: 0 Iset, 1 Jset, 2/3 the last

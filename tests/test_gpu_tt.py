@@ -143,3 +143,57 @@ def test_tensorci2_from_tt_equals_oracle_sweeps(tci, ctx, monkeypatch):
     _same(a.sitetensors, b.sitetensors)
     assert np.array_equal(a.pivoterrors, b.pivoterrors)
     assert a.maxsamplevalue == b.maxsamplevalue
+
+
+# ------------------------------------------------------------ estimatetrueerror
+def _zone_cases(tci):
+    f = O.Functor(LORENTZ, [1.0])
+    t5, _, _ = O.crossinterpolate2(f, [6] * 5, tolerance=1e-3, maxbonddim=3)
+    t8, _, _ = O.crossinterpolate2(f, [10] * 8, tolerance=1e-6, maxbonddim=6)
+    rt = random_tt([1, 3, 4, 3, 1], [3, 5, 4, 2], 21)
+    # a TT functor of one train against another train
+    other = random_tt([1, 2, 5, 2, 1], [3, 5, 4, 2], 22)[2]
+    g_tt = O.Functor(7, rt[0], rt[1])
+    d_tt = tci.TTFunctor(rt[2])
+    return [("lorentz5", f, tci.Lorentz(1.0), t5.sitetensors(), [6] * 5),
+            ("lorentz8", f, tci.Lorentz(1.0), t8.sitetensors(), [10] * 8),
+            ("tt_vs_tt", g_tt, d_tt, other, [3, 5, 4, 2])]
+
+
+def test_floatingzone_equals_oracle(tci, ctx):
+    for name, of, df, cores, ld in _zone_cases(tci):
+        rng = np.random.default_rng(len(name))
+        starts = np.stack([rng.integers(1, d + 1, 300) for d in ld], 1)
+        for kw in (dict(), dict(nsweeps=1), dict(earlystoptol=1e-4)):
+            piv, errs = tci.floatingzone(tci.TensorTrain(cores), df, starts, ctx=ctx, **kw)
+            opiv, oerrs = O.floatingzone(of, cores, starts, **kw)
+            assert np.array_equal(piv, opiv), name
+            assert np.array_equal(errs, oerrs), name
+
+
+def test_estimatetrueerror_device(tci, ctx):
+    name, of, df, cores, ld = _zone_cases(tci)[0]
+    res = tci.estimatetrueerror(tci.TensorTrain(cores), df, nsearch=100, seed=3, ctx=ctx)
+    errs = [e for _, e in res]
+    assert errs == sorted(errs, reverse=True) and len(res) >= 1
+    grid = _grid(ld).astype(np.int32)
+    allerr = np.abs(O.eval_points(of, grid) - O.tt_evaluate(cores, grid))
+    assert errs[0] <= allerr.max() * (1 + 1e-12)
+    # the reference's assertion shape: the estimate is a real error of the train
+    p0 = np.array(res[0][0])[None, :]
+    assert abs(abs(O.eval_points(of, p0)[0] - O.tt_evaluate(cores, p0)[0]) - errs[0]) <= 1e-13
+
+
+def test_floatingzone_nan_stops(tci, ctx):
+    # a table with a NaN: the search that meets it ends with NaN, on both
+    tab = np.arange(1.0, 13.0).reshape(3, 4)
+    tab[2, 1] = np.nan
+    of = O.Functor(6, tab.ravel(order="F"), [3, 4])
+    df = tci.TableFunctor(tab)
+    cores = [np.ones((1, 3, 1)), np.ones((1, 4, 1))]
+    starts = np.array([[1, 1], [3, 3], [2, 4]])
+    piv, errs = tci.floatingzone(tci.TensorTrain(cores), df, starts, ctx=ctx)
+    opiv, oerrs = O.floatingzone(of, cores, starts)
+    assert np.array_equal(piv, opiv)
+    assert np.array_equal(errs, oerrs, equal_nan=True)
+    assert np.isnan(errs).any()

@@ -245,3 +245,82 @@ def test_compress_method_errors():
     tt = tci.TensorTrain(_rtt([1, 2, 1], [2, 2], 0))
     with pytest.raises(tci.TCIError, match="Not implemented yet"):
         tci.compress(tt, "QR")
+
+
+# ------------------------------------------------------------ estimatetrueerror
+def _zone_case():
+    f = O.Functor(LORENTZ, [1.0])
+    ld = [6] * 5
+    t, _, _ = O.crossinterpolate2(f, ld, tolerance=1e-3, maxbonddim=3)
+    return f, ld, t.sitetensors()
+
+
+def test_floatingzone_fixed_point():
+    """_floatingzone (globalsearch.jl:43-95): on return the error is
+    |f − TT| at the found pivot, and no single-site move from it finds a
+    larger error (the sweep that ended the search moved nothing upward)."""
+    f, ld, cores = _zone_case()
+    rng = np.random.default_rng(9)
+    starts = np.stack([rng.integers(1, d + 1, 12) for d in ld], 1)
+    piv, errs = O.floatingzone(f, cores, starts)
+    fmax = np.abs(O.eval_points(f, _grid(ld).astype(np.int32))).max()
+    atol = 1e-13 * fmax
+    for q in range(len(errs)):
+        x = piv[q]
+        e = abs(O.eval_points(f, x[None, :])[0] - O.tt_evaluate(cores, x[None, :])[0])
+        assert abs(e - errs[q]) <= atol
+        for p in range(len(ld)):
+            nb = np.repeat(x[None, :], ld[p], 0)
+            nb[:, p] = np.arange(1, ld[p] + 1)
+            en = np.abs(O.eval_points(f, nb) - O.tt_evaluate(cores, nb))
+            assert en.max() <= errs[q] + atol
+
+
+def test_floatingzone_against_brute_force():
+    f, ld, cores = _zone_case()
+    grid = _grid(ld).astype(np.int32)
+    allerr = np.abs(O.eval_points(f, grid) - O.tt_evaluate(cores, grid))
+    gmax = allerr.max()
+    fmax = np.abs(O.eval_points(f, grid)).max()
+    # from anywhere the search stays below the true maximum; from the
+    # maximizer it returns the maximum itself
+    rng = np.random.default_rng(2)
+    starts = np.stack([rng.integers(1, d + 1, 20) for d in ld], 1)
+    _, errs = O.floatingzone(f, cores, starts)
+    assert errs.max() <= gmax + 1e-13 * fmax
+    _, e1 = O.floatingzone(f, cores, grid[np.argmax(allerr)][None, :])
+    assert abs(e1[0] - gmax) <= 1e-13 * fmax
+
+
+def test_floatingzone_sweep_limits():
+    f, ld, cores = _zone_case()
+    rng = np.random.default_rng(4)
+    starts = np.stack([rng.integers(1, d + 1, 8) for d in ld], 1)
+    _, e_all = O.floatingzone(f, cores, starts)
+    p1, e_one = O.floatingzone(f, cores, starts, nsweeps=1)
+    assert np.all(e_one <= e_all)
+    # earlystoptol 0 stops after the first sweep whose error is positive
+    p0, e0 = O.floatingzone(f, cores, starts, earlystoptol=0.0)
+    assert np.array_equal(p0, p1) and np.array_equal(e0, e_one)
+
+
+def test_estimatetrueerror_host_order(monkeypatch):
+    """globalsearch.jl:37-40: descending error, repeats dropped, over the
+    oracle's per-point search."""
+    f, ld, cores = _zone_case()
+
+    def orc(tt, g, initialpoints, nsweeps=O.INT64_MAX, earlystoptol=float("inf"), ctx=None):
+        return O.floatingzone(f, tt.sitetensors, initialpoints, nsweeps, earlystoptol)
+
+    monkeypatch.setattr(tci, "floatingzone", orc)
+    g = tci.Lorentz(1.0)
+    starts = np.stack([np.random.default_rng(8).integers(1, d + 1, 30) for d in ld], 1)
+    starts = np.vstack([starts, starts[:5]])          # repeats collapse
+    res = tci.estimatetrueerror(tci.TensorTrain(cores), g, initialpoints=starts)
+    errs = [e for _, e in res]
+    assert errs == sorted(errs, reverse=True)
+    assert len({(tuple(p), e) for p, e in res}) == len(res)
+    piv, e = O.floatingzone(f, cores, starts)
+    assert len(res) == len({(tuple(map(int, piv[q])), float(e[q])) for q in range(len(e))})
+    with pytest.raises(tci.TCIError, match="No search"):
+        tci.estimatetrueerror(tci.TensorTrain(cores), g, nsearch=0)
