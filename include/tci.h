@@ -153,10 +153,15 @@ tci_status tci_rrlu_batched_f64(tci_ctx *ctx, tci_lu_job *jobs_device, int64_t c
 /* Which kernel the context's standalone rrLU calls (tci_rrlu_f64,
  * tci_rrlu_batched_f64) use; no counterpart in the reference, and the results
  * are the same bits either way.  AUTO (the default): the register-tile
- * factorization for matrices up to 128×128, the streaming one otherwise;
- * STREAM: always streaming; TILE: always the register tile (jobs larger than
- * 128×128 are refused with TCI_ERR_ARG). */
-enum { TCI_RRLU_AUTO = 0, TCI_RRLU_STREAM = 1, TCI_RRLU_TILE = 2 };
+ * factorization (one workgroup) for matrices up to 128×128; the cooperative
+ * whole-chip factorization (one workgroup per CU) for matrices of at least
+ * 384² elements with sides up to 8192 when a batch holds at most 8 of them;
+ * the streaming one-workgroup factorization otherwise.  STREAM: always
+ * streaming; TILE: always the register tile (jobs larger than 128×128 are
+ * refused with TCI_ERR_ARG); GRID: the cooperative factorization for every
+ * job with sides up to 8192, one job after the other.  The no-allocation
+ * entry below never uses the cooperative kernel. */
+enum { TCI_RRLU_AUTO = 0, TCI_RRLU_STREAM = 1, TCI_RRLU_TILE = 2, TCI_RRLU_GRID = 3 };
 tci_status tci_ctx_set_rrlu_algo(tci_ctx *ctx, int32_t algo);
 
 /* tci_rrlu_batched_f64 that allocates no device memory.  No counterpart in

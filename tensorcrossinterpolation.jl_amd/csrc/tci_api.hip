@@ -270,7 +270,7 @@ tci_status tci_ctx_stream(tci_ctx *ctx, void **stream)
 
 tci_status tci_ctx_set_rrlu_algo(tci_ctx *ctx, int32_t algo)
 {
-    if (!ctx || algo < TCI_RRLU_AUTO || algo > TCI_RRLU_TILE) return set_err(TCI_ERR_ARG, "bad arguments");
+    if (!ctx || algo < TCI_RRLU_AUTO || algo > TCI_RRLU_GRID) return set_err(TCI_ERR_ARG, "bad arguments");
     ctx->rrlu_algo = algo;
     return TCI_OK;
 }
@@ -320,13 +320,18 @@ static hipStream_t pick_stream(tci_ctx *ctx)
 // goes to the streaming kernel (the results are the same bits).
 struct LuPlan {
     std::vector<tci_lu_job> h;
-    std::vector<int32_t> tile, stream;
+    std::vector<int32_t> tile, stream, grid;
     std::vector<int64_t> soff;
     int64_t sc_tot = 0;   // tile scratch, doubles
     int64_t mx = 0;       // largest side of any job
 };
 
-static tci_status lu_partition(tci_ctx *ctx, int64_t tile_budget, LuPlan &P);
+static tci_status lu_partition(tci_ctx *ctx, int64_t tile_budget, LuPlan &P, bool allow_grid = true);
+
+// the cooperative factorization takes a job of at least this many elements
+// when a batch has at most kGridMaxJobs of them (AUTO)
+constexpr int64_t kGridMinElems = (int64_t)384 * 384;
+constexpr int64_t kGridMaxJobs = 8;
 
 // reads and checks the job table (and the largest side), partitions it
 static tci_status lu_read(tci_ctx *ctx, tci_lu_job *jobs, int64_t count, LuPlan &P)
@@ -353,7 +358,7 @@ static tci_status lu_plan(tci_ctx *ctx, tci_lu_job *jobs, int64_t count, int64_t
     return s != TCI_OK ? s : lu_partition(ctx, tile_budget, P);
 }
 
-static tci_status lu_partition(tci_ctx *ctx, int64_t tile_budget, LuPlan &P)
+static tci_status lu_partition(tci_ctx *ctx, int64_t tile_budget, LuPlan &P, bool allow_grid)
 {
     const int64_t count = (int64_t)P.h.size();
     for (int64_t q = 0; q < count; ++q) {
@@ -375,6 +380,55 @@ static tci_status lu_partition(tci_ctx *ctx, int64_t tile_budget, LuPlan &P)
             P.stream.push_back((int32_t)q);
         }
     }
+    // jobs that go to the cooperative whole-chip factorization: with GRID
+    // every job that fits; with AUTO the large ones of a batch too small to
+    // fill the chip with one workgroup per job
+    if (allow_grid && (ctx->rrlu_algo == TCI_RRLU_GRID || ctx->rrlu_algo == TCI_RRLU_AUTO)) {
+        std::vector<int32_t> big, rest;
+        for (int32_t q : P.stream) {
+            const tci_lu_job &j = P.h[(size_t)q];
+            const bool fits = j.m >= 1 && j.n >= 1 && j.m <= kGridMaxSide && j.n <= kGridMaxSide;
+            const bool large = j.m * j.n >= kGridMinElems;
+            if (fits && (ctx->rrlu_algo == TCI_RRLU_GRID || large)) big.push_back(q);
+            else rest.push_back(q);
+        }
+        if (ctx->rrlu_algo == TCI_RRLU_GRID || (!big.empty() && (int64_t)big.size() <= kGridMaxJobs)) {
+            P.grid = big;
+            P.stream = rest;
+        }
+    }
+    return TCI_OK;
+}
+
+// the cooperative factorization of the grid jobs, one after the other
+static tci_status lu_run_grid(tci_ctx *ctx, tci_lu_job *jobs, const LuPlan &P)
+{
+    if (P.grid.empty()) return TCI_OK;
+    hipStream_t st = ctx->stream;
+    int64_t mx = 0, side = 0;
+    for (int32_t q : P.grid) {
+        const tci_lu_job &j = P.h[(size_t)q];
+        mx = j.m * j.n > mx ? j.m * j.n : mx;
+        side = j.m + j.n > side ? j.m + j.n : side;
+    }
+    const int G = rrlu_grid_blocks(ctx->device, rrlu_grid_lds(side, 0));
+    if (G < 1) return set_err(TCI_ERR_DEVICE, "the cooperative rrLU cannot be launched on this device");
+    const size_t b_out = lay_align(sizeof(double) * (size_t)mx), b_cand = lay_align(rrlu_grid_cand_bytes(G));
+    uint8_t *d = nullptr;
+    HIPCHK(hipMallocAsync((void **)&d, b_out + b_cand + 256, st));
+    hipError_t e = hipSuccess;
+    for (int32_t q : P.grid) {
+        const tci_lu_job &j = P.h[(size_t)q];
+        int blocks = G < (int)j.n ? G : (int)j.n;
+        int32_t *flags = (int32_t *)(d + b_out + b_cand);
+        e = hipMemsetAsync(flags, 0, sizeof(int32_t), st);
+        if (e == hipSuccess) e = launch_rrlu_grid(jobs + q, j.m, j.n, (double *)d, d + b_out, flags, blocks, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(j.A, d, sizeof(double) * (size_t)(j.m * j.n), hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) break;
+    }
+    (void)hipFreeAsync(d, st);
+    if (e != hipSuccess) return set_err(TCI_ERR_DEVICE, "cooperative rrlu: %s", hipGetErrorString(e));
     return TCI_OK;
 }
 
@@ -415,6 +469,8 @@ tci_status tci_rrlu_batched_f64(tci_ctx *ctx, tci_lu_job *jobs, int64_t count)
     std::vector<int64_t> &soff = P.soff;
     const int64_t sc_tot = P.sc_tot;
     int64_t mx;
+    ps = lu_run_grid(ctx, jobs, P);
+    if (ps != TCI_OK) return ps;
     if (!tile.empty()) {
         uint8_t *d = nullptr;
         const size_t b_idx = lay_align(sizeof(int32_t) * tile.size());
@@ -482,7 +538,7 @@ tci_status tci_rrlu_batched_noalloc_f64(tci_ctx *ctx, tci_lu_job *jobs, int64_t 
         return set_err(TCI_ERR_ARG, "this batch needs %zu bytes of scratch; the limit is %zu bytes per device",
                        o_tile, kNoAllocScratchBytes);
     // tile jobs while their scratch fits, the others stream (same bits)
-    ps = lu_partition(ctx, (int64_t)((kNoAllocScratchBytes - o_tile) / sizeof(double)), P);
+    ps = lu_partition(ctx, (int64_t)((kNoAllocScratchBytes - o_tile) / sizeof(double)), P, false);
     if (ps != TCI_OK) return ps;
     std::vector<int32_t> order;
     stream_order(P, order);

@@ -29,6 +29,18 @@ hipError_t launch_rrlu(tci_lu_job *jobs_device, int64_t count, int64_t max_side,
 constexpr int kTileSide = 128;
 hipError_t launch_rrlu_tile(tci_lu_job *jobs_device, const int32_t *idx, int64_t count, double *scratch,
                             const int64_t *soff, hipStream_t st);
+// rrlu!(A) of one matrix (each side <= kGridMaxSide) over a cooperative grid
+// of `blocks` workgroups (one per CU, rrlu_grid_blocks); the packed result
+// goes to out (m×n), the job's permutations, npivot, error and status are
+// written; cand: rrlu_grid_cand_bytes(blocks) bytes; flags: one int32, zero
+// on entry
+constexpr int kGridMaxSide = 8192;
+size_t rrlu_grid_lds(int64_t m, int64_t n);
+// the cooperative grid size for that LDS (one workgroup per CU), 0 if none fits
+int rrlu_grid_blocks(int device, size_t lds);
+size_t rrlu_grid_cand_bytes(int blocks);
+hipError_t launch_rrlu_grid(tci_lu_job *job, int64_t m, int64_t n, double *out, void *cand, int32_t *flags,
+                            int blocks, hipStream_t st);
 // C = A·B with each sum in ascending k (M×K · K×N, column-major)
 hipError_t launch_gemm(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
                        double *Cm, int64_t ldc, hipStream_t st);
