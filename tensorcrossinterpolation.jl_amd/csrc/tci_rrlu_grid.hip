@@ -1,6 +1,6 @@
 // tci_rrlu_grid.hip — rrlu!(A) of ONE large matrix spread over the whole
-// chip (matrixlu.jl:141-181): a cooperative grid of one 256-thread
-// workgroup per CU, each owning the physical columns j ≡ g (mod G).
+// chip (matrixlu.jl:141-181): a cooperative grid of one workgroup per CU,
+// each owning the physical columns j ≡ g (mod G).
 //
 // The row and column swaps of addpivot! (matrixlu.jl:98-112) are kept as
 // logical permutations (rowAt/rowPos, colAt/colPos, identical in every
@@ -23,6 +23,7 @@
 // Every result is the same bits as the one-workgroup factorization.
 #include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
 
 #include "tci_kernels.h"
 
@@ -30,8 +31,6 @@ namespace cg = cooperative_groups;
 
 namespace tci {
 
-constexpr int kGridTeam = 256;
-constexpr int kGridRows = kGridMaxSide / kGridTeam;   // rows per thread
 
 struct GridCand {
     double v;              // a², -inf for none
@@ -80,6 +79,7 @@ __device__ inline bool cand_better(const GridCand &a, const GridCand &b)
 }
 
 // team-wide best of one candidate per thread (result in every thread)
+template <int kGridTeam>
 __device__ GridCand cand_reduce(GridCand c, GridCand *red)
 {
     const int tid = threadIdx.x;
@@ -102,11 +102,13 @@ __device__ GridCand cand_reduce(GridCand c, GridCand *red)
     return b;
 }
 
+template <int kGridTeam>
 __global__ __launch_bounds__(kGridTeam, 1) void tci_rrlu_grid_kernel(tci_lu_job *__restrict__ job,
                                                                     double *__restrict__ out,
                                                                     GridCand *__restrict__ cand,
                                                                     int32_t *__restrict__ flags)
 {
+    constexpr int kGridRows = kGridMaxSide / kGridTeam;   // rows per thread
     cg::grid_group grid = cg::this_grid();
     extern __shared__ int32_t perm_lds[];
     __shared__ GridCand red[kGridTeam / 64];
@@ -148,7 +150,7 @@ __global__ __launch_bounds__(kGridTeam, 1) void tci_rrlu_grid_kernel(tci_lu_job 
             }
         }
     }
-    best = cand_reduce(best, red);
+    best = cand_reduce<kGridTeam>(best, red);
     if (tid == 0) cand_store(cand + g, best);
     grid.sync();
 
@@ -168,7 +170,7 @@ __global__ __launch_bounds__(kGridTeam, 1) void tci_rrlu_grid_kernel(tci_lu_job 
             const GridCand o = cand_load(cur + q);
             if (cand_better(o, c)) c = o;
         }
-        c = cand_reduce(c, red);
+        c = cand_reduce<kGridTeam>(c, red);
         if (tid == 0) {
             s_best = c;
             s_a = c.a;
@@ -254,7 +256,7 @@ __global__ __launch_bounds__(kGridTeam, 1) void tci_rrlu_grid_kernel(tci_lu_job 
                 }
             }
         }
-        best = cand_reduce(best, red);   // (its barriers also order the reads of row pr)
+        best = cand_reduce<kGridTeam>(best, red);   // (its barriers also order the reads of row pr)
         if (!lo)
             for (int j = g + G * tid; j < n; j += G * kGridTeam)
                 if (colPos[j] > k) A[pr + (int64_t)m * j] = A[pr + (int64_t)m * j] / a;
@@ -305,16 +307,34 @@ __global__ __launch_bounds__(kGridTeam, 1) void tci_rrlu_grid_kernel(tci_lu_job 
 
 size_t rrlu_grid_lds(int64_t m, int64_t n) { return (size_t)4 * 2 * (size_t)(m + n); }
 
+// threads per workgroup: 1024 (16 waves per CU keep enough loads in
+// flight for the streamed update); TCI_RRLU_GRID_TEAM = 256 or 512 overrides
+static int grid_team()
+{
+    static int t = [] {
+        const char *e = getenv("TCI_RRLU_GRID_TEAM");
+        const int v = e ? atoi(e) : 1024;
+        return (v == 256 || v == 512 || v == 1024) ? v : 1024;
+    }();
+    return t;
+}
+
+static const void *grid_kernel(int team)
+{
+    switch (team) {
+    case 256: return (const void *)tci_rrlu_grid_kernel<256>;
+    case 512: return (const void *)tci_rrlu_grid_kernel<512>;
+    default: return (const void *)tci_rrlu_grid_kernel<1024>;
+    }
+}
+
 int rrlu_grid_blocks(int device, size_t lds)
 {
     int ncu = 0, per = 0;
+    const void *k = grid_kernel(grid_team());
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
-    if (hipFuncSetAttribute((const void *)tci_rrlu_grid_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds) != hipSuccess)
-        return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)tci_rrlu_grid_kernel, kGridTeam, lds) !=
-        hipSuccess)
-        return 0;
+    if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, grid_team(), lds) != hipSuccess) return 0;
     return per < 1 ? 0 : ncu;   // one workgroup per CU
 }
 
@@ -325,12 +345,11 @@ hipError_t launch_rrlu_grid(tci_lu_job *job, int64_t m, int64_t n, double *out, 
 {
     if (m < 1 || n < 1 || m > kGridMaxSide || n > kGridMaxSide || blocks < 1) return hipErrorInvalidValue;
     const size_t lds = rrlu_grid_lds(m, n);
-    hipError_t e = hipFuncSetAttribute((const void *)tci_rrlu_grid_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds);
+    const void *k = grid_kernel(grid_team());
+    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     void *args[] = {(void *)&job, (void *)&out, (void *)&cand, (void *)&flags};
-    return hipLaunchCooperativeKernel((const void *)tci_rrlu_grid_kernel, dim3((unsigned)blocks), dim3(kGridTeam),
-                                      args, (unsigned)lds, st);
+    return hipLaunchCooperativeKernel(k, dim3((unsigned)blocks), dim3(grid_team()), args, (unsigned)lds, st);
 }
 
 }  // namespace tci
