@@ -255,7 +255,11 @@ typedef struct tci_options {
      * refuses a key type that overflows, 19-21).  Values are f's own bits,
      * so results are the same with or without the table. */
     int32_t cache_log2;
-    int32_t reserved;
+    /* updatepivots!'s pivot search (tensorci2.jl:510-595): 0 :full (the
+     * whole Pi matrix), 1 :rook (arrlu, matrixlu.jl:227-293, on Pi entries
+     * evaluated as the search asks for them; its randomsubset draws come
+     * from the solve's PRNG, seeded with `seed`) */
+    int32_t pivotsearch;
 } tci_options;
 
 /* Diagnostic (no counterpart in the reference): end optimize! after its last

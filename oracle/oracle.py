@@ -49,7 +49,7 @@ class _Opts(C.Structure):
                 ("ncheckhistory", C.c_int32), ("maxnglobalpivot", C.c_int32),
                 ("nsearchglobalpivot", C.c_int32), ("tolmarginglobalsearch", C.c_double),
                 ("strictlynested", C.c_int32), ("checkconvglobalpivot", C.c_int32),
-                ("seed", C.c_uint64)]
+                ("seed", C.c_uint64), ("pivotsearch", C.c_int32)]
 
 
 def lib():
@@ -111,6 +111,10 @@ def lib():
                                    C.c_int64, C.c_double, C.c_double, P(C.c_int32), P(C.c_double)]
         L.orc_tt_compress.argtypes = [C.c_int32, P(C.c_int64), P(C.c_int32), P(C.c_double), C.c_int32,
                                       C.c_double, C.c_int64, C.c_int32]
+        L.orc_arrlu.argtypes = [P(C.c_double), C.c_int64, C.c_int64, P(C.c_int64), C.c_int64, P(C.c_int64),
+                                C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_uint64,
+                                P(C.c_int64), P(C.c_int64), P(C.c_double), P(C.c_double), P(C.c_int64),
+                                P(C.c_double)]
         L.orc_floatingzone.argtypes = [C.c_void_p, C.c_int32, P(C.c_int64), P(C.c_double), C.c_int64, P(C.c_int32),
                                        P(C.c_double), C.c_int64, C.c_double]
         L.orc_globalpivotfinder.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_int32, C.c_int32,
@@ -180,6 +184,34 @@ def rrlu(A, maxrank=INT64_MAX, reltol=1e-14, abstol=0.0, leftorthogonal=True) ->
     pv = p.value
     return RRLU(rp[:m].copy(), cp[:n].copy(), np.array(Lb[:m, :pv]), np.array(Ub[:pv, :n]),
                 bool(leftorthogonal), pv, err.value)
+
+
+def arrlu(A, I0=(), J0=(), maxrank=INT64_MAX, reltol=1e-14, abstol=0.0, leftorthogonal=True, numrookiter=5,
+          seed=0) -> RRLU:
+    """arrlu (matrixlu.jl:227-293) of an explicit matrix, rook pivot search
+    from the 1-based starting rows I0 and columns J0, randomsubset drawn
+    from the project PRNG seeded with `seed`."""
+    I0_in, J0_in = I0, J0
+    A = np.asfortranarray(A, dtype=np.float64)
+    m, n = A.shape
+    cap = max(min(m, n), 1)
+    nI, nJ = len(I0), len(J0)
+    I0 = np.zeros(max(m, 1), np.int64)
+    J0 = np.zeros(max(n, 1), np.int64)
+    I0[:nI] = np.asarray(I0_in, np.int64).reshape(-1)
+    J0[:nJ] = np.asarray(J0_in, np.int64).reshape(-1)
+    rp = np.zeros(max(m, 1), np.int64)
+    cp = np.zeros(max(n, 1), np.int64)
+    Lb = np.zeros(max(m * cap, 1))
+    Ub = np.zeros(max(cap * n, 1))
+    p = C.c_int64()
+    err = C.c_double()
+    _check(lib().orc_arrlu(_dp(A), m, n, _ip(I0), nI, _ip(J0), nJ, int(min(maxrank, INT64_MAX)), float(reltol), float(abstol),
+                           int(bool(leftorthogonal)), int(numrookiter), int(seed), _ip(rp), _ip(cp), _dp(Lb), _dp(Ub),
+                           C.byref(p), C.byref(err)))
+    k = p.value
+    return RRLU(rp[:m].copy(), cp[:n].copy(), Lb[:m * k].reshape((m, k), order="F").copy(),
+                Ub[:k * n].reshape((k, n), order="F").copy(), bool(leftorthogonal), k, err.value)
 
 
 def luci_factors(A, **kw):
@@ -318,11 +350,13 @@ def tt_compress(cores, method="LU", tolerance=1e-12, maxbonddim=INT64_MAX, norma
 
 def make_opts(tolerance=1e-8, maxbonddim=INT64_MAX, maxiter=20, sweepstrategy="backandforth",
               normalizeerror=True, ncheckhistory=3, maxnglobalpivot=5, nsearchglobalpivot=5,
-              tolmarginglobalsearch=10.0, strictlynested=False, checkconvglobalpivot=True, seed=0):
+              tolmarginglobalsearch=10.0, strictlynested=False, checkconvglobalpivot=True, seed=0,
+              pivotsearch="full"):
     ss = {"backandforth": 0, "forward": 1, "backward": 2}[sweepstrategy]
+    ps = {"full": 0, "rook": 1}[pivotsearch]
     return _Opts(float(tolerance), int(maxbonddim), int(maxiter), ss, int(normalizeerror),
                  int(ncheckhistory), int(maxnglobalpivot), int(nsearchglobalpivot),
-                 float(tolmarginglobalsearch), int(strictlynested), int(checkconvglobalpivot), int(seed))
+                 float(tolmarginglobalsearch), int(strictlynested), int(checkconvglobalpivot), int(seed), ps)
 
 
 class TCI:

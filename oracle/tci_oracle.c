@@ -380,6 +380,7 @@ struct orc_tci {
     int32_t niter, capiter;
     double ws[6];
     uint64_t rng[4];
+    int32_t pivotsearch;         /* 0 :full, 1 :rook (updatepivots!) */
 };
 
 orc_tci *orc_tci_new(int32_t L, const int32_t *localdims)
@@ -541,10 +542,256 @@ static double *luci_pivoterrors(const luci_res *r, int leftorth)
     return e;
 }
 
+/* ===================================================================== */
+/* Rook pivot search — arrlu (matrixlu.jl:227-293) over a lazily         */
+/* evaluated matrix (SubMatrix, tensorci2.jl:476-503)                    */
+/* ===================================================================== */
+
+static int32_t rng_randint(uint64_t s[4], int32_t d);
+typedef void (*log_fn)(void *ctx, int64_t m, int64_t n, int64_t p);
+
+/* out[i + nr·j] = A(rows[i], cols[j]) (0-based positions), |out| folded
+ * into the caller's running maximum */
+typedef void (*sub_eval_fn)(void *ctx, const int64_t *rows, int64_t nr, const int64_t *cols, int64_t nc,
+                            double *out);
+
+/* pushrandomsubset!(S, 0:N-1, max(1, |S|)) (util.jl:36-58): randomsubset of
+ * setdiff(1:N, S) (ascending), each draw rand(1:length(c)) then deleteat! */
+static void push_random_subset(uint64_t rng[4], int64_t *S, int64_t *ns, int64_t N)
+{
+    int64_t k = *ns > 1 ? *ns : 1;
+    char *in = (char *)calloc((size_t)(N + 1), 1);
+    for (int64_t e = 0; e < *ns; ++e) in[S[e]] = 1;
+    int64_t *c = (int64_t *)malloc(sizeof(int64_t) * (size_t)(N + 1));
+    int64_t nc = 0;
+    for (int64_t q = 0; q < N; ++q)
+        if (!in[q]) c[nc++] = q;
+    if (k > nc) k = nc;
+    for (int64_t i = 0; i < k; ++i) {
+        const int64_t idx = rng_randint(rng, (int32_t)nc) - 1;
+        S[(*ns)++] = c[idx];
+        memmove(c + idx, c + idx + 1, sizeof(int64_t) * (size_t)(nc - idx - 1));
+        nc -= 1;
+    }
+    free(c);
+    free(in);
+}
+
+typedef struct rook_res {
+    int64_t p;
+    int64_t *rowperm, *colperm;   /* 1-based, m and n entries */
+    double *pv;                   /* the p pivot values (diag of U or L) */
+    double error;
+    int64_t ms_last, ns_last;     /* shape of the last factored submatrix */
+    double *L, *U;                /* optional: m×p and p×n (matrixlu.jl:270-290) */
+} rook_res;
+
+static void free_rook(rook_res *r) { free(r->rowperm); free(r->colperm); free(r->pv); free(r->L); free(r->U); }
+
+/* arrlu: I0/J0 (0-based positions, capacity m and n) are updated in place.
+ * The factorization's permutations persist between the submatrices as in
+ * the reference (swaps of rowpermutation/colpermutation entries < the
+ * submatrix size), so a submatrix whose rows or columns are not the first
+ * entries of those permutations yields positions, as the reference does.
+ * want_lu: also the full L and U of the finish (cols2Lmatrix!/rows2Umatrix!). */
+static int arrlu_core(sub_eval_fn ev, void *ectx, uint64_t rng[4], int64_t m, int64_t n, int64_t *I0,
+                      int64_t nI0, int64_t *J0, int64_t nJ0, int64_t maxrank, double reltol, double abstol,
+                      int leftorth, int numrookiter, int want_lu, rook_res *r, log_fn lg, void *lctx)
+{
+    if (maxrank > m) maxrank = m;
+    if (maxrank > n) maxrank = n;
+    int64_t *RP = (int64_t *)malloc(sizeof(int64_t) * (size_t)(m + 1));
+    int64_t *CP = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    int64_t *tmp = (int64_t *)malloc(sizeof(int64_t) * (size_t)((m > n ? m : n) + 1));
+    for (int64_t i = 0; i < m; ++i) RP[i] = i;
+    for (int64_t j = 0; j < n; ++j) CP[j] = j;
+    const int64_t cap = m < n ? m : n;
+    double *S = (double *)malloc(sizeof(double) * (size_t)(m * n + 1));
+    int64_t *rl = (int64_t *)malloc(sizeof(int64_t) * (size_t)(m + 1));
+    int64_t *cl = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    double *Lf = (double *)malloc(sizeof(double) * (size_t)(m * (cap > 0 ? cap : 1) + 1));
+    double *Uf = (double *)malloc(sizeof(double) * (size_t)((cap > 0 ? cap : 1) * n + 1));
+    double *pv = (double *)malloc(sizeof(double) * (size_t)(cap + 1));
+    int islowrank = 0, st = ORC_OK;
+    int64_t p = 0, ms = 0, ns = 0;
+    double err = NAN;
+    for (;;) {
+        if (leftorth) push_random_subset(rng, J0, &nJ0, n);
+        else push_random_subset(rng, I0, &nI0, m);
+        for (int it = 1; it <= numrookiter; ++it) {
+            const int colmove = ((it % 2 == 0) == (leftorth != 0));
+            const int64_t *ri = colmove ? I0 : RP, *ci = colmove ? CP : J0;
+            ms = colmove ? nI0 : m;
+            ns = colmove ? n : nJ0;
+            ev(ectx, ri, ms, ci, ns, S);
+            const int64_t ldu = (ms < ns ? ms : ns) > 0 ? (ms < ns ? ms : ns) : 1;
+            st = orc_rrlu(S, ms, ns, maxrank, reltol, abstol, leftorth, rl, cl, Lf, Uf, ldu, &p, &err);
+            if (lg) lg(lctx, ms, ns, p);
+            if (st != ORC_OK) goto done;
+            for (int64_t k = 0; k < p; ++k) pv[k] = leftorth ? Uf[k + ldu * k] : Lf[k + ms * k];
+            /* the swaps act on the persistent permutations' first ms / ns entries */
+            for (int64_t i = 0; i < ms; ++i) tmp[i] = RP[rl[i] - 1];
+            memcpy(RP, tmp, sizeof(int64_t) * (size_t)ms);
+            for (int64_t j = 0; j < ns; ++j) tmp[j] = CP[cl[j] - 1];
+            memcpy(CP, tmp, sizeof(int64_t) * (size_t)ns);
+            islowrank |= p < (ms < ns ? ms : ns);
+            int same = (p == nI0 && p == nJ0);
+            for (int64_t k = 0; same && k < p; ++k) same = RP[k] == I0[k] && CP[k] == J0[k];
+            if (same) break;
+            for (int64_t k = 0; k < p; ++k) { J0[k] = CP[k]; I0[k] = RP[k]; }
+            nJ0 = p;
+            nI0 = p;
+        }
+        if (islowrank || nI0 >= maxrank) break;
+    }
+    /* the finish: rows outside I0 against J0, columns outside J0 against I0
+     * (their values enter the running maximum as the reference's SubMatrix
+     * calls do) */
+    {
+        double *L2 = NULL, *U2 = NULL;
+        int64_t n2 = 0;
+        char *in = (char *)calloc((size_t)((m > n ? m : n) + 1), 1);
+        if (ms < m) {
+            for (int64_t e = 0; e < nI0; ++e) in[I0[e]] = 1;
+            for (int64_t i = 0; i < m; ++i) if (!in[i]) tmp[n2++] = i;
+            L2 = (double *)malloc(sizeof(double) * (size_t)(n2 * nJ0 + 1));
+            ev(ectx, tmp, n2, J0, nJ0, L2);
+            for (int64_t i = 0; i < n2; ++i) RP[nI0 + i] = tmp[i];   /* rowpermutation = vcat(I0, I2) */
+            for (int64_t k = 0; k < nI0; ++k) RP[k] = I0[k];
+            if (want_lu) {
+                /* cols2Lmatrix!(L2, U[1:p,1:p]) (matrixlu.jl:314-335) */
+                const int64_t ldu = (ms < ns ? ms : ns) > 0 ? (ms < ns ? ms : ns) : 1;
+                for (int64_t k = 0; k < p; ++k) {
+                    for (int64_t i = 0; i < n2; ++i) L2[i + n2 * k] = L2[i + n2 * k] / Uf[k + ldu * k];
+                    for (int64_t j = k + 1; j < p; ++j)
+                        for (int64_t i = 0; i < n2; ++i)
+                            L2[i + n2 * j] = L2[i + n2 * j] - L2[i + n2 * k] * Uf[k + ldu * j];
+                }
+                r->L = (double *)calloc((size_t)(m * p + 1), sizeof(double));
+                for (int64_t j = 0; j < p; ++j) {
+                    for (int64_t i = 0; i < p; ++i) r->L[i + m * j] = Lf[i + ms * j];
+                    for (int64_t i = 0; i < n2; ++i) r->L[p + i + m * j] = L2[i + n2 * j];
+                }
+            }
+            memset(in, 0, (size_t)((m > n ? m : n) + 1));
+        } else if (want_lu) {
+            r->L = (double *)calloc((size_t)(m * p + 1), sizeof(double));
+            memcpy(r->L, Lf, sizeof(double) * (size_t)(m * p));
+        }
+        n2 = 0;
+        if (ns < n) {
+            for (int64_t e = 0; e < nJ0; ++e) in[J0[e]] = 1;
+            for (int64_t j = 0; j < n; ++j) if (!in[j]) tmp[n2++] = j;
+            U2 = (double *)malloc(sizeof(double) * (size_t)(nI0 * n2 + 1));
+            ev(ectx, I0, nI0, tmp, n2, U2);
+            for (int64_t j = 0; j < n2; ++j) CP[nJ0 + j] = tmp[j];
+            for (int64_t k = 0; k < nJ0; ++k) CP[k] = J0[k];
+            if (want_lu) {
+                /* rows2Umatrix!(U2, L[1:p,1:p]) (matrixlu.jl:337-358) */
+                for (int64_t k = 0; k < p; ++k) {
+                    for (int64_t j = 0; j < n2; ++j) U2[k + p * j] = U2[k + p * j] / Lf[k + ms * k];
+                    for (int64_t j = 0; j < n2; ++j)
+                        for (int64_t i = k + 1; i < p; ++i)
+                            U2[i + p * j] = U2[i + p * j] - Lf[i + ms * k] * U2[k + p * j];
+                }
+                const int64_t ldu = (ms < ns ? ms : ns) > 0 ? (ms < ns ? ms : ns) : 1;
+                r->U = (double *)calloc((size_t)(p * n + 1), sizeof(double));
+                for (int64_t j = 0; j < p; ++j)
+                    for (int64_t i = 0; i < p; ++i) r->U[i + p * j] = Uf[i + ldu * j];
+                for (int64_t j = 0; j < n2; ++j)
+                    for (int64_t i = 0; i < p; ++i) r->U[i + p * (p + j)] = U2[i + p * j];
+            }
+        } else if (want_lu) {
+            const int64_t ldu = (ms < ns ? ms : ns) > 0 ? (ms < ns ? ms : ns) : 1;
+            r->U = (double *)calloc((size_t)(p * n + 1), sizeof(double));
+            for (int64_t j = 0; j < n; ++j)
+                for (int64_t i = 0; i < p; ++i) r->U[i + p * j] = Uf[i + ldu * j];
+        }
+        free(in); free(L2); free(U2);
+    }
+done:
+    r->p = p;
+    r->error = err;
+    r->ms_last = ms;
+    r->ns_last = ns;
+    r->rowperm = (int64_t *)malloc(sizeof(int64_t) * (size_t)(m + 1));
+    r->colperm = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    for (int64_t i = 0; i < m; ++i) r->rowperm[i] = RP[i] + 1;
+    for (int64_t j = 0; j < n; ++j) r->colperm[j] = CP[j] + 1;
+    r->pv = pv;
+    free(RP); free(CP); free(tmp); free(S); free(rl); free(cl); free(Lf); free(Uf);
+    return st;
+}
+
+/* the explicit-matrix evaluation of the standalone arrlu */
+typedef struct { const double *A; int64_t m; double maxabs; } mat_ctx;
+static void mat_eval(void *c, const int64_t *rows, int64_t nr, const int64_t *cols, int64_t nc, double *out)
+{
+    mat_ctx *x = (mat_ctx *)c;
+    for (int64_t j = 0; j < nc; ++j)
+        for (int64_t i = 0; i < nr; ++i) {
+            const double v = x->A[rows[i] + x->m * cols[j]];
+            out[i + nr * j] = v;
+            x->maxabs = jl_max(x->maxabs, fabs(v));
+        }
+}
+
+int orc_arrlu(const double *A, int64_t m, int64_t n, const int64_t *I0in, int64_t nI0, const int64_t *J0in,
+              int64_t nJ0, int64_t maxrank, double reltol, double abstol, int32_t leftorth, int32_t numrookiter,
+              uint64_t seed, int64_t *rowperm, int64_t *colperm, double *Lout, double *Uout, int64_t *npivot,
+              double *error)
+{
+    uint64_t rng[4];
+    orc_rng_seed(rng, seed);
+    int64_t *I0 = (int64_t *)malloc(sizeof(int64_t) * (size_t)(m + 1));
+    int64_t *J0 = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    for (int64_t e = 0; e < nI0; ++e) I0[e] = I0in[e] - 1;
+    for (int64_t e = 0; e < nJ0; ++e) J0[e] = J0in[e] - 1;
+    mat_ctx c = {A, m, 0.0};
+    rook_res r;
+    memset(&r, 0, sizeof r);
+    int st = arrlu_core(mat_eval, &c, rng, m, n, I0, nI0, J0, nJ0, maxrank, reltol, abstol, leftorth,
+                        numrookiter, 1, &r, NULL, NULL);
+    if (st == ORC_OK) {
+        memcpy(rowperm, r.rowperm, sizeof(int64_t) * (size_t)m);
+        memcpy(colperm, r.colperm, sizeof(int64_t) * (size_t)n);
+        if (r.L) memcpy(Lout, r.L, sizeof(double) * (size_t)(m * r.p));
+        if (r.U) memcpy(Uout, r.U, sizeof(double) * (size_t)(r.p * n));
+        *npivot = r.p;
+        *error = r.error;
+    }
+    free_rook(&r);
+    free(I0);
+    free(J0);
+    return st;
+}
+
 /* updatepivots! with pivotsearch=:full (tensorci2.jl:510-551, 599-607).
  * The setsitetensor! calls at 601-604 are made only when there are no
  * extras; their tensors are invalidated by the next updatepivots! or
  * recomputed by fillsitetensors!, so this restatement omits them. */
+/* Pi entries f(Icombined[i] ⊕ Jcombined[j]) of the rook search, folded into
+ * maxsamplevalue as SubMatrix does (tensorci2.jl:499-503 and 581) */
+typedef struct { orc_tci *t; const orc_functor *f; const mset *I, *J; } tci_sub_ctx;
+static void tci_sub_eval(void *c, const int64_t *rows, int64_t nr, const int64_t *cols, int64_t nc, double *out)
+{
+    tci_sub_ctx *x = (tci_sub_ctx *)c;
+    const int32_t L = x->t->L, nl = x->I->len;
+    int32_t *pt = (int32_t *)malloc(sizeof(int32_t) * (size_t)(L + 1));
+    for (int64_t j = 0; j < nc; ++j)
+        for (int64_t i = 0; i < nr; ++i) {
+            memcpy(pt, ms_at(x->I, (int32_t)rows[i]), sizeof(int32_t) * (size_t)nl);
+            memcpy(pt + nl, ms_at(x->J, (int32_t)cols[j]), sizeof(int32_t) * (size_t)(L - nl));
+            out[i + nr * j] = feval(x->f, pt, L);
+        }
+    free(pt);
+    updatemaxsample(x->t, out, nr * nc);
+    x->t->ws[0] += (double)(nr * nc);
+    x->t->ws[5] += 1.0;
+}
+
+static void tci_log(void *c, int64_t m, int64_t n, int64_t p) { log_rrlu((orc_tci *)c, m, n, p); }
+
 static int updatepivots_x(orc_tci *t, int32_t b0, const orc_functor *f, int leftorth,
                           double reltol, double abstol, int64_t maxbonddim,
                           const mset *extraI, const mset *extraJ)
@@ -556,6 +803,43 @@ static int updatepivots_x(orc_tci *t, int32_t b0, const orc_functor *f, int left
     ms_kron_left(&Jc, t->d[b0 + 1], &t->J[b0 + 1]);
     ms_union_inplace(&Jc, extraJ);
     int64_t m = Ic.n, n = Jc.n, nc;
+    if (t->pivotsearch == 1) {
+        /* :rook (tensorci2.jl:552-595): I0/J0 = the positions of Iset[b+1] /
+         * Jset[b] in Icombined / Jcombined (findfirst, misses dropped) */
+        int64_t *I0 = (int64_t *)malloc(sizeof(int64_t) * (size_t)(m + 1));
+        int64_t *J0 = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
+        int64_t nI0 = 0, nJ0 = 0;
+        for (int32_t e = 0; e < t->I[b0 + 1].n; ++e) {
+            const int32_t i = ms_find(&Ic, ms_at(&t->I[b0 + 1], e));
+            if (i >= 0) I0[nI0++] = i;
+        }
+        for (int32_t e = 0; e < t->J[b0].n; ++e) {
+            const int32_t j = ms_find(&Jc, ms_at(&t->J[b0], e));
+            if (j >= 0) J0[nJ0++] = j;
+        }
+        tci_sub_ctx ec = {t, f, &Ic, &Jc};
+        rook_res rr;
+        memset(&rr, 0, sizeof rr);
+        int st = arrlu_core(tci_sub_eval, &ec, t->rng, m, n, I0, nI0, J0, nJ0, maxbonddim, reltol, abstol,
+                            leftorth, 5, 0, &rr, tci_log, t);
+        free(I0);
+        free(J0);
+        if (st != ORC_OK) { free_rook(&rr); ms_free(&Ic); ms_free(&Jc); return st; }
+        if (rr.p > 0) {
+            ms_select(&t->I[b0 + 1], &Ic, rr.rowperm, rr.p);
+            ms_select(&t->J[b0], &Jc, rr.colperm, rr.p);
+            double *e = (double *)malloc(sizeof(double) * (size_t)(rr.p + 1));
+            for (int64_t k = 0; k < rr.p; ++k) e[k] = fabs(rr.pv[k]);
+            e[rr.p] = rr.error;
+            updateerrors(t, b0, e, (int32_t)rr.p + 1);
+            free(e);
+            free_rook(&rr);
+            ms_free(&Ic);
+            ms_free(&Jc);
+            return ORC_OK;
+        }
+        free_rook(&rr);   /* no pivot: the full search (tensorci2.jl:584-595) */
+    }
     double *Pi = (double *)malloc(sizeof(double) * (size_t)(m * n + 1));
     filltensor(t, f, &Ic, &Jc, Pi, &nc);
     updatemaxsample(t, Pi, m * n);
@@ -1208,6 +1492,7 @@ int orc_optimize(orc_tci *t, const orc_functor *f, const orc_opts *o)
         return fail(ORC_ERR_ARG, "Specify either tolerance > 0 or some maxbonddim; otherwise, "
                                  "the convergence criterion is not reachable!");
     orc_rng_seed(t->rng, o->seed);
+    t->pivotsearch = o->pivotsearch;
     const int32_t L = t->L;
     int32_t *found = (int32_t *)malloc(sizeof(int32_t) * (size_t)((o->nsearchglobalpivot + 1) * L));
     int st = ORC_OK;

@@ -69,6 +69,7 @@ typedef struct orc_opts {
     int32_t strictlynested;
     int32_t checkconvglobalpivot;
     uint64_t seed;              /* project PRNG seed for the global search     */
+    int32_t pivotsearch;        /* 0 :full, 1 :rook (tensorci2.jl:552-595)     */
 } orc_opts;
 
 const char *orc_last_error(void);
@@ -79,6 +80,15 @@ void orc_submatrixargmax(const double *A, int64_t lda, int64_t m, int64_t n, int
 int orc_rrlu(const double *A, int64_t m, int64_t n, int64_t maxrank, double reltol,
              double abstol, int32_t leftorth, int64_t *rowperm, int64_t *colperm,
              double *L, double *U, int64_t ldu, int64_t *npivot, double *error);
+/* arrlu (matrixlu.jl:227-293) of an explicit matrix: I0/J0 the starting
+ * rows/columns (1-based), randomsubset draws from the project PRNG seeded
+ * with `seed`; L (m×p) and U (p×n) as the reference builds them (the last
+ * submatrix's factors, then cols2Lmatrix!/rows2Umatrix! of the rest),
+ * rowperm/colperm 1-based. */
+int orc_arrlu(const double *A, int64_t m, int64_t n, const int64_t *I0, int64_t nI0, const int64_t *J0,
+              int64_t nJ0, int64_t maxrank, double reltol, double abstol, int32_t leftorth, int32_t numrookiter,
+              uint64_t seed, int64_t *rowperm, int64_t *colperm, double *L, double *U, int64_t *npivot,
+              double *error);
 /* MatrixLUCI left/right factors from an rrLU result (matrixluci.jl:40-84) */
 void orc_luci_left(int64_t m, int64_t n, int64_t p, int32_t leftorth, const int64_t *rowperm,
                    const int64_t *colperm, const double *L, const double *U, int64_t ldu,

@@ -785,8 +785,10 @@ def _resolve_tolerance(tolerance, pivottolerance):
 def _holder(f, localdims, initialpivots, kw):
     kw = dict(kw)
     tol = _resolve_tolerance(kw.pop("tolerance", None), kw.pop("pivottolerance", None))
-    if kw.pop("pivotsearch", "full") != "full":
-        raise ArgumentError(3, "only pivotsearch=:full runs on the device")
+    ps = str(kw.pop("pivotsearch", "full")).lstrip(":")
+    if ps not in _abi.PIVOTSEARCH:
+        raise ArgumentError(3, f"Unknown pivot search strategy {ps}. Choose from :rook, :full.")
+    kw["pivotsearch"] = ps
     kw.pop("checkbatchevaluatable", None)   # device functors are batch evaluators
     kw.pop("verbosity", None)
     kw.pop("loginterval", None)

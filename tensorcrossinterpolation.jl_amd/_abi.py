@@ -13,7 +13,8 @@ import os
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG_DIR, "libtci_hip.so")
+# the in-tree build; TCI_HIP_LIB names another build of the same library
+LIB_PATH = os.environ.get("TCI_HIP_LIB") or os.path.join(PKG_DIR, "libtci_hip.so")
 
 TCI_UNLIMITED = (1 << 63) - 1
 
@@ -51,7 +52,7 @@ class tci_options(C.Structure):
                 ("nsearchglobalpivot", C.c_int32), ("tolmarginglobalsearch", C.c_double),
                 ("strictlynested", C.c_int32), ("checkconvglobalpivot", C.c_int32),
                 ("seed", C.c_uint64), ("chi_cap", C.c_int32), ("flags", C.c_int32),
-                ("cache_log2", C.c_int32), ("reserved", C.c_int32)]
+                ("cache_log2", C.c_int32), ("pivotsearch", C.c_int32)]
 
 
 class tci_state(C.Structure):
@@ -126,19 +127,22 @@ def u8ptr(a):
 
 
 SWEEP = {"backandforth": 0, "forward": 1, "backward": 2}
+PIVOTSEARCH = {"full": 0, "rook": 1}
 
 
 def make_options(tolerance=1e-8, maxbonddim=TCI_UNLIMITED, maxiter=20, sweepstrategy="backandforth",
                  normalizeerror=True, ncheckhistory=3, maxnglobalpivot=5, nsearchglobalpivot=5,
                  tolmarginglobalsearch=10.0, strictlynested=False, checkconvglobalpivot=True,
-                 seed=0, chi_cap=0, flags=0, cache_log2=0):
+                 seed=0, chi_cap=0, flags=0, cache_log2=0, pivotsearch="full"):
     if isinstance(sweepstrategy, str):
         sweepstrategy = SWEEP[sweepstrategy]
+    if isinstance(pivotsearch, str):
+        pivotsearch = PIVOTSEARCH[pivotsearch.lstrip(":")]
     return tci_options(float(tolerance), int(maxbonddim), int(maxiter), int(sweepstrategy),
                        int(bool(normalizeerror)), int(ncheckhistory), int(maxnglobalpivot),
                        int(nsearchglobalpivot), float(tolmarginglobalsearch), int(bool(strictlynested)),
                        int(bool(checkconvglobalpivot)), int(seed) & ((1 << 64) - 1), int(chi_cap), int(flags),
-                       int(cache_log2), 0)
+                       int(cache_log2), int(pivotsearch))
 
 
 class ProblemHolder:

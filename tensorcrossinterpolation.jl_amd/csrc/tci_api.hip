@@ -1100,9 +1100,12 @@ tci_status tci_solve_submit(tci_ctx *ctx, int32_t count, const tci_problem *prob
     size_t tin = 0, tstate = 0, trep = 0;
     int max_mr = 0, max_tab = 0;
     int64_t max_sets = 0, max_lut = 0;
+    // one functor-specialized kernel when every problem has that functor;
+    // the generic kernel otherwise, and for any :rook search or memo table
+    // (the specialized kernels carry only the :full path without a table)
     int fkind = problems[0].f.kind;
     for (int i = 0; i < count; ++i) {
-        if (problems[i].f.kind != fkind) fkind = 0;
+        if (problems[i].f.kind != fkind || problems[i].opts.pivotsearch || problems[i].opts.cache_log2) fkind = 0;
         std::string msg;
         int s = check_problem(problems[i], caps[(size_t)i], &msg);
         if (s != TCI_OK) return set_err((tci_status)s, "problem %d: %s", i, msg.c_str());

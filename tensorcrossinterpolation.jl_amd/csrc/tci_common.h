@@ -72,6 +72,7 @@ struct EngShared {
     int32_t cnt;       // scratch count (union compaction)
     int32_t nf;        // global pivots found
     int32_t npe;       // length of pivoterrors
+    int32_t rk[4];     // :rook search: |I0|, |J0|, a flag, a count
     double pe[kPivErrCap];   // pivoterrors while the solve runs, when they fit
     uint64_t cache_lookups, cache_hits;   // memo table of f (CachedFunction)
 };

@@ -66,6 +66,7 @@ struct SolveDesc {
     int32_t sumd;            // Σ_b d_b
     int32_t maxiter_cap;     // capacity of ranks/errors/nglob
     int32_t optflags;        // TCI_FLAG_* of tci_options
+    int32_t pivotsearch;     // 0 :full, 1 :rook
     double tolerance;
     double tolmargin;
     int64_t maxbonddim;
@@ -84,6 +85,7 @@ struct SolveDesc {
     uint32_t *cache_state;       // 2^cache_log2
     uint64_t *cache_keys;        // 2^cache_log2 × CACHE_LIMBS
     double *cache_vals;          // 2^cache_log2
+    int32_t *rook;               // :rook work arrays, 6 × mrcap (RP, CP, I0, J0, a copy, candidates)
     // ---- state (device) ----
     uint8_t *sets;           // 6L sets × setcap entries × L bytes
     int32_t *setn;           // 6L counts

@@ -311,7 +311,7 @@ struct Engine {
     ENG void build_lut()
     {
         // (with a memo table every evaluation goes through it instead)
-        if (!(FK == TCI_F_LORENTZ_INT || (FK == 0 && D.fkind == TCI_F_LORENTZ_INT)) || !tm.llut || D.cache_log2)
+        if (!(FK == TCI_F_LORENTZ_INT || (FK == 0 && D.fkind == TCI_F_LORENTZ_INT)) || !tm.llut || cache_on())
             return;
         int64_t smax = 0;
         for (int t = 0; t < L; ++t) smax += (int64_t)D.localdims[t] * D.localdims[t];
@@ -530,6 +530,10 @@ struct Engine {
         if (tm.tid() == 0) setn(kind, b) = p;
     }
 
+    // the memo table is compiled into the generic kernel only (FK == 0): a
+    // solve with one runs there
+    ENG_INLINE bool cache_on() const { return FK == 0 && D.cache_log2 != 0; }
+
     // ------------------------------------------------------- memo table of f
     // f(x) through the solve's memo table, as CachedFunction (cachedfunction.jl:
     // 62-110) memoizes it: the key Σ_t coef[t]·(σ_t - 1) (177-184, 256 bits),
@@ -598,7 +602,7 @@ struct Engine {
                     lmax = jl_max(lmax, fabs(v));
                 }
             }
-        } else if (D.cache_log2) {
+        } else if (cache_on()) {
             uint64_t hits = 0;
             for (int j = tm.wave(); j < n; j += tm.nwaves())
                 for (int i = tm.lane(); i < m; i += tm.wsize()) {
@@ -727,7 +731,7 @@ struct Engine {
     // sums of v², never stored.  Returns p or -status like rrlu().
     ENG_INLINE bool lorentz_tile(int m, int n) const
     {
-        return (FK == TCI_F_LORENTZ_INT || (FK == 0 && D.fkind == TCI_F_LORENTZ_INT)) && !D.cache_log2 &&
+        return (FK == TCI_F_LORENTZ_INT || (FK == 0 && D.fkind == TCI_F_LORENTZ_INT)) && !cache_on() &&
                Team::RA > 0 && Tile::fits(m, n) &&
                m <= D.mrcap && n <= D.mrcap;
     }
@@ -824,6 +828,236 @@ struct Engine {
         const uint64_t t2 = tm.clock();
         select_into(SET_I, b + 1, rows, rperm, p, b + 1);
         select_into(SET_J, b, cols, cperm, p, jlen(b));
+        updateerrors(b, p, err);
+        tm.sync();
+        if (tm.tid() == 0) S.tph[PH_SETS] += (t1 - t0) + (tm.clock() - t2);
+        return ST_OK;
+    }
+
+    // ---------------------------------------------- :rook pivot search
+    // out[i + ld·j] = f(rows[ri[i]] ⊕ cols[ci[j]]) (SubMatrix, tensorci2.jl:
+    // 488-503), |values| folded into maxsamplevalue as the reference folds
+    // the SubMatrix maximum into it (581); out may be null
+    ENG void eval_idx(int nl, const int32_t *ri, int ms, const int32_t *ci, int ns, double *out, int64_t ld)
+    {
+        const uint64_t t0 = tm.clock();
+        double lmax = 0.0;
+        uint64_t hits = 0;
+        for (int j = tm.wave(); j < ns; j += tm.nwaves())
+            for (int i = tm.lane(); i < ms; i += tm.wsize()) {
+                tci_point x;
+                x.a = rows + (int64_t)ri[i] * L;
+                x.b = cols + (int64_t)ci[j] * L;
+                x.na = nl;
+                x.L = L;
+                const double v = cache_on() ? cached_f(x, hits) : tci_functor_eval(FK ? FK : D.fkind, D.params,
+                                                                                     D.iparams, &x);
+                if (out) out[i + ld * j] = v;
+                lmax = jl_max(lmax, fabs(v));
+            }
+        if (hits) __atomic_fetch_add(&S.cache_hits, hits, __ATOMIC_RELAXED);
+        const double g = tm.max_nan(lmax);
+        if (tm.tid() == 0) {
+            S.maxsample = jl_max(fabs(S.maxsample), g);
+            S.ws[0] += (double)ms * (double)ns;
+            S.ws[5] += 1.0;
+            if (cache_on()) S.cache_lookups += (uint64_t)ms * (uint64_t)ns;
+            S.tph[PH_EVAL] += tm.clock() - t0;
+        }
+        tm.sync();
+    }
+
+    // out = the q in [0, N) with fl[q] == 0, ascending (wave 0, a ballot per
+    // chunk); returns their count; ends with a barrier
+    ENG_INLINE int compact_free(const uint8_t *fl, int N, int32_t *out)
+    {
+        if (tm.wave() == 0) {
+            int base = 0;
+            for (int q0 = 0; q0 < N; q0 += tm.wsize()) {
+                const int q = q0 + tm.lane();
+                const bool keep = q < N && !fl[q];
+                const uint64_t mask = tm.wave_ballot(keep);
+                if (keep) out[base + __builtin_popcountll(mask & ((((uint64_t)1) << tm.lane()) - 1))] = q;
+                base += __builtin_popcountll(mask);
+            }
+            if (tm.lane() == 0) S.rk[3] = base;
+        }
+        tm.sync();
+        return S.rk[3];
+    }
+
+    // fl[q] = (q ∈ set[0..ns)) for q < N; ends with a barrier
+    ENG_INLINE void mark(uint8_t *fl, int N, const int32_t *set, int ns)
+    {
+        for (int q = tm.tid(); q < N; q += tm.nt()) fl[q] = 0;
+        tm.sync();
+        for (int e = tm.tid(); e < ns; e += tm.nt()) fl[set[e]] = 1;
+        tm.sync();
+    }
+
+    // pushrandomsubset!(S, 1:N, max(1, |S|)) (util.jl:36-58): k draws from
+    // setdiff(1:N, S) in ascending order, each rand(1:length(c)) (the
+    // solve's PRNG: 1 + (32-bit draw mod length)) followed by deleteat!.
+    // Every thread draws the same numbers from its own copy of the state.
+    ENG int push_random_subset(int32_t *set, int ns, int N, int32_t *ca, int32_t *cb)
+    {
+        uint8_t *fl = D.flags;
+        mark(fl, N, set, ns);
+        int c = compact_free(fl, N, ca);
+        int k = ns > 1 ? ns : 1;
+        if (k > c) k = c;
+        uint64_t rng[4] = {S.rng[0], S.rng[1], S.rng[2], S.rng[3]};
+        int32_t *cur = ca, *nxt = cb;
+        for (int i = 0; i < k; ++i) {
+            const uint32_t r = (uint32_t)(eng_rng_next(rng) >> 32);
+            const int idx = (int)(r % (uint32_t)c);
+            if (tm.tid() == 0) set[ns + i] = cur[idx];
+            for (int q = tm.tid(); q < c - 1; q += tm.nt()) nxt[q] = cur[q < idx ? q : q + 1];
+            tm.sync();
+            int32_t *t = cur;
+            cur = nxt;
+            nxt = t;
+            --c;
+        }
+        if (tm.tid() == 0)
+            for (int w = 0; w < 4; ++w) S.rng[w] = rng[w];
+        tm.sync();
+        return k;
+    }
+
+    // arrlu (matrixlu.jl:227-293) on the lazily evaluated Pi of bond b
+    // (tensorci2.jl:552-583), numrookiter = 5.  I0/J0 start as the
+    // positions of Iset[b+1]/Jset[b] in Icombined/Jcombined (findfirst).
+    // The factorization's permutations RP/CP persist between the
+    // submatrices as the reference's do (its swaps act on their first
+    // |submatrix| entries).  Returns p (the pivots are I0/J0[0..p), their
+    // values pvals, *err the last error), 0 for none, or -status.
+    ENG int rook(int b, int m, int n, bool leftorth, double abstol, double *err)
+    {
+        const int mr = D.mrcap;
+        int32_t *RP = D.rook, *CP = RP + mr, *I0 = CP + mr, *J0 = I0 + mr, *ta = J0 + mr, *tb = ta + mr;
+        const int nl = b + 1;
+        int64_t maxrank = D.maxbonddim;
+        if (maxrank > m) maxrank = m;
+        if (maxrank > n) maxrank = n;
+        // positions of Iset[b+1] in Icombined and of Jset[b] in Jcombined
+        const int nIs = setn(SET_I, b + 1), nJs = setn(SET_J, b);
+        const uint8_t *Is = setp(SET_I, b + 1), *Js = setp(SET_J, b);
+        for (int e = tm.tid(); e < nIs + nJs; e += tm.nt()) {
+            // one thread per set entry scans the table in order (findfirst)
+            const bool isI = e < nIs;
+            const uint8_t *x = isI ? Is + (int64_t)e * L : Js + (int64_t)(e - nIs) * L;
+            const uint8_t *tab = isI ? rows : cols;
+            const int cnt = isI ? m : n, len = isI ? nl : jlen(b);
+            int at = INT32_MAX;
+            for (int i = 0; i < cnt; ++i)
+                if (eq(tab + (int64_t)i * L, x, len)) {
+                    at = i;
+                    break;
+                }
+            ta[e] = at;
+        }
+        tm.sync();
+        if (tm.tid() == 0) {
+            int a = 0, c = 0;
+            for (int e = 0; e < nIs; ++e)
+                if (ta[e] != INT32_MAX) I0[a++] = ta[e];
+            for (int e = 0; e < nJs; ++e)
+                if (ta[nIs + e] != INT32_MAX) J0[c++] = ta[nIs + e];
+            S.rk[0] = a;
+            S.rk[1] = c;
+        }
+        for (int i = tm.tid(); i < m; i += tm.nt()) RP[i] = i;
+        for (int j = tm.tid(); j < n; j += tm.nt()) CP[j] = j;
+        tm.sync();
+        bool islowrank = false;
+        int p = 0, ms = 0, ns = 0;
+        double e = 0.0;
+        for (;;) {
+            const int w = leftorth ? 1 : 0;
+            const int k = leftorth ? push_random_subset(J0, S.rk[1], n, ta, tb)
+                                   : push_random_subset(I0, S.rk[0], m, ta, tb);
+            if (tm.tid() == 0) S.rk[w] += k;
+            tm.sync();
+            for (int it = 1; it <= 5; ++it) {
+                const bool colmove = ((it % 2 == 0) == leftorth);
+                const int nI0 = S.rk[0], nJ0 = S.rk[1];
+                const int32_t *ri = colmove ? I0 : RP, *ci = colmove ? CP : J0;
+                ms = colmove ? nI0 : m;
+                ns = colmove ? n : nJ0;
+                double *A = pibuf(ms, ns);
+                perm_area(A == tm.lpi ? (((int64_t)ms * ns + 1) & ~(int64_t)1) : 0, ms, ns);
+                eval_idx(nl, ri, ms, ci, ns, A, ms);
+                p = rrlu(A, ms, ns, maxrank, 1e-14, abstol, leftorth, &e);
+                if (p < 0) return p;
+                // the swaps of this factorization act on RP/CP's first ms/ns entries
+                for (int i = tm.tid(); i < ms; i += tm.nt()) ta[i] = RP[rperm[i]];
+                for (int j = tm.tid(); j < ns; j += tm.nt()) tb[j] = CP[cperm[j]];
+                tm.sync();
+                for (int i = tm.tid(); i < ms; i += tm.nt()) RP[i] = ta[i];
+                for (int j = tm.tid(); j < ns; j += tm.nt()) CP[j] = tb[j];
+                if (tm.tid() == 0) S.rk[2] = 1;
+                tm.sync();
+                islowrank |= p < (ms < ns ? ms : ns);
+                if (p == nI0 && p == nJ0) {
+                    for (int k = tm.tid(); k < p; k += tm.nt())
+                        if (RP[k] != I0[k] || CP[k] != J0[k]) S.rk[2] = 0;
+                    tm.sync();
+                    if (S.rk[2]) break;
+                }
+                for (int k = tm.tid(); k < p; k += tm.nt()) {
+                    I0[k] = RP[k];
+                    J0[k] = CP[k];
+                }
+                tm.sync();
+                if (tm.tid() == 0) S.rk[0] = S.rk[1] = p;
+                tm.sync();
+            }
+            if (islowrank || S.rk[0] >= maxrank) break;
+        }
+        // the finish (matrixlu.jl:270-290): rows outside I0 against J0 and
+        // columns outside J0 against I0 are evaluated (the values enter
+        // maxsamplevalue); the factors they complete are not needed here
+        if (ms < m) {
+            mark(D.flags, m, I0, S.rk[0]);
+            const int c = compact_free(D.flags, m, ta);
+            eval_idx(nl, ta, c, J0, S.rk[1], nullptr, 0);
+        }
+        if (ns < n) {
+            mark(D.flags, n, J0, S.rk[1]);
+            const int c = compact_free(D.flags, n, ta);
+            eval_idx(nl, I0, S.rk[0], ta, c, nullptr, 0);
+        }
+        *err = e;
+        return p;
+    }
+
+    // updatepivots! with pivotsearch = :rook (tensorci2.jl:552-595); the
+    // full search when the rook search finds no pivot (584-595).  Kept out
+    // of line, apart from the :full path.
+    ENG_NOINLINE int updatepivots_rook(int b, bool leftorth, double abstol, int ekind_i, int ekind_j)
+    {
+        const uint64_t t0 = tm.clock();
+        int m = kron_rows(b);
+        int n = kron_cols(b);
+        if (m > D.mrcap || n > D.mrcap) return fail(ST_CAPACITY);
+        tm.sync();
+        if (ekind_i >= 0) {
+            m = union_extras(true, b, m, ekind_i);
+            if (S.status) return S.status;
+            n = union_extras(false, b, n, ekind_j);
+            if (S.status) return S.status;
+        }
+        double err;
+        const uint64_t t1 = tm.clock();
+        const int p = rook(b, m, n, leftorth, abstol, &err);
+        if (p < 0) return fail(-p);
+        if (p == 0) return updatepivots(b, leftorth, abstol, ekind_i, ekind_j);
+        if (p > D.setcap) return fail(ST_CAPACITY);
+        const uint64_t t2 = tm.clock();
+        int32_t *I0 = D.rook + 2 * D.mrcap, *J0 = I0 + D.mrcap;
+        select_into(SET_I, b + 1, rows, I0, p, b + 1);
+        select_into(SET_J, b, cols, J0, p, jlen(b));
         updateerrors(b, p, err);
         tm.sync();
         if (tm.tid() == 0) S.tph[PH_SETS] += (t1 - t0) + (tm.clock() - t2);
@@ -1184,7 +1418,10 @@ struct Engine {
             const bool fwd = D.sweepstrategy == 1 || (D.sweepstrategy == 0 && (iter & 1));
             for (int s = 0; s < L - 1; ++s) {
                 const int b = fwd ? s : L - 2 - s;
-                int st = updatepivots(b, fwd, abstol, extras ? ek_i : -1, extras ? ek_j : -1);
+                // (:rook is compiled into the generic kernel only, FK == 0)
+                int st = (FK == 0 && D.pivotsearch)
+                             ? updatepivots_rook(b, fwd, abstol, extras ? ek_i : -1, extras ? ek_j : -1)
+                             : updatepivots(b, fwd, abstol, extras ? ek_i : -1, extras ? ek_j : -1);
                 if (st != ST_OK) {
                     if (tm.tid() == 0) D.ints[INT_ERRSITE] = b + 1;
                     return st;
@@ -1531,7 +1768,7 @@ struct Engine {
             for (int b = 0; b < L; ++b) { D.Tdims[3 * b] = 0; D.Tdims[3 * b + 1] = 0; D.Tdims[3 * b + 2] = 0; }
         }
         tm.sync();
-        if (D.cache_log2) {
+        if (cache_on()) {
             // an empty memo table (a new CachedFunction per solve)
             const int64_t ne = (int64_t)1 << D.cache_log2;
             for (int64_t q = tm.tid(); q < ne; q += tm.nt()) D.cache_state[q] = CACHE_EMPTY;

@@ -29,6 +29,7 @@ struct SolveCaps {
     int32_t maxiter;
     int32_t has_init;   // the solve starts from given pivot sets (tci_state)
     int32_t cache_log2; // memo table of f: 2^cache_log2 entries, 0 = none
+    int32_t rook;       // pivotsearch = :rook (its work arrays)
 };
 
 // bytes of a starting-state image: the pivot sets in the set arena's own
@@ -80,6 +81,7 @@ inline void layout_solve(const SolveCaps &c, SolveDesc *D, uint8_t *in, uint8_t 
     size_t o_cst = take(ncache * 4);
     size_t o_ckey = take(ncache * CACHE_LIMBS * 8);
     size_t o_cval = take(ncache * 8);
+    size_t o_rook = take(c.rook ? 6 * mr * 4 : 0);
     *state_bytes = o;
 
     size_t r = 0;
@@ -134,6 +136,7 @@ inline void layout_solve(const SolveCaps &c, SolveDesc *D, uint8_t *in, uint8_t 
         D->work = (double *)(base + o_work);
         D->gpts = base + o_gpts;
         D->stage = (double *)(base + o_stage);
+        D->rook = c.rook ? (int32_t *)(base + o_rook) : nullptr;
     }
     if (rep) {
         D->ints = (int32_t *)(rep + r_ints);

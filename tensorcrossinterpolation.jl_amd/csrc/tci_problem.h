@@ -273,6 +273,9 @@ inline int check_problem(const tci_problem &p, SolveCaps &c, std::string *msg)
     if (o.cache_log2 && !cache_key_coefs(L, p.localdims, nullptr))
         return bad(TCI_ERR_ARG, "Overflow in CachedFunction: the domain has more than 2^256 points");
     c.cache_log2 = o.cache_log2;
+    if (o.pivotsearch != 0 && o.pivotsearch != 1)
+        return bad(TCI_ERR_ARG, "Unknown pivot search strategy. Choose from :rook, :full.");
+    c.rook = o.pivotsearch;
     return TCI_OK;
 }
 
@@ -296,6 +299,7 @@ inline void fill_desc_scalars(const tci_problem &p, SolveDesc &D)
     D.maxbonddim = o.maxbonddim;
     D.seed = o.seed;
     D.optflags = o.flags;
+    D.pivotsearch = o.pivotsearch;
     D.init_maxsample = p.state ? p.state->maxsamplevalue : 0.0;
     D.init_hist = (p.state && p.state->nI_hist) ? 1 : 0;
 }

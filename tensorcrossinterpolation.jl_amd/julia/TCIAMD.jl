@@ -214,7 +214,7 @@ struct COptions
     chi_cap::Int32
     flags::Int32
     cache_log2::Int32
-    reserved::Int32
+    pivotsearch::Int32
 end
 
 const FLAG_NO_FINAL_SWEEP = Int32(1)
@@ -284,12 +284,13 @@ function _options(; tolerance::Union{Float64,Nothing}=nothing, pivottolerance::U
         tolerance = pivottolerance
     end
     tol = something(tolerance, 1e-8)
-    pivotsearch == :full || throw(ArgumentError("only pivotsearch=:full runs on the device"))
+    pivotsearch in (:full, :rook) ||
+        throw(ArgumentError("Unknown pivot search strategy $pivotsearch. Choose from :rook, :full."))
     # verbosity / loginterval only print in the reference; checkbatchevaluatable
     # holds trivially for a DeviceFunctor (a BatchEvaluator)
     return COptions(tol, maxbonddim, maxiter, _sweep[sweepstrategy], normalizeerror, ncheckhistory,
                     maxnglobalpivot, nsearchglobalpivot, tolmarginglobalsearch, strictlynested,
-                    checkconvglobalpivot, UInt64(seed), chi_cap, flags, cache_log2, 0)
+                    checkconvglobalpivot, UInt64(seed), chi_cap, flags, cache_log2, pivotsearch === :rook ? 1 : 0)
 end
 
 # one device solve; a solve that outgrows the capacity chosen at submit is

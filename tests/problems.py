@@ -128,3 +128,13 @@ def served_mix(count, seed=5, chi_cap=0):
         out.append((f"served{q}", GSUM, params, [], [d] * n, None,
                     dict(tolerance=tol, maxbonddim=128, chi_cap=chi_cap)))
     return out
+
+
+def rook_specs():
+    """The specs above with pivotsearch = :rook (tensorci2.jl:552-595), for
+    those whose reference tests also run with rook (test_tensorci2.jl:55,
+    247, 343) and a few more shapes."""
+    keep = {"lorentz5_tight", "lorentz5_pivots", "lorentz5_forward", "lorentz6_backward_strict", "gauss_sum",
+            "quantics_gc", "quantics_exp_maxbond1", "random_tt", "diag_table", "lorentz8_nosearch"}
+    return [(name + "_rook", kind, params, ip, ld, piv, dict(opts, pivotsearch="rook"))
+            for name, kind, params, ip, ld, piv, opts in specs() if name in keep]
