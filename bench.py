@@ -423,6 +423,33 @@ def rrlu_bond_section(args, tci, ctx, cpu, m=4352, n=4352, rank=300, maxrank=256
 
 
 # ------------------------------------------------------------------ served
+def slot_leg(args, tci, ctx, P, reqs, batch_cap):
+    """The same requests admitted one at a time into a resident grid
+    (tci_server_*): a request starts as soon as a workgroup is free.  The
+    arrival streams are the batch leg's (lambda x the batch capacity), so the
+    latencies compare like for like."""
+    from importlib import import_module
+    serving = import_module("tci_amd.serving")
+    # the largest request of the mix sizes the grid's on-chip plan
+    shape = (tci.DeviceFunctor(P.GSUM, np.asarray(P.gauss_sum(8, 12, 32, 0), float), np.zeros(0, np.int32)),
+             [32] * 12, None, dict(tolerance=1e-10, maxbonddim=128))
+    srv = serving.SlotServer(ctx, shape, slots=max(16384, len(reqs)), idle_ms=50)
+    try:
+        res, st = srv.run(reqs, np.zeros(len(reqs)))
+        out = {"admission": "slot: a resident grid takes each request from a device queue",
+               "capacity_solves_per_s": round(st.solves_per_s, 2), "failed": int(sum(r.status != 0 for r in res)),
+               "capacity_retries": st.retries, "sweep": []}
+        for lam in args.c5_lambdas:
+            arr = serving.poisson_arrivals(len(reqs), lam * batch_cap, seed=5)
+            res, st = srv.run(reqs, arr)
+            out["sweep"].append({"lambda": lam, "arrival_rate_per_s": round(lam * batch_cap, 1),
+                                 "solves_per_s": round(st.solves_per_s, 2), "p50_ms": round(st.p50_ms, 3),
+                                 "p99_ms": round(st.p99_ms, 3), "failed": int(sum(r.status != 0 for r in res))})
+    finally:
+        srv.close()
+    return out
+
+
 def served_section(args, tci, ctx, cpu):
     """The served workload: 10 000 requests, each a sum of K ~ U{1..8}
     separable Gaussians in n ~ U{4..12} dims, one local dimension
@@ -451,6 +478,8 @@ def served_section(args, tci, ctx, cpu):
                              "p99_ms": round(st.p99_ms, 3), "batches": st.batches,
                              "mean_batch": round(st.mean_batch, 1),
                              "failed": int(sum(r.status != 0 for r in res))})
+    if args.c5_slot:
+        out["slot"] = slot_leg(args, tci, ctx, P, reqs, cap)
     if args.cpu_seconds > 0:
         O = _oracle()
         k = [0]
@@ -473,9 +502,9 @@ def served_section(args, tci, ctx, cpu):
 # ------------------------------------------------------------------ main
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["all", "solve", "rrlu", "bond"], default="all",
+    ap.add_argument("--workload", choices=["all", "solve", "rrlu", "bond", "served"], default="all",
                     help="all: the headline line with every sub-object; solve: the headline only; "
-                         "rrlu: the batched rrLU only")
+                         "rrlu: the batched rrLU only; bond: the C4 bond rrLU; served: the C5 stream")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -494,6 +523,7 @@ def main():
     ap.add_argument("--c5-requests", type=int, default=10000)
     ap.add_argument("--c5-max-batch", type=int, default=4096)
     ap.add_argument("--c5-lambdas", type=float, nargs="*", default=[0.25, 0.5, 1.0, 2.0, 4.0])
+    ap.add_argument("--c5-slot", type=int, default=1, help="1: also serve C5 with slot-level admission")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 plain device copies of 1 GiB (the reference kernel for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -517,6 +547,9 @@ def main():
     cpu = host_cpu()
     if args.workload == "bond":
         print(json.dumps(rrlu_bond_section(args, tci, ctx, cpu)), flush=True)
+        return
+    if args.workload == "served":
+        print(json.dumps(served_section(args, tci, ctx, cpu)), flush=True)
         return
     if args.workload == "rrlu":
         r = rrlu_section(args, tci, ctx, cpu)

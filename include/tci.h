@@ -378,6 +378,40 @@ tci_status tci_batch_cachestats(tci_batch *b, int32_t i, int64_t *entries, int64
 tci_status tci_batch_cachedata(tci_batch *b, int32_t i, int64_t cap, uint8_t *points, double *values,
                                int64_t *count);
 tci_status tci_batch_free(tci_batch *b);
+
+/* Slot-level serving (no counterpart in the reference, whose solves run one
+ * at a time).  A server keeps a persistent grid on the device (two
+ * workgroups per CU, a few slots left free for the runtime's copies) that
+ * takes published requests one at a time from a ring of `slots` entries, so
+ * a request starts as soon as a workgroup is free instead of waiting for a
+ * batch.  `shape` is the largest problem expected (it sizes the grid's LDS
+ * plan; a larger one still runs, with less on chip).  The grid leaves after
+ * `idle_ms` without a running or new request and is started again by the
+ * next submit or poll.  Its stream has a hardware queue of its own, so the
+ * library's own copies never wait behind it; work on the legacy default
+ * stream, which waits for every blocking stream, waits for the grid to leave
+ * (environment TCI_SERVE_QUEUE=shared: a plain non-blocking stream instead,
+ * for processes with spare hardware queues).
+ * tci_server_submit stages one problem in pinned host memory and publishes
+ * it, with no allocation or stream operation once the server's size classes
+ * are warm (the claiming workgroup copies it to the device); the returned
+ * tci_batch (one solve) is read with the tci_batch_* accessors
+ * (tci_batch_query polls its done mark; tci_batch_elapsed_ms is refused)
+ * and freed with tci_batch_free, which returns its memory to the server.  A
+ * full ring, or an HBM budget that cannot hold a new block, returns
+ * TCI_ERR_OOM.
+ * tci_server_poll writes up to cap finished requests not returned by an
+ * earlier poll to out (*count of them) and starts the grid again if requests
+ * wait and it has left.
+ * tci_server_stop runs every published request to completion, copies their
+ * reports to the host and releases the server; the batches stay valid.
+ * Calls on one server, and on its batches, are serialized by the caller. */
+typedef struct tci_server tci_server;
+tci_status tci_server_start(tci_ctx *ctx, const tci_problem *shape, int32_t slots, int32_t idle_ms,
+                            tci_server **out);
+tci_status tci_server_submit(tci_server *s, const tci_problem *p, tci_batch **out);
+tci_status tci_server_poll(tci_server *s, tci_batch **out, int32_t cap, int32_t *count);
+tci_status tci_server_stop(tci_server *s);
 /* Device time of the batch's solve kernel (HIP events), milliseconds. */
 tci_status tci_batch_elapsed_ms(tci_batch *b, float *ms);
 /* Diagnostic (no counterpart in the reference, whose only timing split is the

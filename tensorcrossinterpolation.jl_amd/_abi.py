@@ -226,6 +226,10 @@ def lib():
                                       C.c_double, C.c_int64, C.c_int32]
     L.tci_tt_sweep_f64.argtypes = [vp, C.c_int32, P(C.c_int64), P(C.c_int32), P(C.c_double), C.c_int32, C.c_int32,
                                    C.c_int64, C.c_double, C.c_double, P(C.c_int32), P(C.c_double)]
+    L.tci_server_start.argtypes = [vp, P(tci_problem), C.c_int32, C.c_int32, P(vp)]
+    L.tci_server_submit.argtypes = [vp, P(tci_problem), P(vp)]
+    L.tci_server_stop.argtypes = [vp]
+    L.tci_server_poll.argtypes = [vp, P(vp), C.c_int32, P(C.c_int32)]
     L.tci_tt_floatingzone_f64.argtypes = [vp, P(tci_functor), C.c_int32, P(C.c_int64), P(C.c_int32), P(C.c_double),
                                           C.c_int64, P(C.c_uint8), P(C.c_double), C.c_int64, C.c_double]
     _LIB = L
@@ -242,4 +246,5 @@ EXPORTED_SYMBOLS = [
     "tci_batch_profile", "tci_capacity_plan", "tci_tt_evaluate_f64", "tci_batch_evaluate",
     "tci_ctx_set_rrlu_algo", "tci_rrlu_batched_noalloc_f64", "tci_batch_cachestats", "tci_batch_cachedata",
     "tci_gemm_f64", "tci_tt_compress_f64", "tci_tt_sweep_f64", "tci_tt_floatingzone_f64",
+    "tci_server_start", "tci_server_submit", "tci_server_poll", "tci_server_stop",
 ]

@@ -11,6 +11,9 @@
 
 #include <string>
 #include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <thread>
 #include <mutex>
 #include <vector>
 
@@ -60,6 +63,7 @@ struct tci_ctx {
     hipStream_t stream;                 // synchronous calls and the first solve stream
     std::vector<int32_t> lu_order;      // host copy of the last batched rrLU job order
     std::vector<hipStream_t> extra;     // more solve streams (tci_ctx_set_streams)
+    hipStream_t dl = nullptr;           // downloads of results (never behind a solve)
     size_t next = 0;
     uint64_t budget;
     uint64_t used;
@@ -69,6 +73,14 @@ struct tci_ctx {
     int pi_lds_kb;      // LDS budget for Pi matrices (KiB)
     int rrlu_algo = TCI_RRLU_AUTO;   // factorization kernel of the standalone rrLU
 };
+
+// a device-to-host copy of a finished result on the context's download
+// stream: not behind later solves, and not on the legacy default stream
+static hipError_t ctx_download(tci_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->dl);
+    return e == hipSuccess ? hipStreamSynchronize(ctx->dl) : e;
+}
 
 // Launch shape of the solve kernel: a team of `team` threads per solve; LDS
 // holds the rrLU staging vectors, the index tables and as much of a Pi
@@ -131,6 +143,21 @@ static void pool_put(tci_ctx *c, void *p, size_t bytes)
     }
 }
 
+// a block of a slot-level server's memory: device memory, or pinned host
+// memory with its device alias
+struct ServeChunk {
+    void *base = nullptr;
+    size_t bytes = 0;
+    bool device = true;
+    int inuse = 0;           // its blocks held by requests
+    bool orphan = false;     // the server stopped: freed with its last block
+};
+struct ServeBlock {
+    uint8_t *ptr = nullptr;  // host view (pinned) or device pointer
+    uint8_t *dev = nullptr;  // the device's view
+    ServeChunk *chunk = nullptr;
+};
+
 struct tci_batch {
     tci_ctx *ctx;
     int count;
@@ -144,6 +171,37 @@ struct tci_batch {
     bool fetched = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipStream_t stream = nullptr;
+    // a request of a slot-level server (no events: its done mark is polled)
+    bool served = false;
+    bool polled = false;               // returned by tci_server_poll
+    tci_server *server = nullptr;      // null once the server stopped (the ticket ran)
+    uint64_t ticket = 0;
+    ServeBlock dblk{}, hblk{};         // its device block and pinned staging block
+    int dcls = 0, hcls = 0;            // their size classes
+    const uint8_t *h_rep_pin = nullptr;   // the report's pinned host copy
+};
+
+// A slot-level server: the persistent serving grid and its queue.
+struct tci_server {
+    tci_ctx *ctx = nullptr;
+    hipStream_t kst = nullptr;                  // the grid's stream (a hardware queue of its own)
+    ServeCtl *ctl = nullptr, *d_ctl = nullptr;  // coherent host memory and its device alias
+    uint64_t *done = nullptr, *d_done = nullptr;
+    ServeSlot *ring = nullptr, *d_ring = nullptr;
+    ServeDev *d_g = nullptr;                    // the grid's words in device memory
+    int cap = 0, team = 256, blocks = 0;
+    uint64_t next = 0, idle_ticks = 0;
+    uint32_t epoch = 0;                         // launches so far
+    LdsPlan plan{};
+    hipEvent_t kev = nullptr;
+    bool running = false;
+    std::vector<tci_batch *> slot;              // the request of each ring slot in flight
+    std::vector<tci_batch *> pending;           // submitted and not yet returned by a poll
+    // cached blocks per size class (device blocks, pinned staging blocks):
+    // carved from chunks that live until the server stops and their last
+    // block is released
+    std::vector<std::vector<ServeBlock>> dfree, hfree;
+    std::vector<ServeChunk *> chunks;
 };
 
 // ---- scratch of tci_rrlu_batched_noalloc_f64, the entry point that
@@ -206,7 +264,9 @@ tci_status tci_ctx_create(tci_ctx **out, int32_t device, uint64_t hbm_budget_byt
     c->device = device;
     c->used = 0;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->dl, hipStreamNonBlocking);
     if (e != hipSuccess) {
+        if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return set_err(TCI_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
     }
@@ -248,6 +308,7 @@ tci_status tci_ctx_destroy(tci_ctx *ctx)
         (void)hipStreamDestroy(s);
     }
     for (PoolBuf &b : ctx->pool) (void)hipFree(b.ptr);
+    (void)hipStreamDestroy(ctx->dl);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return TCI_OK;
@@ -1071,10 +1132,15 @@ tci_status tci_capacity_plan(const tci_problem *p, int64_t *chi_used, int64_t *c
     return TCI_OK;
 }
 
+static void serve_block_put(tci_server *s, tci_ctx *ctx, const ServeBlock &k, int cls, bool device);
+
 static void batch_release(tci_batch *b)
 {
     if (!b) return;
-    if (b->ctx) {
+    if (b->served) {
+        if (b->dblk.ptr) serve_block_put(b->server, b->ctx, b->dblk, b->dcls, true);
+        if (b->hblk.ptr) serve_block_put(b->server, b->ctx, b->hblk, b->hcls, false);
+    } else if (b->ctx) {
         pool_put(b->ctx, b->d_desc, b->desc_bytes);
         pool_put(b->ctx, b->d_in, b->in_bytes);
         pool_put(b->ctx, b->d_state, b->state_bytes);
@@ -1091,7 +1157,16 @@ static void batch_release(tci_batch *b)
     delete b;
 }
 
-tci_status tci_solve_submit(tci_ctx *ctx, int32_t count, const tci_problem *problems, tci_batch **out)
+// the launch-independent part of a submit: the problems checked and laid
+// out, the regions taken from the pool and the inputs and descriptors
+// uploaded on st; *sh gets what the LDS plan and the kernel choice need
+struct BatchShape {
+    int max_mr = 0, max_tab = 0, fkind = 0;
+    int64_t max_sets = 0, max_lut = 0;
+};
+
+static tci_status batch_prepare(tci_ctx *ctx, int32_t count, const tci_problem *problems, hipStream_t st,
+                                tci_batch **out, BatchShape *sh)
 {
     if (!ctx || !out || count < 1 || !problems) return set_err(TCI_ERR_ARG, "bad arguments");
     *out = nullptr;
@@ -1146,10 +1221,11 @@ tci_status tci_solve_submit(tci_ctx *ctx, int32_t count, const tci_problem *prob
     b->in_bytes = tin;
     b->state_bytes = tstate;
     b->desc_bytes = sizeof(SolveDesc) * (size_t)count;
-    if ((e = pool_get(ctx, tin, (void **)&b->d_in)) != hipSuccess) return fail(e, "hipMalloc(inputs)");
-    if ((e = pool_get(ctx, tstate, (void **)&b->d_state)) != hipSuccess) return fail(e, "hipMalloc(state)");
-    if ((e = pool_get(ctx, trep, (void **)&b->d_rep)) != hipSuccess) return fail(e, "hipMalloc(report)");
-    if ((e = pool_get(ctx, b->desc_bytes, (void **)&b->d_desc)) != hipSuccess) return fail(e, "hipMalloc(desc)");
+    auto get = [&](size_t bytes, void **p) { return pool_get(ctx, bytes, p); };
+    if ((e = get(tin, (void **)&b->d_in)) != hipSuccess) return fail(e, "hipMalloc(inputs)");
+    if ((e = get(tstate, (void **)&b->d_state)) != hipSuccess) return fail(e, "hipMalloc(state)");
+    if ((e = get(trep, (void **)&b->d_rep)) != hipSuccess) return fail(e, "hipMalloc(report)");
+    if ((e = get(b->desc_bytes, (void **)&b->d_desc)) != hipSuccess) return fail(e, "hipMalloc(desc)");
     b->rep_bytes = trep;
     b->desc.resize((size_t)count);
     b->rep_off.resize((size_t)count);
@@ -1168,30 +1244,70 @@ tci_status tci_solve_submit(tci_ctx *ctx, int32_t count, const tci_problem *prob
         ost += sb[(size_t)i];
         orp += rb[(size_t)i];
     }
-    hipStream_t st = pick_stream(ctx);
     b->stream = st;
     if ((e = hipMemcpyAsync(b->d_in, himg, tin, hipMemcpyHostToDevice, st)) != hipSuccess) return fail(e, "upload inputs");
     if ((e = hipMemcpyAsync(b->d_desc, b->desc.data(), sizeof(SolveDesc) * (size_t)count, hipMemcpyHostToDevice, st)) != hipSuccess)
         return fail(e, "upload descriptors");
+    sh->max_mr = max_mr;
+    sh->max_tab = max_tab;
+    sh->max_sets = max_sets;
+    sh->max_lut = max_lut;
+    sh->fkind = fkind;
+    *out = b;
+    return TCI_OK;
+}
+
+tci_status tci_solve_submit(tci_ctx *ctx, int32_t count, const tci_problem *problems, tci_batch **out)
+{
+    if (!ctx || !out || count < 1 || !problems) return set_err(TCI_ERR_ARG, "bad arguments");
+    *out = nullptr;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = pick_stream(ctx);
+    BatchShape sh;
+    tci_batch *b = nullptr;
+    tci_status s = batch_prepare(ctx, count, problems, st, &b, &sh);
+    if (s != TCI_OK) return s;
+    auto fail = [&](hipError_t e, const char *what) {
+        batch_release(b);
+        return set_err(e == hipErrorOutOfMemory ? TCI_ERR_OOM : TCI_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+    };
+    hipError_t e;
     if ((e = hipEventCreate(&b->ev0)) != hipSuccess) return fail(e, "hipEventCreate");
     if ((e = hipEventCreate(&b->ev1)) != hipSuccess) return fail(e, "hipEventCreate");
     if ((e = hipEventRecord(b->ev0, st)) != hipSuccess) return fail(e, "hipEventRecord");
-    const int stat = solve_static_lds(ctx->team, fkind);
+    const int stat = solve_static_lds(ctx->team, sh.fkind);
     if (stat < 0) return fail(hipErrorInvalidValue, "solve kernel attributes");
-    const LdsPlan plan = make_plan(ctx, (int64_t)ctx->lds_wg - stat, max_mr, max_tab, max_sets, max_lut);
-    if ((e = launch_solve(b->d_desc, count, ctx->team, fkind, plan, st)) != hipSuccess) return fail(e, "solve launch");
+    const LdsPlan plan = make_plan(ctx, (int64_t)ctx->lds_wg - stat, sh.max_mr, sh.max_tab, sh.max_sets, sh.max_lut);
+    if ((e = launch_solve(b->d_desc, count, ctx->team, sh.fkind, plan, st)) != hipSuccess) return fail(e, "solve launch");
     if ((e = hipEventRecord(b->ev1, st)) != hipSuccess) return fail(e, "hipEventRecord");
     *out = b;
     return TCI_OK;
 }
 
+static tci_status serve_ensure(tci_server *s);
+static tci_status served_query(tci_batch *b);
+
 static tci_status fetch_reports(tci_batch *b)
 {
     if (b->fetched) return TCI_OK;
     HIPCHK(hipSetDevice(b->ctx->device));
-    HIPCHK(hipEventSynchronize(b->ev1));
     b->h_rep.resize(b->rep_bytes);
-    HIPCHK(hipMemcpy(b->h_rep.data(), b->d_rep, b->rep_bytes, hipMemcpyDeviceToHost));
+    if (b->served) {
+        // poll the done mark (relaunching the grid if it went idle); the
+        // workgroup copied the report to pinned host memory before it
+        for (;;) {
+            const tci_status q = served_query(b);
+            if (q == TCI_OK) break;
+            if (q != TCI_ERR_PENDING) return q;
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+        }
+        std::atomic_thread_fence(std::memory_order_acquire);
+        memcpy(b->h_rep.data(), b->h_rep_pin, b->rep_bytes);
+        b->fetched = true;
+        return TCI_OK;
+    }
+    HIPCHK(hipEventSynchronize(b->ev1));
+    HIPCHK(ctx_download(b->ctx, b->h_rep.data(), b->d_rep, b->rep_bytes));
     b->fetched = true;
     return TCI_OK;
 }
@@ -1199,6 +1315,7 @@ static tci_status fetch_reports(tci_batch *b)
 tci_status tci_batch_query(tci_batch *b)
 {
     if (!b) return set_err(TCI_ERR_ARG, "batch is NULL");
+    if (b->served) return served_query(b);
     hipError_t e = hipEventQuery(b->ev1);
     if (e == hipErrorNotReady) {
         (void)hipGetLastError();   // not finished is not an error: clear it
@@ -1286,7 +1403,7 @@ tci_status tci_batch_indexset(tci_batch *b, int32_t i, int32_t which, int32_t si
     if (n == 0 || len == 0) return TCI_OK;
     std::vector<uint8_t> raw((size_t)n * L);
     const uint8_t *src = R.sets + ((int64_t)(kind * L + b0) * R.setcap) * L;
-    HIPCHK(hipMemcpy(raw.data(), src, raw.size(), hipMemcpyDeviceToHost));
+    HIPCHK(ctx_download(b->ctx, raw.data(), src, raw.size()));
     for (int e = 0; e < n; ++e) memcpy(out + (size_t)e * len, raw.data() + (size_t)e * L, (size_t)len);
     return TCI_OK;
 }
@@ -1295,10 +1412,10 @@ tci_status tci_batch_sitetensor(tci_batch *b, int32_t i, int32_t site, int64_t *
 {
     SolveDesc R;
     if (!dims3 || !report_view(b, i, R) || site < 1 || site > R.L) return set_err(TCI_ERR_ARG, "bad arguments");
-    HIPCHK(hipMemcpy(dims3, R.Tdims + 3 * (site - 1), sizeof(int64_t) * 3, hipMemcpyDeviceToHost));
+    HIPCHK(ctx_download(b->ctx, dims3, R.Tdims + 3 * (site - 1), sizeof(int64_t) * 3));
     const int64_t n = dims3[0] * dims3[1] * dims3[2];
     if (n > cap || (!out && n > 0)) return set_err(TCI_ERR_ARG, "output buffer too small");
-    if (n > 0) HIPCHK(hipMemcpy(out, R.T + R.tstride * (site - 1), sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (n > 0) HIPCHK(ctx_download(b->ctx, out, R.T + R.tstride * (site - 1), sizeof(double) * n));
     return TCI_OK;
 }
 
@@ -1323,6 +1440,7 @@ tci_status tci_batch_bonderrors(tci_batch *b, int32_t i, double *out)
 tci_status tci_batch_elapsed_ms(tci_batch *b, float *ms)
 {
     if (!b || !ms) return set_err(TCI_ERR_ARG, "bad arguments");
+    if (b->served) return set_err(TCI_ERR_ARG, "a served request has no launch of its own to time");
     HIPCHK(hipEventSynchronize(b->ev1));
     HIPCHK(hipEventElapsedTime(ms, b->ev0, b->ev1));
     return TCI_OK;
@@ -1363,10 +1481,10 @@ tci_status tci_batch_cachedata(tci_batch *b, int32_t i, int64_t cap, uint8_t *po
     std::vector<double> vals((size_t)ne);
     std::vector<int32_t> ld((size_t)L);
     HIPCHK(hipSetDevice(b->ctx->device));
-    HIPCHK(hipMemcpy(st.data(), R.cache_state, sizeof(uint32_t) * st.size(), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(keys.data(), R.cache_keys, sizeof(uint64_t) * keys.size(), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(vals.data(), R.cache_vals, sizeof(double) * vals.size(), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(ld.data(), R.localdims, sizeof(int32_t) * ld.size(), hipMemcpyDeviceToHost));
+    HIPCHK(ctx_download(b->ctx, st.data(), R.cache_state, sizeof(uint32_t) * st.size()));
+    HIPCHK(ctx_download(b->ctx, keys.data(), R.cache_keys, sizeof(uint64_t) * keys.size()));
+    HIPCHK(ctx_download(b->ctx, vals.data(), R.cache_vals, sizeof(double) * vals.size()));
+    HIPCHK(ctx_download(b->ctx, ld.data(), R.localdims, sizeof(int32_t) * ld.size()));
     int64_t n = 0;
     for (int64_t e = 0; e < ne; ++e) {
         if (st[(size_t)e] != CACHE_READY) continue;
@@ -1589,8 +1707,8 @@ tci_status tci_batch_evaluate(tci_batch *b, int32_t i, int64_t npts, const uint8
     const int L = R.L;
     std::vector<int32_t> ld((size_t)L);
     std::vector<int64_t> dims3((size_t)(3 * L)), coff((size_t)L);
-    HIPCHK(hipMemcpy(ld.data(), R.localdims, sizeof(int32_t) * (size_t)L, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(dims3.data(), R.Tdims, sizeof(int64_t) * dims3.size(), hipMemcpyDeviceToHost));
+    HIPCHK(ctx_download(ctx, ld.data(), R.localdims, sizeof(int32_t) * (size_t)L));
+    HIPCHK(ctx_download(ctx, dims3.data(), R.Tdims, sizeof(int64_t) * dims3.size()));
     if (!points_in_range(L, ld.data(), npts, pts)) return set_err(TCI_ERR_ARG, "point index out of range");
     int64_t chimax = 1;
     for (int s = 0; s < L; ++s) {
@@ -1618,8 +1736,367 @@ tci_status tci_batch_free(tci_batch *b)
         (void)hipSetDevice(b->ctx->device);
         if (b->ev1) (void)hipEventSynchronize(b->ev1);
     }
+    if (b->served) {
+        // the grid may still be running it: its blocks are released once done
+        const tci_status w = fetch_reports(b);
+        if (w != TCI_OK) return w;
+        if (tci_server *s = b->server) {
+            s->slot[(size_t)(b->ticket % (uint64_t)s->cap)] = nullptr;
+            if (!b->polled) {
+                auto it = std::find(s->pending.begin(), s->pending.end(), b);
+                if (it != s->pending.end()) {
+                    *it = s->pending.back();
+                    s->pending.pop_back();
+                }
+            }
+        }
+    }
     batch_release(b);
     return TCI_OK;
+}
+
+// ---------------------------------------------------------------- serving
+// Memory of served requests.  A request takes one device block (its
+// descriptor, inputs, state and report) and one pinned staging block (the
+// image the claiming workgroup copies in, and the report it copies out), each
+// rounded up to a power-of-two size class and carved from chunks that stay
+// allocated while the server runs, so a submit calls no allocator and no
+// stream operation once the classes it needs are warm (a device-wide
+// synchronizing free would wait for the resident grid to leave).  A block
+// returns to its class when its request is freed; the grid finished with it
+// before the done mark the host saw.
+constexpr int kServeMinClass = 12;                    // 4 KiB
+constexpr size_t kServeChunkBytes = (size_t)32 << 20; // blocks are carved from 32 MiB chunks
+
+static int serve_class(size_t bytes)
+{
+    int c = kServeMinClass;
+    while (((size_t)1 << c) < bytes) ++c;
+    return c;
+}
+
+static void serve_chunk_free(tci_ctx *ctx, ServeChunk *k)
+{
+    if (k->device) {
+        (void)hipFree(k->base);
+        if (ctx) ctx->used -= k->bytes;
+    } else {
+        (void)hipHostFree(k->base);
+    }
+    delete k;
+}
+
+static void serve_block_put(tci_server *s, tci_ctx *ctx, const ServeBlock &k, int cls, bool device)
+{
+    ServeChunk *c = k.chunk;
+    c->inuse -= 1;
+    if (s) {
+        (device ? s->dfree : s->hfree)[(size_t)cls].push_back(k);
+    } else if (c->orphan && c->inuse == 0) {
+        serve_chunk_free(ctx, c);
+    }
+}
+
+// a block of class cls; TCI_ERR_OOM when a new device chunk would exceed the
+// context's HBM budget
+static tci_status serve_block_get(tci_server *s, int cls, bool device, ServeBlock *out)
+{
+    auto &fl = (device ? s->dfree : s->hfree)[(size_t)cls];
+    if (fl.empty()) {
+        const size_t bsz = (size_t)1 << cls;
+        const size_t csz = bsz > kServeChunkBytes ? bsz : kServeChunkBytes;
+        tci_ctx *ctx = s->ctx;
+        if (device && ctx->used + csz > ctx->budget)
+            return set_err(TCI_ERR_OOM, "served request needs a %zu-byte block, HBM budget has %llu left", bsz,
+                           (unsigned long long)(ctx->budget - ctx->used));
+        ServeChunk *c = new ServeChunk();
+        c->bytes = csz;
+        c->device = device;
+        hipError_t e;
+        uint8_t *dev = nullptr;
+        if (device) {
+            e = hipMalloc(&c->base, csz);
+            dev = (uint8_t *)c->base;
+        } else {
+            e = hipHostMalloc(&c->base, csz, hipHostMallocCoherent | hipHostMallocMapped);
+            if (e == hipSuccess) e = hipHostGetDevicePointer((void **)&dev, c->base, 0);
+        }
+        if (e != hipSuccess) {
+            if (c->base) (void)(device ? hipFree(c->base) : hipHostFree(c->base));
+            delete c;
+            return set_err(e == hipErrorOutOfMemory ? TCI_ERR_OOM : TCI_ERR_DEVICE, "served memory: %s",
+                           hipGetErrorString(e));
+        }
+        if (device) ctx->used += csz;
+        s->chunks.push_back(c);
+        for (size_t o = 0; o + bsz <= csz; o += bsz)
+            fl.push_back(ServeBlock{(uint8_t *)c->base + o, dev + o, c});
+    }
+    *out = fl.back();
+    fl.pop_back();
+    out->chunk->inuse += 1;
+    return TCI_OK;
+}
+
+static tci_status serve_ensure(tci_server *s)
+{
+    if (s->running) {
+        const hipError_t q = hipEventQuery(s->kev);
+        if (q == hipErrorNotReady) {
+            (void)hipGetLastError();
+            return TCI_OK;
+        }
+        if (q != hipSuccess) return set_err(TCI_ERR_DEVICE, "serving grid: %s", hipGetErrorString(q));
+        s->running = false;
+    }
+    // nothing published beyond what ran: stay idle
+    if (__atomic_load_n(&s->ctl->tail, __ATOMIC_ACQUIRE) == 0) return TCI_OK;
+    s->epoch += 1;
+    hipError_t e = launch_serve(s->d_ring, s->d_g, s->d_ctl, s->d_done, s->cap, s->plan, s->idle_ticks, s->epoch,
+                                s->team, s->blocks, s->kst);
+    if (e == hipSuccess) e = hipEventRecord(s->kev, s->kst);
+    if (e != hipSuccess) return set_err(TCI_ERR_DEVICE, "serving grid launch: %s", hipGetErrorString(e));
+    s->running = true;
+    return TCI_OK;
+}
+
+static bool served_done(const tci_server *s, const tci_batch *b)
+{
+    return __atomic_load_n(s->done + b->ticket % (uint64_t)s->cap, __ATOMIC_ACQUIRE) == b->ticket + 1;
+}
+
+static tci_status served_query(tci_batch *b)
+{
+    tci_server *s = b->server;
+    if (!s) return TCI_OK;   // the server stopped after draining every ticket
+    if (served_done(s, b)) return TCI_OK;
+    const tci_status e = serve_ensure(s);
+    return e != TCI_OK ? e : TCI_ERR_PENDING;
+}
+
+static void server_release(tci_server *s)
+{
+    if (s->kev) (void)hipEventDestroy(s->kev);
+    if (s->d_g) (void)hipFree(s->d_g);
+    if (s->ring) (void)hipHostFree(s->ring);
+    if (s->done) (void)hipHostFree(s->done);
+    if (s->ctl) (void)hipHostFree(s->ctl);
+    if (s->kst) (void)hipStreamDestroy(s->kst);
+    for (ServeChunk *c : s->chunks) {
+        if (c->inuse == 0) serve_chunk_free(s->ctx, c);
+        else c->orphan = true;   // freed with the last request that holds a block of it
+    }
+    delete s;
+}
+
+tci_status tci_server_start(tci_ctx *ctx, const tci_problem *shape, int32_t slots, int32_t idle_ms, tci_server **out)
+{
+    if (!ctx || !shape || !out || slots < 1 || slots > (1 << 20) || idle_ms < 1 || idle_ms > 60000)
+        return set_err(TCI_ERR_ARG, "bad arguments");
+    *out = nullptr;
+    SolveCaps c;
+    std::string msg;
+    int st = check_problem(*shape, c, &msg);
+    if (st != TCI_OK) return set_err((tci_status)st, "shape: %s", msg.c_str());
+    HIPCHK(hipSetDevice(ctx->device));
+    tci_server *s = new tci_server();
+    s->ctx = ctx;
+    s->cap = slots;
+    s->team = ctx->team == 512 ? 512 : 256;
+    s->dfree.resize(64);
+    s->hfree.resize(64);
+    auto fail = [&](hipError_t e, const char *what) {
+        server_release(s);
+        return set_err(TCI_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+    };
+    hipError_t e;
+    int ncu = 0, khz = 0;
+    if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device)) != hipSuccess)
+        return fail(e, "device attributes");
+    if ((e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device)) != hipSuccess)
+        return fail(e, "device attributes");
+    // a full grid holds every CU's register file, and the runtime's own copy
+    // kernels (downloads of results) would wait for it to leave: keep a few
+    // workgroup slots free for them
+    const int reserve = ncu / 32 > 4 ? ncu / 32 : 4;
+    s->blocks = (s->team == 512 ? 1 : 2) * ncu - reserve;
+    if (s->blocks < 1) s->blocks = 1;
+    s->idle_ticks = (uint64_t)idle_ms * (uint64_t)(khz > 0 ? khz : 100000);
+    {
+        // the grid's stream on a hardware queue of its own (a stream with a
+        // CU mask is never given a shared one): a queue it shared would hold
+        // every later command of the other stream behind the resident grid
+        const char *qv = getenv("TCI_SERVE_QUEUE");
+        if (qv && !strcmp(qv, "shared")) {
+            e = hipStreamCreateWithFlags(&s->kst, hipStreamNonBlocking);
+        } else {
+            std::vector<uint32_t> mask((size_t)(ncu + 31) / 32, 0u);
+            for (int q = 0; q < ncu; ++q) mask[(size_t)q / 32] |= 1u << (q % 32);
+            e = hipExtStreamCreateWithCUMask(&s->kst, (uint32_t)mask.size(), mask.data());
+        }
+        if (e != hipSuccess) return fail(e, "stream");
+    }
+    if ((e = hipEventCreateWithFlags(&s->kev, hipEventDisableTiming)) != hipSuccess) return fail(e, "event");
+    const unsigned hf = hipHostMallocCoherent | hipHostMallocMapped;
+    if ((e = hipHostMalloc((void **)&s->ctl, sizeof(ServeCtl), hf)) != hipSuccess) return fail(e, "hipHostMalloc");
+    if ((e = hipHostMalloc((void **)&s->done, sizeof(uint64_t) * (size_t)slots, hf)) != hipSuccess)
+        return fail(e, "hipHostMalloc");
+    if ((e = hipHostMalloc((void **)&s->ring, sizeof(ServeSlot) * (size_t)slots, hf)) != hipSuccess)
+        return fail(e, "hipHostMalloc");
+    memset(s->ctl, 0, sizeof(ServeCtl));
+    memset(s->done, 0, sizeof(uint64_t) * (size_t)slots);
+    memset(s->ring, 0, sizeof(ServeSlot) * (size_t)slots);
+    if ((e = hipHostGetDevicePointer((void **)&s->d_ctl, s->ctl, 0)) != hipSuccess) return fail(e, "device alias");
+    if ((e = hipHostGetDevicePointer((void **)&s->d_done, s->done, 0)) != hipSuccess) return fail(e, "device alias");
+    if ((e = hipHostGetDevicePointer((void **)&s->d_ring, s->ring, 0)) != hipSuccess) return fail(e, "device alias");
+    if ((e = hipMalloc((void **)&s->d_g, sizeof(ServeDev))) != hipSuccess) return fail(e, "hipMalloc");
+    if ((e = hipMemsetAsync(s->d_g, 0, sizeof(ServeDev), s->kst)) != hipSuccess) return fail(e, "hipMemset");
+    if ((e = hipStreamSynchronize(s->kst)) != hipSuccess) return fail(e, "hipStreamSynchronize");
+    // one LDS plan for the grid, from the largest problem the caller expects
+    // (a larger one still runs: what does not fit stays in its arena)
+    int64_t lut = 0;
+    if (shape->f.kind == TCI_F_LORENTZ_INT) {
+        lut = 2;
+        for (int t = 0; t < shape->L; ++t) lut += (int64_t)shape->localdims[t] * shape->localdims[t];
+    }
+    const int64_t sbytes = ((((int64_t)NSETKIND * c.L * c.setcap * c.L) + 15) & ~(int64_t)15) + 4 * (int64_t)NSETKIND * c.L;
+    const int stat = serve_static_lds(s->team);
+    if (stat < 0) return fail(hipErrorInvalidValue, "serving kernel attributes");
+    s->plan = make_plan(ctx, (int64_t)ctx->lds_wg - stat, c.mrcap, c.mrcap * c.L, sbytes, lut);
+    s->slot.assign((size_t)slots, nullptr);
+    *out = s;
+    return TCI_OK;
+}
+
+static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+tci_status tci_server_submit(tci_server *s, const tci_problem *p, tci_batch **out)
+{
+    if (!s || !p || !out) return set_err(TCI_ERR_ARG, "bad arguments");
+    *out = nullptr;
+    const size_t k = (size_t)(s->next % (uint64_t)s->cap);
+    if (s->slot[k]) return set_err(TCI_ERR_OOM, "server queue full: free finished requests first");
+    SolveCaps c;
+    std::string msg;
+    const int chk = check_problem(*p, c, &msg);
+    if (chk != TCI_OK) return set_err((tci_status)chk, "%s", msg.c_str());
+    HIPCHK(hipSetDevice(s->ctx->device));
+    size_t ib = 0, sb = 0, rb = 0;
+    layout_solve(c, nullptr, nullptr, nullptr, nullptr, &ib, &sb, &rb);
+    // device block: [descriptor][inputs][state][report], each 256-aligned;
+    // staging block: [descriptor][inputs] as in the device block, then the
+    // report's host copy
+    const size_t oin = align_up(sizeof(SolveDesc), 256);
+    const size_t ost = oin + align_up(ib, 256), orp = ost + align_up(sb, 256);
+    const size_t dbytes = orp + align_up(rb, 256);
+    const size_t staged = align_up(oin + ib, 16);
+    const size_t hrep = align_up(staged, 256);
+    const size_t hbytes = hrep + align_up(rb, 256);
+    tci_batch *b = new tci_batch();
+    b->ctx = s->ctx;
+    b->count = 1;
+    b->served = true;
+    b->dcls = serve_class(dbytes);
+    b->hcls = serve_class(hbytes);
+    tci_status st = serve_block_get(s, b->dcls, true, &b->dblk);
+    if (st == TCI_OK) st = serve_block_get(s, b->hcls, false, &b->hblk);
+    if (st != TCI_OK) {
+        b->server = s;
+        batch_release(b);
+        return st;
+    }
+    uint8_t *dblk = b->dblk.dev, *hblk = b->hblk.ptr;
+    memset(hblk, 0, staged);
+    SolveDesc &D = *(SolveDesc *)hblk;
+    size_t x, y, z;
+    layout_solve(c, &D, dblk + oin, dblk + ost, dblk + orp, &x, &y, &z);
+    fill_desc_scalars(*p, D);
+    pack_inputs(*p, c, D, hblk + oin, dblk + oin);
+    b->desc.assign(1, D);
+    b->rep_off.assign(1, 0);
+    b->d_desc = (SolveDesc *)dblk;
+    b->d_in = dblk + oin;
+    b->d_state = dblk + ost;
+    b->d_rep = dblk + orp;
+    b->in_bytes = ib;
+    b->state_bytes = sb;
+    b->rep_bytes = rb;
+    b->desc_bytes = sizeof(SolveDesc);
+    b->h_rep_pin = hblk + hrep;
+    ServeSlot &E = s->ring[k];
+    E.src = b->hblk.dev;
+    E.dst = dblk;
+    E.rep = dblk + orp;
+    E.rep_out = b->hblk.dev + hrep;
+    E.bytes = staged;
+    E.rep_bytes = align_up(rb, 16);
+    b->server = s;
+    b->ticket = s->next;
+    s->slot[k] = b;
+    s->pending.push_back(b);
+    s->next += 1;
+    __atomic_store_n(&s->ctl->tail, s->next, __ATOMIC_RELEASE);   // publish
+    st = serve_ensure(s);
+    *out = b;
+    return st;
+}
+
+tci_status tci_server_poll(tci_server *s, tci_batch **out, int32_t cap, int32_t *count)
+{
+    if (!s || !count || cap < 0 || (cap > 0 && !out)) return set_err(TCI_ERR_ARG, "bad arguments");
+    *count = 0;
+    for (size_t i = 0; i < s->pending.size() && *count < cap;) {
+        tci_batch *b = s->pending[i];
+        if (served_done(s, b)) {
+            b->polled = true;
+            out[(*count)++] = b;
+            s->pending[i] = s->pending.back();
+            s->pending.pop_back();
+        } else {
+            ++i;
+        }
+    }
+    // requests still waiting and the grid gone idle: launch it again
+    if (!s->pending.empty()) return serve_ensure(s);
+    return TCI_OK;
+}
+
+tci_status tci_server_stop(tci_server *s)
+{
+    if (!s) return TCI_OK;
+    tci_status r = TCI_OK;
+    if (s->ctx) (void)hipSetDevice(s->ctx->device);
+    if (s->ctl) __atomic_store_n(&s->ctl->stop, 1u, __ATOMIC_RELEASE);
+    // a grid that was leaving idle when stop came may leave tickets behind:
+    // the next launch sees stop and drains them all, so two rounds suffice
+    for (int round = 0; round < 3 && r == TCI_OK; ++round) {
+        if (s->running) {
+            const hipError_t e = hipEventSynchronize(s->kev);
+            if (e != hipSuccess) {
+                r = set_err(TCI_ERR_DEVICE, "serving grid: %s", hipGetErrorString(e));
+                break;
+            }
+            s->running = false;
+        }
+        bool pending = false;
+        if (s->ctl)
+            for (tci_batch *b : s->slot)
+                if (b && !served_done(s, b)) pending = true;
+        if (!pending) break;
+        r = serve_ensure(s);
+    }
+    for (tci_batch *b : s->slot)
+        if (b) {
+            // every ticket ran: its report comes to the host now, and its
+            // blocks stay with it (their chunks outlive the server)
+            if (r == TCI_OK) {
+                const tci_status w = fetch_reports(b);
+                if (w != TCI_OK) r = w;
+            }
+            b->server = nullptr;
+        }
+    server_release(s);
+    return r;
 }
 
 }  // extern "C"

@@ -21,6 +21,44 @@ size_t lds_plan_bytes(const LdsPlan &p);
 int solve_static_lds(int team, int fkind);
 hipError_t launch_solve(const SolveDesc *descs_device, int count, int team, int fkind,
                         const LdsPlan &plan, hipStream_t st);
+// slot-level serving: the queue's control words in coherent host memory
+struct ServeCtl {
+    uint64_t tail;   // host: tickets published (ring entries valid below it)
+    uint32_t stop;   // host: 1 = leave once idle
+    uint32_t pad;
+};
+// the serving grid's own words in device memory (zero when the server starts)
+struct ServeDev {
+    uint64_t head;       // next ticket to claim
+    uint64_t dtail;      // mirror of ServeCtl::tail (only ever raised)
+    uint64_t lastpoll;   // wall clock of the last read of the host's words
+    uint64_t last;       // wall clock of the last claim or finished solve
+    uint32_t busy;       // workgroups running a solve
+    uint32_t closing;    // the launch epoch that is leaving for lack of work
+    uint32_t dstop;      // the launch epoch that has seen ServeCtl::stop
+    uint32_t pad;
+};
+// one published request (a ring entry in coherent host memory).  The
+// claiming workgroup copies the staged image (the descriptor, then the
+// inputs) into the device block, runs the solve, and copies the report out
+// before it sets the done mark
+struct ServeSlot {
+    const uint8_t *src;    // the staged image (device alias of pinned host memory)
+    uint8_t *dst;          // the request's device block: the descriptor at dst
+    const uint8_t *rep;    // the report in the device block
+    uint8_t *rep_out;      // its host copy (device alias of pinned host memory)
+    uint64_t bytes;        // staged bytes, a multiple of 16
+    uint64_t rep_bytes;    // report bytes, a multiple of 16
+};
+int serve_static_lds(int team);
+// a persistent grid of `blocks` workgroups (generic functor kernel) that runs
+// published tickets; it leaves, all of it, once stop is set and every
+// published ticket is claimed, or once no solve has run, been claimed or
+// finished for idle_ticks of the wall clock.  epoch: this launch's number
+// (each launch a new one, from 1)
+hipError_t launch_serve(const ServeSlot *ring, ServeDev *g, const ServeCtl *ctl, uint64_t *done, int cap,
+                        const LdsPlan &plan, uint64_t idle_ticks, uint32_t epoch, int team, int blocks,
+                        hipStream_t st);
 // order (device, count entries, or null): the job index each workgroup takes
 hipError_t launch_rrlu(tci_lu_job *jobs_device, int64_t count, int64_t max_side, double *gstage,
                        int64_t gstage_stride, const int32_t *order, hipStream_t st);

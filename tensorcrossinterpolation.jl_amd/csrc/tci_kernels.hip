@@ -67,6 +67,191 @@ __global__ __launch_bounds__(NT, WPE) void tci_solve_kernel(const SolveDesc *__r
     eng.run();
 }
 
+// ------------------------------------------------------------------ serving
+// Slot-level serving: a persistent grid whose workgroups each take one
+// published solve at a time from a ring of requests.  The host publishes
+// ticket t by writing ring[t % cap] and the staged image it names (coherent
+// host memory, before) and then ctl->tail = t + 1.  One workgroup at a time, at most
+// every kServePollTicks, reads the host's words and raises the device mirror
+// g->dtail, so the idle workgroups poll device memory, not the bus.  A
+// workgroup claims the next ticket below the mirror by a compare-and-swap on
+// g->head, so a ticket is claimed only once it is published and every claimed
+// ticket is run; after the solve its done slot (coherent host memory) becomes
+// t + 1.  The grid leaves as a whole: with stop seen, once every published
+// ticket is claimed; otherwise once no workgroup has run, claimed or finished
+// a solve for idle_ticks (the first to see that marks the epoch closing and
+// the others follow; one that claimed meanwhile still runs its ticket).  A
+// ticket published while the grid leaves is run by the next launch.
+constexpr uint64_t kServePollTicks = 50;   // 0.5 us of the 100 MHz wall clock
+
+// n bytes (a multiple of 16, both ends 16-byte aligned) by the whole team
+template <int NT>
+__device__ __forceinline__ void team_copy16(uint8_t *dst, const uint8_t *src, uint64_t n)
+{
+    for (uint64_t o = (uint64_t)threadIdx.x * 16; o < n; o += (uint64_t)NT * 16)
+        *(uint4 *)(dst + o) = *(const uint4 *)(src + o);
+}
+
+template <int NT, int WPE>
+__global__ __launch_bounds__(NT, WPE) void tci_serve_kernel(const ServeSlot *__restrict__ ring, ServeDev *g,
+                                                            const ServeCtl *ctl, uint64_t *done, int cap,
+                                                            LdsPlan plan, uint64_t idle_ticks, uint32_t epoch)
+{
+    __shared__ EngShared S;
+    __shared__ uint64_t s_ticket;
+    __shared__ int s_go;
+    __shared__ ServeSlot s_slot;
+    extern __shared__ double dyn[];
+    const uint64_t born = (uint64_t)wall_clock64();
+    for (;;) {
+        if (threadIdx.x == 0) {
+            int go = 0, nap = 1;
+            uint64_t t = 0;
+            for (;;) {
+                uint64_t tail = __hip_atomic_load(&g->dtail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                uint64_t h = __hip_atomic_load(&g->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint64_t now = (uint64_t)wall_clock64();
+                if (h >= tail) {
+                    uint64_t lp = __hip_atomic_load(&g->lastpoll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (now - lp >= kServePollTicks &&
+                        __hip_atomic_compare_exchange_strong(&g->lastpoll, &lp, now, __ATOMIC_RELAXED,
+                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        // stop first: a tail read after it holds every ticket
+                        // published before the stop
+                        const bool st =
+                            __hip_atomic_load(&ctl->stop, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) != 0;
+                        const uint64_t ht = __hip_atomic_load(&ctl->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+                        if (st) __hip_atomic_store(&g->dstop, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_fetch_max(&g->dtail, ht, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                        if (ht > tail) tail = ht;
+                    }
+                }
+                const bool stopping =
+                    __hip_atomic_load(&g->dstop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
+                if (!stopping && __hip_atomic_load(&g->closing, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch)
+                    break;
+                if (h < tail) {
+                    if (__hip_atomic_compare_exchange_strong(&g->head, &h, h + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT)) {
+                        __hip_atomic_fetch_add(&g->busy, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_store(&g->last, now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        t = h;
+                        go = 1;
+                        break;
+                    }
+                    continue;
+                }
+                if (stopping) break;
+                if (__hip_atomic_load(&g->busy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+                    // the last finish's clock is published before its busy decrement
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    const uint64_t last = __hip_atomic_load(&g->last, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const uint64_t ref = last > born ? last : born;
+                    if (now > ref && now - ref > idle_ticks) {
+                        __hip_atomic_store(&g->closing, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                }
+                // back off while idle: about 0.9 us, doubling to about 7 us
+                for (int k = 0; k < nap; ++k) __builtin_amdgcn_s_sleep(32);
+                if (nap < 8) nap *= 2;
+            }
+            if (go) {
+                // the ring entry and its staged image were written before
+                // the publication: read them after it
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+                const uint64_t *e = (const uint64_t *)(ring + t % (uint64_t)cap);
+                uint64_t *w = (uint64_t *)&s_slot;
+                for (int k = 0; k < (int)(sizeof(ServeSlot) / 8); ++k)
+                    w[k] = __hip_atomic_load(e + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+            s_ticket = t;
+            s_go = go;
+        }
+        __syncthreads();
+        if (!s_go) return;
+        const uint64_t ticket = s_ticket;
+        // the descriptor and inputs into the request's device block
+        team_copy16<NT>(s_slot.dst, s_slot.src, s_slot.bytes);
+        __threadfence();
+        __syncthreads();
+        const SolveDesc &D = *(const SolveDesc *)s_slot.dst;
+        DevTeam<NT, WPE> tm;
+        double *stage = dyn;
+        uint8_t *tabs = (uint8_t *)(dyn + 2 * plan.stage_cap);
+        uint8_t *lsets = tabs + 2 * plan.tab_cap;
+        double *lut = (double *)(lsets + plan.sets_cap);
+        double *pi = lut + plan.lut_cap;
+        if (D.mrcap <= plan.stage_cap) {
+            tm.init(stage, stage + D.mrcap);
+            tm.stage_lds = true;
+        } else {
+            tm.init(D.stage, D.stage + D.mrcap);
+        }
+        if ((int64_t)D.mrcap * D.L <= plan.tab_cap) {
+            tm.lrows = tabs;
+            tm.lcols = tabs + plan.tab_cap;
+        }
+        const int64_t setbytes = (int64_t)NSETKIND * D.L * D.setcap * D.L;
+        if (((setbytes + 15) & ~(int64_t)15) + 4 * NSETKIND * D.L <= plan.sets_cap) {
+            tm.lsets = lsets;
+            tm.lsetn = (int32_t *)(lsets + ((setbytes + 15) & ~(int64_t)15));
+        }
+        tm.llut = plan.lut_cap ? lut : nullptr;
+        tm.llut_cap = plan.lut_cap;
+        tm.lpi = pi;
+        tm.lpi_cap = plan.pi_cap;
+        {
+            Engine<DevTeam<NT, WPE>, 0> eng(tm, D, S);
+            eng.run();
+        }
+        __syncthreads();
+        team_copy16<NT>(s_slot.rep_out, s_slot.rep, s_slot.rep_bytes);
+        __threadfence_system();   // every thread's part of the report before the done mark
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __hip_atomic_store(done + ticket % (uint64_t)cap, ticket + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&g->last, (uint64_t)wall_clock64(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_sub(&g->busy, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+    }
+}
+
+static const void *serve_kernel_ptr(int team)
+{
+    return team == 512 ? (const void *)tci_serve_kernel<512, 2> : (const void *)tci_serve_kernel<256, 2>;
+}
+
+int serve_static_lds(int team)
+{
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, serve_kernel_ptr(team)) != hipSuccess) return -1;
+    return (int)a.sharedSizeBytes;
+}
+
+hipError_t launch_serve(const ServeSlot *ring, ServeDev *g, const ServeCtl *ctl, uint64_t *done, int cap,
+                        const LdsPlan &plan, uint64_t idle_ticks, uint32_t epoch, int team, int blocks,
+                        hipStream_t st)
+{
+    const size_t lds = lds_plan_bytes(plan);
+    int lds_blk = 0, dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e != hipSuccess) return e;
+    const int stat = serve_static_lds(team);
+    if (stat < 0 || (size_t)stat + lds > (size_t)lds_blk) return hipErrorInvalidValue;
+    const void *k = serve_kernel_ptr(team);
+    if ((e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)) != hipSuccess) return e;
+    if (team == 512)
+        hipLaunchKernelGGL((tci_serve_kernel<512, 2>), dim3(blocks), dim3(512), lds, st, ring, g, ctl, done, cap,
+                           plan, idle_ticks, epoch);
+    else
+        hipLaunchKernelGGL((tci_serve_kernel<256, 2>), dim3(blocks), dim3(256), lds, st, ring, g, ctl, done, cap,
+                           plan, idle_ticks, epoch);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ rrLU
 // order: the job each workgroup takes (largest first, so the long
 // factorizations start early and the launch ends evenly), or null

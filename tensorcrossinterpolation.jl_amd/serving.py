@@ -195,6 +195,131 @@ class TCIServer:
         return results, stats
 
 
+class SlotServer:
+    """Slot-level serving (tci_server_*): a persistent grid on the device takes
+    each request from a queue as soon as a workgroup is free, so a request
+    waits neither for a batch to form nor for a batch's slowest solve.
+
+    shape: the largest request expected, as (f, localdims, initialpivots,
+    kwargs) — it sizes the grid's on-chip plan.  slots: the queue's ring
+    (requests in flight at most).  idle_ms: the grid leaves after this long
+    without a request (and starts again with the next one)."""
+
+    def __init__(self, ctx, shape, slots=16384, idle_ms=50):
+        from . import _holder
+        self.ctx = ctx
+        h = _holder(*shape)
+        self.h = C.c_void_p()
+        st = _abi.lib().tci_server_start(ctx.handle, C.byref(h.st), int(slots), int(idle_ms), C.byref(self.h))
+        if st:
+            raise ServeError(st, _abi.lib().tci_last_error().decode())
+
+    def close(self):
+        if self.h:
+            st = _abi.lib().tci_server_stop(self.h)
+            self.h = C.c_void_p()
+            if st:
+                raise ServeError(st, _abi.lib().tci_last_error().decode())
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, requests, arrivals, keep_tci=False):
+        """requests[i] = (f, localdims, initialpivots_or_None, kwargs); arrivals
+        sorted, seconds from the start.  Each request is submitted when it
+        arrives; a full queue or an exhausted HBM budget holds new ones back
+        until results are retired.  A request that outgrew its capacity with
+        chi_cap unset is submitted again at twice the capacity.  Returns
+        (results, ServeStats)."""
+        from . import TensorCI2, _holder, capacity_plan
+        lib = _abi.lib()
+        n = len(requests)
+        results = [None] * n
+        inflight = {}                       # batch handle value -> (request id, batch handle, holder)
+        retry = []
+        retry_holder = {}                   # request id -> holder at the larger capacity
+        nxt = 0
+        nretries = 0
+        oom_waits = 0
+        pcap = 4096
+        polled = (C.c_void_p * pcap)()
+        npolled = C.c_int32()
+        t0 = time.perf_counter()
+        while nxt < n or inflight or retry:
+            now = time.perf_counter() - t0
+            progressed = False
+            blocked = False
+            while (retry or (nxt < n and arrivals[nxt] <= now)) and not blocked:
+                i = retry[0] if retry else nxt
+                f, ld, piv, kw = requests[i]
+                h = retry_holder.pop(i) if retry else _holder(f, ld, piv, kw)
+                b = C.c_void_p()
+                st = lib.tci_server_submit(self.h, C.byref(h.st), C.byref(b))
+                if st == 0:
+                    inflight[b.value] = (i, b, h)
+                    if retry:
+                        retry.pop(0)
+                    else:
+                        nxt += 1
+                    progressed = True
+                elif st == 11 and inflight:   # queue or HBM budget full: retire first
+                    if retry:
+                        retry_holder[i] = h
+                    oom_waits += 1
+                    blocked = True
+                else:
+                    for _, b2, _ in inflight.values():
+                        lib.tci_batch_free(b2)
+                    raise ServeError(st, f"submit of request {i} failed: {lib.tci_last_error().decode()}")
+            st = lib.tci_server_poll(self.h, polled, pcap, C.byref(npolled))
+            if st:
+                msg = lib.tci_last_error().decode()
+                for _, b2, _ in inflight.values():
+                    lib.tci_batch_free(b2)
+                raise ServeError(st, f"poll failed: {msg}")
+            done = time.perf_counter() - t0
+            for q in range(npolled.value):
+                i, b, h = inflight.pop(polled[q])
+                progressed = True
+                r = _abi.tci_result()
+                lib.tci_batch_result(b, 0, C.byref(r))
+                if r.status == TCI_ERR_CAPACITY and not requests[i][3].get("chi_cap"):
+                    used, limit = capacity_plan(h)
+                    chi = int(min(2 * used, limit, 8192))
+                    if chi > used:
+                        f, ld, piv, kw = requests[i]
+                        retry_holder[i] = _holder(f, ld, piv, dict(kw, chi_cap=chi))
+                        retry.append(i)
+                        nretries += 1
+                        lib.tci_batch_free(b)
+                        continue
+                m = max(r.niter, 1)
+                ranks = np.zeros(m, np.int64)
+                errs = np.zeros(m)
+                lib.tci_batch_history(b, 0, _abi.i64ptr(ranks), _abi.dptr(errs), None)
+                t = TensorCI2(b, 0, requests[i][1]) if keep_tci else None
+                results[i] = ServedResult(r.status, ranks[:r.niter], errs[:r.niter], r.maxsamplevalue,
+                                          float(arrivals[i]), done, t)
+                lib.tci_batch_free(b)
+            if not progressed:
+                if nxt < n and not inflight and not retry:
+                    wait = arrivals[nxt] - (time.perf_counter() - t0)
+                    if wait > 0:
+                        time.sleep(min(wait, 1e-3))
+                else:
+                    time.sleep(20e-6)
+        wall = time.perf_counter() - t0
+        lat = np.array([r.latency for r in results]) * 1e3
+        stats = ServeStats(n=n, wall_s=wall, solves_per_s=n / wall if wall > 0 else 0.0,
+                           p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                           batches=n, mean_batch=1.0, oom_waits=oom_waits, retries=nretries,
+                           latencies_ms=lat.tolist())
+        return results, stats
+
+
 def poisson_arrivals(n, rate_per_s, seed=5):
     """n arrival times of a Poisson stream (exponential gaps), seconds."""
     rng = np.random.default_rng(seed)
