@@ -1286,7 +1286,8 @@ struct Engine {
     // fillsitetensors! (globalsearch.jl:97-104) → setsitetensor!(tci, f, b)
     // (tensorci2.jl:367-394).  When no global search follows (nsearch == 0)
     // the tensors are never read before sweep1site! rebuilds all of them, so
-    // only max|Pi1| (which feeds maxsamplevalue) is computed.  Otherwise each
+    // only max|Pi1| (which feeds maxsamplevalue) and the shape check are
+    // computed.  Otherwise each
     // site's Pi1 is evaluated into its tensor slot, and the independent solves
     // T_b = Pi1·P⁻¹ run in groups of sites whose Pᵀ fit in LDS together, one
     // wave per site.  Errors keep the reference's order: a singular P at a
@@ -1302,6 +1303,7 @@ struct Engine {
             tm.sync();
             if (!need_tensors) {
                 eval(rows, b + 1, m, cols, nJ, nullptr, 0, false, true);
+                if (b + 1 < L && setn(SET_I, b + 1) != nJ) { nsq = b; break; }
                 continue;
             }
             if ((int64_t)m * nJ > D.tstride) return fail(ST_CAPACITY);
@@ -1309,7 +1311,15 @@ struct Engine {
             setTdims(b, nI, d, nJ);
             if (b + 1 < L && setn(SET_I, b + 1) != nJ) { nsq = b; break; }
         }
-        if (!need_tensors) return ST_OK;
+        if (!need_tensors) {
+            // the reference's "not square" check still holds; its LU of each
+            // P (SingularException) is not repeated when no tensor is needed
+            if (nsq >= 0) {
+                if (tm.tid() == 0) D.ints[INT_ERRSITE] = nsq + 1;
+                return fail(ST_NONSQUARE);
+            }
+            return ST_OK;
+        }
         const int nsolve = nsq >= 0 ? nsq : L - 1;
         // per-wave row scratch for the substitutions (null: in place in HBM)
         int cmax = 1;

@@ -2,8 +2,9 @@
 
 Bit-exact for pivot sequences, permutations, ranks, pivot sets and every
 FP64 value the engine computes with a fixed operation order (the oracle
-restates the same order); see tests/test_gpu_tolerance.py for the checks
-that are to rounding tolerance.
+restates the same order); the checks to a tolerance (TT evaluation against
+the functor, the error estimates) are in tests/test_gpu_tt_eval.py and
+tests/test_gpu_tt.py.
 """
 import json
 import os
