@@ -438,7 +438,8 @@ def slot_leg(args, tci, ctx, P, reqs, batch_cap):
         res, st = srv.run(reqs, np.zeros(len(reqs)))
         out = {"admission": "slot: a resident grid takes each request from a device queue",
                "capacity_solves_per_s": round(st.solves_per_s, 2), "failed": int(sum(r.status != 0 for r in res)),
-               "capacity_retries": st.retries, "sweep": []}
+               "capacity_retries": st.retries, "capacity_wall_ms": round(st.wall_s * 1e3, 1),
+               "capacity_host_ms": {k: round(v, 1) for k, v in st.host_ms.items()}, "sweep": []}
         for lam in args.c5_lambdas:
             arr = serving.poisson_arrivals(len(reqs), lam * batch_cap, seed=5)
             res, st = srv.run(reqs, arr)

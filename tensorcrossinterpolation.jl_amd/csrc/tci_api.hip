@@ -67,7 +67,8 @@ struct tci_ctx {
     size_t next = 0;
     uint64_t budget;
     uint64_t used;
-    std::vector<PoolBuf> pool;
+    std::vector<PoolBuf> pool;          // cached device buffers of finished batches
+    uint64_t pooled = 0;                // their bytes
     int lds_wg;         // LDS bytes (static + dynamic) a solve workgroup may use
     int team;           // threads per solve workgroup
     int pi_lds_kb;      // LDS budget for Pi matrices (KiB)
@@ -118,7 +119,11 @@ static LdsPlan make_plan(const tci_ctx *c, int64_t left, int max_mrcap, int max_
     return p;
 }
 
-static hipError_t pool_get(tci_ctx *c, size_t bytes, void **out)
+// Device buffers of finished batches are cached and reused best-fit (up to
+// twice the size asked for).  hipFree synchronizes the whole device, so a
+// cached buffer is freed only when the cache and the buffers in use would
+// exceed the context's HBM budget, never to keep the cache small.
+static hipError_t pool_get(tci_ctx *c, size_t bytes, void **out, size_t *got)
 {
     size_t best = (size_t)-1;
     for (size_t i = 0; i < c->pool.size(); ++i)
@@ -127,9 +132,17 @@ static hipError_t pool_get(tci_ctx *c, size_t bytes, void **out)
             best = i;
     if (best != (size_t)-1) {
         *out = c->pool[best].ptr;
+        *got = c->pool[best].bytes;
+        c->pooled -= c->pool[best].bytes;
         c->pool.erase(c->pool.begin() + (long)best);
         return hipSuccess;
     }
+    while (!c->pool.empty() && c->used + c->pooled > c->budget) {
+        (void)hipFree(c->pool.front().ptr);
+        c->pooled -= c->pool.front().bytes;
+        c->pool.erase(c->pool.begin());
+    }
+    *got = bytes;
     return hipMalloc(out, bytes);
 }
 
@@ -137,10 +150,7 @@ static void pool_put(tci_ctx *c, void *p, size_t bytes)
 {
     if (!p) return;
     c->pool.push_back({p, bytes});
-    while (c->pool.size() > 16) {
-        (void)hipFree(c->pool.front().ptr);
-        c->pool.erase(c->pool.begin());
-    }
+    c->pooled += bytes;
 }
 
 // a block of a slot-level server's memory: device memory, or pinned host
@@ -166,6 +176,7 @@ struct tci_batch {
     uint8_t *d_in = nullptr, *d_state = nullptr, *d_rep = nullptr;
     SolveDesc *d_desc = nullptr;
     size_t bytes = 0, rep_bytes = 0, in_bytes = 0, state_bytes = 0, desc_bytes = 0;
+    size_t pool_bytes[4] = {0, 0, 0, 0};   // sizes of the pool buffers held (inputs, state, report, descriptors)
     std::vector<uint8_t> h_in;        // upload image, alive until the copy ran
     std::vector<uint8_t> h_rep;
     bool fetched = false;
@@ -1141,10 +1152,10 @@ static void batch_release(tci_batch *b)
         if (b->dblk.ptr) serve_block_put(b->server, b->ctx, b->dblk, b->dcls, true);
         if (b->hblk.ptr) serve_block_put(b->server, b->ctx, b->hblk, b->hcls, false);
     } else if (b->ctx) {
-        pool_put(b->ctx, b->d_desc, b->desc_bytes);
-        pool_put(b->ctx, b->d_in, b->in_bytes);
-        pool_put(b->ctx, b->d_state, b->state_bytes);
-        pool_put(b->ctx, b->d_rep, b->rep_bytes);
+        pool_put(b->ctx, b->d_desc, b->pool_bytes[3]);
+        pool_put(b->ctx, b->d_in, b->pool_bytes[0]);
+        pool_put(b->ctx, b->d_state, b->pool_bytes[1]);
+        pool_put(b->ctx, b->d_rep, b->pool_bytes[2]);
         b->ctx->used -= b->bytes;
     } else {
         if (b->d_desc) (void)hipFree(b->d_desc);
@@ -1221,11 +1232,12 @@ static tci_status batch_prepare(tci_ctx *ctx, int32_t count, const tci_problem *
     b->in_bytes = tin;
     b->state_bytes = tstate;
     b->desc_bytes = sizeof(SolveDesc) * (size_t)count;
-    auto get = [&](size_t bytes, void **p) { return pool_get(ctx, bytes, p); };
-    if ((e = get(tin, (void **)&b->d_in)) != hipSuccess) return fail(e, "hipMalloc(inputs)");
-    if ((e = get(tstate, (void **)&b->d_state)) != hipSuccess) return fail(e, "hipMalloc(state)");
-    if ((e = get(trep, (void **)&b->d_rep)) != hipSuccess) return fail(e, "hipMalloc(report)");
-    if ((e = get(b->desc_bytes, (void **)&b->d_desc)) != hipSuccess) return fail(e, "hipMalloc(desc)");
+    if ((e = pool_get(ctx, tin, (void **)&b->d_in, &b->pool_bytes[0])) != hipSuccess) return fail(e, "hipMalloc(inputs)");
+    if ((e = pool_get(ctx, tstate, (void **)&b->d_state, &b->pool_bytes[1])) != hipSuccess)
+        return fail(e, "hipMalloc(state)");
+    if ((e = pool_get(ctx, trep, (void **)&b->d_rep, &b->pool_bytes[2])) != hipSuccess) return fail(e, "hipMalloc(report)");
+    if ((e = pool_get(ctx, b->desc_bytes, (void **)&b->d_desc, &b->pool_bytes[3])) != hipSuccess)
+        return fail(e, "hipMalloc(desc)");
     b->rep_bytes = trep;
     b->desc.resize((size_t)count);
     b->rep_off.resize((size_t)count);

@@ -30,14 +30,8 @@ constexpr int kTeam = 256;
 //   [Lorentzian value table: lut_cap doubles][Pi area: pi_cap doubles]
 // Anything that does not fit stays in the solve's HBM arena.
 template <int NT, int FK, int WPE>
-__global__ __launch_bounds__(NT, WPE) void tci_solve_kernel(const SolveDesc *__restrict__ descs,
-                                                       int count, LdsPlan plan)
+__device__ __forceinline__ void solve_on_team(const SolveDesc &D, const LdsPlan &plan, EngShared &S, double *dyn)
 {
-    const int s = blockIdx.x;
-    if (s >= count) return;
-    __shared__ EngShared S;
-    extern __shared__ double dyn[];
-    const SolveDesc &D = descs[s];
     DevTeam<NT, WPE> tm;
     double *stage = dyn;
     uint8_t *tabs = (uint8_t *)(dyn + 2 * plan.stage_cap);
@@ -67,6 +61,17 @@ __global__ __launch_bounds__(NT, WPE) void tci_solve_kernel(const SolveDesc *__r
     eng.run();
 }
 
+template <int NT, int FK, int WPE>
+__global__ __launch_bounds__(NT, WPE) void tci_solve_kernel(const SolveDesc *__restrict__ descs,
+                                                       int count, LdsPlan plan)
+{
+    const int s = blockIdx.x;
+    if (s >= count) return;
+    __shared__ EngShared S;
+    extern __shared__ double dyn[];
+    solve_on_team<NT, FK, WPE>(descs[s], plan, S, dyn);
+}
+
 // ------------------------------------------------------------------ serving
 // Slot-level serving: a persistent grid whose workgroups each take one
 // published solve at a time from a ring of requests.  The host publishes
@@ -92,12 +97,21 @@ __device__ __forceinline__ void team_copy16(uint8_t *dst, const uint8_t *src, ui
         *(uint4 *)(dst + o) = *(const uint4 *)(src + o);
 }
 
+// the engine run of one served request, compiled apart from the claim loop
+// so that the loop's live values do not crowd the engine's registers
+template <int NT, int WPE>
+__device__ __noinline__ void serve_solve(const SolveDesc *D, LdsPlan plan)
+{
+    __shared__ EngShared S;
+    extern __shared__ double dyn[];
+    solve_on_team<NT, 0, WPE>(*D, plan, S, dyn);
+}
+
 template <int NT, int WPE>
 __global__ __launch_bounds__(NT, WPE) void tci_serve_kernel(const ServeSlot *__restrict__ ring, ServeDev *g,
                                                             const ServeCtl *ctl, uint64_t *done, int cap,
                                                             LdsPlan plan, uint64_t idle_ticks, uint32_t epoch)
 {
-    __shared__ EngShared S;
     __shared__ uint64_t s_ticket;
     __shared__ int s_go;
     __shared__ ServeSlot s_slot;
@@ -175,36 +189,7 @@ __global__ __launch_bounds__(NT, WPE) void tci_serve_kernel(const ServeSlot *__r
         team_copy16<NT>(s_slot.dst, s_slot.src, s_slot.bytes);
         __threadfence();
         __syncthreads();
-        const SolveDesc &D = *(const SolveDesc *)s_slot.dst;
-        DevTeam<NT, WPE> tm;
-        double *stage = dyn;
-        uint8_t *tabs = (uint8_t *)(dyn + 2 * plan.stage_cap);
-        uint8_t *lsets = tabs + 2 * plan.tab_cap;
-        double *lut = (double *)(lsets + plan.sets_cap);
-        double *pi = lut + plan.lut_cap;
-        if (D.mrcap <= plan.stage_cap) {
-            tm.init(stage, stage + D.mrcap);
-            tm.stage_lds = true;
-        } else {
-            tm.init(D.stage, D.stage + D.mrcap);
-        }
-        if ((int64_t)D.mrcap * D.L <= plan.tab_cap) {
-            tm.lrows = tabs;
-            tm.lcols = tabs + plan.tab_cap;
-        }
-        const int64_t setbytes = (int64_t)NSETKIND * D.L * D.setcap * D.L;
-        if (((setbytes + 15) & ~(int64_t)15) + 4 * NSETKIND * D.L <= plan.sets_cap) {
-            tm.lsets = lsets;
-            tm.lsetn = (int32_t *)(lsets + ((setbytes + 15) & ~(int64_t)15));
-        }
-        tm.llut = plan.lut_cap ? lut : nullptr;
-        tm.llut_cap = plan.lut_cap;
-        tm.lpi = pi;
-        tm.lpi_cap = plan.pi_cap;
-        {
-            Engine<DevTeam<NT, WPE>, 0> eng(tm, D, S);
-            eng.run();
-        }
+        serve_solve<NT, WPE>((const SolveDesc *)s_slot.dst, plan);
         __syncthreads();
         team_copy16<NT>(s_slot.rep_out, s_slot.rep, s_slot.rep_bytes);
         __threadfence_system();   // every thread's part of the report before the done mark

@@ -62,6 +62,7 @@ class ServeStats:
     oom_waits: int = 0
     retries: int = 0
     latencies_ms: list = field(default_factory=list)
+    host_ms: dict = field(default_factory=dict)   # host time per phase of the loop
 
 
 class TCIServer:
@@ -247,11 +248,14 @@ class SlotServer:
         pcap = 4096
         polled = (C.c_void_p * pcap)()
         npolled = C.c_int32()
+        host = {"submit": 0.0, "poll": 0.0, "retire": 0.0}
+        clock = time.perf_counter
         t0 = time.perf_counter()
         while nxt < n or inflight or retry:
             now = time.perf_counter() - t0
             progressed = False
             blocked = False
+            ts = clock()
             while (retry or (nxt < n and arrivals[nxt] <= now)) and not blocked:
                 i = retry[0] if retry else nxt
                 f, ld, piv, kw = requests[i]
@@ -274,7 +278,11 @@ class SlotServer:
                     for _, b2, _ in inflight.values():
                         lib.tci_batch_free(b2)
                     raise ServeError(st, f"submit of request {i} failed: {lib.tci_last_error().decode()}")
+            tp = clock()
+            host["submit"] += tp - ts
             st = lib.tci_server_poll(self.h, polled, pcap, C.byref(npolled))
+            tr = clock()
+            host["poll"] += tr - tp
             if st:
                 msg = lib.tci_last_error().decode()
                 for _, b2, _ in inflight.values():
@@ -304,6 +312,7 @@ class SlotServer:
                 results[i] = ServedResult(r.status, ranks[:r.niter], errs[:r.niter], r.maxsamplevalue,
                                           float(arrivals[i]), done, t)
                 lib.tci_batch_free(b)
+            host["retire"] += clock() - tr
             if not progressed:
                 if nxt < n and not inflight and not retry:
                     wait = arrivals[nxt] - (time.perf_counter() - t0)
@@ -316,7 +325,7 @@ class SlotServer:
         stats = ServeStats(n=n, wall_s=wall, solves_per_s=n / wall if wall > 0 else 0.0,
                            p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
                            batches=n, mean_batch=1.0, oom_waits=oom_waits, retries=nretries,
-                           latencies_ms=lat.tolist())
+                           latencies_ms=lat.tolist(), host_ms={k: v * 1e3 for k, v in host.items()})
         return results, stats
 
 
