@@ -164,6 +164,36 @@ tci_status tci_rrlu_batched_f64(tci_ctx *ctx, tci_lu_job *jobs_device, int64_t c
 enum { TCI_RRLU_AUTO = 0, TCI_RRLU_STREAM = 1, TCI_RRLU_TILE = 2, TCI_RRLU_GRID = 3 };
 tci_status tci_ctx_set_rrlu_algo(tci_ctx *ctx, int32_t algo);
 
+/* The autotuner of the standalone rrLU (no counterpart in the reference).
+ * For square matrices of each bucket side (32, 64, ..., 4096; rank
+ * min(side/2, 64), reltol 1e-10) it times on this context's device: the
+ * register tile against the streaming kernel for a batch of 512 (sides up
+ * to 128), the cooperative kernel at 256, 512 and 1024 threads for one
+ * matrix, and k cooperative runs one after the other against one streaming
+ * launch of k matrices (k = 1, 2, 4, ..., 64).  The winners are stored in
+ * the context, and AUTO then follows them: a job of largest side s uses the
+ * first entry whose side is at least s (the last one beyond it).  The
+ * register tile is used when that entry's tile is set.  The cooperative
+ * kernel (with that entry's team) is used when the batch holds no more such
+ * jobs than every one of their entries' grid_max_jobs.  Choices change
+ * speed only: every kernel gives the same bits.  Writes up to cap entries
+ * to table (may be NULL) and their number to *count.  Takes a few seconds
+ * and up to about 9 GB of the HBM budget while it runs.
+ * tci_ctx_set_rrlu_tuning installs a table (one measured earlier and saved);
+ * n = 0 returns AUTO to its fixed rules. */
+typedef struct tci_rrlu_tune {
+    int32_t side;            /* bucket: square matrices of this side           */
+    int32_t tile;            /* 1: the register tile for these (side <= 128)   */
+    int32_t grid_team;       /* threads per workgroup of the cooperative kernel */
+    int32_t grid_max_jobs;   /* most such jobs the cooperative kernel takes (0: none) */
+    float ms_stream;         /* one matrix, streaming kernel                   */
+    float ms_grid;           /* one matrix, cooperative kernel at grid_team    */
+    float ms_tile_batch;     /* 512 matrices, register tile (side <= 128)      */
+    float ms_stream_batch;   /* 512 matrices, streaming kernel (side <= 128)   */
+} tci_rrlu_tune;
+tci_status tci_ctx_autotune_rrlu(tci_ctx *ctx, tci_rrlu_tune *table, int32_t cap, int32_t *count);
+tci_status tci_ctx_set_rrlu_tuning(tci_ctx *ctx, const tci_rrlu_tune *table, int32_t n);
+
 /* tci_rrlu_batched_f64 that allocates no device memory.  No counterpart in
  * the reference: it is for callers that cannot call hipMalloc (code under
  * HIP graph capture, tight serving loops, harnesses that forbid it).  The

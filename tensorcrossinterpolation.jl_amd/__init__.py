@@ -86,6 +86,27 @@ class Context:
     def synchronize(self):
         _check(_abi.lib().tci_ctx_synchronize(self._h))
 
+    _TUNE_FIELDS = ("side", "tile", "grid_team", "grid_max_jobs", "ms_stream", "ms_grid", "ms_tile_batch",
+                    "ms_stream_batch")
+
+    def autotune_rrlu(self):
+        """Measure the standalone rrLU's kernel choices per size bucket on
+        this device (tci_ctx_autotune_rrlu); AUTO follows them from now on.
+        Returns the table as a list of dicts."""
+        tab = (_abi.tci_rrlu_tune * 32)()
+        n = C.c_int32()
+        _check(_abi.lib().tci_ctx_autotune_rrlu(self._h, tab, 32, C.byref(n)))
+        return [{k: getattr(tab[q], k) for k in self._TUNE_FIELDS} for q in range(n.value)]
+
+    def set_rrlu_tuning(self, table):
+        """Install a table returned by autotune_rrlu (an empty one returns
+        AUTO to its fixed rules)."""
+        tab = (_abi.tci_rrlu_tune * max(len(table), 1))()
+        for q, row in enumerate(table):
+            for k in self._TUNE_FIELDS:
+                setattr(tab[q], k, row.get(k, 0))
+        _check(_abi.lib().tci_ctx_set_rrlu_tuning(self._h, tab, len(table)))
+
     def close(self):
         if self._h:
             _abi.lib().tci_ctx_destroy(self._h)

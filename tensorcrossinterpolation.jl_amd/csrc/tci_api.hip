@@ -73,6 +73,8 @@ struct tci_ctx {
     int team;           // threads per solve workgroup
     int pi_lds_kb;      // LDS budget for Pi matrices (KiB)
     int rrlu_algo = TCI_RRLU_AUTO;   // factorization kernel of the standalone rrLU
+    std::vector<tci_rrlu_tune> tune; // measured AUTO choices (tci_ctx_autotune_rrlu), by side
+    int grid_team = 0;               // the autotuner's forced cooperative team (0: none)
 };
 
 // a device-to-host copy of a finished result on the context's download
@@ -430,12 +432,27 @@ static tci_status lu_plan(tci_ctx *ctx, tci_lu_job *jobs, int64_t count, int64_t
     return s != TCI_OK ? s : lu_partition(ctx, tile_budget, P);
 }
 
+// the tuned entry for a job of largest side s: the first entry at least s,
+// the last one beyond them; null when the context is not tuned
+static const tci_rrlu_tune *tune_for(const tci_ctx *c, int64_t m, int64_t n)
+{
+    if (c->tune.empty()) return nullptr;
+    const int64_t s = m > n ? m : n;
+    for (const tci_rrlu_tune &t : c->tune)
+        if (t.side >= s) return &t;
+    return &c->tune.back();
+}
+
 static tci_status lu_partition(tci_ctx *ctx, int64_t tile_budget, LuPlan &P, bool allow_grid)
 {
     const int64_t count = (int64_t)P.h.size();
     for (int64_t q = 0; q < count; ++q) {
         const tci_lu_job &j = P.h[(size_t)q];
-        const bool fits = j.m >= 1 && j.n >= 1 && j.m <= kTileSide && j.n <= kTileSide;
+        bool fits = j.m >= 1 && j.n >= 1 && j.m <= kTileSide && j.n <= kTileSide;
+        if (fits && ctx->rrlu_algo == TCI_RRLU_AUTO) {
+            const tci_rrlu_tune *t = tune_for(ctx, j.m, j.n);
+            if (t && !t->tile) fits = false;
+        }
         if (ctx->rrlu_algo == TCI_RRLU_TILE && !fits && j.m > 0 && j.n > 0)
             return set_err(TCI_ERR_ARG, "job %lld (%lld x %lld) does not fit the 128 x 128 register tile",
                            (long long)q, (long long)j.m, (long long)j.n);
@@ -457,14 +474,21 @@ static tci_status lu_partition(tci_ctx *ctx, int64_t tile_budget, LuPlan &P, boo
     // fill the chip with one workgroup per job
     if (allow_grid && (ctx->rrlu_algo == TCI_RRLU_GRID || ctx->rrlu_algo == TCI_RRLU_AUTO)) {
         std::vector<int32_t> big, rest;
+        // tuned: the fewest jobs any of the large ones' entries allows
+        int64_t maxjobs = ctx->tune.empty() ? kGridMaxJobs : INT64_MAX;
         for (int32_t q : P.stream) {
             const tci_lu_job &j = P.h[(size_t)q];
             const bool fits = j.m >= 1 && j.n >= 1 && j.m <= kGridMaxSide && j.n <= kGridMaxSide;
-            const bool large = j.m * j.n >= kGridMinElems;
-            if (fits && (ctx->rrlu_algo == TCI_RRLU_GRID || large)) big.push_back(q);
-            else rest.push_back(q);
+            const tci_rrlu_tune *t = tune_for(ctx, j.m, j.n);
+            const bool large = t ? t->grid_max_jobs > 0 : j.m * j.n >= kGridMinElems;
+            if (fits && (ctx->rrlu_algo == TCI_RRLU_GRID || large)) {
+                big.push_back(q);
+                if (t && t->grid_max_jobs < maxjobs) maxjobs = t->grid_max_jobs;
+            } else {
+                rest.push_back(q);
+            }
         }
-        if (ctx->rrlu_algo == TCI_RRLU_GRID || (!big.empty() && (int64_t)big.size() <= kGridMaxJobs)) {
+        if (ctx->rrlu_algo == TCI_RRLU_GRID || (!big.empty() && (int64_t)big.size() <= maxjobs)) {
             P.grid = big;
             P.stream = rest;
         }
@@ -477,24 +501,31 @@ static tci_status lu_run_grid(tci_ctx *ctx, tci_lu_job *jobs, const LuPlan &P)
 {
     if (P.grid.empty()) return TCI_OK;
     hipStream_t st = ctx->stream;
-    int64_t mx = 0, side = 0;
+    int64_t mx = 0;
     for (int32_t q : P.grid) {
         const tci_lu_job &j = P.h[(size_t)q];
         mx = j.m * j.n > mx ? j.m * j.n : mx;
-        side = j.m + j.n > side ? j.m + j.n : side;
     }
-    const int G = rrlu_grid_blocks(ctx->device, rrlu_grid_lds(side, 0));
-    if (G < 1) return set_err(TCI_ERR_DEVICE, "the cooperative rrLU cannot be launched on this device");
-    const size_t b_out = lay_align(sizeof(double) * (size_t)mx), b_cand = lay_align(rrlu_grid_cand_bytes(G));
+    int ncu = 0;
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const size_t b_out = lay_align(sizeof(double) * (size_t)mx), b_cand = lay_align(rrlu_grid_cand_bytes(ncu));
     uint8_t *d = nullptr;
     HIPCHK(hipMallocAsync((void **)&d, b_out + b_cand + 256, st));
     hipError_t e = hipSuccess;
     for (int32_t q : P.grid) {
         const tci_lu_job &j = P.h[(size_t)q];
+        // the team: the autotuner's forced one, the tuned one, or the default
+        const tci_rrlu_tune *t = tune_for(ctx, j.m, j.n);
+        const int team = ctx->grid_team ? ctx->grid_team : t ? t->grid_team : rrlu_grid_default_team();
+        const int G = rrlu_grid_blocks(ctx->device, rrlu_grid_lds(j.m, j.n), team);
+        if (G < 1 || G > ncu) {
+            e = hipErrorInvalidConfiguration;
+            break;
+        }
         int blocks = G < (int)j.n ? G : (int)j.n;
         int32_t *flags = (int32_t *)(d + b_out + b_cand);
         e = hipMemsetAsync(flags, 0, sizeof(int32_t), st);
-        if (e == hipSuccess) e = launch_rrlu_grid(jobs + q, j.m, j.n, (double *)d, d + b_out, flags, blocks, st);
+        if (e == hipSuccess) e = launch_rrlu_grid(jobs + q, j.m, j.n, (double *)d, d + b_out, flags, blocks, team, st);
         if (e == hipSuccess)
             e = hipMemcpyAsync(j.A, d, sizeof(double) * (size_t)(j.m * j.n), hipMemcpyDeviceToDevice, st);
         if (e != hipSuccess) break;
@@ -577,6 +608,155 @@ tci_status tci_rrlu_batched_f64(tci_ctx *ctx, tci_lu_job *jobs, int64_t count)
     (void)hipFreeAsync(dorder, ctx->stream);
     if (gstage) (void)hipFreeAsync(gstage, ctx->stream);
     if (e != hipSuccess) return set_err(TCI_ERR_DEVICE, "rrlu launch: %s", hipGetErrorString(e));
+    return TCI_OK;
+}
+
+// ---- the autotuner of the standalone rrLU (tci_ctx_autotune_rrlu)
+static void tune_lcg(uint64_t &x, double *v, size_t n)
+{
+    for (size_t i = 0; i < n; ++i) {
+        x = x * 6364136223846793005ULL + 1442695040888963407ULL;
+        v[i] = (double)(int64_t)(x >> 11) * 0x1.0p-52 - 1.0;   // in [-1, 1)
+    }
+}
+
+tci_status tci_ctx_autotune_rrlu(tci_ctx *ctx, tci_rrlu_tune *table, int32_t cap, int32_t *count)
+{
+    if (!ctx || !count || cap < 0 || (cap > 0 && !table)) return set_err(TCI_ERR_ARG, "bad arguments");
+    *count = 0;
+    HIPCHK(hipSetDevice(ctx->device));
+    static const int kSides[] = {32, 64, 128, 256, 512, 1024, 2048, 4096};
+    static const int kTeams[] = {256, 512, 1024};
+    const int algo0 = ctx->rrlu_algo;
+    std::vector<tci_rrlu_tune> tuned0 = ctx->tune;
+    ctx->tune.clear();
+    std::vector<tci_rrlu_tune> out;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    uint8_t *mem = nullptr;
+    tci_status st = TCI_OK;
+    hipError_t he = hipEventCreate(&e0);
+    if (he == hipSuccess) he = hipEventCreate(&e1);
+    uint64_t seed = 0x243F6A8885A308D3ULL;
+    for (int s : kSides) {
+        if (he != hipSuccess || st != TCI_OK) break;
+        const int r = s / 2 < 64 ? s / 2 : 64;
+        const int kmax = s <= kTileSide ? 512 : 64;
+        const size_t mat = (size_t)s * s;
+        // pristine matrix, kmax working copies, X, Y, permutations, jobs
+        const size_t b_mat = lay_align(sizeof(double) * mat);
+        const size_t b_xy = lay_align(sizeof(double) * (size_t)s * r);
+        const size_t b_perm = lay_align(sizeof(int32_t) * 2 * (size_t)s);
+        const size_t b_jobs = lay_align(sizeof(tci_lu_job) * (size_t)kmax);
+        const size_t total = b_mat * (1 + (size_t)kmax) + 2 * b_xy + b_perm * (size_t)kmax + b_jobs;
+        if (ctx->used + total > ctx->budget) break;   // larger buckets do not fit the budget
+        if ((he = hipMalloc((void **)&mem, total)) != hipSuccess) break;
+        double *A0 = (double *)mem, *W = (double *)(mem + b_mat);
+        double *X = (double *)(mem + b_mat * (1 + (size_t)kmax)), *Y = (double *)((uint8_t *)X + b_xy);
+        int32_t *perm = (int32_t *)((uint8_t *)Y + b_xy);
+        tci_lu_job *djobs = (tci_lu_job *)((uint8_t *)perm + b_perm * (size_t)kmax);
+        // A0 = X·Y, rank r
+        std::vector<double> hx((size_t)s * r), hy((size_t)s * r);
+        tune_lcg(seed, hx.data(), hx.size());
+        tune_lcg(seed, hy.data(), hy.size());
+        he = hipMemcpy(X, hx.data(), sizeof(double) * hx.size(), hipMemcpyHostToDevice);
+        if (he == hipSuccess) he = hipMemcpy(Y, hy.data(), sizeof(double) * hy.size(), hipMemcpyHostToDevice);
+        if (he == hipSuccess) he = launch_gemm(s, s, r, X, s, Y, r, A0, s, ctx->stream);
+        std::vector<tci_lu_job> hj((size_t)kmax);
+        for (int q = 0; q < kmax; ++q) {
+            tci_lu_job &j = hj[(size_t)q];
+            memset(&j, 0, sizeof j);
+            j.A = (double *)((uint8_t *)W + b_mat * (size_t)q);
+            j.rowperm = (int32_t *)((uint8_t *)perm + b_perm * (size_t)q);
+            j.colperm = j.rowperm + s;
+            j.m = j.n = s;
+            j.maxrank = s;
+            j.reltol = 1e-10;
+            j.abstol = 0.0;
+            j.leftorth = 1;
+        }
+        if (he == hipSuccess) he = hipMemcpy(djobs, hj.data(), sizeof(tci_lu_job) * hj.size(), hipMemcpyHostToDevice);
+        if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+        // k fresh copies of A0, then the factorization timed on the context's stream
+        auto run = [&](int algo, int team, int k, float *ms) -> bool {
+            for (int q = 0; q < k && he == hipSuccess; ++q)
+                he = hipMemcpyAsync((uint8_t *)W + b_mat * (size_t)q, A0, sizeof(double) * mat,
+                                    hipMemcpyDeviceToDevice, ctx->stream);
+            if (he != hipSuccess) return false;
+            ctx->rrlu_algo = algo;
+            ctx->grid_team = team;
+            he = hipEventRecord(e0, ctx->stream);
+            if (he != hipSuccess) return false;
+            st = tci_rrlu_batched_f64(ctx, djobs, k);
+            if (st != TCI_OK) return false;
+            if ((he = hipEventRecord(e1, ctx->stream)) != hipSuccess) return false;
+            if ((he = hipEventSynchronize(e1)) != hipSuccess) return false;
+            return (he = hipEventElapsedTime(ms, e0, e1)) == hipSuccess;
+        };
+        tci_rrlu_tune t;
+        memset(&t, 0, sizeof t);
+        t.side = s;
+        float ms = 0.0f, warm = 0.0f;
+        bool ok = run(TCI_RRLU_STREAM, 0, 1, &warm) && run(TCI_RRLU_STREAM, 0, 1, &t.ms_stream);
+        if (ok && s <= kTileSide) {
+            ok = run(TCI_RRLU_TILE, 0, 1, &warm) && run(TCI_RRLU_TILE, 0, 512, &t.ms_tile_batch) &&
+                 run(TCI_RRLU_STREAM, 0, 512, &t.ms_stream_batch);
+            t.tile = t.ms_tile_batch < t.ms_stream_batch;
+        }
+        t.ms_grid = -1.0f;
+        for (int team : kTeams) {
+            if (!ok) break;
+            ok = run(TCI_RRLU_GRID, team, 1, &warm) && run(TCI_RRLU_GRID, team, 1, &ms);
+            if (ok && (t.ms_grid < 0.0f || ms < t.ms_grid)) {
+                t.ms_grid = ms;
+                t.grid_team = team;
+            }
+        }
+        // the most jobs k for which k cooperative runs beat one streaming
+        // launch of k (a launch is never faster than its slowest job)
+        for (int k = 1; ok && k <= 64; k *= 2) {
+            if ((double)k * t.ms_grid < t.ms_stream) {
+                t.grid_max_jobs = k;
+                continue;
+            }
+            ok = run(TCI_RRLU_STREAM, 0, k, &ms);
+            if (!ok || (double)k * t.ms_grid >= ms) break;
+            t.grid_max_jobs = k;
+        }
+        (void)hipFree(mem);
+        mem = nullptr;
+        if (!ok) break;
+        out.push_back(t);
+    }
+    if (mem) (void)hipFree(mem);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    ctx->rrlu_algo = algo0;
+    ctx->grid_team = 0;
+    if (st != TCI_OK) {
+        ctx->tune = tuned0;
+        return st;
+    }
+    if (he != hipSuccess) {
+        ctx->tune = tuned0;
+        return set_err(TCI_ERR_DEVICE, "rrlu autotune: %s", hipGetErrorString(he));
+    }
+    ctx->tune = out;
+    for (size_t q = 0; q < out.size() && (int32_t)q < cap; ++q) table[q] = out[q];
+    *count = (int32_t)out.size();
+    return TCI_OK;
+}
+
+tci_status tci_ctx_set_rrlu_tuning(tci_ctx *ctx, const tci_rrlu_tune *table, int32_t n)
+{
+    if (!ctx || n < 0 || (n > 0 && !table)) return set_err(TCI_ERR_ARG, "bad arguments");
+    for (int32_t q = 0; q < n; ++q) {
+        const tci_rrlu_tune &t = table[q];
+        if (t.side < 1 || (q > 0 && t.side <= table[q - 1].side) || t.grid_max_jobs < 0 ||
+            (t.grid_team != 256 && t.grid_team != 512 && t.grid_team != 1024) || (t.tile && t.side > kTileSide))
+            return set_err(TCI_ERR_ARG, "tuning entry %d: sides ascending, team 256/512/1024, tile only up to 128",
+                           (int)q);
+    }
+    ctx->tune.assign(table, table + n);
     return TCI_OK;
 }
 

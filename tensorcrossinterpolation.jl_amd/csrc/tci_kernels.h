@@ -74,11 +74,14 @@ hipError_t launch_rrlu_tile(tci_lu_job *jobs_device, const int32_t *idx, int64_t
 // on entry
 constexpr int kGridMaxSide = 8192;
 size_t rrlu_grid_lds(int64_t m, int64_t n);
-// the cooperative grid size for that LDS (one workgroup per CU), 0 if none fits
-int rrlu_grid_blocks(int device, size_t lds);
+// team: 256, 512 or 1024 threads (anything else: the default,
+// TCI_RRLU_GRID_TEAM or 1024); the cooperative grid size for that LDS (one
+// workgroup per CU), 0 if none fits
+int rrlu_grid_default_team();
+int rrlu_grid_blocks(int device, size_t lds, int team);
 size_t rrlu_grid_cand_bytes(int blocks);
 hipError_t launch_rrlu_grid(tci_lu_job *job, int64_t m, int64_t n, double *out, void *cand, int32_t *flags,
-                            int blocks, hipStream_t st);
+                            int blocks, int team, hipStream_t st);
 // C = A·B with each sum in ascending k (M×K · K×N, column-major)
 hipError_t launch_gemm(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
                        double *Cm, int64_t ldc, hipStream_t st);

@@ -328,28 +328,34 @@ static const void *grid_kernel(int team)
     }
 }
 
-int rrlu_grid_blocks(int device, size_t lds)
+int rrlu_grid_default_team() { return grid_team(); }
+
+static int valid_team(int team) { return (team == 256 || team == 512 || team == 1024) ? team : grid_team(); }
+
+int rrlu_grid_blocks(int device, size_t lds, int team)
 {
+    team = valid_team(team);
     int ncu = 0, per = 0;
-    const void *k = grid_kernel(grid_team());
+    const void *k = grid_kernel(team);
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
     if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, grid_team(), lds) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, team, lds) != hipSuccess) return 0;
     return per < 1 ? 0 : ncu;   // one workgroup per CU
 }
 
 size_t rrlu_grid_cand_bytes(int blocks) { return sizeof(GridCand) * 2 * (size_t)blocks; }
 
 hipError_t launch_rrlu_grid(tci_lu_job *job, int64_t m, int64_t n, double *out, void *cand, int32_t *flags,
-                            int blocks, hipStream_t st)
+                            int blocks, int team, hipStream_t st)
 {
     if (m < 1 || n < 1 || m > kGridMaxSide || n > kGridMaxSide || blocks < 1) return hipErrorInvalidValue;
+    team = valid_team(team);
     const size_t lds = rrlu_grid_lds(m, n);
-    const void *k = grid_kernel(grid_team());
+    const void *k = grid_kernel(team);
     hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     void *args[] = {(void *)&job, (void *)&out, (void *)&cand, (void *)&flags};
-    return hipLaunchCooperativeKernel(k, dim3((unsigned)blocks), dim3(grid_team()), args, (unsigned)lds, st);
+    return hipLaunchCooperativeKernel(k, dim3((unsigned)blocks), dim3(team), args, (unsigned)lds, st);
 }
 
 }  // namespace tci
