@@ -178,6 +178,19 @@ def sample_error(tci, spec, npts=100_000, seed=12345):
             "ranks": list(map(int, t.ranks))}
 
 
+def tune_team(tci, ctx, holder, batch):
+    """The solve team for this workload (tci_ctx_autotune_solve: each team
+    timed on the batch, outside the timed region); later batches of this
+    size use it until the table is reset."""
+    lib = tci._abi.lib()
+    ms = (C.c_float * 3)()
+    team = C.c_int32()
+    if lib.tci_ctx_autotune_solve(ctx.handle, C.byref(holder.st), int(batch), ms, C.byref(team)):
+        raise RuntimeError(lib.tci_last_error().decode())
+    return {"team": team.value, "kernel_ms_by_team": {"256": round(ms[0], 3), "512": round(ms[1], 3),
+                                                       "1024": round(ms[2], 3)}}
+
+
 def solve_config(tci, ctx, spec, batch, steps, plain_reps, cpu, cpu_seconds, cpu_reps):
     """Batched and plain GPU rates of one solve configuration, its sample
     error, and the oracle's single-thread latency and multi-thread rate."""
@@ -185,6 +198,7 @@ def solve_config(tci, ctx, spec, batch, steps, plain_reps, cpu, cpu_seconds, cpu
     holder = tci._abi.ProblemHolder(ld, kind, params, ip, piv, **opts)
     arr = (tci._abi.tci_problem * batch)(*([holder.st] * batch))
     one = (tci._abi.tci_problem * 1)(holder.st)
+    tuned = tune_team(tci, ctx, holder, batch)
     run_batch(tci, ctx, one, 1)
     plain = [run_batch(tci, ctx, one, 1)[0] for _ in range(plain_reps)]
     run_batch(tci, ctx, arr, batch)
@@ -201,7 +215,8 @@ def solve_config(tci, ctx, spec, batch, steps, plain_reps, cpu, cpu_seconds, cpu
            "plain": {"solves_per_s": round(1.0 / statistics.median(plain), 2),
                      "latency_ms_p50": round(statistics.median(plain) * 1e3, 3)},
            "batched_vs_plain": round(value * statistics.median(plain), 1), "ranks": ranks,
-           "error": sample_error(tci, spec)}
+           "solve_team": tuned, "error": sample_error(tci, spec)}
+    ctx.set_solve_tuning([])
     if cpu_seconds > 0:
         O = _oracle()
         of = O.Functor(kind, params, ip)
@@ -568,6 +583,7 @@ def main():
     holder = tci._abi.ProblemHolder([10] * 8, tci._abi.F_LORENTZ_INT, [1.0], tolerance=1e-8)
     arr = (tci._abi.tci_problem * B)(*([holder.st] * B))
     one = (tci._abi.tci_problem * 1)(holder.st)
+    tuned = tune_team(tci, ctx, holder, B)
     for _ in range(args.warmup):
         run_batch(tci, ctx, arr, B)
     run_batch(tci, ctx, one, 1)
@@ -668,6 +684,8 @@ def main():
                                          f"nproc {cpu['nproc']})",
                                "latency_ms_1thread": round(lat1 * 1e3, 3)}
         out["vs_cpu"] = round(value / cps, 1)
+    out["solve_team"] = tuned
+    ctx.set_solve_tuning([])
     if args.workload == "all":
         out["rrlu_batched"] = rrlu_section(args, tci, ctx, cpu)
         out["rrlu_c4_bond"] = rrlu_bond_section(args, tci, ctx, cpu)

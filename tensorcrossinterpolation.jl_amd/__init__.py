@@ -107,6 +107,25 @@ class Context:
                 setattr(tab[q], k, row.get(k, 0))
         _check(_abi.lib().tci_ctx_set_rrlu_tuning(self._h, tab, len(table)))
 
+    def autotune_solve(self, problem, batch=256):
+        """Time `batch` copies of problem = (f, localdims, initialpivots,
+        kwargs) with 256, 512 and 1024 threads per solve and keep the fastest
+        for batches at least as large (tci_ctx_autotune_solve).  Returns
+        (team, [ms at 256, 512, 1024])."""
+        h = _holder(*problem)
+        ms = (C.c_float * 3)()
+        team = C.c_int32()
+        _check(_abi.lib().tci_ctx_autotune_solve(self._h, C.byref(h.st), int(batch), ms, C.byref(team)))
+        return team.value, list(ms)
+
+    def set_solve_tuning(self, entries):
+        """entries: [(min_mrcap, team)] (tci_ctx_set_solve_tuning); [] gives
+        every batch the default team."""
+        n = len(entries)
+        keys = (C.c_int32 * max(n, 1))(*[int(k) for k, _ in entries])
+        teams = (C.c_int32 * max(n, 1))(*[int(t) for _, t in entries])
+        _check(_abi.lib().tci_ctx_set_solve_tuning(self._h, keys, teams, n))
+
     def close(self):
         if self._h:
             _abi.lib().tci_ctx_destroy(self._h)

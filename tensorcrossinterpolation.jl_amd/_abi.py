@@ -221,6 +221,8 @@ def lib():
     L.tci_ctx_set_rrlu_algo.argtypes = [vp, C.c_int32]
     L.tci_ctx_autotune_rrlu.argtypes = [vp, P(tci_rrlu_tune), C.c_int32, P(C.c_int32)]
     L.tci_ctx_set_rrlu_tuning.argtypes = [vp, P(tci_rrlu_tune), C.c_int32]
+    L.tci_ctx_autotune_solve.argtypes = [vp, P(tci_problem), C.c_int32, P(C.c_float), P(C.c_int32)]
+    L.tci_ctx_set_solve_tuning.argtypes = [vp, P(C.c_int32), P(C.c_int32), C.c_int32]
     L.tci_batch_cachestats.argtypes = [vp, C.c_int32, P(C.c_int64), P(C.c_int64), P(C.c_int64)]
     L.tci_batch_cachedata.argtypes = [vp, C.c_int32, C.c_int64, P(C.c_uint8), P(C.c_double), P(C.c_int64)]
     L.tci_rrlu_batched_noalloc_f64.argtypes = [vp, vp, C.c_int64]
@@ -252,7 +254,7 @@ EXPORTED_SYMBOLS = [
     "tci_batch_history", "tci_batch_linkdims", "tci_batch_indexset", "tci_batch_sitetensor",
     "tci_batch_pivoterrors", "tci_batch_bonderrors", "tci_batch_free", "tci_batch_elapsed_ms",
     "tci_batch_profile", "tci_capacity_plan", "tci_tt_evaluate_f64", "tci_batch_evaluate",
-    "tci_ctx_set_rrlu_algo", "tci_ctx_autotune_rrlu", "tci_ctx_set_rrlu_tuning", "tci_rrlu_batched_noalloc_f64", "tci_batch_cachestats", "tci_batch_cachedata",
+    "tci_ctx_set_rrlu_algo", "tci_ctx_autotune_rrlu", "tci_ctx_set_rrlu_tuning", "tci_ctx_autotune_solve", "tci_ctx_set_solve_tuning", "tci_rrlu_batched_noalloc_f64", "tci_batch_cachestats", "tci_batch_cachedata",
     "tci_gemm_f64", "tci_tt_compress_f64", "tci_tt_sweep_f64", "tci_tt_floatingzone_f64",
     "tci_server_start", "tci_server_submit", "tci_server_poll", "tci_server_stop",
 ]

@@ -69,8 +69,11 @@ struct tci_ctx {
     uint64_t used;
     std::vector<PoolBuf> pool;          // cached device buffers of finished batches
     uint64_t pooled = 0;                // their bytes
-    int lds_wg;         // LDS bytes (static + dynamic) a solve workgroup may use
-    int team;           // threads per solve workgroup
+    int lds_wg;         // LDS bytes (static + dynamic) a solve workgroup of the default team may use
+    int team;           // threads per solve workgroup (the default team)
+    int lds_cu = 0, lds_blk = 0, wg_env = 0;   // LDS per CU and per workgroup; TCI_WG_PER_CU (0: unset)
+    // measured solve teams (tci_ctx_autotune_solve): (smallest largest-mrcap, team), ascending
+    std::vector<std::pair<int32_t, int32_t>> solve_tune;
     int pi_lds_kb;      // LDS budget for Pi matrices (KiB)
     int rrlu_algo = TCI_RRLU_AUTO;   // factorization kernel of the standalone rrLU
     std::vector<tci_rrlu_tune> tune; // measured AUTO choices (tci_ctx_autotune_rrlu), by side
@@ -83,6 +86,27 @@ static hipError_t ctx_download(tci_ctx *ctx, void *dst, const void *src, size_t 
 {
     hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->dl);
     return e == hipSuccess ? hipStreamSynchronize(ctx->dl) : e;
+}
+
+// LDS bytes (static + dynamic) a solve workgroup of `team` threads may use:
+// the CU's LDS shared by the workgroups that should be resident together
+// (TCI_WG_PER_CU; two of 256 threads, else one), at most the per-workgroup
+// limit; the kernel's static LDS is subtracted at launch
+static int lds_for_team(const tci_ctx *c, int team)
+{
+    const int wgs = c->wg_env > 0 ? c->wg_env : (team == 256 ? 2 : 1);
+    const int per = (c->lds_cu > 0 ? c->lds_cu : c->lds_blk) / wgs;
+    return per > c->lds_blk ? c->lds_blk : per;
+}
+
+// the team of a batch whose largest Pi-row capacity is max_mr: the last
+// tuned entry at or below it, the context's default team without one
+static int solve_team_for(const tci_ctx *c, int max_mr)
+{
+    int team = c->team;
+    for (const auto &e : c->solve_tune)
+        if (e.first <= max_mr) team = e.second;
+    return team;
 }
 
 // Launch shape of the solve kernel: a team of `team` threads per solve; LDS
@@ -300,11 +324,10 @@ tci_status tci_ctx_create(tci_ctx **out, int32_t device, uint64_t hbm_budget_byt
     c->team = ev ? atoi(ev) : 256;
     if (c->team != 256 && c->team != 512 && c->team != 1024) c->team = 256;
     ev = getenv("TCI_WG_PER_CU");
-    int wgs = ev ? atoi(ev) : (c->team == 256 ? 2 : 1);
-    if (wgs < 1) wgs = 1;
-    int per_wg = (lds_cu > 0 ? lds_cu : lds_blk) / wgs;
-    if (per_wg > lds_blk) per_wg = lds_blk;
-    c->lds_wg = per_wg;
+    c->wg_env = ev && atoi(ev) > 0 ? atoi(ev) : 0;
+    c->lds_cu = lds_cu;
+    c->lds_blk = lds_blk;
+    c->lds_wg = lds_for_team(c, c->team);
     ev = getenv("TCI_PI_LDS_KB");
     c->pi_lds_kb = ev ? atoi(ev) : 1 << 20;
     *out = c;
@@ -1467,12 +1490,75 @@ tci_status tci_solve_submit(tci_ctx *ctx, int32_t count, const tci_problem *prob
     if ((e = hipEventCreate(&b->ev0)) != hipSuccess) return fail(e, "hipEventCreate");
     if ((e = hipEventCreate(&b->ev1)) != hipSuccess) return fail(e, "hipEventCreate");
     if ((e = hipEventRecord(b->ev0, st)) != hipSuccess) return fail(e, "hipEventRecord");
-    const int stat = solve_static_lds(ctx->team, sh.fkind);
+    const int team = solve_team_for(ctx, sh.max_mr);
+    const int stat = solve_static_lds(team, sh.fkind);
     if (stat < 0) return fail(hipErrorInvalidValue, "solve kernel attributes");
-    const LdsPlan plan = make_plan(ctx, (int64_t)ctx->lds_wg - stat, sh.max_mr, sh.max_tab, sh.max_sets, sh.max_lut);
-    if ((e = launch_solve(b->d_desc, count, ctx->team, sh.fkind, plan, st)) != hipSuccess) return fail(e, "solve launch");
+    const LdsPlan plan = make_plan(ctx, (int64_t)lds_for_team(ctx, team) - stat, sh.max_mr, sh.max_tab, sh.max_sets,
+                                   sh.max_lut);
+    if ((e = launch_solve(b->d_desc, count, team, sh.fkind, plan, st)) != hipSuccess) return fail(e, "solve launch");
     if ((e = hipEventRecord(b->ev1, st)) != hipSuccess) return fail(e, "hipEventRecord");
     *out = b;
+    return TCI_OK;
+}
+
+// ---- the solve team's autotuner (tci_ctx_autotune_solve)
+static void solve_tune_put(tci_ctx *ctx, int32_t key, int32_t team)
+{
+    auto &v = ctx->solve_tune;
+    for (auto &e : v)
+        if (e.first == key) {
+            e.second = team;
+            return;
+        }
+    v.emplace_back(key, team);
+    std::sort(v.begin(), v.end());
+}
+
+tci_status tci_ctx_autotune_solve(tci_ctx *ctx, const tci_problem *sample, int32_t batch, float *ms3, int32_t *team_out)
+{
+    if (!ctx || !sample || batch < 1 || batch > 65536) return set_err(TCI_ERR_ARG, "bad arguments");
+    SolveCaps c;
+    std::string msg;
+    const int chk = check_problem(*sample, c, &msg);
+    if (chk != TCI_OK) return set_err((tci_status)chk, "sample: %s", msg.c_str());
+    static const int kTeams[3] = {256, 512, 1024};
+    std::vector<tci_problem> ps((size_t)batch, *sample);
+    const auto saved = ctx->solve_tune;
+    float best = -1.0f;
+    int bt = ctx->team;
+    tci_status st = TCI_OK;
+    for (int q = 0; q < 3 && st == TCI_OK; ++q) {
+        ctx->solve_tune.assign(1, std::make_pair(0, kTeams[q]));   // every batch on this team
+        float ms = 0.0f;
+        for (int rep = 0; rep < 2 && st == TCI_OK; ++rep) {   // a warm-up launch, then the timed one
+            tci_batch *b = nullptr;
+            st = tci_solve_submit(ctx, batch, ps.data(), &b);
+            if (st == TCI_OK) st = tci_batch_wait(b);
+            if (st == TCI_OK) st = tci_batch_elapsed_ms(b, &ms);
+            if (b) (void)tci_batch_free(b);
+        }
+        if (st != TCI_OK) break;
+        if (ms3) ms3[q] = ms;
+        if (best < 0.0f || ms < best) {
+            best = ms;
+            bt = kTeams[q];
+        }
+    }
+    ctx->solve_tune = saved;
+    if (st != TCI_OK) return st;
+    solve_tune_put(ctx, c.mrcap, bt);
+    if (team_out) *team_out = bt;
+    return TCI_OK;
+}
+
+tci_status tci_ctx_set_solve_tuning(tci_ctx *ctx, const int32_t *min_mrcap, const int32_t *team, int32_t n)
+{
+    if (!ctx || n < 0 || (n > 0 && (!min_mrcap || !team))) return set_err(TCI_ERR_ARG, "bad arguments");
+    for (int32_t q = 0; q < n; ++q)
+        if (min_mrcap[q] < 0 || (team[q] != 256 && team[q] != 512 && team[q] != 1024))
+            return set_err(TCI_ERR_ARG, "entry %d: mrcap >= 0, team 256, 512 or 1024", (int)q);
+    ctx->solve_tune.clear();
+    for (int32_t q = 0; q < n; ++q) solve_tune_put(ctx, min_mrcap[q], team[q]);
     return TCI_OK;
 }
 
