@@ -518,9 +518,10 @@ def served_section(args, tci, ctx, cpu):
 # ------------------------------------------------------------------ main
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["all", "solve", "rrlu", "bond", "served"], default="all",
+    ap.add_argument("--workload", choices=["all", "solve", "rrlu", "bond", "served", "c3", "c4"], default="all",
                     help="all: the headline line with every sub-object; solve: the headline only; "
-                         "rrlu: the batched rrLU only; bond: the C4 bond rrLU; served: the C5 stream")
+                         "rrlu: the batched rrLU only; bond: the C4 bond rrLU; served: the C5 stream; "
+                         "c3, c4: one solve configuration")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -566,6 +567,13 @@ def main():
         return
     if args.workload == "served":
         print(json.dumps(served_section(args, tci, ctx, cpu)), flush=True)
+        return
+    if args.workload in ("c3", "c4"):
+        P = _problems()
+        spec, batch, cs = ((P.c3_spec(5), args.c3_batch, args.cpu_seconds / 2) if args.workload == "c3"
+                           else (P.c4_spec(5), args.c4_batch, args.cpu_seconds))
+        steps, plain = (3, 5) if args.workload == "c3" else (2, 3)
+        print(json.dumps(solve_config(tci, ctx, spec, batch, steps, plain, cpu, cs, 3)), flush=True)
         return
     if args.workload == "rrlu":
         r = rrlu_section(args, tci, ctx, cpu)
