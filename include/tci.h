@@ -456,6 +456,12 @@ tci_status tci_server_start(tci_ctx *ctx, const tci_problem *shape, int32_t slot
                             tci_server **out);
 tci_status tci_server_submit(tci_server *s, const tci_problem *p, tci_batch **out);
 tci_status tci_server_poll(tci_server *s, tci_batch **out, int32_t cap, int32_t *count);
+/* tci_server_submit for problems[0..count) in order, published together:
+ * out[q] receives each request; *accepted = how many were taken.  A full
+ * ring or budget ends the burst early with TCI_OK (retire, then submit the
+ * rest); any other failure returns its status. */
+tci_status tci_server_submit_many(tci_server *s, int32_t count, const tci_problem *problems, tci_batch **out,
+                                  int32_t *accepted);
 tci_status tci_server_stop(tci_server *s);
 /* Device time of the batch's solve kernel (HIP events), milliseconds. */
 tci_status tci_batch_elapsed_ms(tci_batch *b, float *ms);

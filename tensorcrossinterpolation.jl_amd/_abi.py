@@ -240,6 +240,7 @@ def lib():
     L.tci_server_submit.argtypes = [vp, P(tci_problem), P(vp)]
     L.tci_server_stop.argtypes = [vp]
     L.tci_server_poll.argtypes = [vp, P(vp), C.c_int32, P(C.c_int32)]
+    L.tci_server_submit_many.argtypes = [vp, C.c_int32, P(tci_problem), P(vp), P(C.c_int32)]
     L.tci_tt_floatingzone_f64.argtypes = [vp, P(tci_functor), C.c_int32, P(C.c_int64), P(C.c_int32), P(C.c_double),
                                           C.c_int64, P(C.c_uint8), P(C.c_double), C.c_int64, C.c_double]
     _LIB = L
@@ -256,5 +257,6 @@ EXPORTED_SYMBOLS = [
     "tci_batch_profile", "tci_capacity_plan", "tci_tt_evaluate_f64", "tci_batch_evaluate",
     "tci_ctx_set_rrlu_algo", "tci_ctx_autotune_rrlu", "tci_ctx_set_rrlu_tuning", "tci_ctx_autotune_solve", "tci_ctx_set_solve_tuning", "tci_rrlu_batched_noalloc_f64", "tci_batch_cachestats", "tci_batch_cachedata",
     "tci_gemm_f64", "tci_tt_compress_f64", "tci_tt_sweep_f64", "tci_tt_floatingzone_f64",
-    "tci_server_start", "tci_server_submit", "tci_server_poll", "tci_server_stop",
+    "tci_server_start", "tci_server_submit", "tci_server_submit_many", "tci_server_poll",
+    "tci_server_stop",
 ]

@@ -2248,9 +2248,9 @@ tci_status tci_server_start(tci_ctx *ctx, const tci_problem *shape, int32_t slot
 
 static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-tci_status tci_server_submit(tci_server *s, const tci_problem *p, tci_batch **out)
+// one request staged and its ring entry written, not yet published
+static tci_status server_stage(tci_server *s, const tci_problem *p, tci_batch **out)
 {
-    if (!s || !p || !out) return set_err(TCI_ERR_ARG, "bad arguments");
     *out = nullptr;
     const size_t k = (size_t)(s->next % (uint64_t)s->cap);
     if (s->slot[k]) return set_err(TCI_ERR_OOM, "server queue full: free finished requests first");
@@ -2313,10 +2313,37 @@ tci_status tci_server_submit(tci_server *s, const tci_problem *p, tci_batch **ou
     s->slot[k] = b;
     s->pending.push_back(b);
     s->next += 1;
-    __atomic_store_n(&s->ctl->tail, s->next, __ATOMIC_RELEASE);   // publish
-    st = serve_ensure(s);
     *out = b;
-    return st;
+    return TCI_OK;
+}
+
+tci_status tci_server_submit(tci_server *s, const tci_problem *p, tci_batch **out)
+{
+    if (!s || !p || !out) return set_err(TCI_ERR_ARG, "bad arguments");
+    const tci_status st = server_stage(s, p, out);
+    if (st != TCI_OK) return st;
+    __atomic_store_n(&s->ctl->tail, s->next, __ATOMIC_RELEASE);   // publish
+    return serve_ensure(s);
+}
+
+tci_status tci_server_submit_many(tci_server *s, int32_t count, const tci_problem *problems, tci_batch **out,
+                                  int32_t *accepted)
+{
+    if (!s || !accepted || count < 0 || (count > 0 && (!problems || !out))) return set_err(TCI_ERR_ARG, "bad arguments");
+    *accepted = 0;
+    tci_status st = TCI_OK;
+    for (int32_t q = 0; q < count; ++q) {
+        st = server_stage(s, problems + q, out + q);
+        if (st != TCI_OK) break;
+        *accepted = q + 1;
+    }
+    if (*accepted > 0) {
+        __atomic_store_n(&s->ctl->tail, s->next, __ATOMIC_RELEASE);   // publish them all
+        const tci_status e = serve_ensure(s);
+        if (e != TCI_OK) return e;
+    }
+    // a full ring or budget stops the burst without an error: retire and go on
+    return st == TCI_ERR_OOM ? TCI_OK : st;
 }
 
 tci_status tci_server_poll(tci_server *s, tci_batch **out, int32_t cap, int32_t *count)

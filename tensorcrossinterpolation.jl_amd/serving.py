@@ -210,6 +210,7 @@ class SlotServer:
         from . import _holder
         self.ctx = ctx
         h = _holder(*shape)
+        self.slots = int(slots)
         self.h = C.c_void_p()
         st = _abi.lib().tci_server_start(ctx.handle, C.byref(h.st), int(slots), int(idle_ms), C.byref(self.h))
         if st:
@@ -228,13 +229,14 @@ class SlotServer:
         except Exception:
             pass
 
-    def run(self, requests, arrivals, keep_tci=False):
+    def run(self, requests, arrivals, keep_tci=False, keep_history=True):
         """requests[i] = (f, localdims, initialpivots_or_None, kwargs); arrivals
-        sorted, seconds from the start.  Each request is submitted when it
-        arrives; a full queue or an exhausted HBM budget holds new ones back
-        until results are retired.  A request that outgrew its capacity with
-        chi_cap unset is submitted again at twice the capacity.  Returns
-        (results, ServeStats)."""
+        sorted, seconds from the start.  The requests that have arrived are
+        submitted together (tci_server_submit_many); a full queue or an
+        exhausted HBM budget holds the rest back until results are retired.
+        A request that outgrew its capacity with chi_cap unset is submitted
+        again at twice the capacity.  keep_history=False skips the per-request
+        rank/error history.  Returns (results, ServeStats)."""
         from . import TensorCI2, _holder, capacity_plan
         lib = _abi.lib()
         n = len(requests)
@@ -242,52 +244,65 @@ class SlotServer:
         inflight = {}                       # batch handle value -> (request id, batch handle, holder)
         retry = []
         retry_holder = {}                   # request id -> holder at the larger capacity
+        hcache = {}                         # id(request) -> holder: repeated requests share one
         nxt = 0
         nretries = 0
         oom_waits = 0
         pcap = 4096
         polled = (C.c_void_p * pcap)()
         npolled = C.c_int32()
+        accepted = C.c_int32()
         host = {"submit": 0.0, "poll": 0.0, "retire": 0.0}
         clock = time.perf_counter
         t0 = time.perf_counter()
+
+        def fail_all(st, msg):
+            for _, b2, _ in inflight.values():
+                lib.tci_batch_free(b2)
+            raise ServeError(st, msg)
+
         while nxt < n or inflight or retry:
             now = time.perf_counter() - t0
             progressed = False
-            blocked = False
             ts = clock()
-            while (retry or (nxt < n and arrivals[nxt] <= now)) and not blocked:
-                i = retry[0] if retry else nxt
-                f, ld, piv, kw = requests[i]
-                h = retry_holder.pop(i) if retry else _holder(f, ld, piv, kw)
-                b = C.c_void_p()
-                st = lib.tci_server_submit(self.h, C.byref(h.st), C.byref(b))
-                if st == 0:
-                    inflight[b.value] = (i, b, h)
-                    if retry:
-                        retry.pop(0)
-                    else:
-                        nxt += 1
-                    progressed = True
-                elif st == 11 and inflight:   # queue or HBM budget full: retire first
-                    if retry:
-                        retry_holder[i] = h
+            ids = list(retry)
+            hs = [retry_holder[i] for i in retry]
+            j = nxt
+            room = min(4096, self.slots - len(inflight))   # the ring's free entries at most
+            while j < n and arrivals[j] <= now and len(ids) < room:
+                key = id(requests[j])
+                h = hcache.get(key)
+                if h is None:
+                    h = hcache[key] = _holder(*requests[j])
+                ids.append(j)
+                hs.append(h)
+                j += 1
+            if ids:
+                arr = (_abi.tci_problem * len(ids))(*[h.st for h in hs])
+                outb = (C.c_void_p * len(ids))()
+                st = lib.tci_server_submit_many(self.h, len(ids), arr, outb, C.byref(accepted))
+                if st:
+                    fail_all(st, f"submit failed: {lib.tci_last_error().decode()}")
+                acc = accepted.value
+                for q in range(acc):
+                    inflight[outb[q]] = (ids[q], C.c_void_p(outb[q]), hs[q])
+                taken = min(acc, len(retry))
+                for i in retry[:taken]:
+                    del retry_holder[i]
+                retry = retry[taken:]
+                nxt += max(0, acc - taken)
+                progressed = acc > 0
+                if acc < len(ids):
+                    if not inflight:
+                        fail_all(11, f"request {ids[acc]} does not fit the queue or the HBM budget alone")
                     oom_waits += 1
-                    blocked = True
-                else:
-                    for _, b2, _ in inflight.values():
-                        lib.tci_batch_free(b2)
-                    raise ServeError(st, f"submit of request {i} failed: {lib.tci_last_error().decode()}")
             tp = clock()
             host["submit"] += tp - ts
             st = lib.tci_server_poll(self.h, polled, pcap, C.byref(npolled))
             tr = clock()
             host["poll"] += tr - tp
             if st:
-                msg = lib.tci_last_error().decode()
-                for _, b2, _ in inflight.values():
-                    lib.tci_batch_free(b2)
-                raise ServeError(st, f"poll failed: {msg}")
+                fail_all(st, f"poll failed: {lib.tci_last_error().decode()}")
             done = time.perf_counter() - t0
             for q in range(npolled.value):
                 i, b, h = inflight.pop(polled[q])
@@ -304,13 +319,16 @@ class SlotServer:
                         nretries += 1
                         lib.tci_batch_free(b)
                         continue
-                m = max(r.niter, 1)
-                ranks = np.zeros(m, np.int64)
-                errs = np.zeros(m)
-                lib.tci_batch_history(b, 0, _abi.i64ptr(ranks), _abi.dptr(errs), None)
+                if keep_history:
+                    m = max(r.niter, 1)
+                    ranks = np.zeros(m, np.int64)
+                    errs = np.zeros(m)
+                    lib.tci_batch_history(b, 0, _abi.i64ptr(ranks), _abi.dptr(errs), None)
+                    ranks, errs = ranks[:r.niter], errs[:r.niter]
+                else:
+                    ranks, errs = None, None
                 t = TensorCI2(b, 0, requests[i][1]) if keep_tci else None
-                results[i] = ServedResult(r.status, ranks[:r.niter], errs[:r.niter], r.maxsamplevalue,
-                                          float(arrivals[i]), done, t)
+                results[i] = ServedResult(r.status, ranks, errs, r.maxsamplevalue, float(arrivals[i]), done, t)
                 lib.tci_batch_free(b)
             host["retire"] += clock() - tr
             if not progressed:
