@@ -130,3 +130,44 @@ def test_slot_server_ring_wraps_and_reuses_blocks(tci, ctx):
     for i, r in enumerate(reqs):
         (alone,) = tci.crossinterpolate2_batch([r], ctx=ctx)
         _same(res[i].tci, alone, f"request {i}")
+
+
+def test_slot_server_submit_many_partial_bursts(tci, ctx):
+    """A burst larger than the ring is taken in part (TCI_OK, accepted < count);
+    the rest goes in after retirements; every result equals its solve alone."""
+    lib = tci._abi.lib()
+    reqs = [_req(tci, s) for s in served_mix(10, seed=31)]
+    holders = [tci._holder(*q) for q in reqs]
+    shape = tci._holder(*_shape(tci))
+    h = C.c_void_p()
+    assert lib.tci_server_start(ctx.handle, C.byref(shape.st), 4, 50, C.byref(h)) == 0
+    got = {}
+    try:
+        nxt = 0
+        out = (C.c_void_p * 10)()
+        acc = C.c_int32()
+        polled = (C.c_void_p * 10)()
+        n = C.c_int32()
+        handles = {}
+        t_end = time.time() + 60
+        while len(got) < len(reqs) and time.time() < t_end:
+            if nxt < len(reqs):
+                rest = len(reqs) - nxt
+                arr = (tci._abi.tci_problem * rest)(*[hd.st for hd in holders[nxt:]])
+                assert lib.tci_server_submit_many(h, rest, arr, out, C.byref(acc)) == 0
+                assert 0 <= acc.value <= min(rest, 4)
+                for q in range(acc.value):
+                    handles[out[q]] = (nxt + q, C.c_void_p(out[q]))
+                nxt += acc.value
+            assert lib.tci_server_poll(h, polled, 10, C.byref(n)) == 0
+            for q in range(n.value):
+                k, b = handles.pop(polled[q])
+                got[k] = tci.TensorCI2(b, 0, reqs[k][1])
+                lib.tci_batch_free(b)
+            time.sleep(1e-3)
+        assert sorted(got) == list(range(len(reqs)))
+    finally:
+        assert lib.tci_server_stop(h) == 0
+    for k, r in enumerate(reqs):
+        (alone,) = tci.crossinterpolate2_batch([r], ctx=ctx)
+        _same(got[k], alone, f"burst {k}")
