@@ -171,3 +171,27 @@ def test_slot_server_submit_many_partial_bursts(tci, ctx):
     for k, r in enumerate(reqs):
         (alone,) = tci.crossinterpolate2_batch([r], ctx=ctx)
         _same(got[k], alone, f"burst {k}")
+
+
+def test_slot_server_capacity_retry(tci, ctx):
+    """A request whose rank outgrows its default set capacity (a random 10^4
+    table, middle rank 100) ends with TCI_ERR_CAPACITY on the device and is
+    submitted again at twice the capacity; the served result equals
+    crossinterpolate2 with its own retries, and the stream's other requests
+    are unaffected."""
+    serving = _serving()
+    rng = np.random.default_rng(7)
+    table = (tci.TableFunctor(rng.random((10, 10, 10, 10))), [10] * 4, None, dict(tolerance=1e-12))
+    reqs = [table] + [_req(tci, s) for s in served_mix(5, seed=37)]
+    srv = serving.SlotServer(ctx, _shape(tci), slots=16, idle_ms=20)
+    try:
+        res, stats = srv.run(reqs, np.zeros(len(reqs)), keep_tci=True)
+    finally:
+        srv.close()
+    assert stats.retries >= 1
+    assert res[0].status == 0 and max(res[0].tci.linkdims()) > 64
+    t, _, _ = tci.crossinterpolate2(float, *table[:1], table[1], **table[3])
+    _same(res[0].tci, t, "table")
+    for i, r in enumerate(reqs[1:], 1):
+        (alone,) = tci.crossinterpolate2_batch([r], ctx=ctx)
+        _same(res[i].tci, alone, f"request {i}")
