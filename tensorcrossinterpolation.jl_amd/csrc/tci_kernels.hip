@@ -166,9 +166,11 @@ __global__ __launch_bounds__(NT, WPE) void tci_serve_kernel(const ServeSlot *__r
                         break;
                     }
                 }
-                // back off while idle: about 0.9 us, doubling to about 7 us
+                // back off while idle: about 0.9 us, doubling to about 55 us,
+                // so a mostly idle grid does not keep the memory channel of
+                // these words busy under the solves still running
                 for (int k = 0; k < nap; ++k) __builtin_amdgcn_s_sleep(32);
-                if (nap < 8) nap *= 2;
+                if (nap < 64) nap *= 2;
             }
             if (go) {
                 // the ring entry and its staged image were written before
