@@ -183,12 +183,11 @@ def tune_team(tci, ctx, holder, batch):
     timed on the batch, outside the timed region); later batches of this
     size use it until the table is reset."""
     lib = tci._abi.lib()
-    ms = (C.c_float * 3)()
+    ms = (C.c_float * 4)()
     team = C.c_int32()
     if lib.tci_ctx_autotune_solve(ctx.handle, C.byref(holder.st), int(batch), ms, C.byref(team)):
         raise RuntimeError(lib.tci_last_error().decode())
-    return {"team": team.value, "kernel_ms_by_team": {"256": round(ms[0], 3), "512": round(ms[1], 3),
-                                                       "1024": round(ms[2], 3)}}
+    return {"team": team.value, "kernel_ms_by_team": {str(t): round(ms[q], 3) for q, t in enumerate((64, 256, 512, 1024))}}
 
 
 def solve_config(tci, ctx, spec, batch, steps, plain_reps, cpu, cpu_seconds, cpu_reps):

@@ -109,11 +109,11 @@ class Context:
 
     def autotune_solve(self, problem, batch=256):
         """Time `batch` copies of problem = (f, localdims, initialpivots,
-        kwargs) with 256, 512 and 1024 threads per solve and keep the fastest
-        for batches at least as large (tci_ctx_autotune_solve).  Returns
-        (team, [ms at 256, 512, 1024])."""
+        kwargs) with 64, 256, 512 and 1024 threads per solve and keep the
+        fastest for batches at least as large (tci_ctx_autotune_solve).
+        Returns (team, [ms at 64, 256, 512, 1024])."""
         h = _holder(*problem)
-        ms = (C.c_float * 3)()
+        ms = (C.c_float * 4)()
         team = C.c_int32()
         _check(_abi.lib().tci_ctx_autotune_solve(self._h, C.byref(h.st), int(batch), ms, C.byref(team)))
         return team.value, list(ms)

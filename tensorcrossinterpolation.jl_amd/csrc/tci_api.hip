@@ -90,11 +90,11 @@ static hipError_t ctx_download(tci_ctx *ctx, void *dst, const void *src, size_t 
 
 // LDS bytes (static + dynamic) a solve workgroup of `team` threads may use:
 // the CU's LDS shared by the workgroups that should be resident together
-// (TCI_WG_PER_CU; two of 256 threads, else one), at most the per-workgroup
+// (TCI_WG_PER_CU; eight one-wave teams, two of 256 threads, else one), at most the per-workgroup
 // limit; the kernel's static LDS is subtracted at launch
 static int lds_for_team(const tci_ctx *c, int team)
 {
-    const int wgs = c->wg_env > 0 ? c->wg_env : (team == 256 ? 2 : 1);
+    const int wgs = c->wg_env > 0 ? c->wg_env : (team == 256 ? 2 : team == 64 ? 8 : 1);
     const int per = (c->lds_cu > 0 ? c->lds_cu : c->lds_blk) / wgs;
     return per > c->lds_blk ? c->lds_blk : per;
 }
@@ -322,7 +322,7 @@ tci_status tci_ctx_create(tci_ctx **out, int32_t device, uint64_t hbm_budget_byt
     HIPCHK(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device));
     const char *ev = getenv("TCI_TEAM");
     c->team = ev ? atoi(ev) : 256;
-    if (c->team != 256 && c->team != 512 && c->team != 1024) c->team = 256;
+    if (c->team != 64 && c->team != 256 && c->team != 512 && c->team != 1024) c->team = 256;
     ev = getenv("TCI_WG_PER_CU");
     c->wg_env = ev && atoi(ev) > 0 ? atoi(ev) : 0;
     c->lds_cu = lds_cu;
@@ -1514,20 +1514,20 @@ static void solve_tune_put(tci_ctx *ctx, int32_t key, int32_t team)
     std::sort(v.begin(), v.end());
 }
 
-tci_status tci_ctx_autotune_solve(tci_ctx *ctx, const tci_problem *sample, int32_t batch, float *ms3, int32_t *team_out)
+tci_status tci_ctx_autotune_solve(tci_ctx *ctx, const tci_problem *sample, int32_t batch, float *ms4, int32_t *team_out)
 {
     if (!ctx || !sample || batch < 1 || batch > 65536) return set_err(TCI_ERR_ARG, "bad arguments");
     SolveCaps c;
     std::string msg;
     const int chk = check_problem(*sample, c, &msg);
     if (chk != TCI_OK) return set_err((tci_status)chk, "sample: %s", msg.c_str());
-    static const int kTeams[3] = {256, 512, 1024};
+    static const int kTeams[4] = {64, 256, 512, 1024};
     std::vector<tci_problem> ps((size_t)batch, *sample);
     const auto saved = ctx->solve_tune;
     float best = -1.0f;
     int bt = ctx->team;
     tci_status st = TCI_OK;
-    for (int q = 0; q < 3 && st == TCI_OK; ++q) {
+    for (int q = 0; q < 4 && st == TCI_OK; ++q) {
         ctx->solve_tune.assign(1, std::make_pair(0, kTeams[q]));   // every batch on this team
         float ms = 0.0f;
         for (int rep = 0; rep < 2 && st == TCI_OK; ++rep) {   // a warm-up launch, then the timed one
@@ -1538,7 +1538,7 @@ tci_status tci_ctx_autotune_solve(tci_ctx *ctx, const tci_problem *sample, int32
             if (b) (void)tci_batch_free(b);
         }
         if (st != TCI_OK) break;
-        if (ms3) ms3[q] = ms;
+        if (ms4) ms4[q] = ms;
         if (best < 0.0f || ms < best) {
             best = ms;
             bt = kTeams[q];
@@ -1555,8 +1555,8 @@ tci_status tci_ctx_set_solve_tuning(tci_ctx *ctx, const int32_t *min_mrcap, cons
 {
     if (!ctx || n < 0 || (n > 0 && (!min_mrcap || !team))) return set_err(TCI_ERR_ARG, "bad arguments");
     for (int32_t q = 0; q < n; ++q)
-        if (min_mrcap[q] < 0 || (team[q] != 256 && team[q] != 512 && team[q] != 1024))
-            return set_err(TCI_ERR_ARG, "entry %d: mrcap >= 0, team 256, 512 or 1024", (int)q);
+        if (min_mrcap[q] < 0 || (team[q] != 64 && team[q] != 256 && team[q] != 512 && team[q] != 1024))
+            return set_err(TCI_ERR_ARG, "entry %d: mrcap >= 0, team 64, 256, 512 or 1024", (int)q);
     ctx->solve_tune.clear();
     for (int32_t q = 0; q < n; ++q) solve_tune_put(ctx, min_mrcap[q], team[q]);
     return TCI_OK;

@@ -11,6 +11,7 @@ namespace tci {
 // register-tile shape per team size (tci_regtile.h): wave w, lane l holds rows
 // l + 64·i (i < RA) of columns w + NW·j (j < CB); capacity (64·RA)×(NW·CB)
 template <int NT> struct TileShape;
+template <> struct TileShape<64> { static constexpr int RA = 1, CB = 64; };     // 64×64 (one wave)
 template <> struct TileShape<256> { static constexpr int RA = 2, CB = 32; };    // 128×128
 template <> struct TileShape<512> { static constexpr int RA = 2, CB = 16; };    // 128×128
 template <> struct TileShape<1024> { static constexpr int RA = 2, CB = 8; };    // 128×128

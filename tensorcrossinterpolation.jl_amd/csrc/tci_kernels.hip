@@ -803,6 +803,9 @@ size_t lds_plan_bytes(const LdsPlan &p)
 static const void *solve_kernel_ptr(int team, int fkind)
 {
     switch (team) {
+    case 64:
+        return fkind == TCI_F_QUANTICS_GAUSS_COS ? (const void *)tci_solve_kernel<64, TCI_F_QUANTICS_GAUSS_COS, 2>
+                                                 : (const void *)tci_solve_kernel<64, 0, 2>;
     case 1024:
         return fkind == TCI_F_LORENTZ_INT ? (const void *)tci_solve_kernel<1024, TCI_F_LORENTZ_INT, 4>
                                           : (const void *)tci_solve_kernel<1024, 0, 4>;
@@ -866,6 +869,10 @@ hipError_t launch_solve(const SolveDesc *descs, int count, int team, int fkind, 
     // every functor specialized); 512 threads at 256 VGPRs, one solve per CU;
     // 1024 threads at 128 VGPRs
     switch (team) {
+    case 64:
+        if (fkind == TCI_F_QUANTICS_GAUSS_COS) launch_one<64, TCI_F_QUANTICS_GAUSS_COS, 2>(descs, count, plan, lds, st);
+        else launch_one<64, 0, 2>(descs, count, plan, lds, st);
+        break;
     case 1024:
         if (fkind == TCI_F_LORENTZ_INT) launch_one<1024, TCI_F_LORENTZ_INT, 4>(descs, count, plan, lds, st);
         else launch_one<1024, 0, 4>(descs, count, plan, lds, st);

@@ -357,7 +357,7 @@ struct TileLU {
             // latency overlaps; with half of it (two workgroups per CU) each
             // is read as its column is updated, to keep the tile in registers
             lmax = -1.0;
-            if (Team::WPE <= 2) {
+            if (Team::WPE <= 2 && CB <= 32) {
                 double yv[CB];
 #pragma unroll
                 for (int j = 0; j < CB; ++j) yv[j] = tm.tY(tm.wave(), j);
