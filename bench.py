@@ -214,7 +214,10 @@ def solve_config(tci, ctx, spec, batch, steps, plain_reps, cpu, cpu_seconds, cpu
            "plain": {"solves_per_s": round(1.0 / statistics.median(plain), 2),
                      "latency_ms_p50": round(statistics.median(plain) * 1e3, 3)},
            "batched_vs_plain": round(value * statistics.median(plain), 1), "ranks": ranks,
-           "solve_team": tuned, "error": sample_error(tci, spec)}
+           "solve_team": tuned,
+           "solve_phases_us": dict(zip(["eval", "rrlu", "sets", "fill", "gsearch", "sweep1", "total", "pinv"],
+                                       [round(v, 1) for v in ws[6][:8]])),
+           "error": sample_error(tci, spec)}
     ctx.set_solve_tuning([])
     if cpu_seconds > 0:
         O = _oracle()
