@@ -179,9 +179,9 @@ def sample_error(tci, spec, npts=100_000, seed=12345):
 
 
 def tune_team(tci, ctx, holder, batch):
-    """The solve team for this workload (tci_ctx_autotune_solve: each team
-    timed on the batch, outside the timed region); later batches of this
-    size use it until the table is reset."""
+    """The solve team for batches of this workload of at least `batch`
+    solves (tci_ctx_autotune_solve: each team timed on such a batch, outside
+    the timed region), used until the table is reset."""
     lib = tci._abi.lib()
     ms = (C.c_float * 4)()
     team = C.c_int32()
@@ -198,6 +198,7 @@ def solve_config(tci, ctx, spec, batch, steps, plain_reps, cpu, cpu_seconds, cpu
     arr = (tci._abi.tci_problem * batch)(*([holder.st] * batch))
     one = (tci._abi.tci_problem * 1)(holder.st)
     tuned = tune_team(tci, ctx, holder, batch)
+    tuned_plain = tune_team(tci, ctx, holder, 1)
     run_batch(tci, ctx, one, 1)
     plain = [run_batch(tci, ctx, one, 1)[0] for _ in range(plain_reps)]
     run_batch(tci, ctx, arr, batch)
@@ -214,7 +215,7 @@ def solve_config(tci, ctx, spec, batch, steps, plain_reps, cpu, cpu_seconds, cpu
            "plain": {"solves_per_s": round(1.0 / statistics.median(plain), 2),
                      "latency_ms_p50": round(statistics.median(plain) * 1e3, 3)},
            "batched_vs_plain": round(value * statistics.median(plain), 1), "ranks": ranks,
-           "solve_team": tuned,
+           "solve_team": tuned, "solve_team_plain": tuned_plain,
            "solve_phases_us": dict(zip(["eval", "rrlu", "sets", "fill", "gsearch", "sweep1", "total", "pinv"],
                                        [round(v, 1) for v in ws[6][:8]])),
            "error": sample_error(tci, spec)}
@@ -594,6 +595,7 @@ def main():
     arr = (tci._abi.tci_problem * B)(*([holder.st] * B))
     one = (tci._abi.tci_problem * 1)(holder.st)
     tuned = tune_team(tci, ctx, holder, B)
+    tuned_plain = tune_team(tci, ctx, holder, 1)
     for _ in range(args.warmup):
         run_batch(tci, ctx, arr, B)
     run_batch(tci, ctx, one, 1)
@@ -708,6 +710,7 @@ def main():
                                "latency_ms_1thread": round(lat1 * 1e3, 3)}
         out["vs_cpu"] = round(value / cps, 1)
     out["solve_team"] = tuned
+    out["solve_team_plain"] = tuned_plain
     ctx.set_solve_tuning([])
     if args.workload == "all":
         out["rrlu_batched"] = rrlu_section(args, tci, ctx, cpu)

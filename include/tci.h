@@ -359,18 +359,21 @@ tci_status tci_solve_submit(tci_ctx *ctx, int32_t count, const tci_problem *prob
 /* The solve engine's team autotuner (no counterpart in the reference).  A
  * batch of solves runs with 256 threads per solve by default (two solves per
  * CU); larger Pi matrices can run faster with 512 or 1024, small ones with a
- * single wave of 64 (eight solves per CU).  This times `batch` copies of
+ * single wave of 64 (eight solves per CU), and the best team for one solve
+ * alone is not the best for thousands.  This times `batch` copies of
  * `sample` with each team (a warm-up launch and a timed one, kernel time in
- * ms4[0..3] for 64, 256, 512, 1024; may be NULL), keeps the
- * fastest (*team, may be NULL) and records it for batches whose largest
- * Pi-row capacity (the `mrcap` of their largest problem) is at least the
- * sample's: a batch takes the team of the last such entry at or below its
- * own, the default team without one.  Results are the same bits with every
- * team.  tci_ctx_set_solve_tuning installs n entries (min_mrcap[i],
- * team[i]) instead (n = 0: the default team for every batch). */
+ * ms4[0..3] for 64, 256, 512, 1024; may be NULL), keeps the fastest (*team,
+ * may be NULL) and records it for batches of at least `batch` solves whose
+ * largest Pi-row capacity (the `mrcap` of their largest problem) is at least
+ * the sample's.  A batch takes the team of the entry with the largest such
+ * capacity and then the largest such batch size, the default team without
+ * one.  Results are the same bits with every team.  tci_ctx_set_solve_tuning
+ * installs n entries (min_mrcap[i], min_count[i], team[i]) instead (n = 0:
+ * the default team for every batch). */
 tci_status tci_ctx_autotune_solve(tci_ctx *ctx, const tci_problem *sample, int32_t batch, float *ms4,
                                   int32_t *team);
-tci_status tci_ctx_set_solve_tuning(tci_ctx *ctx, const int32_t *min_mrcap, const int32_t *team, int32_t n);
+tci_status tci_ctx_set_solve_tuning(tci_ctx *ctx, const int32_t *min_mrcap, const int32_t *min_count,
+                                    const int32_t *team, int32_t n);
 /* TCI_OK when finished, TCI_ERR_PENDING while running. */
 tci_status tci_batch_query(tci_batch *b);
 tci_status tci_batch_wait(tci_batch *b);

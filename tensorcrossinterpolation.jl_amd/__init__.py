@@ -110,8 +110,9 @@ class Context:
     def autotune_solve(self, problem, batch=256):
         """Time `batch` copies of problem = (f, localdims, initialpivots,
         kwargs) with 64, 256, 512 and 1024 threads per solve and keep the
-        fastest for batches at least as large (tci_ctx_autotune_solve).
-        Returns (team, [ms at 64, 256, 512, 1024])."""
+        fastest for batches of at least `batch` solves of at least this
+        problem's capacity (tci_ctx_autotune_solve).  Returns (team, [ms at
+        64, 256, 512, 1024])."""
         h = _holder(*problem)
         ms = (C.c_float * 4)()
         team = C.c_int32()
@@ -119,12 +120,13 @@ class Context:
         return team.value, list(ms)
 
     def set_solve_tuning(self, entries):
-        """entries: [(min_mrcap, team)] (tci_ctx_set_solve_tuning); [] gives
-        every batch the default team."""
+        """entries: [(min_mrcap, min_count, team)] (tci_ctx_set_solve_tuning);
+        [] gives every batch the default team."""
         n = len(entries)
-        keys = (C.c_int32 * max(n, 1))(*[int(k) for k, _ in entries])
-        teams = (C.c_int32 * max(n, 1))(*[int(t) for _, t in entries])
-        _check(_abi.lib().tci_ctx_set_solve_tuning(self._h, keys, teams, n))
+        keys = (C.c_int32 * max(n, 1))(*[int(e[0]) for e in entries])
+        counts = (C.c_int32 * max(n, 1))(*[int(e[1]) for e in entries])
+        teams = (C.c_int32 * max(n, 1))(*[int(e[2]) for e in entries])
+        _check(_abi.lib().tci_ctx_set_solve_tuning(self._h, keys, counts, teams, n))
 
     def close(self):
         if self._h:

@@ -222,7 +222,7 @@ def lib():
     L.tci_ctx_autotune_rrlu.argtypes = [vp, P(tci_rrlu_tune), C.c_int32, P(C.c_int32)]
     L.tci_ctx_set_rrlu_tuning.argtypes = [vp, P(tci_rrlu_tune), C.c_int32]
     L.tci_ctx_autotune_solve.argtypes = [vp, P(tci_problem), C.c_int32, P(C.c_float), P(C.c_int32)]
-    L.tci_ctx_set_solve_tuning.argtypes = [vp, P(C.c_int32), P(C.c_int32), C.c_int32]
+    L.tci_ctx_set_solve_tuning.argtypes = [vp, P(C.c_int32), P(C.c_int32), P(C.c_int32), C.c_int32]
     L.tci_batch_cachestats.argtypes = [vp, C.c_int32, P(C.c_int64), P(C.c_int64), P(C.c_int64)]
     L.tci_batch_cachedata.argtypes = [vp, C.c_int32, C.c_int64, P(C.c_uint8), P(C.c_double), P(C.c_int64)]
     L.tci_rrlu_batched_noalloc_f64.argtypes = [vp, vp, C.c_int64]

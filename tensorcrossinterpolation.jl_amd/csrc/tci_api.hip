@@ -72,8 +72,13 @@ struct tci_ctx {
     int lds_wg;         // LDS bytes (static + dynamic) a solve workgroup of the default team may use
     int team;           // threads per solve workgroup (the default team)
     int lds_cu = 0, lds_blk = 0, wg_env = 0;   // LDS per CU and per workgroup; TCI_WG_PER_CU (0: unset)
-    // measured solve teams (tci_ctx_autotune_solve): (smallest largest-mrcap, team), ascending
-    std::vector<std::pair<int32_t, int32_t>> solve_tune;
+    // measured solve teams (tci_ctx_autotune_solve), ascending by
+    // (smallest largest-mrcap, smallest batch)
+    struct SolveTune {
+        int32_t mrcap, count, team;
+        bool operator<(const SolveTune &o) const { return mrcap != o.mrcap ? mrcap < o.mrcap : count < o.count; }
+    };
+    std::vector<SolveTune> solve_tune;
     int pi_lds_kb;      // LDS budget for Pi matrices (KiB)
     int rrlu_algo = TCI_RRLU_AUTO;   // factorization kernel of the standalone rrLU
     std::vector<tci_rrlu_tune> tune; // measured AUTO choices (tci_ctx_autotune_rrlu), by side
@@ -99,13 +104,14 @@ static int lds_for_team(const tci_ctx *c, int team)
     return per > c->lds_blk ? c->lds_blk : per;
 }
 
-// the team of a batch whose largest Pi-row capacity is max_mr: the last
-// tuned entry at or below it, the context's default team without one
-static int solve_team_for(const tci_ctx *c, int max_mr)
+// the team of a batch of `count` solves whose largest Pi-row capacity is
+// max_mr: among the tuned entries at or below both, the one with the largest
+// capacity and then the largest batch; the context's default team without one
+static int solve_team_for(const tci_ctx *c, int max_mr, int count)
 {
     int team = c->team;
     for (const auto &e : c->solve_tune)
-        if (e.first <= max_mr) team = e.second;
+        if (e.mrcap <= max_mr && e.count <= count) team = e.team;
     return team;
 }
 
@@ -1490,7 +1496,7 @@ tci_status tci_solve_submit(tci_ctx *ctx, int32_t count, const tci_problem *prob
     if ((e = hipEventCreate(&b->ev0)) != hipSuccess) return fail(e, "hipEventCreate");
     if ((e = hipEventCreate(&b->ev1)) != hipSuccess) return fail(e, "hipEventCreate");
     if ((e = hipEventRecord(b->ev0, st)) != hipSuccess) return fail(e, "hipEventRecord");
-    const int team = solve_team_for(ctx, sh.max_mr);
+    const int team = solve_team_for(ctx, sh.max_mr, count);
     const int stat = solve_static_lds(team, sh.fkind);
     if (stat < 0) return fail(hipErrorInvalidValue, "solve kernel attributes");
     const LdsPlan plan = make_plan(ctx, (int64_t)lds_for_team(ctx, team) - stat, sh.max_mr, sh.max_tab, sh.max_sets,
@@ -1502,15 +1508,15 @@ tci_status tci_solve_submit(tci_ctx *ctx, int32_t count, const tci_problem *prob
 }
 
 // ---- the solve team's autotuner (tci_ctx_autotune_solve)
-static void solve_tune_put(tci_ctx *ctx, int32_t key, int32_t team)
+static void solve_tune_put(tci_ctx *ctx, int32_t key, int32_t count, int32_t team)
 {
     auto &v = ctx->solve_tune;
     for (auto &e : v)
-        if (e.first == key) {
-            e.second = team;
+        if (e.mrcap == key && e.count == count) {
+            e.team = team;
             return;
         }
-    v.emplace_back(key, team);
+    v.push_back({key, count, team});
     std::sort(v.begin(), v.end());
 }
 
@@ -1528,7 +1534,7 @@ tci_status tci_ctx_autotune_solve(tci_ctx *ctx, const tci_problem *sample, int32
     int bt = ctx->team;
     tci_status st = TCI_OK;
     for (int q = 0; q < 4 && st == TCI_OK; ++q) {
-        ctx->solve_tune.assign(1, std::make_pair(0, kTeams[q]));   // every batch on this team
+        ctx->solve_tune.assign(1, tci_ctx::SolveTune{0, 0, kTeams[q]});   // every batch on this team
         float ms = 0.0f;
         for (int rep = 0; rep < 2 && st == TCI_OK; ++rep) {   // a warm-up launch, then the timed one
             tci_batch *b = nullptr;
@@ -1546,19 +1552,21 @@ tci_status tci_ctx_autotune_solve(tci_ctx *ctx, const tci_problem *sample, int32
     }
     ctx->solve_tune = saved;
     if (st != TCI_OK) return st;
-    solve_tune_put(ctx, c.mrcap, bt);
+    solve_tune_put(ctx, c.mrcap, batch, bt);
     if (team_out) *team_out = bt;
     return TCI_OK;
 }
 
-tci_status tci_ctx_set_solve_tuning(tci_ctx *ctx, const int32_t *min_mrcap, const int32_t *team, int32_t n)
+tci_status tci_ctx_set_solve_tuning(tci_ctx *ctx, const int32_t *min_mrcap, const int32_t *min_count,
+                                    const int32_t *team, int32_t n)
 {
-    if (!ctx || n < 0 || (n > 0 && (!min_mrcap || !team))) return set_err(TCI_ERR_ARG, "bad arguments");
+    if (!ctx || n < 0 || (n > 0 && (!min_mrcap || !min_count || !team))) return set_err(TCI_ERR_ARG, "bad arguments");
     for (int32_t q = 0; q < n; ++q)
-        if (min_mrcap[q] < 0 || (team[q] != 64 && team[q] != 256 && team[q] != 512 && team[q] != 1024))
-            return set_err(TCI_ERR_ARG, "entry %d: mrcap >= 0, team 64, 256, 512 or 1024", (int)q);
+        if (min_mrcap[q] < 0 || min_count[q] < 0 ||
+            (team[q] != 64 && team[q] != 256 && team[q] != 512 && team[q] != 1024))
+            return set_err(TCI_ERR_ARG, "entry %d: mrcap >= 0, count >= 0, team 64, 256, 512 or 1024", (int)q);
     ctx->solve_tune.clear();
-    for (int32_t q = 0; q < n; ++q) solve_tune_put(ctx, min_mrcap[q], team[q]);
+    for (int32_t q = 0; q < n; ++q) solve_tune_put(ctx, min_mrcap[q], min_count[q], team[q]);
     return TCI_OK;
 }
 

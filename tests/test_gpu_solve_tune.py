@@ -1,5 +1,5 @@
 """The solve engine's team choice (tci_ctx_autotune_solve,
-tci_ctx_set_solve_tuning): a batch gives the same bits with 256, 512 or 1024
+tci_ctx_set_solve_tuning): a batch gives the same bits with 64, 256, 512 or 1024
 threads per solve, the autotuner keeps the fastest team, and bad tables are
 refused."""
 import pytest
@@ -27,7 +27,7 @@ def test_every_team_gives_the_same_bits(tci, own_ctx):
     own_ctx.set_solve_tuning([])
     base = tci.crossinterpolate2_batch(reqs, ctx=own_ctx)
     for team in (64, 512, 1024):
-        own_ctx.set_solve_tuning([(0, team)])
+        own_ctx.set_solve_tuning([(0, 0, team)])
         got = tci.crossinterpolate2_batch(reqs, ctx=own_ctx)
         for i, (a, b) in enumerate(zip(got, base)):
             _same(a, b, f"team {team}, request {i}")
@@ -47,12 +47,12 @@ def test_autotune_keeps_the_fastest_team(tci, own_ctx):
     # the tuned context still solves to the same bits
     own_ctx.set_solve_tuning([])
     (a,) = tci.crossinterpolate2_batch([_req(tci, spec)], ctx=own_ctx)
-    own_ctx.set_solve_tuning([(0, team)])
+    own_ctx.set_solve_tuning([(0, 0, team)])
     (b,) = tci.crossinterpolate2_batch([_req(tci, spec)], ctx=own_ctx)
     _same(a, b, "tuned")
 
 
 def test_bad_solve_tables_are_refused(tci, own_ctx):
-    for bad in ([(0, 300)], [(-1, 256)]):
+    for bad in ([(0, 0, 300)], [(-1, 0, 256)], [(0, -1, 256)]):
         with pytest.raises(tci.TCIError):
             own_ctx.set_solve_tuning(bad)
