@@ -136,3 +136,129 @@ def test_pi_eval_checks_functor_for_one_site():
     f2 = pkg._abi.tci_functor(99, 0, None, 0, None)
     assert lib.tci_pi_eval_f64(C.c_void_p(1), C.byref(f2), 1, pkg._abi.i32ptr(ld), None, 1, 0, None, 1, 0,
                                pkg._abi.dptr(out), None) == 3
+
+
+class _StubServerLib:
+    """Stands in for the slot server of libtci_hip.so: a ring of `ring`
+    entries, requests finishing `per_poll` at a time in submission order;
+    result statuses from a script keyed by submission number."""
+
+    def __init__(self, ring=2, per_poll=1, statuses=None, poll_code=0, room_limit=None):
+        self.ring, self.per_poll = ring, per_poll
+        self.statuses = dict(statuses or {})
+        self.poll_code = poll_code
+        self.room_limit = room_limit
+        self.inflight = []          # handles submitted, not yet polled
+        self.unfreed = set()
+        self.freed = []
+        self.nsub = 0
+        self.status_of = {}
+        self.chi_caps = []
+        self.bursts = []
+
+    def tci_server_start(self, ctx, shape, slots, idle, out):
+        out._obj.value = 77
+        return 0
+
+    def tci_server_stop(self, h):
+        return 0
+
+    def tci_server_submit_many(self, h, count, arr, out, acc):
+        self.bursts.append(count)
+        room = self.ring - len(self.unfreed)
+        if self.room_limit is not None:
+            room = min(room, self.room_limit)
+        k = max(0, min(count, room))
+        for q in range(k):
+            self.nsub += 1
+            hv = 5000 + self.nsub
+            out[q] = hv
+            self.inflight.append(hv)
+            self.unfreed.add(hv)
+            self.status_of[hv] = self.statuses.get(self.nsub, 0)
+            self.chi_caps.append(arr[q].opts.chi_cap)
+        acc._obj.value = k
+        return 0
+
+    def tci_server_poll(self, h, out, cap, n):
+        if self.poll_code:
+            return self.poll_code
+        k = min(self.per_poll, len(self.inflight), cap)
+        for q in range(k):
+            out[q] = self.inflight.pop(0)
+        n._obj.value = k
+        return 0
+
+    def tci_batch_result(self, b, i, r):
+        r._obj.status = self.status_of[b.value]
+        r._obj.niter = 0
+        return 0
+
+    def tci_batch_history(self, *a):
+        return 0
+
+    def tci_capacity_plan(self, p, used, limit):
+        used._obj.value = 32
+        limit._obj.value = 1000
+        return 0
+
+    def tci_batch_free(self, b):
+        self.freed.append(b.value)
+        self.unfreed.discard(b.value)
+        return 0
+
+    def tci_last_error(self):
+        return b"stub failure"
+
+
+def _slot(monkeypatch, stub, n=5):
+    pkg = load_pkg()
+    serving = __import__("tci_amd.serving", fromlist=["SlotServer"])
+    monkeypatch.setattr(pkg._abi, "lib", lambda: stub)
+    shape = (pkg.Lorentz(1.0), [4] * 4, None, {})
+    srv = serving.SlotServer(_Ctx(), shape, slots=stub.ring, idle_ms=5)
+    reqs = [(pkg.Lorentz(1.0), [4] * 4, None, {}) for _ in range(n)]
+    return serving, srv, reqs
+
+
+def test_slot_serving_bursts_fill_the_ring_and_free_each_once(monkeypatch):
+    stub = _StubServerLib(ring=2, per_poll=1)
+    serving, srv, reqs = _slot(monkeypatch, stub)
+    results, stats = srv.run(reqs, np.zeros(len(reqs)), keep_history=False)
+    srv.close()   # while the stub stands in for the library
+    assert stats.n == 5 and all(r is not None and r.status == 0 for r in results)
+    assert max(stub.bursts) <= 2                       # bursts never exceed the ring's free entries
+    assert sorted(stub.freed) == sorted(set(stub.freed)) and len(stub.freed) == 5
+    assert not stub.unfreed and stub.nsub == 5
+
+
+def test_slot_serving_resubmits_a_capacity_failure_at_twice_the_capacity(monkeypatch):
+    # the second submission ends with TCI_ERR_CAPACITY (10): resubmitted with chi_cap 64
+    stub = _StubServerLib(ring=4, per_poll=2, statuses={2: 10})
+    serving, srv, reqs = _slot(monkeypatch, stub, n=3)
+    results, stats = srv.run(reqs, np.zeros(3))
+    srv.close()
+    assert stats.retries == 1 and stub.nsub == 4
+    assert all(r.status == 0 for r in results)
+    assert stub.chi_caps[:3] == [0, 0, 0] and stub.chi_caps[3] == 64
+    assert not stub.unfreed
+
+
+def test_slot_serving_poll_error_frees_everything_and_raises(monkeypatch):
+    stub = _StubServerLib(ring=4, poll_code=12)
+    serving, srv, reqs = _slot(monkeypatch, stub, n=3)
+    with pytest.raises(serving.ServeError) as e:
+        srv.run(reqs, np.zeros(3))
+    srv.close()
+    assert e.value.status == 12
+    assert not stub.unfreed and len(stub.freed) == 3
+
+
+def test_slot_serving_refuses_a_request_that_never_fits(monkeypatch):
+    # nothing in flight and still no room: raise instead of waiting forever
+    stub = _StubServerLib(ring=4, room_limit=0)
+    serving, srv, reqs = _slot(monkeypatch, stub, n=2)
+    with pytest.raises(serving.ServeError) as e:
+        srv.run(reqs, np.zeros(2))
+    srv.close()
+    assert e.value.status == 11
