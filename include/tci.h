@@ -454,7 +454,10 @@ tci_status tci_batch_free(tci_batch *b);
  * wait and it has left.
  * tci_server_stop runs every published request to completion, copies their
  * reports to the host and releases the server; the batches stay valid.
- * Calls on one server, and on its batches, are serialized by the caller. */
+ * Calls on one server, and on its batches, are serialized by the caller.
+ * One server per device at a time: a second server's grid becomes resident
+ * only as the first one's workgroups leave (after its idle_ms without work),
+ * and large solve launches on the device share it the same way. */
 typedef struct tci_server tci_server;
 tci_status tci_server_start(tci_ctx *ctx, const tci_problem *shape, int32_t slots, int32_t idle_ms,
                             tci_server **out);
