@@ -178,6 +178,42 @@ def sample_error(tci, spec, npts=100_000, seed=12345):
             "ranks": list(map(int, t.ranks))}
 
 
+def true_error_search(tci, ctx, spec, cpu_seconds, nsearch=2000, seed=7):
+    """estimatetrueerror (globalsearch.jl:18-41) on one solve's train: the
+    floating-zone search from nsearch seeded random start points on the
+    device (one workgroup per start), the largest |f - TT| it finds, and
+    the oracle's _floatingzone on a bounded sample of the same starts (its
+    pivots and errors compared with the device's)."""
+    name, kind, params, ip, ld, piv, opts = spec
+    f = tci.DeviceFunctor(kind, np.asarray(params, float), np.asarray(ip, np.int32))
+    (t,) = tci.crossinterpolate2_batch([(f, ld, piv, dict(opts))], ctx=ctx)
+    tt = tci.TensorTrain(t)
+    starts = tci.sample_points(ld, nsearch, seed)
+    tci.floatingzone(tt, f, starts[:8], ctx=ctx)   # warm-up
+    t0 = time.perf_counter()
+    pv, errs = tci.floatingzone(tt, f, starts, ctx=ctx)
+    dt = time.perf_counter() - t0
+    worst = float(np.nanmax(errs))
+    out = {"nsearch": nsearch, "seed": seed, "ms": round(dt * 1e3, 3), "starts_per_s": round(nsearch / dt, 1),
+           "max_abs_err_found": worst, "over_maxsample": worst / t.maxsamplevalue,
+           "tolerance": opts["tolerance"]}
+    if cpu_seconds > 0:
+        O = _oracle()
+        of = O.Functor(kind, params, ip)
+        cores = [np.asarray(c) for c in t.sitetensors]
+        k, t1 = 0, time.perf_counter()
+        same = True
+        while k < nsearch and time.perf_counter() - t1 < cpu_seconds:
+            op, oe = O.floatingzone(of, cores, starts[k:k + 1])
+            same = same and np.array_equal(op[0], pv[k]) and oe[0] == errs[k]
+            k += 1
+        ct = time.perf_counter() - t1
+        out["cpu_baseline"] = {"value": round(k / ct, 2), "unit": "starts/s", "cores": 1, "kind": "port",
+                               "sample": f"the first {k} starts in {ct:.1f} s on one thread"}
+        out["oracle_equal_on_sample"] = bool(same)
+    return out
+
+
 def tune_team(tci, ctx, holder, batch):
     """The solve team for batches of this workload of at least `batch`
     solves (tci_ctx_autotune_solve: each team timed on such a batch, outside
@@ -218,7 +254,8 @@ def solve_config(tci, ctx, spec, batch, steps, plain_reps, cpu, cpu_seconds, cpu
            "solve_team": tuned, "solve_team_plain": tuned_plain,
            "solve_phases_us": dict(zip(["eval", "rrlu", "sets", "fill", "gsearch", "sweep1", "total", "pinv"],
                                        [round(v, 1) for v in ws[6][:8]])),
-           "error": sample_error(tci, spec)}
+           "error": sample_error(tci, spec),
+           "true_error_search": true_error_search(tci, ctx, spec, min(cpu_seconds, 5.0))}
     ctx.set_solve_tuning([])
     if cpu_seconds > 0:
         O = _oracle()
