@@ -214,6 +214,48 @@ def true_error_search(tci, ctx, spec, cpu_seconds, nsearch=2000, seed=7):
     return out
 
 
+def tt_post_section(tci, ctx):
+    """compress!(tt, :LU) and TensorCI2(tt) (tensortrain.jl:147-183,
+    conversion.jl:158-176) on the device for two trains: the C4 solve's
+    (L=40, d=16, bond dimensions up to 16) and a random train with L=20,
+    d=16, bond dimension 128.  Each result is compared with the oracle's on
+    the same input (cores bitwise for compress!, pivot sets for the
+    conversion), and the oracle is timed beside it on one thread."""
+    P = _problems()
+    O = _oracle()
+    name, kind, params, ip, ld, piv, opts = P.c4_spec(5)
+    f = tci.DeviceFunctor(kind, np.asarray(params, float), np.asarray(ip, np.int32))
+    (t,) = tci.crossinterpolate2_batch([(f, ld, piv, dict(opts))], ctx=ctx)
+    rng = np.random.default_rng(11)
+    chi, L, d = 128, 20, 16
+    dims = [1] + [chi] * (L - 1) + [1]
+    rand = [rng.standard_normal((dims[b], d, dims[b + 1])) / np.sqrt(dims[b] * d) for b in range(L)]
+    trains = {"c4_solve_train": [np.asarray(c) for c in t.sitetensors], "random_L20_d16_chi128": rand}
+    out = {}
+    for tname, cores in trains.items():
+        row = {"linkdims": [int(c.shape[2]) for c in cores[:-1]]}
+        # compress!(:LU) with a truncating tolerance
+        tt = tci.TensorTrain(cores)
+        tci.compress(tci.TensorTrain(cores), "LU", tolerance=1e-8, ctx=ctx)   # warm-up
+        t0 = time.perf_counter()
+        tci.compress(tt, "LU", tolerance=1e-8, ctx=ctx)
+        row["compress_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
+        row["compressed_linkdims_max"] = int(max(tt.linkdims()))
+        t0 = time.perf_counter()
+        oc = O.tt_compress([np.array(c, order="F") for c in cores], "LU", tolerance=1e-8)
+        row["compress_cpu_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
+        row["compress_equal_oracle"] = bool(all(np.array_equal(a, b) for a, b in zip(tt.sitetensors, oc)))
+        # TensorCI2(tt)
+        t0 = time.perf_counter()
+        conv = tci.tensorci2_from_tt(tci.TensorTrain(cores), tolerance=1e-12, ctx=ctx)
+        row["to_tensorci2_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
+        row["to_tensorci2_linkdims_max"] = int(max(len(s_) for s_ in conv.Iset[1:])) if hasattr(conv, "Iset") else None
+        out[tname] = row
+    out["note"] = ("host-driven per-bond sweeps: one factorization and one product per bond, each a device call; "
+                   "the oracle runs the same steps on one host thread")
+    return out
+
+
 def tune_team(tci, ctx, holder, batch):
     """The solve team for batches of this workload of at least `batch`
     solves (tci_ctx_autotune_solve: each team timed on such a batch, outside
@@ -558,7 +600,7 @@ def served_section(args, tci, ctx, cpu):
 # ------------------------------------------------------------------ main
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["all", "solve", "rrlu", "bond", "served", "c3", "c4"], default="all",
+    ap.add_argument("--workload", choices=["all", "solve", "rrlu", "bond", "served", "c3", "c4", "ttpost"], default="all",
                     help="all: the headline line with every sub-object; solve: the headline only; "
                          "rrlu: the batched rrLU only; bond: the C4 bond rrLU; served: the C5 stream; "
                          "c3, c4: one solve configuration")
@@ -607,6 +649,9 @@ def main():
         return
     if args.workload == "served":
         print(json.dumps(served_section(args, tci, ctx, cpu)), flush=True)
+        return
+    if args.workload == "ttpost":
+        print(json.dumps(tt_post_section(tci, ctx)), flush=True)
         return
     if args.workload in ("c3", "c4"):
         P = _problems()
@@ -755,6 +800,7 @@ def main():
         out["c3_quantics"] = solve_config(tci, ctx, P.c3_spec(5), args.c3_batch, 3, 5, cpu, args.cpu_seconds / 2, 3)
         out["c4_highdim"] = solve_config(tci, ctx, P.c4_spec(5), args.c4_batch, 2, 3, cpu, args.cpu_seconds, 3)
         out["c5_served"] = served_section(args, tci, ctx, cpu)
+        out["tt_post"] = tt_post_section(tci, ctx)
     print(json.dumps(out), flush=True)
 
 
