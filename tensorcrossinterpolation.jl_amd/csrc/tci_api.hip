@@ -1117,6 +1117,84 @@ static tci_status tt_download(TtDevice &T, int64_t *linkdims, double *cores)
 // per bond, the rrLU of the grouped site matrix, its left/right factors
 // (method 0 rrLU with permutations, 1 MatrixLUCI), the factored site
 // replaced by one of them and the other multiplied into the neighbour.
+// The whole sweep in one workgroup (launch_tt_sweep) for trains whose every
+// bond matrix holds at most kTtFusedElems entries: there the host-driven
+// sweep spends its time in per-bond round trips (about 0.3 ms each), not in
+// the factorizations.  Same operations per entry, so the same bits.
+constexpr int64_t kTtFusedElems = (int64_t)256 * 256;
+constexpr int64_t kTtStageCap = 48 * 1024 / 16;   // doubles per LDS staging vector
+constexpr int64_t kTtMatCap = 96 * 1024 / 8;       // doubles of LDS for a bond matrix
+
+static tci_status tt_sweep_fused(TtDevice &T, int32_t method, int32_t forward, int64_t maxrank, double reltol,
+                                 double abstol, int64_t R, int32_t *pivots, double *pivoterrors)
+{
+    tci_ctx *ctx = T.ctx;
+    hipStream_t st = ctx->stream;
+    const int L = T.L;
+    const size_t b_links = lay_align(sizeof(int64_t) * (size_t)(L + 1)), b_d = lay_align(sizeof(int32_t) * (size_t)L);
+    const size_t b_off = lay_align(sizeof(int64_t) * (size_t)L);
+    const size_t b_piv = lay_align(sizeof(int32_t) * (size_t)(L - 1) * 2 * (size_t)R);
+    const size_t b_pe = lay_align(sizeof(double) * (size_t)(R + 1)), b_st = lay_align(sizeof(int32_t) * 2);
+    const size_t b_stage = lay_align(sizeof(double) * 2 * (size_t)T.maxside);
+    uint8_t *d = nullptr;
+    HIPCHK(hipMallocAsync((void **)&d, b_links + b_d + b_off + b_piv + b_pe + b_st + b_stage, st));
+    TtSweepArgs a;
+    memset(&a, 0, sizeof a);
+    a.L = L;
+    a.method = method;
+    a.forward = forward;
+    a.stage_cap = (int32_t)(T.maxside < kTtStageCap ? T.maxside : kTtStageCap);
+    a.mat_cap = (int32_t)(T.maxsite < kTtMatCap ? T.maxsite : kTtMatCap);
+    a.maxrank = maxrank;
+    a.R = R;
+    a.reltol = reltol;
+    a.abstol = abstol;
+    a.links = (int64_t *)d;
+    a.d = (const int32_t *)(d + b_links);
+    a.off = (const int64_t *)(d + b_links + b_d);
+    a.piv = (int32_t *)(d + b_links + b_d + b_off);
+    a.pe = (double *)(d + b_links + b_d + b_off + b_piv);
+    a.status = (int32_t *)(d + b_links + b_d + b_off + b_piv + b_pe);
+    a.gstage = (double *)(d + b_links + b_d + b_off + b_piv + b_pe + b_st);
+    a.cores = T.cores;
+    a.wl = T.wl;
+    a.wu = T.wu;
+    a.wc = T.wc;
+    a.wp = T.wp;
+    a.perm = T.perm;
+    std::vector<int32_t> hst(2, 0), hpiv((size_t)(L - 1) * 2 * (size_t)R);
+    std::vector<double> hpe((size_t)(R + 1), 0.0);
+    hipError_t e = hipMemcpyAsync(a.links, T.links.data(), sizeof(int64_t) * (size_t)(L + 1), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync((void *)a.d, T.d.data(), sizeof(int32_t) * (size_t)L, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync((void *)a.off, T.off.data(), sizeof(int64_t) * (size_t)L, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(a.pe, hpe.data(), sizeof(double) * (size_t)(R + 1), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(a.status, hst.data(), sizeof(int32_t) * 2, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_tt_sweep(a, st);
+    std::vector<int64_t> links((size_t)(L + 1));
+    if (e == hipSuccess) e = hipMemcpyAsync(links.data(), a.links, sizeof(int64_t) * (size_t)(L + 1), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(hpiv.data(), a.piv, sizeof(int32_t) * hpiv.size(), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(hpe.data(), a.pe, sizeof(double) * hpe.size(), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(hst.data(), a.status, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFreeAsync(d, st);
+    if (e != hipSuccess) return set_err(TCI_ERR_DEVICE, "fused tt sweep: %s", hipGetErrorString(e));
+    if (hst[0] == TCI_ERR_NAN_L) return set_err(TCI_ERR_NAN_L, "lu.L contains NaNs");
+    if (hst[0] == TCI_ERR_NAN_U) return set_err(TCI_ERR_NAN_U, "lu.U contains NaNs");
+    if (hst[0] != TCI_OK) return set_err((tci_status)hst[0], "rrlu failed at bond %d", hst[1] + 1);
+    T.links = links;
+    if (pivoterrors)
+        for (int64_t k = 0; k <= R; ++k) pivoterrors[k] = hpe[(size_t)k];
+    if (pivots) {
+        int64_t q = 0;
+        for (int b = 0; b < L - 1; ++b) {
+            const int64_t p = links[(size_t)b + 1];
+            const int32_t *src = hpiv.data() + (size_t)b * 2 * (size_t)R;
+            for (int64_t k = 0; k < 2 * p; ++k) pivots[q++] = src[k];
+        }
+    }
+    return TCI_OK;
+}
+
 static tci_status tt_sweep(TtDevice &T, int32_t method, int32_t forward, int64_t maxrank, double reltol,
                            double abstol, int32_t *pivots, double *pivoterrors)
 {
@@ -1125,6 +1203,17 @@ static tci_status tt_sweep(TtDevice &T, int32_t method, int32_t forward, int64_t
     const int L = T.L;
     int64_t R = 0;
     for (int b = 1; b < L; ++b) R = T.links[(size_t)b] > R ? T.links[(size_t)b] : R;
+    {
+        int64_t big = 0;
+        for (int b = 0; b + 1 < L; ++b) {
+            const int64_t mm = forward ? T.links[(size_t)b] * T.d[(size_t)b] : T.links[(size_t)b + 1];
+            const int64_t nn = forward ? T.links[(size_t)b + 1] : T.d[(size_t)b + 1] * T.links[(size_t)b + 2];
+            big = mm * nn > big ? mm * nn : big;
+        }
+        const char *ev = getenv("TCI_TT_FUSED");
+        const bool fused = ev ? atoi(ev) != 0 : big <= kTtFusedElems;
+        if (fused && L > 1 && R > 0) return tt_sweep_fused(T, method, forward, maxrank, reltol, abstol, R, pivots, pivoterrors);
+    }
     if (pivoterrors)
         for (int64_t k = 0; k <= R; ++k) pivoterrors[k] = 0.0;
     std::vector<std::vector<int32_t>> piv((size_t)(L > 1 ? L - 1 : 0));

@@ -82,6 +82,24 @@ int rrlu_grid_blocks(int device, size_t lds, int team);
 size_t rrlu_grid_cand_bytes(int blocks);
 hipError_t launch_rrlu_grid(tci_lu_job *job, int64_t m, int64_t n, double *out, void *cand, int32_t *flags,
                             int blocks, int team, hipStream_t st);
+// One whole TT sweep of factorizations (the host-driven sweep's steps, the
+// same operations per entry) in a single workgroup, for trains whose bonds
+// are small enough that per-bond host round trips would dominate
+struct TtSweepArgs {
+    int32_t L, method, forward, stage_cap;   // stage_cap: doubles per LDS staging vector
+    int32_t mat_cap;                         // doubles of LDS for the bond matrix (0: factored in place)
+    int64_t maxrank, R;                      // R: the largest bond on entry (pivot slots per bond: 2R)
+    double reltol, abstol;
+    int64_t *links;                          // device, L+1; each bond's new dimension written back
+    const int32_t *d;                        // device, L
+    const int64_t *off;                      // device, L: each site's slot in cores (doubles)
+    double *cores, *wl, *wu, *wc, *wp, *gstage;
+    int32_t *perm;                           // device, m+n of the largest bond
+    int32_t *piv;                            // device, (L-1)·2R: bond b's row, then column pivots (1-based)
+    double *pe;                              // device, R+1: running pivot errors (zero on entry)
+    int32_t *status;                         // device, 2: status and bond of a NaN verdict (zero on entry)
+};
+hipError_t launch_tt_sweep(const TtSweepArgs &a, hipStream_t st);
 // C = A·B with each sum in ascending k (M×K · K×N, column-major)
 hipError_t launch_gemm(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
                        double *Cm, int64_t ldc, hipStream_t st);

@@ -338,42 +338,56 @@ __global__ __launch_bounds__(kTeam, 2) void tci_rrlu_tile_kernel(tci_lu_job *__r
     else tile_job<TileLU<Team, 2, 32>>(tm, J, m, n, Lsc, Usc, pv);
 }
 
+// entry q of the unpacked factors: q < m·p an entry of L, else of U.  The
+// bodies below are shared by the standalone kernels and the fused TT sweep,
+// so both compute every entry with the same operations
+__device__ __forceinline__ void lu_unpack_at(int64_t q, const double *A, int64_t m, int64_t p, int leftorth,
+                                             double *L, int64_t ldl, double *U, int64_t ldu)
+{
+    const int64_t nl = m * p;
+    if (q < nl) {
+        int64_t j = q / m, i = q - j * m;
+        double v = i >= j ? A[i + m * j] : 0.0;
+        if (leftorth && i == j) v = 1.0;
+        L[i + ldl * j] = v;
+    } else {
+        int64_t r = q - nl;
+        int64_t j = r / p, i = r - j * p;
+        double v = i <= j ? A[i + m * j] : 0.0;
+        if (!leftorth && i == j) v = 1.0;
+        U[i + ldu * j] = v;
+    }
+}
+
 __global__ void tci_lu_unpack_kernel(const double *__restrict__ A, int64_t m, int64_t n, int64_t p,
                                      int leftorth, double *__restrict__ L, int64_t ldl,
                                      double *__restrict__ U, int64_t ldu)
 {
     const int64_t nl = m * p, nu = p * n;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nl + nu;
-         q += (int64_t)gridDim.x * blockDim.x) {
-        if (q < nl) {
-            int64_t j = q / m, i = q - j * m;
-            double v = i >= j ? A[i + m * j] : 0.0;
-            if (leftorth && i == j) v = 1.0;
-            L[i + ldl * j] = v;
-        } else {
-            int64_t r = q - nl;
-            int64_t j = r / p, i = r - j * p;
-            double v = i <= j ? A[i + m * j] : 0.0;
-            if (!leftorth && i == j) v = 1.0;
-            U[i + ldu * j] = v;
-        }
-    }
+         q += (int64_t)gridDim.x * blockDim.x)
+        lu_unpack_at(q, A, m, p, leftorth, L, ldl, U, ldu);
 }
 
 // ------------------------------------------------------------ products
 // C = A·B (M×K times K×N, column-major), one thread per entry of C, each
 // sum in ascending k from 0.0 with a rounding per product and per sum (the
 // order of the oracle's loops; the reference calls BLAS here).
+__device__ __forceinline__ void gemm_at(int64_t q, int64_t M, int64_t K, const double *A, int64_t lda,
+                                        const double *B, int64_t ldb, double *Cm, int64_t ldc)
+{
+    const int64_t j = q / M, i = q - j * M;
+    double acc = 0.0;
+    for (int64_t k = 0; k < K; ++k) acc = acc + A[i + lda * k] * B[k + ldb * j];
+    Cm[i + ldc * j] = acc;
+}
+
 __global__ void tci_gemm_kernel(int64_t M, int64_t N, int64_t K, const double *__restrict__ A, int64_t lda,
                                 const double *__restrict__ B, int64_t ldb, double *__restrict__ Cm, int64_t ldc)
 {
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < M * N;
-         q += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t j = q / M, i = q - j * M;
-        double acc = 0.0;
-        for (int64_t k = 0; k < K; ++k) acc = acc + A[i + lda * k] * B[k + ldb * j];
-        Cm[i + ldc * j] = acc;
-    }
+         q += (int64_t)gridDim.x * blockDim.x)
+        gemm_at(q, M, K, A, lda, B, ldb, Cm, ldc);
 }
 
 hipError_t launch_gemm(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
@@ -395,61 +409,67 @@ hipError_t launch_gemm(int64_t M, int64_t N, int64_t K, const double *A, int64_t
 // (position j, written to column colperm[j]-1); each substitution or dot
 // product runs in a fixed order (ascending t for products from 0.0, back to
 // front for the triangular solves).
+__device__ __forceinline__ void luci_at(int64_t q, int64_t m, int64_t p, int leftorth, const int32_t *rowperm,
+                                        const int32_t *colperm, const double *L, const double *U, double *left,
+                                        int64_t ldl, double *right, int64_t ldr)
+{
+    if (q < m) {
+        const int64_t i = q, r = rowperm[i] - 1;
+        double *out = left + r;   // row r of left, stride ldl
+        if (leftorth) {
+            // colstimespivotinv (48-57): [I_p; L21·L11⁻¹]; x·L11 = L[i,1:p]
+            if (i < p) {
+                for (int64_t j = 0; j < p; ++j) out[ldl * j] = i == j ? 1.0 : 0.0;
+            } else {
+                for (int64_t j = p - 1; j >= 0; --j) {
+                    double s = L[i + m * j];
+                    for (int64_t t = j + 1; t < p; ++t) s = s - out[ldl * t] * L[t + m * j];
+                    out[ldl * j] = s / L[j + m * j];
+                }
+            }
+        } else {
+            // colmatrix (40-42): L · U[:,1:p]
+            for (int64_t j = 0; j < p; ++j) {
+                double s = 0.0;
+                const int64_t tmax = i < j ? i : j;
+                for (int64_t t = 0; t <= tmax && t < p; ++t) s = s + L[i + m * t] * U[t + p * j];
+                out[ldl * j] = s;
+            }
+        }
+    } else {
+        const int64_t j = q - m, c = colperm[j] - 1;
+        double *out = right + ldr * c;   // column c of right
+        if (leftorth) {
+            // rowmatrix (44-46): L[1:p,:] · U
+            for (int64_t i = 0; i < p; ++i) {
+                double s = 0.0;
+                const int64_t tmax = i < j ? i : j;
+                for (int64_t t = 0; t <= tmax; ++t) s = s + L[i + m * t] * U[t + p * j];
+                out[i] = s;
+            }
+        } else {
+            // pivotinvtimesrows (59-68): [I_p  U11⁻¹·U12]
+            if (j < p) {
+                for (int64_t i = 0; i < p; ++i) out[i] = i == j ? 1.0 : 0.0;
+            } else {
+                for (int64_t i = p - 1; i >= 0; --i) {
+                    double s = U[i + p * j];
+                    for (int64_t t = i + 1; t < p; ++t) s = s - U[i + p * t] * out[t];
+                    out[i] = s / U[i + p * i];
+                }
+            }
+        }
+    }
+}
+
 __global__ void tci_luci_kernel(int64_t m, int64_t n, int64_t p, int leftorth, const int32_t *__restrict__ rowperm,
                                 const int32_t *__restrict__ colperm, const double *__restrict__ L,
                                 const double *__restrict__ U, double *__restrict__ left, int64_t ldl,
                                 double *__restrict__ right, int64_t ldr)
 {
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < m + n;
-         q += (int64_t)gridDim.x * blockDim.x) {
-        if (q < m) {
-            const int64_t i = q, r = rowperm[i] - 1;
-            double *out = left + r;   // row r of left, stride ldl
-            if (leftorth) {
-                // colstimespivotinv (48-57): [I_p; L21·L11⁻¹]; x·L11 = L[i,1:p]
-                if (i < p) {
-                    for (int64_t j = 0; j < p; ++j) out[ldl * j] = i == j ? 1.0 : 0.0;
-                } else {
-                    for (int64_t j = p - 1; j >= 0; --j) {
-                        double s = L[i + m * j];
-                        for (int64_t t = j + 1; t < p; ++t) s = s - out[ldl * t] * L[t + m * j];
-                        out[ldl * j] = s / L[j + m * j];
-                    }
-                }
-            } else {
-                // colmatrix (40-42): L · U[:,1:p]
-                for (int64_t j = 0; j < p; ++j) {
-                    double s = 0.0;
-                    const int64_t tmax = i < j ? i : j;
-                    for (int64_t t = 0; t <= tmax && t < p; ++t) s = s + L[i + m * t] * U[t + p * j];
-                    out[ldl * j] = s;
-                }
-            }
-        } else {
-            const int64_t j = q - m, c = colperm[j] - 1;
-            double *out = right + ldr * c;   // column c of right
-            if (leftorth) {
-                // rowmatrix (44-46): L[1:p,:] · U
-                for (int64_t i = 0; i < p; ++i) {
-                    double s = 0.0;
-                    const int64_t tmax = i < j ? i : j;
-                    for (int64_t t = 0; t <= tmax; ++t) s = s + L[i + m * t] * U[t + p * j];
-                    out[i] = s;
-                }
-            } else {
-                // pivotinvtimesrows (59-68): [I_p  U11⁻¹·U12]
-                if (j < p) {
-                    for (int64_t i = 0; i < p; ++i) out[i] = i == j ? 1.0 : 0.0;
-                } else {
-                    for (int64_t i = p - 1; i >= 0; --i) {
-                        double s = U[i + p * j];
-                        for (int64_t t = i + 1; t < p; ++t) s = s - U[i + p * t] * out[t];
-                        out[i] = s / U[i + p * i];
-                    }
-                }
-            }
-        }
-    }
+         q += (int64_t)gridDim.x * blockDim.x)
+        luci_at(q, m, p, leftorth, rowperm, colperm, L, U, left, ldl, right, ldr);
 }
 
 hipError_t launch_luci(int64_t m, int64_t n, int64_t p, int leftorth, const int32_t *rowperm, const int32_t *colperm,
@@ -468,6 +488,20 @@ hipError_t launch_luci(int64_t m, int64_t n, int64_t p, int leftorth, const int3
 // left(lu) and right(lu) with permute=true (matrixlu.jl:374-392): row i of
 // L goes to row rowperm[i]-1 of left, column j of U to column colperm[j]-1
 // of right (L m×p with lda m, U p×n with lda p; permutations 1-based)
+__device__ __forceinline__ void lu_permute_at(int64_t q, int64_t m, int64_t p, const int32_t *rowperm,
+                                              const int32_t *colperm, const double *L, const double *U,
+                                              double *left, int64_t ldl, double *right, int64_t ldr)
+{
+    const int64_t nl = m * p;
+    if (q < nl) {
+        const int64_t j = q / m, i = q - j * m;
+        left[(rowperm[i] - 1) + ldl * j] = L[i + m * j];
+    } else {
+        const int64_t r = q - nl, j = r / p, i = r - j * p;
+        right[i + ldr * (colperm[j] - 1)] = U[i + p * j];
+    }
+}
+
 __global__ void tci_lu_permute_kernel(int64_t m, int64_t n, int64_t p, const int32_t *__restrict__ rowperm,
                                       const int32_t *__restrict__ colperm, const double *__restrict__ L,
                                       const double *__restrict__ U, double *__restrict__ left, int64_t ldl,
@@ -475,15 +509,117 @@ __global__ void tci_lu_permute_kernel(int64_t m, int64_t n, int64_t p, const int
 {
     const int64_t nl = m * p, nu = p * n;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nl + nu;
-         q += (int64_t)gridDim.x * blockDim.x) {
-        if (q < nl) {
-            const int64_t j = q / m, i = q - j * m;
-            left[(rowperm[i] - 1) + ldl * j] = L[i + m * j];
-        } else {
-            const int64_t r = q - nl, j = r / p, i = r - j * p;
-            right[i + ldr * (colperm[j] - 1)] = U[i + p * j];
+         q += (int64_t)gridDim.x * blockDim.x)
+        lu_permute_at(q, m, p, rowperm, colperm, L, U, left, ldl, right, ldr);
+}
+
+// ------------------------------------------------------ fused TT sweep
+// The host-driven sweep (per bond: rrlu!, unpack, left/right of rrLU or
+// MatrixLUCI, the product into the neighbour; tensortrain.jl:155-182,
+// conversion.jl:105-141) in one workgroup: the streaming rrLU of the team
+// (whose packed result equals every other rrLU kernel's) and the entry bodies
+// of the standalone kernels above, so every value has the same bits.  Each
+// bond's new dimension p is returned to every thread by the factorization,
+// so the shapes of the next bond need no memory round trip.
+constexpr int kTtTeam = 256;
+
+__device__ __forceinline__ double tt_nan_max(double a, double b)
+{
+    if (a != a || b != b) return a + b;
+    return a > b ? a : b;
+}
+
+__global__ __launch_bounds__(kTtTeam) void tci_tt_sweep_kernel(TtSweepArgs a)
+{
+    extern __shared__ double stage[];
+    DevTeam<kTtTeam> tm;
+    const int L = a.L;
+    const bool fw = a.forward != 0;
+    int64_t carried = -1;   // the previous bond's new dimension
+    for (int it = 0; it < L - 1; ++it) {
+        const int b = fw ? it : L - 2 - it;
+        const int s = fw ? b : b + 1;
+        const int64_t m = fw ? (carried >= 0 ? carried : a.links[b]) * a.d[b] : a.links[b + 1];
+        const int64_t n = fw ? a.links[b + 1] : a.d[b + 1] * (carried >= 0 ? carried : a.links[b + 2]);
+        double *A = a.cores + a.off[s];
+        const int64_t mx = m > n ? m : n;
+        if (mx <= a.stage_cap) tm.init(stage, stage + mx);
+        else tm.init(a.gstage, a.gstage + mx);
+        // a bond matrix that fits is factored in LDS (the same operations on
+        // a copy with a shorter latency) and copied back
+        double *Aw = A;
+        if (m * n <= a.mat_cap) {
+            Aw = stage + 2 * (int64_t)a.stage_cap;
+            for (int64_t q = tm.tid(); q < m * n; q += kTtTeam) Aw[q] = A[q];
+            tm.sync();
         }
+        double err, st2[2];
+        const int p = rrlu_team(tm, Aw, (int)m, (int)n, a.maxrank, a.reltol, a.abstol, fw, a.perm, a.perm + m, nullptr,
+                                &err, st2);
+        if (Aw != A) {
+            tm.sync();
+            for (int64_t q = tm.tid(); q < m * n; q += kTtTeam) A[q] = Aw[q];
+        }
+        if (p < 0) {
+            if (tm.tid() == 0) {
+                a.status[0] = -p;
+                a.status[1] = b;
+            }
+            return;
+        }
+        int32_t *rp = a.perm, *cp = a.perm + m;
+        for (int64_t i = tm.tid(); i < m; i += kTtTeam) rp[i] += 1;
+        for (int64_t j = tm.tid(); j < n; j += kTtTeam) cp[j] += 1;
+        tm.sync();
+        // running pivot errors (the diagonal of the packed factors) and the pivots
+        if (tm.tid() == 0) {
+            for (int k = 0; k < p; ++k) a.pe[k] = tt_nan_max(a.pe[k], fabs(A[k + m * k]));
+            a.pe[p] = tt_nan_max(a.pe[p], err);
+        }
+        int32_t *pv = a.piv + (int64_t)b * 2 * a.R;
+        for (int k = tm.tid(); k < p; k += kTtTeam) {
+            pv[k] = rp[k];
+            pv[p + k] = cp[k];
+        }
+        for (int64_t q = tm.tid(); q < m * p + (int64_t)p * n; q += kTtTeam)
+            lu_unpack_at(q, A, m, p, fw ? 1 : 0, a.wl, m, a.wu, p);
+        tm.sync();
+        double *lft = fw ? A : a.wc, *rgt = fw ? a.wc : A;
+        if (a.method == 0) {
+            for (int64_t q = tm.tid(); q < m * p + (int64_t)p * n; q += kTtTeam)
+                lu_permute_at(q, m, p, rp, cp, a.wl, a.wu, lft, m, rgt, p);
+        } else if (p > 0) {
+            for (int64_t q = tm.tid(); q < m + n; q += kTtTeam)
+                luci_at(q, m, p, fw ? 1 : 0, rp, cp, a.wl, a.wu, lft, m, rgt, p);
+        }
+        tm.sync();
+        if (fw) {
+            const int64_t nn = a.d[b + 1] * a.links[b + 2];
+            double *nx = a.cores + a.off[b + 1];
+            for (int64_t q = tm.tid(); q < (int64_t)p * nn; q += kTtTeam) gemm_at(q, p, n, a.wc, p, nx, n, a.wp, p);
+            tm.sync();
+            for (int64_t q = tm.tid(); q < (int64_t)p * nn; q += kTtTeam) nx[q] = a.wp[q];
+        } else {
+            const int64_t mm = a.links[b] * a.d[b];
+            double *pv2 = a.cores + a.off[b];
+            for (int64_t q = tm.tid(); q < mm * p; q += kTtTeam) gemm_at(q, mm, m, pv2, mm, a.wc, m, a.wp, mm);
+            tm.sync();
+            for (int64_t q = tm.tid(); q < mm * p; q += kTtTeam) pv2[q] = a.wp[q];
+        }
+        if (tm.tid() == 0) a.links[b + 1] = p;
+        carried = p;
+        tm.sync();
     }
+}
+
+hipError_t launch_tt_sweep(const TtSweepArgs &a, hipStream_t st)
+{
+    const size_t lds = sizeof(double) * (2 * (size_t)a.stage_cap + (size_t)a.mat_cap);
+    hipError_t e = hipFuncSetAttribute((const void *)tci_tt_sweep_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(tci_tt_sweep_kernel, dim3(1), dim3(kTtTeam), lds, st, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_lu_permute(int64_t m, int64_t n, int64_t p, const int32_t *rowperm, const int32_t *colperm,

@@ -3,7 +3,9 @@ compress! (tci_tt_compress_f64, tensortrain.jl:95-183) for :LU and :CI with
 every truncation mode, one sweep of sweep1sitegetindices!
 (tci_tt_sweep_f64, conversion.jl:73-156) with its pivots and pivot errors,
 the products (tci_gemm_f64), and TensorCI2(tt) (conversion.jl:158-176)
-as test_conversion.jl:75-99 checks it, with a real f."""
+as test_conversion.jl:75-99 checks it, with a real f.  The sweeps are
+checked both in the fused one-workgroup kernel and driven bond by bond from
+the host."""
 import numpy as np
 import pytest
 
@@ -40,6 +42,14 @@ def _trains():
 _TRAINS = _trains()
 
 
+@pytest.fixture(params=["fused", "host"])
+def route(request, monkeypatch):
+    """The TT sweeps run either in one fused workgroup or driven from the
+    host bond by bond (TCI_TT_FUSED); both must give the oracle's bits."""
+    monkeypatch.setenv("TCI_TT_FUSED", "1" if request.param == "fused" else "0")
+    return request.param
+
+
 def test_gemm_equals_oracle_order(tci, ctx):
     rng = np.random.default_rng(2)
     for M, N, K in ((1, 1, 1), (7, 5, 3), (130, 70, 257), (3, 200, 1)):
@@ -56,7 +66,7 @@ def test_gemm_equals_oracle_order(tci, ctx):
 
 @pytest.mark.parametrize("name", sorted(_TRAINS))
 @pytest.mark.parametrize("method", ["LU", "CI"])
-def test_compress_equals_oracle(tci, ctx, name, method):
+def test_compress_equals_oracle(tci, ctx, name, method, route):
     cores = _TRAINS[name]
     for kw in (dict(tolerance=1e-12), dict(maxbonddim=5), dict(tolerance=1.0),
                dict(tolerance=1e-6, normalizeerror=False), dict(tolerance=1e-9, maxbonddim=12)):
@@ -72,7 +82,7 @@ def test_compress_equals_oracle(tci, ctx, name, method):
 @pytest.mark.parametrize("name", sorted(_TRAINS))
 @pytest.mark.parametrize("method", ["LU", "CI"])
 @pytest.mark.parametrize("forward", [True, False])
-def test_sweep_equals_oracle(tci, ctx, name, method, forward):
+def test_sweep_equals_oracle(tci, ctx, name, method, forward, route):
     cores = _TRAINS[name]
     for kw in (dict(maxrank=O.INT64_MAX, reltol=1e-14, abstol=0.0), dict(maxrank=4, reltol=1e-14, abstol=0.0),
                dict(maxrank=O.INT64_MAX, reltol=1e-14, abstol=1e-7)):
@@ -85,7 +95,7 @@ def test_sweep_equals_oracle(tci, ctx, name, method, forward):
         assert np.array_equal(pe, ope)
 
 
-def test_zero_pivot_error_matches(tci, ctx):
+def test_zero_pivot_error_matches(tci, ctx, route):
     col = np.array([1.0, 2.0, 0.5, 4.0])
     c0 = np.stack([col, col], 1).reshape((1, 4, 2), order="F")
     c1 = np.arange(1.0, 7.0).reshape((2, 3, 1), order="F")
