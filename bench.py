@@ -416,15 +416,17 @@ def rrlu_section(args, tci, ctx, cpu):
     permh = perm.cpu().numpy()
     rng = np.random.default_rng(77)
     sample = sorted(rng.choice(cnt, size=min(args.rrlu_check, cnt), replace=False).tolist())
-    mism = 0
-    for i in sample:
+    def check(i):
         m, n = shapes[i]
         o = O.rrlu(host[offs[i]:offs[i] + m * n].reshape((m, n), order="F"), reltol=1e-10)
-        same = (o.npivot == jobs[i].npivot and
+        return (o.npivot == jobs[i].npivot and
                 np.array_equal(permh[poff[i]:poff[i] + m], o.rowpermutation) and
                 np.array_equal(permh[poff[i] + m:poff[i] + m + n], o.colpermutation) and
                 o.error == jobs[i].error)
-        mism += not same
+
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=cpu["threads"]) as ex:   # the oracle's C calls release the GIL
+        mism = sum(not ok for ok in ex.map(check, sample))
     out["pivot_check"] = {"matrices": len(sample), "mismatches": int(mism),
                           "note": "seeded sample of the same launch's matrices; rowpermutation, colpermutation, "
                                   "npivot and error compared with the oracle bit for bit, outside the timed region"}
@@ -616,7 +618,7 @@ def main():
     ap.add_argument("--rrlu-count", type=int, default=4096)
     ap.add_argument("--rrlu-steps", type=int, default=3)
     ap.add_argument("--rrlu-warmup", type=int, default=1)
-    ap.add_argument("--rrlu-check", type=int, default=48, help="matrices whose pivots are checked against the oracle")
+    ap.add_argument("--rrlu-check", type=int, default=512, help="matrices whose pivots are checked against the oracle")
     ap.add_argument("--c3-batch", type=int, default=2048)
     ap.add_argument("--c4-batch", type=int, default=512)
     ap.add_argument("--c5-requests", type=int, default=10000)
