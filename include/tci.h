@@ -436,7 +436,10 @@ tci_status tci_batch_free(tci_batch *b);
  * batch.  `shape` is the largest problem expected (it sizes the grid's LDS
  * plan; a larger one still runs, with less on chip).  The grid leaves after
  * `idle_ms` without a running or new request and is started again by the
- * next submit or poll.  Its stream has a hardware queue of its own, so the
+ * next submit or poll.  team: threads per solve, 256 (two solves per CU,
+ * the most requests in flight) or 512 (one per CU, each solve faster; for
+ * streams that keep fewer requests in flight than there are CUs); 0: the
+ * context's default team.  Its stream has a hardware queue of its own, so the
  * library's own copies never wait behind it; work on the legacy default
  * stream, which waits for every blocking stream, waits for the grid to leave
  * (environment TCI_SERVE_QUEUE=shared: a plain non-blocking stream instead,
@@ -460,7 +463,7 @@ tci_status tci_batch_free(tci_batch *b);
  * and large solve launches on the device share it the same way. */
 typedef struct tci_server tci_server;
 tci_status tci_server_start(tci_ctx *ctx, const tci_problem *shape, int32_t slots, int32_t idle_ms,
-                            tci_server **out);
+                            int32_t team, tci_server **out);
 tci_status tci_server_submit(tci_server *s, const tci_problem *p, tci_batch **out);
 tci_status tci_server_poll(tci_server *s, tci_batch **out, int32_t cap, int32_t *count);
 /* tci_server_submit for problems[0..count) in order, published together:

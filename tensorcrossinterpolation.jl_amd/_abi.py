@@ -236,7 +236,7 @@ def lib():
                                       C.c_double, C.c_int64, C.c_int32]
     L.tci_tt_sweep_f64.argtypes = [vp, C.c_int32, P(C.c_int64), P(C.c_int32), P(C.c_double), C.c_int32, C.c_int32,
                                    C.c_int64, C.c_double, C.c_double, P(C.c_int32), P(C.c_double)]
-    L.tci_server_start.argtypes = [vp, P(tci_problem), C.c_int32, C.c_int32, P(vp)]
+    L.tci_server_start.argtypes = [vp, P(tci_problem), C.c_int32, C.c_int32, C.c_int32, P(vp)]
     L.tci_server_submit.argtypes = [vp, P(tci_problem), P(vp)]
     L.tci_server_stop.argtypes = [vp]
     L.tci_server_poll.argtypes = [vp, P(vp), C.c_int32, P(C.c_int32)]

@@ -2264,9 +2264,11 @@ static void server_release(tci_server *s)
     delete s;
 }
 
-tci_status tci_server_start(tci_ctx *ctx, const tci_problem *shape, int32_t slots, int32_t idle_ms, tci_server **out)
+tci_status tci_server_start(tci_ctx *ctx, const tci_problem *shape, int32_t slots, int32_t idle_ms, int32_t team,
+                            tci_server **out)
 {
-    if (!ctx || !shape || !out || slots < 1 || slots > (1 << 20) || idle_ms < 1 || idle_ms > 60000)
+    if (!ctx || !shape || !out || slots < 1 || slots > (1 << 20) || idle_ms < 1 || idle_ms > 60000 ||
+        (team != 0 && team != 256 && team != 512))
         return set_err(TCI_ERR_ARG, "bad arguments");
     *out = nullptr;
     SolveCaps c;
@@ -2277,7 +2279,7 @@ tci_status tci_server_start(tci_ctx *ctx, const tci_problem *shape, int32_t slot
     tci_server *s = new tci_server();
     s->ctx = ctx;
     s->cap = slots;
-    s->team = ctx->team == 512 ? 512 : 256;
+    s->team = team ? team : ctx->team == 512 ? 512 : 256;
     s->dfree.resize(64);
     s->hfree.resize(64);
     auto fail = [&](hipError_t e, const char *what) {
@@ -2337,7 +2339,7 @@ tci_status tci_server_start(tci_ctx *ctx, const tci_problem *shape, int32_t slot
     const int64_t sbytes = ((((int64_t)NSETKIND * c.L * c.setcap * c.L) + 15) & ~(int64_t)15) + 4 * (int64_t)NSETKIND * c.L;
     const int stat = serve_static_lds(s->team);
     if (stat < 0) return fail(hipErrorInvalidValue, "serving kernel attributes");
-    s->plan = make_plan(ctx, (int64_t)ctx->lds_wg - stat, c.mrcap, c.mrcap * c.L, sbytes, lut);
+    s->plan = make_plan(ctx, (int64_t)lds_for_team(ctx, s->team) - stat, c.mrcap, c.mrcap * c.L, sbytes, lut);
     s->slot.assign((size_t)slots, nullptr);
     *out = s;
     return TCI_OK;

@@ -204,15 +204,19 @@ class SlotServer:
     shape: the largest request expected, as (f, localdims, initialpivots,
     kwargs) — it sizes the grid's on-chip plan.  slots: the queue's ring
     (requests in flight at most).  idle_ms: the grid leaves after this long
-    without a request (and starts again with the next one)."""
+    without a request (and starts again with the next one).  team: threads
+    per solve, 256 (two solves per CU: the most in flight) or 512 (one per
+    CU, each solve faster: for streams with fewer requests in flight than
+    CUs); 0 takes the context's default."""
 
-    def __init__(self, ctx, shape, slots=16384, idle_ms=50):
+    def __init__(self, ctx, shape, slots=16384, idle_ms=50, team=0):
         from . import _holder
         self.ctx = ctx
         h = _holder(*shape)
         self.slots = int(slots)
         self.h = C.c_void_p()
-        st = _abi.lib().tci_server_start(ctx.handle, C.byref(h.st), int(slots), int(idle_ms), C.byref(self.h))
+        st = _abi.lib().tci_server_start(ctx.handle, C.byref(h.st), int(slots), int(idle_ms), int(team),
+                                         C.byref(self.h))
         if st:
             raise ServeError(st, _abi.lib().tci_last_error().decode())
 

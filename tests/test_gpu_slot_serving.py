@@ -61,7 +61,7 @@ def test_slot_server_stop_drains_and_queue_full(tci, ctx):
     holders = [tci._holder(*q) for q in reqs]
     shape = tci._holder(*_shape(tci))
     h = C.c_void_p()
-    assert lib.tci_server_start(ctx.handle, C.byref(shape.st), 2, 50, C.byref(h)) == 0
+    assert lib.tci_server_start(ctx.handle, C.byref(shape.st), 2, 50, 0, C.byref(h)) == 0
     bs = []
     try:
         for k in range(2):
@@ -86,7 +86,7 @@ def test_slot_server_poll_returns_each_once(tci, ctx):
     holders = [tci._holder(*q) for q in reqs]
     shape = tci._holder(*_shape(tci))
     h = C.c_void_p()
-    assert lib.tci_server_start(ctx.handle, C.byref(shape.st), 16, 50, C.byref(h)) == 0
+    assert lib.tci_server_start(ctx.handle, C.byref(shape.st), 16, 50, 0, C.byref(h)) == 0
     handles = {}
     try:
         for k, hd in enumerate(holders):
@@ -140,7 +140,7 @@ def test_slot_server_submit_many_partial_bursts(tci, ctx):
     holders = [tci._holder(*q) for q in reqs]
     shape = tci._holder(*_shape(tci))
     h = C.c_void_p()
-    assert lib.tci_server_start(ctx.handle, C.byref(shape.st), 4, 50, C.byref(h)) == 0
+    assert lib.tci_server_start(ctx.handle, C.byref(shape.st), 4, 50, 512, C.byref(h)) == 0
     got = {}
     try:
         nxt = 0

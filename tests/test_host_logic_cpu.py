@@ -156,7 +156,7 @@ class _StubServerLib:
         self.chi_caps = []
         self.bursts = []
 
-    def tci_server_start(self, ctx, shape, slots, idle, out):
+    def tci_server_start(self, ctx, shape, slots, idle, team, out):
         out._obj.value = 77
         return 0
 
