@@ -585,14 +585,18 @@ def compress(tt, method="LU", tolerance=1e-12, maxbonddim=TCI_UNLIMITED, normali
     tt._unpack(links, ld, flat)
 
 
-def _kron_left(Iset, d):
-    """kronecker(Iset, localdim) (tensorci2.jl:315-320): Iset index fastest."""
-    return [list(i) + [s] for s in range(1, d + 1) for i in Iset]
+def _kron_left_at(Iset, d, q):
+    """Entry q (0-based) of kronecker(Iset, localdim) (tensorci2.jl:315-320),
+    whose Iset index runs fastest, without building the product."""
+    s, i = divmod(q, len(Iset))
+    return list(Iset[i]) + [s + 1]
 
 
-def _kron_right(d, Jset):
-    """kronecker(localdim, Jset) (tensorci2.jl:322-327): local index fastest."""
-    return [[s] + list(j) for j in Jset for s in range(1, d + 1)]
+def _kron_right_at(d, Jset, q):
+    """Entry q (0-based) of kronecker(localdim, Jset) (tensorci2.jl:322-327),
+    whose local index runs fastest, without building the product."""
+    j, s = divmod(q, d)
+    return [s + 1] + list(Jset[j])
 
 
 def tt_sweep(tt, method="CI", forward=True, maxrank=TCI_UNLIMITED, reltol=1e-14, abstol=0.0, ctx=None):
@@ -634,12 +638,12 @@ def _sweep1sitegetindices(tt, forward, spectator=None, maxbonddim=TCI_UNLIMITED,
     for i in range(L - 1):
         if forward:
             b = i   # site b, bond b
-            indexset.append([_kron_left(indexset[-1], ld[b])[r - 1] for r in rows[b]])
+            indexset.append([_kron_left_at(indexset[-1], ld[b], r - 1) for r in rows[b]])
             if spectator:
                 spectator[b] = [spectator[b][c - 1] for c in cols[b]]
         else:
             b = L - 2 - i   # site b+1, bond b
-            indexset.append([_kron_right(ld[b + 1], indexset[-1])[c - 1] for c in cols[b]])
+            indexset.append([_kron_right_at(ld[b + 1], indexset[-1], c - 1) for c in cols[b]])
             if spectator:
                 spectator[b + 1] = [spectator[b + 1][r - 1] for r in rows[b]]
     return (indexset if forward else indexset[::-1]), pe

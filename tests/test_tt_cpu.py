@@ -324,3 +324,18 @@ def test_estimatetrueerror_host_order(monkeypatch):
     assert len(res) == len({(tuple(map(int, piv[q])), float(e[q])) for q in range(len(e))})
     with pytest.raises(tci.TCIError, match="No search"):
         tci.estimatetrueerror(tci.TensorTrain(cores), g, nsearch=0)
+
+
+def test_kronecker_entries_follow_the_reference_order():
+    """kronecker(Iset, d) and kronecker(d, Jset) (tensorci2.jl:315-327) are
+    Julia comprehensions over (set, local index) flattened column-major: the
+    set index runs fastest on the left, the local index on the right.  The
+    index sweep reads single entries of them by index arithmetic."""
+    I = [[1], [2]]
+    left = [[1, 1], [2, 1], [1, 2], [2, 2], [1, 3], [2, 3]]
+    assert [tci._kron_left_at(I, 3, q) for q in range(6)] == left
+    J = [[4, 5], [6, 7]]
+    right = [[1, 4, 5], [2, 4, 5], [3, 4, 5], [1, 6, 7], [2, 6, 7], [3, 6, 7]]
+    assert [tci._kron_right_at(3, J, q) for q in range(6)] == right
+    assert tci._kron_left_at([[]], 4, 2) == [3]
+    assert tci._kron_right_at(4, [[]], 3) == [4]
