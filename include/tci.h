@@ -456,7 +456,10 @@ tci_status tci_batch_free(tci_batch *b);
  * earlier poll to out (*count of them) and starts the grid again if requests
  * wait and it has left.
  * tci_server_stop runs every published request to completion, copies their
- * reports to the host and releases the server; the batches stay valid.
+ * reports to the host and releases the server; the batches stay valid.  If
+ * the grid fails first, stop returns TCI_ERR_DEVICE and every request that
+ * had not run reports TCI_ERR_DEVICE from its queries, waits and results
+ * (tci_batch_free still releases it).
  * Calls on one server, and on its batches, are serialized by the caller.
  * One server per device at a time: a second server's grid becomes resident
  * only as the first one's workgroups leave (after its idle_ms without work),

@@ -217,7 +217,8 @@ struct tci_batch {
     // a request of a slot-level server (no events: its done mark is polled)
     bool served = false;
     bool polled = false;               // returned by tci_server_poll
-    tci_server *server = nullptr;      // null once the server stopped (the ticket ran)
+    bool lost = false;                 // its server stopped on an error before the ticket ran
+    tci_server *server = nullptr;      // null once the server stopped
     uint64_t ticket = 0;
     ServeBlock dblk{}, hblk{};         // its device block and pinned staging block
     int dcls = 0, hcls = 0;            // their size classes
@@ -2113,7 +2114,8 @@ tci_status tci_batch_free(tci_batch *b)
     }
     if (b->served) {
         // the grid may still be running it: its blocks are released once done
-        const tci_status w = fetch_reports(b);
+        // (a lost request has no grid left to wait for)
+        const tci_status w = b->lost ? TCI_OK : fetch_reports(b);
         if (w != TCI_OK) return w;
         if (tci_server *s = b->server) {
             s->slot[(size_t)(b->ticket % (uint64_t)s->cap)] = nullptr;
@@ -2243,7 +2245,11 @@ static bool served_done(const tci_server *s, const tci_batch *b)
 static tci_status served_query(tci_batch *b)
 {
     tci_server *s = b->server;
-    if (!s) return TCI_OK;   // the server stopped after draining every ticket
+    if (!s) {
+        // the server stopped: it drained every ticket unless it failed first
+        if (b->lost) return set_err(TCI_ERR_DEVICE, "the server stopped on an error before this request ran");
+        return TCI_OK;
+    }
     if (served_done(s, b)) return TCI_OK;
     const tci_status e = serve_ensure(s);
     return e != TCI_OK ? e : TCI_ERR_PENDING;
@@ -2492,11 +2498,14 @@ tci_status tci_server_stop(tci_server *s)
     for (tci_batch *b : s->slot)
         if (b) {
             // every ticket ran: its report comes to the host now, and its
-            // blocks stay with it (their chunks outlive the server)
+            // blocks stay with it (their chunks outlive the server).  After
+            // a failure, a ticket without its done mark never ran: its
+            // queries, waits and results report the failure from then on
             if (r == TCI_OK) {
                 const tci_status w = fetch_reports(b);
                 if (w != TCI_OK) r = w;
             }
+            if (!b->fetched && !served_done(s, b)) b->lost = true;
             b->server = nullptr;
         }
     server_release(s);
