@@ -571,11 +571,10 @@ __global__ __launch_bounds__(kTtTeam) void tci_tt_sweep_kernel(TtSweepArgs a)
         for (int64_t i = tm.tid(); i < m; i += kTtTeam) rp[i] += 1;
         for (int64_t j = tm.tid(); j < n; j += kTtTeam) cp[j] += 1;
         tm.sync();
-        // running pivot errors (the diagonal of the packed factors) and the pivots
-        if (tm.tid() == 0) {
-            for (int k = 0; k < p; ++k) a.pe[k] = tt_nan_max(a.pe[k], fabs(A[k + m * k]));
-            a.pe[p] = tt_nan_max(a.pe[p], err);
-        }
+        // running pivot errors (the diagonal of the packed factors, one
+        // thread per entry) and the pivots
+        for (int k = tm.tid(); k < p; k += kTtTeam) a.pe[k] = tt_nan_max(a.pe[k], fabs(A[k + m * k]));
+        if (tm.tid() == 0) a.pe[p] = tt_nan_max(a.pe[p], err);
         int32_t *pv = a.piv + (int64_t)b * 2 * a.R;
         for (int k = tm.tid(); k < p; k += kTtTeam) {
             pv[k] = rp[k];
