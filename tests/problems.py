@@ -102,8 +102,8 @@ def c4_spec(nsearch=5):
 
 
 def config_specs():
-    """The solve configurations at their full sizes, in the survey's parity
-    mode (no global search, fully deterministic) and with the optimize!
+    """The solve configurations at their full sizes, in parity mode (no
+    global search, fully deterministic) and with the optimize!
     defaults (global search with the project PRNG)."""
     return [c1_spec(0), c1_spec(5), c3_spec(0), c3_spec(5), c4_spec(0), c4_spec(5)]
 
